@@ -1,0 +1,225 @@
+/*
+ * rbw.h -- C ABI of the MI355X wavefront path tracer (librbw.so).
+ *
+ * The library replaces the render call of rainbow-cpu's path integrator:
+ *
+ *   integrator::render(camera, scene)        rainbow-cpu/integrators/integrator.hpp:64-66
+ *   sampler_integrator::render               rainbow-cpu/integrators/sampler_integrator.cpp:19-144
+ *   path_integrator::trace                   rainbow-cpu/integrators/path_integrator.cpp:15-47
+ *
+ * The caller (the C++ wavefront_path_integrator, or any FFI) flattens the
+ * scene and camera into the plain descriptors below.  Every pointer is a
+ * host pointer owned by the caller; the library copies what it needs.
+ * Errors are reported as rbw_status codes, with a message available from
+ * rbw_last_error() on the calling thread; no C++ exception crosses the ABI.
+ * One context owns one HIP stream; calls on one context are not thread-safe.
+ */
+#ifndef RBW_H
+#define RBW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rbw_status;
+
+enum {
+    RBW_OK = 0,
+    RBW_ERR_INVALID_ARGUMENT = 1,
+    RBW_ERR_OUT_OF_MEMORY = 2,
+    RBW_ERR_DEVICE = 3,
+    RBW_ERR_NO_SCENE = 4,
+    RBW_ERR_UNSUPPORTED = 5
+};
+
+#define RBW_NONE 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------ scene */
+
+/* shape kinds: mesh (rainbow-cpu/shapes/mesh.hpp:11) and sphere (shapes/sphere.hpp:7) */
+enum { RBW_SHAPE_NONE = 0, RBW_SHAPE_MESH = 1, RBW_SHAPE_SPHERE = 2 };
+
+/* entity flags */
+enum {
+    RBW_ENTITY_EMITTER = 1u,             /* surface_light component (emitters/surface_light.hpp:7) */
+    RBW_ENTITY_REVERSE_ORIENTATION = 2u, /* shape::reverse_orientation() (shapes/shape.hpp:81) */
+    RBW_ENTITY_HAS_NORMALS = 4u,         /* mesh::has_normal() (shapes/mesh.cpp:215) */
+    RBW_ENTITY_HAS_UVS = 8u              /* mesh::has_uv() (shapes/mesh.cpp:220) */
+};
+
+/* materials: matte (materials/matte_material.cpp:13-29, Lambert only: sigma must be 0)
+ *            metal (materials/metal_material.cpp:17-37, GGX conductor, visible-normal sampling) */
+enum { RBW_MATERIAL_MATTE = 1, RBW_MATERIAL_METAL = 2 };
+
+typedef struct rbw_material {
+    uint32_t type;
+    float diffuse[3];   /* matte reflectance */
+    float sigma;        /* matte Oren-Nayar sigma (only 0 is supported) */
+    float eta[3];       /* metal: conductor eta */
+    float k[3];         /* metal: conductor absorption */
+    float alpha_u;      /* metal: Trowbridge-Reitz alpha_x (after any roughness map) */
+    float alpha_v;      /* metal: Trowbridge-Reitz alpha_y */
+} rbw_material;
+
+/* One rainbow entity (scenes/entity.hpp:45-70): shape + material + emitter.
+ * Mesh entities hold world-space triangles [first, first+count) and carry the
+ * identity transform; sphere entities hold sphere index `first` and carry the
+ * translation to the sphere centre (the reference's sphere lives at the
+ * origin of its local space, shapes/sphere.cpp:17-86). */
+typedef struct rbw_entity {
+    uint32_t shape;     /* RBW_SHAPE_* */
+    uint32_t first;
+    uint32_t count;
+    uint32_t material;  /* index into materials, or RBW_NONE (invisible pass-through) */
+    uint32_t flags;     /* RBW_ENTITY_* */
+    float radiance[3];  /* surface_light radiance when RBW_ENTITY_EMITTER */
+} rbw_entity;
+
+typedef struct rbw_scene_desc {
+    uint32_t num_vertices;
+    const float* positions;          /* 3 * num_vertices, world space */
+    const float* normals;            /* 3 * num_vertices or NULL */
+    const float* uvs;                /* 2 * num_vertices or NULL */
+    uint32_t num_triangles;
+    const uint32_t* indices;         /* 3 * num_triangles */
+    const uint32_t* triangle_entity; /* entity of each triangle */
+    const uint32_t* triangle_material; /* per-triangle material or NULL (entity material) */
+    uint32_t num_spheres;
+    const float* spheres;            /* 4 * num_spheres: centre xyz, radius */
+    const uint32_t* sphere_entity;
+    uint32_t num_entities;
+    const rbw_entity* entities;      /* in scene::add_entity order (scenes/scene.cpp:15-28) */
+    uint32_t num_materials;
+    const rbw_material* materials;
+} rbw_scene_desc;
+
+/* ----------------------------------------------------------------- camera */
+
+/* perspective_camera (cameras/perspective_camera.cpp:78-99) flattened:
+ * both matrices are 4x4 column-major float32 as built by the host. */
+typedef struct rbw_camera_desc {
+    float raster_to_camera[16];
+    float camera_to_world[16];
+    uint32_t width, height;          /* film resolution */
+    int32_t bound_min[2];            /* film pixels_bound (cameras/film.cpp:186-204) */
+    int32_t bound_max[2];
+    float lens_radius;               /* 0 = pinhole */
+    float focus_distance;
+} rbw_camera_desc;
+
+/* ----------------------------------------------------------------- render */
+
+enum {
+    /* one pcg32 stream per camera sample, seeded from (seed, pixel, sample);
+     * the draw order per sample is the reference's */
+    RBW_SAMPLER_PER_SAMPLE_PCG = 0,
+    /* replay of recorded per-sample draws (the reference's per-tile stream) */
+    RBW_SAMPLER_REPLAY = 1
+};
+
+enum { RBW_TRI_MOLLER_TRUMBORE = 0, RBW_TRI_WATERTIGHT = 1 };
+
+enum {
+    RBW_SHADOW_CLOSEST_VISIBLE = 0,  /* scene.cpp:97-112 semantics */
+    RBW_SHADOW_ANY_HIT = 1           /* occlusion test with early exit */
+};
+
+/* debug_hits records, per camera sample and bounce, three u32 primitive ids:
+ * [extension hit, shadow-ray hit, MIS-ray hit].  Triangles are numbered
+ * 0..num_triangles-1, spheres num_triangles+i.  RBW_NONE = ray not traced,
+ * RBW_MISS = traced and missed. */
+#define RBW_MISS 0xFFFFFFFEu
+
+typedef struct rbw_render_params {
+    uint32_t spp;
+    uint32_t max_depth;
+    float rr_threshold;              /* path_integrator threshold (path_integrator.hpp:9-12) */
+    uint32_t seed;
+    uint32_t sampler;                /* RBW_SAMPLER_* */
+    uint32_t triangle_test;          /* RBW_TRI_* */
+    uint32_t shadow_mode;            /* RBW_SHADOW_* */
+    uint32_t samples_per_pass;       /* camera samples per pixel in flight per pass; 0 = auto */
+    /* RBW_SAMPLER_REPLAY: draws of sample (pixel p = y*width + x, index s) are
+     * replay_values[replay_offsets[p*spp+s] .. replay_offsets[p*spp+s+1]) */
+    const uint64_t* replay_offsets;  /* width*height*spp + 1 entries */
+    const float* replay_values;
+    uint64_t replay_count;
+    uint32_t* debug_hits;            /* NULL or width*height*spp*max_depth*3 entries */
+    uint32_t count_visits;           /* nonzero: count BVH node visits and primitive tests */
+} rbw_render_params;
+
+enum {
+    RBW_KERNEL_RAYGEN = 0,
+    RBW_KERNEL_EXTEND = 1,
+    RBW_KERNEL_SHADE = 2,
+    RBW_KERNEL_CONNECT = 3,
+    RBW_KERNEL_FILM = 4,
+    RBW_KERNEL_COUNT = 8
+};
+
+typedef struct rbw_stats {
+    double render_ms;                /* HIP-event time of the whole render on the context stream */
+    double msamples_per_s;
+    double mrays_per_s;
+    uint64_t samples;
+    uint64_t camera_rays;
+    uint64_t extension_rays;         /* includes camera rays */
+    uint64_t shadow_rays;
+    uint64_t mis_rays;
+    uint64_t light_casts;            /* light-local pdf casts (shapes/shape.cpp:46-60) */
+    uint64_t node_visits;            /* when count_visits */
+    uint64_t prim_tests;             /* when count_visits */
+    uint64_t extend_node_visits;     /* node visits of extension rays only */
+    uint64_t extend_prim_tests;
+    double kernel_ms[RBW_KERNEL_COUNT];
+    uint64_t kernel_launches[RBW_KERNEL_COUNT];
+} rbw_stats;
+
+typedef struct rbw_device_opts {
+    int32_t device;                  /* HIP device ordinal */
+} rbw_device_opts;
+
+typedef struct rbw_ctx rbw_ctx;
+
+typedef struct rbw_hit {
+    uint32_t prim;                   /* primitive id or RBW_MISS */
+    uint32_t entity;                 /* entity id or RBW_NONE */
+    float t;                         /* entity-local ray parameter of the hit */
+    float b1, b2;                    /* triangle barycentrics (0 for spheres) */
+} rbw_hit;
+
+/* Context with its own HIP stream on `opts->device` (NULL = device 0). */
+rbw_status rbw_create(const rbw_device_opts* opts, rbw_ctx** out);
+void rbw_destroy(rbw_ctx* ctx);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* rbw_last_error(void);
+
+/* Flattens, builds the BVH and uploads the scene (scene::build_accelerator,
+ * scenes/scene.cpp:30-80).  Replaces any previous scene of the context. */
+rbw_status rbw_upload_scene(rbw_ctx* ctx, const rbw_scene_desc* scene);
+
+/* The render call: writes the film's pixel values, width*height*3 floats in
+ * row-major RGB, into out_rgb (host memory; NULL keeps the film on the device).
+ * Pixels outside the camera's bound are written as 0. */
+rbw_status rbw_render(rbw_ctx* ctx, const rbw_camera_desc* camera,
+                      const rbw_render_params* params, float* out_rgb,
+                      rbw_stats* stats);
+
+/* Closest-hit (visible_only = 0, scenes/scene.cpp:82-95) or closest-visible
+ * (visible_only = 1, scenes/scene.cpp:97-112) queries for n world-space rays:
+ * origins/directions 3*n floats (directions normalised by the caller as
+ * shared::ray does, shared/ray.cpp:7-10), tmax n floats (world length). */
+rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
+                          const float* directions, const float* tmax,
+                          uint32_t visible_only, uint32_t triangle_test,
+                          rbw_hit* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBW_H */

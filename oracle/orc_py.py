@@ -1,0 +1,166 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (oracle/liborc.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product path (rainbow_amd, librbw.so) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from rainbow_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+REF_PCG = os.path.join(HERE, "_ref", "libref_pcg.so")
+
+ORC_SAMPLER_TILE, ORC_SAMPLER_PER_SAMPLE, ORC_SAMPLER_REPLAY = 0, 1, 2
+
+
+class orc_render_opts(C.Structure):
+    _fields_ = [("sampler", C.c_uint32), ("threads", C.c_uint32), ("tile_stride", C.c_uint32),
+                ("record_offsets", abi.u64p), ("record_values", abi.f32p), ("record_capacity", C.c_uint64),
+                ("record_count", C.c_uint64)]
+
+
+class orc_stats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("samples", C.c_uint64), ("camera_rays", C.c_uint64),
+                ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("mis_rays", C.c_uint64),
+                ("light_casts", C.c_uint64), ("threads", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+    if os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_scene_create.argtypes = [C.POINTER(abi.rbw_scene_desc), C.POINTER(C.c_void_p)]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_scene_num_bvh_nodes.argtypes = [C.c_void_p]
+        L.orc_scene_num_bvh_nodes.restype = C.c_uint32
+        L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.rbw_camera_desc), C.POINTER(abi.rbw_render_params),
+                                 C.POINTER(orc_render_opts), abi.f32p, C.POINTER(orc_stats)]
+        for fn in (L.orc_trace_rays, L.orc_trace_rays_linear):
+            fn.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, abi.f32p, C.c_uint32, C.c_void_p]
+        L.orc_pcg32_reals.argtypes = [C.c_uint64, C.c_uint32, abi.f32p]
+        L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint32, abi.u32p]
+        L.orc_concentric_sample_disk.argtypes = [C.c_float, C.c_float, abi.f32p]
+        L.orc_cosine_sample_hemisphere.argtypes = [C.c_float, C.c_float, abi.f32p]
+        L.orc_uniform_sample_triangle.argtypes = [C.c_float, C.c_float, abi.f32p]
+        L.orc_ggx_D.argtypes = [C.c_float, C.c_float, abi.f32p]
+        L.orc_ggx_D.restype = C.c_float
+        L.orc_ggx_lambda.argtypes = [C.c_float, C.c_float, abi.f32p]
+        L.orc_ggx_lambda.restype = C.c_float
+        L.orc_ggx_sample_wh.argtypes = [C.c_float, C.c_float, abi.f32p, C.c_float, C.c_float, abi.f32p]
+        L.orc_ggx_sample_f.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, C.c_float, C.c_float,
+                                       abi.f32p, abi.f32p, abi.f32p]
+        L.orc_ggx_pdf.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p]
+        L.orc_ggx_pdf.restype = C.c_float
+        L.orc_ggx_eval.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, abi.f32p, abi.f32p]
+        L.orc_fresnel_conductor.argtypes = [C.c_float, abi.f32p, abi.f32p, abi.f32p]
+        _lib = L
+    return _lib
+
+
+def f3(v):
+    return np.ascontiguousarray(np.asarray(v, np.float32).reshape(3))
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self.scene = scene
+        self.desc = scene.desc()
+        self.h = C.c_void_p()
+        rc = lib().orc_scene_create(C.byref(self.desc), C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"orc_scene_create failed: {rc}")
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            try:
+                lib().orc_scene_destroy(self.h)
+            except Exception:
+                pass
+
+    def num_bvh_nodes(self):
+        return lib().orc_scene_num_bvh_nodes(self.h)
+
+    def trace(self, origins, directions, tmax, visible_only=False, linear=False):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tmax, np.float32).reshape(-1)
+        out = np.zeros(len(o), abi.HIT_DTYPE)
+        fn = lib().orc_trace_rays_linear if linear else lib().orc_trace_rays
+        rc = fn(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p), abi.ptr(t, abi.f32p),
+                int(visible_only), out.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        return out
+
+    def render(self, cam, params, sampler=ORC_SAMPLER_TILE, threads=0, tile_stride=1, record=False,
+               debug=False):
+        W, H = cam.width, cam.height
+        out = np.zeros((H, W, 3), np.float32)
+        opts = orc_render_opts(sampler=sampler, threads=threads, tile_stride=tile_stride)
+        keep = []
+        if record:
+            n = W * H * params.spp
+            offs = np.zeros(n + 1, np.uint64)
+            cap = n * (4 + 8 * params.max_depth)
+            vals = np.zeros(cap, np.float32)
+            opts.record_offsets = abi.ptr(offs, abi.u64p)
+            opts.record_values = abi.ptr(vals, abi.f32p)
+            opts.record_capacity = cap
+            keep += [offs, vals]
+        dbg = None
+        if debug:
+            dbg = np.full(W * H * params.spp * params.max_depth * 3, abi.RBW_NONE, np.uint32)
+            params.debug_hits = abi.ptr(dbg, abi.u32p)
+        st = orc_stats()
+        rc = lib().orc_render(self.h, C.byref(cam), C.byref(params), C.byref(opts), abi.ptr(out, abi.f32p),
+                              C.byref(st))
+        if debug:
+            params.debug_hits = abi.ptr(None, abi.u32p)
+        if rc != 0:
+            raise RuntimeError(f"orc_render failed: {rc}")
+        res = {"image": out, "stats": st.as_dict()}
+        if record:
+            res["replay_offsets"] = keep[0]
+            res["replay_values"] = keep[1][: opts.record_count].copy()
+        if debug:
+            res["debug_hits"] = dbg.reshape(H * W * params.spp, params.max_depth, 3)
+        return res
+
+
+def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
+    p = abi.rbw_render_params(spp=spp, max_depth=max_depth, rr_threshold=rr_threshold, seed=seed)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def pcg32_reals(seed, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_pcg32_reals(seed, n, abi.ptr(out, abi.f32p))
+    return out
+
+
+def pcg32_u32(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32_u32(seed, n, abi.ptr(out, abi.u32p))
+    return out
