@@ -1,0 +1,97 @@
+"""Python handle over librbw.so: the MI355X wavefront path tracer.
+
+Mirrors the reference's render call (integrator::render, rainbow-cpu/
+integrators/integrator.hpp:64-66): a scene is uploaded once, then render()
+fills a film for a camera.  Every call goes through the C ABI; there is no
+Python or CPU fallback -- a missing library or device is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class RbwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rbw error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is None:
+        _lib = abi.load_library()
+    return _lib
+
+
+def _check(rc):
+    if rc != abi.RBW_OK:
+        raise RbwError(rc, library().rbw_last_error().decode())
+
+
+class Tracer:
+    def __init__(self, device=0):
+        self.lib = library()
+        self.h = C.c_void_p()
+        opts = abi.rbw_device_opts(device=device)
+        _check(self.lib.rbw_create(C.byref(opts), C.byref(self.h)))
+        self._scene_desc = None
+
+    def close(self):
+        if self.h:
+            self.lib.rbw_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene):
+        desc = scene.desc()
+        _check(self.lib.rbw_upload_scene(self.h, C.byref(desc)))
+        self._scene_desc = desc
+        return self
+
+    def render(self, cam, params, debug=False, stats=True):
+        W, H = cam.width, cam.height
+        out = np.zeros((H, W, 3), np.float32)
+        dbg = None
+        if debug:
+            dbg = np.full(W * H * params.spp * params.max_depth * 3, abi.RBW_NONE, np.uint32)
+            params.debug_hits = abi.ptr(dbg, abi.u32p)
+        st = abi.rbw_stats()
+        try:
+            _check(self.lib.rbw_render(self.h, C.byref(cam), C.byref(params), abi.ptr(out, abi.f32p),
+                                       C.byref(st) if stats else None))
+        finally:
+            if debug:
+                params.debug_hits = abi.ptr(None, abi.u32p)
+        res = {"image": out, "stats": st.as_dict()}
+        if debug:
+            res["debug_hits"] = dbg.reshape(H * W * params.spp, params.max_depth, 3)
+        return res
+
+    def trace(self, origins, directions, tmax, visible_only=False, triangle_test=abi.RBW_TRI_MOLLER_TRUMBORE):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tmax, np.float32).reshape(-1)
+        out = np.zeros(len(o), abi.HIT_DTYPE)
+        _check(self.lib.rbw_trace_rays(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p),
+                                       abi.ptr(t, abi.f32p), int(visible_only), int(triangle_test),
+                                       out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
+    p = abi.rbw_render_params(spp=spp, max_depth=max_depth, rr_threshold=rr_threshold, seed=seed)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
