@@ -1,0 +1,44 @@
+"""Algorithmic byte model of the tracer kernels (for roofline fractions).
+
+Per unit of work each kernel must move at least these bytes between HBM and
+the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
+
+  node visit      64 B   one bvh2 node (two child boxes + child refs)
+  primitive test  48 B   one leaf slot (three float4: p0|id, p1, p2)
+  extend ray      36 B   queue index 4 + ray origin/direction 24 (read)
+                         + hit prim/t 8 (write)
+  shade path     264 B   queue 4, hit 8, ray 24, beta 12, value 12, rng 8,
+                         pixel/sample/bounce 12 (read); 3 vertex positions 36,
+                         3 normals 36, 3 indices 12, entity 48, material 48 (gather);
+                         ray 24, beta 12, value 12, rng 8, bounce 4 (write) -- the
+                         connect record writes are counted in connect
+  connect record 132 B   queue 4, flags 4, shadow ray 28 + light id 4, contribution 12,
+                         vertex point/normal 24, BSDF direction/value/pdf 28, beta 12,
+                         value 12 (read), value 12 (write) ... written once by shade
+                         and read once here: both sides counted here (x1)
+  light cast      96 B   two light triangles' vertex positions and indices
+"""
+S_NODE = 64
+S_PRIM = 48
+B_EXTEND_RAY = 36
+B_SHADE_PATH = 264
+B_CONNECT_REC = 132
+B_LIGHT_CAST = 96
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def kernel_bytes(stats):
+    """Algorithmic bytes of a render per kernel kind, from an instrumented run's counters."""
+    ext_nodes = stats["extend_node_visits"]
+    ext_prims = stats["extend_prim_tests"]
+    con_nodes = stats["node_visits"] - ext_nodes
+    con_prims = stats["prim_tests"] - ext_prims
+    ext_rays = stats["extension_rays"]
+    records = max(stats["shadow_rays"], stats["mis_rays"])
+    return {
+        "extend": ext_rays * B_EXTEND_RAY + ext_nodes * S_NODE + ext_prims * S_PRIM,
+        "shade": ext_rays * B_SHADE_PATH,
+        "connect": records * B_CONNECT_REC + con_nodes * S_NODE + con_prims * S_PRIM
+        + stats["light_casts"] * B_LIGHT_CAST,
+    }

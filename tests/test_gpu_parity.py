@@ -125,3 +125,30 @@ def test_render_deterministic(tracer, cornell):
     assert not pixel_mismatch(a, a2).any()
     assert np.count_nonzero(a != a2) <= 64
     assert not pixel_mismatch(a, b).any()
+
+
+@pytest.fixture(scope="module")
+def icosphere(tracer):
+    sc, cam = scenes.displaced_icosphere_box(level=8, width=1024, height=1024)
+    tracer.upload(sc)
+    orc = _oracle()
+    return sc, cam, orc.OracleScene(sc)
+
+
+def test_render_icosphere_window(tracer, icosphere):
+    """1.31M-triangle instanced mesh with per-triangle materials: a 64x64 window of the
+    1024x1024 film at 64 spp, depth 8 (Russian roulette active from depth 4)."""
+    sc, cam, os_ = icosphere
+    orc = _oracle()
+    cam.bound_min[:] = (480, 440)
+    cam.bound_max[:] = (544, 504)
+    try:
+        c = os_.render(cam, orc.render_params(spp=64, max_depth=8, seed=5), sampler=orc.ORC_SAMPLER_PER_SAMPLE,
+                       debug=True)
+        g = tracer.render(cam, render_params(spp=64, max_depth=8, seed=5), debug=True)
+    finally:
+        cam.bound_min[:] = (0, 0)
+        cam.bound_max[:] = (1024, 1024)
+    frac, idf = _compare_render(g, c, "icosphere window")
+    assert idf < 1e-5
+    assert frac < 1e-3
