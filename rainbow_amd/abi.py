@@ -15,6 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librbw.so")
 
 RBW_OK = 0
+RBW_ERR_INVALID_ARGUMENT = 1
+RBW_ERR_OUT_OF_MEMORY = 2
+RBW_ERR_DEVICE = 3
+RBW_ERR_NO_SCENE = 4
+RBW_ERR_UNSUPPORTED = 5
 RBW_NONE = 0xFFFFFFFF
 RBW_MISS = 0xFFFFFFFE
 

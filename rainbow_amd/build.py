@@ -17,6 +17,29 @@ HIPCC_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-cont
                "-Wno-unused-variable", "-Wno-unused-result"]
 
 
+HOST_LIB = os.path.join(HERE, "librbw_host.so")
+TEST_BIN = os.path.join(HERE, "..", "tests", "cpp", "build", "cornell_integrator")
+
+
+def build_host(verbose=False):
+    """The rainbow-style C++ host interface (librbw_host.so) and its C++ test driver."""
+    cxx = os.environ.get("CXX", "g++")
+    host = os.path.join(HERE, "host", "rainbow_host.cpp")
+    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", host, "-L" + HERE, "-lrbw",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.makedirs(os.path.dirname(TEST_BIN), exist_ok=True)
+    src = os.path.join(HERE, "..", "tests", "cpp", "cornell_integrator.cpp")
+    cmd = [cxx, "-std=c++17", "-O2", "-o", TEST_BIN, src, "-L" + HERE, "-lrbw_host", "-lrbw",
+           "-Wl,-rpath," + os.path.relpath(HERE, os.path.dirname(TEST_BIN)).replace("..", "$ORIGIN/..", 1)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(CSRC, "tracer.hip"), os.path.join(CSRC, "bvh.cpp")]
@@ -26,6 +49,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    build_host(verbose)
     return OUT
 
 

@@ -1,0 +1,371 @@
+// Host-side mirror of rainbow's render interfaces and the wavefront path
+// integrator that renders through the C ABI (include/rbw.h).
+#include "rainbow_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+namespace rainbow::core::math {
+
+matrix4x4 matrix4x4::inverse() const {
+    // Gauss-Jordan with partial pivoting on the row-major view (double)
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            a[r][c] = m[c * 4 + r];
+            a[r][c + 4] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 4; r++)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) throw std::runtime_error("singular matrix");
+        if (piv != col)
+            for (int c = 0; c < 8; c++) std::swap(a[piv][c], a[col][c]);
+        const double inv = 1.0 / a[col][col];
+        for (int c = 0; c < 8; c++) a[col][c] *= inv;
+        for (int r = 0; r < 4; r++) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f == 0.0) continue;
+            for (int c = 0; c < 8; c++) a[r][c] -= f * a[col][c];
+        }
+    }
+    matrix4x4 out;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out.m[c * 4 + r] = a[r][c + 4];
+    return out;
+}
+
+}  // namespace rainbow::core::math
+
+namespace rainbow::cpus {
+
+namespace shared {
+
+bool transform::is_identity() const {
+    const matrix4x4 I = matrix4x4::identity();
+    for (int i = 0; i < 16; i++)
+        if ((float)mTransform.m[i] != (float)I.m[i]) return false;
+    return true;
+}
+
+bool transform::is_translation(vector3& t) const {
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 4; r++)
+            if ((float)mTransform.m[c * 4 + r] != (c == r ? 1.0f : 0.0f)) return false;
+    if ((float)mTransform.m[15] != 1.0f) return false;
+    t = vector3{(float)mTransform.m[12], (float)mTransform.m[13], (float)mTransform.m[14]};
+    return true;
+}
+
+transform translate(const vector3& v) {
+    matrix4x4 m = matrix4x4::identity(), i = matrix4x4::identity();
+    m.m[12] = v.x; m.m[13] = v.y; m.m[14] = v.z;
+    i.m[12] = -v.x; i.m[13] = -v.y; i.m[14] = -v.z;
+    return transform(m, i);
+}
+
+transform scale(const vector3& v) {
+    matrix4x4 m = matrix4x4::identity(), i = matrix4x4::identity();
+    m.m[0] = v.x; m.m[5] = v.y; m.m[10] = v.z;
+    i.m[0] = 1.0 / v.x; i.m[5] = 1.0 / v.y; i.m[10] = 1.0 / v.z;
+    return transform(m, i);
+}
+
+// glm::perspectiveFovRH_NO
+transform perspective_right_hand(real fov, real width, real height, real near, real far) {
+    const double h = std::cos(0.5 * fov) / std::sin(0.5 * fov);
+    const double w = h * height / width;
+    matrix4x4 m;
+    m.m[0] = w;
+    m.m[5] = h;
+    m.m[10] = -(double(far) + near) / (double(far) - near);
+    m.m[11] = -1.0;
+    m.m[14] = -(2.0 * far * near) / (double(far) - near);
+    return transform(m, m.inverse());
+}
+
+// glm::lookAtRH
+transform look_at_right_hand(const vector3& origin, const vector3& target, const vector3& up) {
+    double f[3] = {double(target.x) - origin.x, double(target.y) - origin.y, double(target.z) - origin.z};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& x : f) x /= fl;
+    const double u0[3] = {up.x, up.y, up.z};
+    double s[3] = {f[1] * u0[2] - f[2] * u0[1], f[2] * u0[0] - f[0] * u0[2], f[0] * u0[1] - f[1] * u0[0]};
+    double sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (double& x : s) x /= sl;
+    const double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    const double e[3] = {origin.x, origin.y, origin.z};
+    matrix4x4 m = matrix4x4::identity();
+    for (int k = 0; k < 3; k++) {
+        m.m[k * 4 + 0] = s[k];
+        m.m[k * 4 + 1] = u[k];
+        m.m[k * 4 + 2] = -f[k];
+    }
+    m.m[12] = -(s[0] * e[0] + s[1] * e[1] + s[2] * e[2]);
+    m.m[13] = -(u[0] * e[0] + u[1] * e[1] + u[2] * e[2]);
+    m.m[14] = (f[0] * e[0] + f[1] * e[1] + f[2] * e[2]);
+    return transform(m, m.inverse());
+}
+
+}  // namespace shared
+
+namespace materials {
+real roughness_to_alpha(real roughness) {
+    roughness = std::max(roughness, static_cast<real>(1e-3));
+    const real x = std::log(roughness);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+}  // namespace materials
+
+namespace cameras {
+
+film::film(const std::shared_ptr<filters::filter>& filter, const vector2i& resolution, const bound2& crop_window,
+           real scale)
+    : mFilter(filter), mResolution(resolution) {
+    (void)scale;
+    // cameras/film.cpp:186-204
+    mPixelsBound.min = vector2i{(int32_t)std::ceil(mResolution.x * crop_window.min.x),
+                                (int32_t)std::ceil(mResolution.y * crop_window.min.y)};
+    mPixelsBound.max = vector2i{(int32_t)std::ceil(mResolution.x * crop_window.max.x),
+                                (int32_t)std::ceil(mResolution.y * crop_window.max.y)};
+    mPixels.assign(size_t(resolution.x) * resolution.y, spectrum(0));
+}
+
+perspective_camera::perspective_camera(const std::shared_ptr<cameras::film>& film, const camera_system& system,
+                                       const bound2& sw, const shared::transform& projective,
+                                       const shared::transform& transform, real focus, real lens)
+    : camera(film, system, transform), mCameraToScreen(projective), mFocus(focus), mLens(lens) {
+    // cameras/camera.cpp:30-51: ScreenToRaster = scale(res) * scale(1/w, 1/-h) * translate(-min.x, -max.y)
+    const vector2i res = mFilm->resolution();
+    mScreenToRaster = shared::scale(vector3{(float)res.x, (float)res.y, 1.0f});
+    matrix4x4 s2 = matrix4x4::identity();
+    s2.m[0] = 1.0 / (double(sw.max.x) - sw.min.x);
+    s2.m[5] = 1.0 / (double(sw.min.y) - sw.max.y);
+    mScreenToRaster = mScreenToRaster * shared::transform(s2, s2.inverse());
+    matrix4x4 t = matrix4x4::identity();
+    t.m[12] = -double(sw.min.x);
+    t.m[13] = -double(sw.max.y);
+    mScreenToRaster = mScreenToRaster * shared::transform(t, t.inverse());
+    mRasterToCamera = shared::transform(mCameraToScreen.matrix().inverse() * mScreenToRaster.matrix().inverse(),
+                                        matrix4x4::identity());
+}
+
+}  // namespace cameras
+
+namespace integrators {
+
+namespace {
+
+void require(bool c, const char* msg) {
+    if (!c) throw std::runtime_error(msg);
+}
+
+// glm mat4 * vec4 in float: (m0*x + m1*y) + (m2*z + m3*w)
+void mat_mul_f(const float* m, const float* v, float w, float* out) {
+    for (int r = 0; r < 3; r++) out[r] = (m[r] * v[0] + m[4 + r] * v[1]) + (m[8 + r] * v[2] + m[12 + r] * w);
+}
+
+void to_float(const matrix4x4& m, float* out) {
+    for (int i = 0; i < 16; i++) out[i] = (float)m.m[i];
+}
+
+}  // namespace
+
+rbw_scene_desc flat_scene::desc() const {
+    rbw_scene_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.num_vertices = uint32_t(positions.size() / 3);
+    d.positions = positions.data();
+    d.normals = has_normals ? normals.data() : nullptr;
+    d.uvs = nullptr;
+    d.num_triangles = uint32_t(indices.size() / 3);
+    d.indices = indices.data();
+    d.triangle_entity = triangle_entity.data();
+    d.triangle_material = nullptr;
+    d.num_spheres = uint32_t(spheres.size() / 4);
+    d.spheres = spheres.empty() ? nullptr : spheres.data();
+    d.sphere_entity = sphere_entity.empty() ? nullptr : sphere_entity.data();
+    d.num_entities = uint32_t(entities.size());
+    d.entities = entities.data();
+    d.num_materials = uint32_t(materials.size());
+    d.materials = materials.data();
+    return d;
+}
+
+flat_scene flatten(const scene& s) {
+    flat_scene f;
+    std::map<const materials::material*, uint32_t> mat_index;
+    for (const auto& ep : s.entities()) {
+        const scenes::entity& e = *ep;
+        rbw_entity out;
+        std::memset(&out, 0, sizeof(out));
+        out.material = RBW_NONE;
+        if (auto m = e.material()) {
+            auto it = mat_index.find(m.get());
+            if (it == mat_index.end()) {
+                rbw_material rm;
+                std::memset(&rm, 0, sizeof(rm));
+                if (auto mm = std::dynamic_pointer_cast<materials::matte_material>(m)) {
+                    rm.type = RBW_MATERIAL_MATTE;
+                    for (int c = 0; c < 3; c++) rm.diffuse[c] = mm->diffuse->value()[c];
+                    rm.sigma = std::min(std::max(mm->sigma->value(), 0.0f), 90.0f);
+                } else if (auto mt = std::dynamic_pointer_cast<materials::metal_material>(m)) {
+                    rm.type = RBW_MATERIAL_METAL;
+                    for (int c = 0; c < 3; c++) {
+                        rm.eta[c] = mt->eta->value()[c];
+                        rm.k[c] = mt->k->value()[c];
+                    }
+                    const real ru = mt->roughness_u->value(), rv = mt->roughness_v->value();
+                    rm.alpha_u = mt->map_roughness_to_alpha ? materials::roughness_to_alpha(ru) : ru;
+                    rm.alpha_v = mt->map_roughness_to_alpha ? materials::roughness_to_alpha(rv) : rv;
+                } else {
+                    throw std::runtime_error("unsupported material");
+                }
+                it = mat_index.emplace(m.get(), uint32_t(f.materials.size())).first;
+                f.materials.push_back(rm);
+            }
+            out.material = it->second;
+        }
+        if (auto em = e.emitter()) {
+            auto sl = std::dynamic_pointer_cast<emitters::surface_light>(em);
+            require(sl != nullptr, "only surface_light emitters are supported");
+            out.flags |= RBW_ENTITY_EMITTER;
+            for (int c = 0; c < 3; c++) out.radiance[c] = sl->radiance[c];
+        }
+        const uint32_t eid = uint32_t(f.entities.size());
+        auto sh = e.shape();
+        if (!sh) {
+            out.shape = RBW_SHAPE_NONE;
+        } else if (auto me = std::dynamic_pointer_cast<shapes::mesh>(sh)) {
+            require(me->tangents.empty(), "mesh tangents are not supported");
+            const uint32_t base = uint32_t(f.positions.size() / 3);
+            const bool ident = e.transform().is_identity();
+            float M[16], Minv[16];
+            to_float(e.transform().matrix(), M);
+            to_float(e.transform().inverse_matrix(), Minv);
+            for (const vector3& p : me->positions) {
+                float q[3] = {p.x, p.y, p.z}, r[3];
+                if (ident) {
+                    f.positions.insert(f.positions.end(), {p.x, p.y, p.z});
+                } else {
+                    mat_mul_f(M, q, 1.0f, r);  // transform_point (shared/transform.cpp:172-177)
+                    f.positions.insert(f.positions.end(), {r[0], r[1], r[2]});
+                }
+            }
+            if (!me->normals.empty()) {
+                require(me->normals.size() == me->positions.size(), "normals must match positions");
+                f.has_normals = true;
+                out.flags |= RBW_ENTITY_HAS_NORMALS;
+                for (const vector3& n : me->normals) {
+                    if (ident) {
+                        f.normals.insert(f.normals.end(), {n.x, n.y, n.z});
+                    } else {
+                        // transform_normal: normalize(transpose(inverse) * n)
+                        float T[16];
+                        for (int c = 0; c < 4; c++)
+                            for (int r = 0; r < 4; r++) T[c * 4 + r] = Minv[r * 4 + c];
+                        float q[3] = {n.x, n.y, n.z}, r[3];
+                        mat_mul_f(T, q, 0.0f, r);
+                        const float l = 1.0f / std::sqrt((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
+                        f.normals.insert(f.normals.end(), {r[0] * l, r[1] * l, r[2] * l});
+                    }
+                }
+            } else {
+                f.normals.insert(f.normals.end(), me->positions.size() * 3, 0.0f);
+            }
+            require(me->uvs.empty(), "mesh uvs are not carried by this flattening");
+            if (sh->reverse_orientation()) out.flags |= RBW_ENTITY_REVERSE_ORIENTATION;
+            out.shape = RBW_SHAPE_MESH;
+            out.first = uint32_t(f.indices.size() / 3);
+            out.count = uint32_t(me->indices.size() / 3);
+            for (unsigned i : me->indices) f.indices.push_back(base + i);
+            f.triangle_entity.insert(f.triangle_entity.end(), out.count, eid);
+        } else if (auto sp = std::dynamic_pointer_cast<shapes::sphere>(sh)) {
+            vector3 t;
+            require(e.transform().is_translation(t), "sphere entities need a translation transform");
+            out.shape = RBW_SHAPE_SPHERE;
+            out.first = uint32_t(f.spheres.size() / 4);
+            out.count = 1;
+            if (sh->reverse_orientation()) out.flags |= RBW_ENTITY_REVERSE_ORIENTATION;
+            f.spheres.insert(f.spheres.end(), {t.x, t.y, t.z, sp->radius()});
+            f.sphere_entity.push_back(eid);
+        } else {
+            throw std::runtime_error("unsupported shape");
+        }
+        f.entities.push_back(out);
+    }
+    f.triangle_material.assign(f.triangle_entity.size(), RBW_NONE);
+    return f;
+}
+
+rbw_camera_desc flatten(const cameras::perspective_camera& c) {
+    rbw_camera_desc d;
+    std::memset(&d, 0, sizeof(d));
+    to_float(c.raster_to_camera().matrix(), d.raster_to_camera);
+    to_float(c.camera_to_world().matrix(), d.camera_to_world);
+    const auto film = c.film();
+    d.width = uint32_t(film->resolution().x);
+    d.height = uint32_t(film->resolution().y);
+    const bound2i b = film->pixels_bound();
+    d.bound_min[0] = b.min.x;
+    d.bound_min[1] = b.min.y;
+    d.bound_max[0] = b.max.x;
+    d.bound_max[1] = b.max.y;
+    d.lens_radius = c.lens();
+    d.focus_distance = c.focus();
+    return d;
+}
+
+wavefront_path_integrator::wavefront_path_integrator(const std::shared_ptr<samplers::sampler2d>& sampler2d,
+                                                     const std::shared_ptr<samplers::sampler1d>& sampler1d,
+                                                     size_t max_depth, real threshold, int device)
+    : mSampler2D(sampler2d), mSampler1D(sampler1d), mMaxDepth(max_depth), mThreshold(threshold), mDevice(device) {}
+
+wavefront_path_integrator::~wavefront_path_integrator() {
+    if (mCtx) rbw_destroy(mCtx);
+}
+
+void wavefront_path_integrator::render(const std::shared_ptr<camera>& cam, const std::shared_ptr<scene>& sc) {
+    auto pc = std::dynamic_pointer_cast<cameras::perspective_camera>(cam);
+    require(pc != nullptr, "wavefront_path_integrator renders perspective cameras");
+    require(sc != nullptr, "scene is null");
+    auto check = [](rbw_status s) {
+        if (s != RBW_OK) throw std::runtime_error(std::string("rbw: ") + rbw_last_error());
+    };
+    if (!mCtx) {
+        rbw_device_opts o{mDevice};
+        check(rbw_create(&o, &mCtx));
+    }
+    if (mUploaded != sc.get() || mUploadedGeneration != sc->generation()) {
+        const flat_scene fs = flatten(*sc);
+        const rbw_scene_desc d = fs.desc();
+        check(rbw_upload_scene(mCtx, &d));
+        mUploaded = sc.get();
+        mUploadedGeneration = sc->generation();
+    }
+    const rbw_camera_desc cd = flatten(*pc);
+    rbw_render_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.spp = uint32_t(mSampler2D->samples_per_pixel());
+    p.max_depth = uint32_t(mMaxDepth);
+    p.rr_threshold = mThreshold;
+    p.seed = seed;
+    p.sampler = RBW_SAMPLER_PER_SAMPLE_PCG;
+    p.triangle_test = triangle_test;
+    p.shadow_mode = shadow_mode;
+    std::vector<float> rgb(size_t(cd.width) * cd.height * 3);
+    check(rbw_render(mCtx, &cd, &p, rgb.data(), &last_stats));
+    // film::set_pixel (cameras/film.cpp:202-205) for every pixel in bounds
+    auto film = pc->film();
+    for (int32_t y = cd.bound_min[1]; y < cd.bound_max[1]; y++)
+        for (int32_t x = cd.bound_min[0]; x < cd.bound_max[0]; x++) {
+            const size_t i = size_t(y) * cd.width + x;
+            film->set_pixel(i, cameras::spectrum(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]));
+        }
+}
+
+}  // namespace integrators
+}  // namespace rainbow::cpus

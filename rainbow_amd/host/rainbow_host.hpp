@@ -1,0 +1,369 @@
+// Host-side mirror of rainbow's render interfaces (C++17), and the drop-in
+// integrator that renders through librbw.so.
+//
+// The classes keep the reference's names, namespaces, constructor arguments
+// and ownership (std::shared_ptr everywhere):
+//   integrators::integrator::render(camera, scene)   rainbow-cpu/integrators/integrator.hpp:58-71
+//   integrators::wavefront_path_integrator           in place of path_integrator (path_integrator.hpp:7-27)
+//   scenes::scene / scenes::entity                   scenes/scene.hpp:18-44, scenes/entity.hpp:36-70
+//   shapes::mesh / shapes::sphere                    shapes/mesh.hpp:13-38, shapes/sphere.hpp:7-39
+//   materials::matte_material / metal_material       materials/matte_material.hpp, metal_material.hpp
+//   emitters::surface_light                          emitters/surface_light.hpp:7-13
+//   cameras::film / cameras::perspective_camera      cameras/film.hpp:52-91, cameras/perspective_camera.hpp:7-27
+//   filters::box_filter, samplers::random_sampler    filters/box_filter.hpp, samplers/random_sampler.hpp
+// The reference builds its matrices with glm; this mirror composes them in
+// double and rounds to float.  Accessors the reference keeps private (sphere
+// radius, material textures, camera matrices) are public here because the
+// flattening into the C ABI needs them.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rbw.h"
+
+namespace rainbow::core {
+using real = float;
+using uint32 = unsigned;
+using uint64 = unsigned long long;
+namespace math {
+struct vector2 { real x = 0, y = 0; };
+struct vector2i { int32_t x = 0, y = 0; };
+struct vector3 { real x = 0, y = 0, z = 0; };
+struct bound2 { vector2 min{0, 0}, max{1, 1}; };
+struct bound2i { vector2i min, max; };
+// column-major m[col * 4 + row], as glm
+struct matrix4x4 {
+    std::array<double, 16> m{};
+    static matrix4x4 identity() {
+        matrix4x4 r;
+        for (int i = 0; i < 4; i++) r.m[i * 5] = 1.0;
+        return r;
+    }
+    matrix4x4 operator*(const matrix4x4& b) const {
+        matrix4x4 r;
+        for (int c = 0; c < 4; c++)
+            for (int w = 0; w < 4; w++) {
+                double s = 0;
+                for (int k = 0; k < 4; k++) s += m[k * 4 + w] * b.m[c * 4 + k];
+                r.m[c * 4 + w] = s;
+            }
+        return r;
+    }
+    matrix4x4 inverse() const;
+};
+}  // namespace math
+}  // namespace rainbow::core
+
+namespace rainbow::cpus {
+
+using core::real;
+using core::math::bound2;
+using core::math::bound2i;
+using core::math::matrix4x4;
+using core::math::vector2;
+using core::math::vector2i;
+using core::math::vector3;
+
+namespace interfaces {
+class noncopyable {
+public:
+    noncopyable() = default;
+    virtual ~noncopyable() = default;
+    noncopyable(const noncopyable&) = delete;
+    noncopyable& operator=(const noncopyable&) = delete;
+};
+}  // namespace interfaces
+
+namespace shared {
+namespace spectrums {
+struct spectrum {
+    std::array<real, 3> c{0, 0, 0};
+    spectrum() = default;
+    spectrum(real v) : c{v, v, v} {}
+    spectrum(real r, real g, real b) : c{r, g, b} {}
+    real operator[](size_t i) const { return c[i]; }
+};
+}  // namespace spectrums
+
+// shared/transform.hpp:16-48: a matrix with its inverse
+class transform {
+public:
+    transform() : mTransform(matrix4x4::identity()), mInverse(matrix4x4::identity()) {}
+    transform(const matrix4x4& m, const matrix4x4& inv) : mTransform(m), mInverse(inv) {}
+    transform operator*(const transform& r) const { return transform(mTransform * r.mTransform, r.mInverse * mInverse); }
+    transform inverse() const { return transform(mInverse, mTransform); }
+    const matrix4x4& matrix() const noexcept { return mTransform; }
+    const matrix4x4& inverse_matrix() const noexcept { return mInverse; }
+    bool is_identity() const;
+    bool is_translation(vector3& t) const;
+private:
+    matrix4x4 mTransform, mInverse;
+};
+
+transform translate(const vector3& v);
+transform scale(const vector3& v);
+transform perspective_right_hand(real fov, real width, real height, real near = 0.01f, real far = 1000.0f);
+transform look_at_right_hand(const vector3& origin, const vector3& target, const vector3& up);
+}  // namespace shared
+
+namespace textures {
+template <typename T>
+class constant_texture2d final : public interfaces::noncopyable {
+public:
+    explicit constant_texture2d(const T& value) : mValue(value) {}
+    const T& value() const noexcept { return mValue; }
+private:
+    T mValue;
+};
+template <typename T>
+using texture2d = constant_texture2d<T>;
+}  // namespace textures
+
+namespace shapes {
+class shape : public interfaces::noncopyable, public std::enable_shared_from_this<shape> {
+public:
+    explicit shape(bool reverse_orientation, size_t count) : mReverseOrientation(reverse_orientation), mCount(count) {}
+    bool reverse_orientation() const noexcept { return mReverseOrientation; }
+    size_t count() const noexcept { return mCount; }
+protected:
+    bool mReverseOrientation = false;
+    size_t mCount = 0;
+};
+
+// shapes/mesh.hpp: per-vertex positions, optional tangents/normals/uvs, 3 indices per face
+class mesh final : public shape {
+public:
+    mesh(const std::vector<vector3>& positions, const std::vector<vector3>& tangents,
+         const std::vector<vector3>& normals, const std::vector<vector3>& uvs, const std::vector<unsigned>& indices,
+         bool reverse_orientation = false)
+        : shape(reverse_orientation, indices.size() / 3), positions(positions), tangents(tangents), normals(normals),
+          uvs(uvs), indices(indices) {}
+    std::vector<vector3> positions, tangents, normals, uvs;
+    std::vector<unsigned> indices;
+};
+
+class sphere final : public shape {
+public:
+    explicit sphere(real radius, bool reverse_orientation = false) : shape(reverse_orientation, 1), mRadius(radius) {}
+    real radius() const noexcept { return mRadius; }
+private:
+    real mRadius;
+};
+}  // namespace shapes
+
+namespace materials {
+using textures::texture2d;
+using shared::spectrums::spectrum;
+class material : public interfaces::noncopyable {};
+
+class matte_material final : public material {
+public:
+    matte_material(const std::shared_ptr<texture2d<spectrum>>& diffuse, const std::shared_ptr<texture2d<real>>& sigma)
+        : diffuse(diffuse), sigma(sigma) {}
+    std::shared_ptr<texture2d<spectrum>> diffuse;
+    std::shared_ptr<texture2d<real>> sigma;
+};
+
+class metal_material final : public material {
+public:
+    metal_material(const std::shared_ptr<texture2d<spectrum>>& eta, const std::shared_ptr<texture2d<spectrum>>& k,
+                   const std::shared_ptr<texture2d<real>>& roughness_u,
+                   const std::shared_ptr<texture2d<real>>& roughness_v, bool map_roughness_to_alpha = true)
+        : eta(eta), k(k), roughness_u(roughness_u), roughness_v(roughness_v), map_roughness_to_alpha(map_roughness_to_alpha) {}
+    std::shared_ptr<texture2d<spectrum>> eta, k;
+    std::shared_ptr<texture2d<real>> roughness_u, roughness_v;
+    bool map_roughness_to_alpha;
+};
+
+// trowbridge_reitz_distribution.cpp:128-136
+real roughness_to_alpha(real roughness);
+}  // namespace materials
+
+namespace emitters {
+using shared::spectrums::spectrum;
+class emitter : public interfaces::noncopyable {};
+class surface_light final : public emitter {
+public:
+    explicit surface_light(const spectrum& radiance) : radiance(radiance) {}
+    spectrum radiance;
+};
+}  // namespace emitters
+
+namespace media {
+class media : public interfaces::noncopyable {};
+}  // namespace media
+
+namespace scenes {
+class entity final : public interfaces::noncopyable, public std::enable_shared_from_this<entity> {
+public:
+    entity(const std::shared_ptr<materials::material>& material, const std::shared_ptr<emitters::emitter>& emitter,
+           const std::shared_ptr<shapes::shape>& shape, const std::shared_ptr<media::media>& media,
+           const shared::transform& transform)
+        : mMaterial(material), mEmitter(emitter), mShape(shape), mMedia(media), mTransform(transform) {}
+    std::shared_ptr<materials::material> material() const noexcept { return mMaterial; }
+    std::shared_ptr<emitters::emitter> emitter() const noexcept { return mEmitter; }
+    std::shared_ptr<shapes::shape> shape() const noexcept { return mShape; }
+    std::shared_ptr<media::media> media() const noexcept { return mMedia; }
+    shared::transform transform() const noexcept { return mTransform; }
+private:
+    std::shared_ptr<materials::material> mMaterial;
+    std::shared_ptr<emitters::emitter> mEmitter;
+    std::shared_ptr<shapes::shape> mShape;
+    std::shared_ptr<media::media> mMedia;
+    shared::transform mTransform;
+};
+
+class scene final : public interfaces::noncopyable {
+public:
+    void add_entity(const std::shared_ptr<entity>& e) {
+        if (e->emitter()) mEmitters.push_back(e);
+        mEntities.push_back(e);
+        mGeneration++;
+    }
+    void build_accelerator() { mGeneration++; }
+    const std::vector<std::shared_ptr<entity>>& entities() const noexcept { return mEntities; }
+    const std::vector<std::shared_ptr<entity>>& emitters() const noexcept { return mEmitters; }
+    uint64_t generation() const noexcept { return mGeneration; }
+private:
+    std::vector<std::shared_ptr<entity>> mEntities, mEmitters;
+    uint64_t mGeneration = 0;
+};
+}  // namespace scenes
+
+namespace filters {
+class filter : public interfaces::noncopyable {
+public:
+    explicit filter(const vector2& radius) : mRadius(radius) {}
+    vector2 radius() const noexcept { return mRadius; }
+private:
+    vector2 mRadius;
+};
+class box_filter final : public filter {
+public:
+    explicit box_filter(const vector2& radius = vector2{0.5f, 0.5f}) : filter(radius) {}
+};
+}  // namespace filters
+
+namespace cameras {
+using shared::spectrums::spectrum;
+// cameras/film.hpp:52-91 (pixel values as the render leaves them)
+class film final : public interfaces::noncopyable {
+public:
+    film(const std::shared_ptr<filters::filter>& filter, const vector2i& resolution, const bound2& crop_window,
+         real scale = 1);
+    void set_pixel(size_t index, const spectrum& value) { mPixels[index] = value; }
+    spectrum pixel(size_t index) const { return mPixels[index]; }
+    vector2i resolution() const noexcept { return mResolution; }
+    bound2i pixels_bound() const noexcept { return mPixelsBound; }
+    std::shared_ptr<filters::filter> filter() const noexcept { return mFilter; }
+private:
+    std::shared_ptr<filters::filter> mFilter;
+    vector2i mResolution;
+    bound2i mPixelsBound;
+    std::vector<spectrum> mPixels;
+};
+
+enum class camera_system : uint32_t { right_hand = 0, left_hand = 1 };
+
+class camera : public interfaces::noncopyable {
+public:
+    camera(const std::shared_ptr<cameras::film>& film, const camera_system& system, const shared::transform& transform)
+        : mFilm(film), mCameraSystem(system), mCameraToWorld(transform) {}
+    std::shared_ptr<cameras::film> film() const noexcept { return mFilm; }
+    const shared::transform& camera_to_world() const noexcept { return mCameraToWorld; }
+protected:
+    std::shared_ptr<cameras::film> mFilm;
+    camera_system mCameraSystem;
+    shared::transform mCameraToWorld;
+};
+
+// cameras/camera.cpp:21-52 + perspective_camera.cpp:1-20
+class perspective_camera final : public camera {
+public:
+    perspective_camera(const std::shared_ptr<cameras::film>& film, const camera_system& system,
+                       const bound2& screen_window, const shared::transform& projective,
+                       const shared::transform& transform, real focus = 1e6, real lens = 0);
+    const shared::transform& raster_to_camera() const noexcept { return mRasterToCamera; }
+    real focus() const noexcept { return mFocus; }
+    real lens() const noexcept { return mLens; }
+private:
+    shared::transform mCameraToScreen, mScreenToRaster, mRasterToCamera;
+    real mFocus, mLens;
+};
+}  // namespace cameras
+
+namespace samplers {
+template <size_t Dimension>
+class random_sampler_t : public interfaces::noncopyable {
+public:
+    explicit random_sampler_t(size_t samples_per_pixel) : mSamplesPerPixel(samples_per_pixel) {}
+    size_t samples_per_pixel() const noexcept { return mSamplesPerPixel; }
+private:
+    size_t mSamplesPerPixel;
+};
+using sampler1d = random_sampler_t<1>;
+using sampler2d = random_sampler_t<2>;
+using random_sampler1d = random_sampler_t<1>;
+using random_sampler2d = random_sampler_t<2>;
+}  // namespace samplers
+
+namespace integrators {
+using cameras::camera;
+using scenes::scene;
+
+// integrators/integrator.hpp:58-71
+class integrator : public interfaces::noncopyable {
+public:
+    virtual void render(const std::shared_ptr<camera>& camera, const std::shared_ptr<scene>& scene) = 0;
+};
+
+// Flat copy of a rainbow scene in the C ABI's form, owning its arrays.
+struct flat_scene {
+    std::vector<float> positions, normals;
+    std::vector<uint32_t> indices, triangle_entity, triangle_material;
+    std::vector<float> spheres;
+    std::vector<uint32_t> sphere_entity;
+    std::vector<rbw_entity> entities;
+    std::vector<rbw_material> materials;
+    bool has_normals = false;
+    rbw_scene_desc desc() const;
+};
+
+// Flattens entities in scene order (scenes/scene.cpp:15-28).  Meshes with a
+// non-identity transform are baked to world space; spheres need a translation.
+flat_scene flatten(const scene& s);
+rbw_camera_desc flatten(const cameras::perspective_camera& c);
+
+// The drop-in: path_integrator's constructor arguments (path_integrator.hpp:9-12),
+// rendered by the MI355X wavefront tracer.
+class wavefront_path_integrator final : public integrator {
+public:
+    explicit wavefront_path_integrator(const std::shared_ptr<samplers::sampler2d>& sampler2d,
+                                       const std::shared_ptr<samplers::sampler1d>& sampler1d, size_t max_depth = 5,
+                                       real threshold = 1, int device = 0);
+    ~wavefront_path_integrator() override;
+    void render(const std::shared_ptr<camera>& camera, const std::shared_ptr<scene>& scene) override;
+
+    uint32_t seed = 0;
+    uint32_t triangle_test = RBW_TRI_MOLLER_TRUMBORE;
+    uint32_t shadow_mode = RBW_SHADOW_CLOSEST_VISIBLE;
+    rbw_stats last_stats{};
+private:
+    std::shared_ptr<samplers::sampler2d> mSampler2D;
+    std::shared_ptr<samplers::sampler1d> mSampler1D;
+    size_t mMaxDepth;
+    real mThreshold;
+    int mDevice;
+    rbw_ctx* mCtx = nullptr;
+    const scene* mUploaded = nullptr;
+    uint64_t mUploadedGeneration = 0;
+};
+}  // namespace integrators
+
+}  // namespace rainbow::cpus

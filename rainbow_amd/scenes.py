@@ -207,11 +207,15 @@ def perspective_camera(width, height, fov_deg, eye, target, up=(0, 1, 0), lens_r
     aspect = width / height
     sw_min = (-aspect, -1.0) if aspect >= 1 else (-1.0, -1.0 / aspect)
     sw_max = (aspect, 1.0) if aspect >= 1 else (1.0, 1.0 / aspect)
-    cam_to_screen = _perspective_fov_rh(math.radians(fov_deg), width, height)
+    # the reference's perspective_right_hand takes float fov/near/far (shared/transform.hpp:113)
+    fov = float(np.float32(math.radians(fov_deg)))
+    cam_to_screen = _perspective_fov_rh(fov, float(np.float32(width)), float(np.float32(height)),
+                                        near=float(np.float32(0.01)), far=1000.0)
     s2r = _scale((width, height, 1.0))
     s2r = _mul(s2r, _scale((1.0 / (sw_max[0] - sw_min[0]), 1.0 / (sw_min[1] - sw_max[1]), 1.0)))
     s2r = _mul(s2r, _translate((-sw_min[0], -sw_max[1], 0.0)))
     r2c = _mul(_inv(cam_to_screen), _inv(s2r))
+    r2c[np.abs(r2c) < 1e-12] = 0.0  # inversion residue of entries that are exactly zero
     c2w = _inv(_look_at_rh(eye, target, up))
     cam = abi.rbw_camera_desc()
     cam.raster_to_camera[:] = [float(x) for x in r2c.astype(np.float32).reshape(-1)]

@@ -1,0 +1,80 @@
+"""The rainbow-style C++ host interface (rainbow_amd/host): a scene built with
+entities, meshes, spheres, materials and a perspective camera flattens to the
+same C-ABI descriptors as the Python builders, and (on the GPU) renders the
+same film through wavefront_path_integrator::render."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rainbow_amd import abi, scenes
+from rainbow_amd.build import TEST_BIN
+
+MATERIAL_DTYPE = scenes.MATERIAL_DTYPE
+
+
+def _read_flat(path):
+    import ctypes as C
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, n):
+        nonlocal off
+        a = np.frombuffer(b, dtype, n, off)
+        off += a.nbytes
+        return a
+
+    nv, nt, ns, ne, nm = take(np.uint32, 5)
+    out = {"P": take(np.float32, 3 * nv), "N": take(np.float32, 3 * nv), "I": take(np.uint32, 3 * nt),
+           "TE": take(np.uint32, nt), "S": take(np.float32, 4 * ns), "SE": take(np.uint32, ns)}
+    out["E"] = b[off: off + ne * C.sizeof(abi.rbw_entity)]
+    off += ne * C.sizeof(abi.rbw_entity)
+    out["M"] = np.frombuffer(b, MATERIAL_DTYPE, nm, off)
+    off += nm * MATERIAL_DTYPE.itemsize
+    out["CAM"] = abi.rbw_camera_desc.from_buffer_copy(b[off: off + C.sizeof(abi.rbw_camera_desc)])
+    return out
+
+
+def _binary():
+    if not os.path.exists(TEST_BIN):
+        pytest.skip("C++ driver not built")
+    return TEST_BIN
+
+
+def test_host_flatten_matches_python_builder(tmp_path):
+    import ctypes as C
+    f = tmp_path / "flat.bin"
+    subprocess.run([_binary(), "flatten", str(f)], check=True)
+    got = _read_flat(f)
+    sc, cam = scenes.cornell_box()
+    P, N, I, TE, TM, S, SE = sc.arrays()
+    assert np.array_equal(got["P"], P.reshape(-1))
+    assert np.array_equal(got["N"], N.reshape(-1))
+    assert np.array_equal(got["I"], I.reshape(-1))
+    assert np.array_equal(got["TE"], TE)
+    assert np.array_equal(got["S"], S.reshape(-1))
+    assert np.array_equal(got["SE"], SE)
+    d = sc.desc()
+    ents = bytes((abi.rbw_entity * d.num_entities).from_address(C.addressof(d.entities.contents)))
+    assert got["E"] == ents
+    mats = np.concatenate(sc.materials)
+    assert got["M"].tobytes() == mats.tobytes()
+    gc = got["CAM"]
+    assert np.allclose(np.array(gc.raster_to_camera), np.array(cam.raster_to_camera), rtol=2e-7, atol=0)
+    assert list(gc.camera_to_world) == list(cam.camera_to_world)
+    assert (gc.width, gc.height) == (cam.width, cam.height)
+
+
+@pytest.mark.gpu
+def test_host_integrator_render_matches_c_abi(tracer, tmp_path):
+    """wavefront_path_integrator::render(camera, scene) == rbw_render on the Python-built scene."""
+    from rainbow_amd.tracer import render_params
+    f = tmp_path / "film.bin"
+    subprocess.run([_binary(), "render", str(f), "16", "5", "3"], check=True)
+    img = np.fromfile(f, np.float32).reshape(256, 256, 3)
+    sc, cam = scenes.cornell_box()
+    tracer.upload(sc)
+    ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3))["image"]
+    bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
+    assert bad.mean() < 1e-3, bad.sum()
