@@ -114,7 +114,8 @@ typedef struct rbw_camera_desc {
 
 enum {
     /* one pcg32 stream per camera sample, seeded from (seed, pixel, sample);
-     * the draw order per sample is the reference's */
+     * draws in the reference's order (sampler_integrator.cpp:104-126,
+     * integrator.cpp:118-165, 380, path_integrator.cpp:43) */
     RBW_SAMPLER_PER_SAMPLE_PCG = 0,
     /* replay of recorded per-sample draws (the reference's per-tile stream) */
     RBW_SAMPLER_REPLAY = 1
@@ -133,6 +134,22 @@ enum {
  * RBW_MISS = traced and missed. */
 #define RBW_MISS 0xFFFFFFFEu
 
+/* Learned path guiding (no counterpart in the reference): the continuation
+ * sample of every vertex with a BSDF is drawn from the MLP's vMF mixture with
+ * probability mis_fraction, else from the BSDF, and weighted by the one-sample
+ * MIS density mis_fraction * q + (1 - mis_fraction) * p_bsdf (unbiased).  With
+ * train set, the context's network is trained online between passes on the
+ * radiance the render's own paths carried back along their sampled directions. */
+typedef struct rbw_guiding_opts {
+    uint32_t enable;
+    float mis_fraction;              /* probability of sampling the guide, e.g. 0.5 */
+    uint32_t train;                  /* train between passes */
+    float learning_rate;             /* Adam learning rate, e.g. 1e-2 */
+    uint32_t batch;                  /* training samples per Adam step (<= 2^20), e.g. 2^18 */
+    uint32_t steps_per_pass;         /* Adam steps after each pass */
+    float scene_min[3], scene_max[3];/* position normalisation; all zero = scene bounds */
+} rbw_guiding_opts;
+
 typedef struct rbw_render_params {
     uint32_t spp;
     uint32_t max_depth;
@@ -149,6 +166,7 @@ typedef struct rbw_render_params {
     uint64_t replay_count;
     uint32_t* debug_hits;            /* NULL or width*height*spp*max_depth*3 entries */
     uint32_t count_visits;           /* nonzero: count BVH node visits and primitive tests */
+    const rbw_guiding_opts* guiding; /* NULL = plain BSDF continuation (the reference's) */
 } rbw_render_params;
 
 enum {
@@ -157,6 +175,9 @@ enum {
     RBW_KERNEL_SHADE = 2,
     RBW_KERNEL_CONNECT = 3,
     RBW_KERNEL_FILM = 4,
+    RBW_KERNEL_GUIDE_INFER = 5,      /* MLP inference over the guided vertices */
+    RBW_KERNEL_CONTINUE = 6,         /* guided continuation sample */
+    RBW_KERNEL_GUIDE_TRAIN = 7,      /* training-sample collection and Adam steps */
     RBW_KERNEL_COUNT = 8
 };
 
@@ -217,6 +238,26 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
                           const float* directions, const float* tmax,
                           uint32_t visible_only, uint32_t triangle_test,
                           rbw_hit* out);
+
+/* ---------------------------------------------------------------- guiding
+ * The guiding network of a context (no counterpart in the reference).
+ * Parameters use a padded layout of RBW_GUIDE_PARAMS floats: for each layer
+ * l = 0..5, W_l[out][in] row-major then b_l[out], with in = 224, 64, 64, 64,
+ * 64, 64 and out = 64, 64, 64, 64, 64, 48 (216 live inputs, 40 live outputs). */
+#define RBW_GUIDE_PARAMS 34160
+
+/* He-uniform re-initialisation from `seed`, Adam state cleared */
+rbw_status rbw_guide_reset(rbw_ctx* ctx, uint32_t seed);
+rbw_status rbw_guide_get_params(rbw_ctx* ctx, float* params /* RBW_GUIDE_PARAMS */);
+rbw_status rbw_guide_set_params(rbw_ctx* ctx, const float* params /* RBW_GUIDE_PARAMS */);
+/* batched inference: inputs n x 9 in [0,1] (position, (normal+1)/2, (wo+1)/2),
+ * out n x 48 raw head outputs (lobe j: mean xyz, kappa, weight logit at 5j..5j+4) */
+rbw_status rbw_guide_forward(rbw_ctx* ctx, uint32_t n, const float* inputs, float* out);
+/* one training step on n samples: loss = -sum c_i log q(d_i) / sum c_i.  grad
+ * (nullable, RBW_GUIDE_PARAMS) receives d loss / d params; apply = 1 also takes
+ * an Adam step with learning_rate.  loss (nullable) receives the loss. */
+rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const float* dirs, const float* c,
+                           float learning_rate, uint32_t apply, float* loss, float* grad);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,8 @@ RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL = 1, 2
 RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY = 0, 1
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
-RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "k5", "k6", "k7"]
+RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "continue", "guide_train"]
+RBW_GUIDE_PARAMS = 34160
 RBW_KERNEL_COUNT = 8
 
 f32p = C.POINTER(C.c_float)
@@ -66,12 +67,19 @@ class rbw_camera_desc(C.Structure):
                 ("bound_max", C.c_int32 * 2), ("lens_radius", C.c_float), ("focus_distance", C.c_float)]
 
 
+class rbw_guiding_opts(C.Structure):
+    _fields_ = [("enable", C.c_uint32), ("mis_fraction", C.c_float), ("train", C.c_uint32),
+                ("learning_rate", C.c_float), ("batch", C.c_uint32), ("steps_per_pass", C.c_uint32),
+                ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3)]
+
+
 class rbw_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_threshold", C.c_float),
                 ("seed", C.c_uint32), ("sampler", C.c_uint32), ("triangle_test", C.c_uint32),
                 ("shadow_mode", C.c_uint32), ("samples_per_pass", C.c_uint32),
                 ("replay_offsets", u64p), ("replay_values", f32p), ("replay_count", C.c_uint64),
-                ("debug_hits", u32p), ("count_visits", C.c_uint32)]
+                ("debug_hits", u32p), ("count_visits", C.c_uint32),
+                ("guiding", C.POINTER(rbw_guiding_opts))]
 
 
 class rbw_stats(C.Structure):
@@ -121,6 +129,11 @@ RBW_SIGNATURES = {
                                f32p, C.POINTER(rbw_stats)]),
     "rbw_trace_rays": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_uint32,
                                    C.c_void_p]),
+    "rbw_guide_reset": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "rbw_guide_get_params": (C.c_int32, [C.c_void_p, f32p]),
+    "rbw_guide_set_params": (C.c_int32, [C.c_void_p, f32p]),
+    "rbw_guide_forward": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p]),
+    "rbw_guide_train": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_uint32, f32p, f32p]),
 }
 
 

@@ -1,0 +1,467 @@
+// FP16-MFMA kernels of the path-guiding MLP (inference, training forward,
+// backward, Adam).  One wavefront owns 16 rows at a time and runs them through
+// the whole network with v_mfma_f32_16x16x16_f16 (FP32 accumulation); layer
+// outputs go through a per-wave LDS tile to become the next layer's A operand.
+//
+// MFMA operand layout (wave64, 16x16x16):
+//   A (16x16, M x K): lane l holds A[l % 16][4 * (l / 16) + 0..3]
+//   B (16x16, K x N): lane l holds B[4 * (l / 16) + 0..3][l % 16]
+//   D (16x16):        lane l holds D[4 * (l / 16) + 0..3][l % 16]
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dmath.hpp"
+#include "guiding.hpp"
+
+namespace rbw {
+namespace guide {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int ACT_STRIDE = N_FEAT_PAD + 5 * HIDDEN;  // 544 halfs per row: inputs of layers 0..5
+__host__ __device__ constexpr int act_offset(int l) { return l == 0 ? 0 : N_FEAT_PAD + (l - 1) * HIDDEN; }
+constexpr int TILE_PAD = HIDDEN + 8;  // LDS row pitch (halfs)
+
+#define GCHECK(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+__device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+
+__device__ __forceinline__ float pick9(const float* x, int v) {
+    float r = x[0];
+#pragma unroll
+    for (int i = 1; i < N_IN; i++) r = (v == i) ? x[i] : r;
+    return r;
+}
+
+// four consecutive encoded features starting at f0 (a multiple of 4): features of one
+// input scalar v = f0 / 24 at frequencies k and k+1 as (sin, cos, sin, cos)
+__device__ __forceinline__ h4 encode4(const float* x, int f0) {
+    h4 a;
+    if (f0 >= N_FEAT) {
+        a[0] = a[1] = a[2] = a[3] = (_Float16)0.0f;
+        return a;
+    }
+    const int v = f0 / 24;
+    const int k = (f0 % 24) >> 1;
+    const float xv = pick9(x, v);
+    // sin/cos(2^k pi x) = sin/cos(2 pi t) with t = 2^(k-1) x; v_sin/v_cos take turns
+    float t0 = xv * (float)(1 << k) * 0.5f;
+    float t1 = xv * (float)(1 << (k + 1)) * 0.5f;
+    t0 = t0 - floorf(t0);
+    t1 = t1 - floorf(t1);
+    a[0] = (_Float16)__builtin_amdgcn_sinf(t0);
+    a[1] = (_Float16)__builtin_amdgcn_cosf(t0);
+    a[2] = (_Float16)__builtin_amdgcn_sinf(t1);
+    a[3] = (_Float16)__builtin_amdgcn_cosf(t1);
+    return a;
+}
+
+__device__ __forceinline__ h4 load_h4(const _Float16* p) { return *reinterpret_cast<const h4*>(p); }
+
+// Forward pass of 16-row chunks.  STORE writes every layer's input (training) for
+// all rows up to the padded count; rows >= n use the last valid input row.
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w16, const float* __restrict__ w32,
+                                                 const float* __restrict__ inputs, uint32_t n, uint32_t n_pad,
+                                                 float* __restrict__ out, _Float16* __restrict__ acts) {
+    __shared__ _Float16 tile[4][16][TILE_PAD];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    const uint32_t nwaves = gridDim.x * 4;
+    for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
+        const uint32_t r0 = chunk * 16;
+        const uint32_t arow = r0 + li;  // the row this lane feeds as A operand
+        const uint32_t src = arow < n ? arow : n - 1;
+        float x[N_IN];
+#pragma unroll
+        for (int i = 0; i < N_IN; i++) x[i] = inputs[(size_t)src * N_IN + i];
+        f4 acc[4];
+        // ---- layer 0: encoded inputs (224) -> 64
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        const _Float16* W0 = w16 + layer_w_offset(0);
+        for (int s = 0; s < N_FEAT_PAD / 16; s++) {
+            const h4 a = encode4(x, 16 * s + 4 * lg);
+            if (STORE) *reinterpret_cast<h4*>(acts + (size_t)arow * ACT_STRIDE + 16 * s + 4 * lg) = a;
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                acc[t] = mfma16(a, load_h4(W0 + (16 * t + li) * N_FEAT_PAD + 16 * s + 4 * lg), acc[t]);
+        }
+        // ---- hidden layers 1..4 and the output layer 5
+        for (int l = 1; l < N_LAYERS; l++) {
+            // bias + ReLU of layer l-1, to the LDS tile (and the activation store)
+            const float* bprev = w32 + layer_b_offset(l - 1);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float b = bprev[16 * t + li];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float z = acc[t][j] + b;
+                    const _Float16 h = (_Float16)(z > 0.0f ? z : 0.0f);
+                    tile[wave][4 * lg + j][16 * t + li] = h;
+                    if (STORE) acts[(size_t)(r0 + 4 * lg + j) * ACT_STRIDE + act_offset(l) + 16 * t + li] = h;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int nt = layer_out(l) / 16;
+            const _Float16* Wl = w16 + layer_w_offset(l);
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < HIDDEN / 16; s++) {
+                const h4 a = load_h4(&tile[wave][li][16 * s + 4 * lg]);
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (t < nt) acc[t] = mfma16(a, load_h4(Wl + (16 * t + li) * HIDDEN + 16 * s + 4 * lg), acc[t]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // ---- raw outputs (48 columns, 40 live) with bias
+        const float* b5 = w32 + layer_b_offset(N_LAYERS - 1);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const float b = b5[16 * t + li];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t r = r0 + 4 * lg + j;
+                if (r < n_pad) out[(size_t)r * N_OUT_PAD + 16 * t + li] = acc[t][j] + b;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+
+// d loss / d raw output of the vMF-mixture head for one row; returns log q
+__device__ float mixture_grad(const float* o, const float* d, float scale, float* g) {
+    float w[N_LOBES], vv[N_LOBES], kap[N_LOBES], mu[N_LOBES][3], nm[N_LOBES], cosv[N_LOBES], sig[N_LOBES];
+    bool kcap[N_LOBES];
+    float amax = -3.4e38f;
+    for (int j = 0; j < N_LOBES; j++) amax = fmaxf(amax, o[5 * j + 4]);
+    float wsum = 0.0f;
+    for (int j = 0; j < N_LOBES; j++) {
+        w[j] = expf(o[5 * j + 4] - amax);
+        wsum += w[j];
+    }
+    float q = 0.0f;
+    for (int j = 0; j < N_LOBES; j++) {
+        w[j] /= wsum;
+        const float mx = o[5 * j], my = o[5 * j + 1], mz = o[5 * j + 2];
+        nm[j] = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-12f);
+        mu[j][0] = mx / nm[j];
+        mu[j][1] = my / nm[j];
+        mu[j][2] = mz / nm[j];
+        const float sp = softplus(o[5 * j + 3]) + KAPPA_MIN;
+        kcap[j] = sp >= KAPPA_MAX;
+        kap[j] = fminf(sp, KAPPA_MAX);
+        sig[j] = 1.0f / (1.0f + expf(-o[5 * j + 3]));
+        cosv[j] = mu[j][0] * d[0] + mu[j][1] * d[1] + mu[j][2] * d[2];
+        const float norm = kap[j] / (2.0f * 3.14159265f * (-expm1f(-2.0f * kap[j])));
+        vv[j] = norm * expf(kap[j] * (cosv[j] - 1.0f));
+        q += w[j] * vv[j];
+    }
+    q = fmaxf(q, 1e-30f);
+    const float gq = -scale;  // d loss / d log q (scaled)
+    for (int j = 0; j < N_LOBES; j++) {
+        const float r = w[j] * vv[j] / q;
+        const float e = expf(-2.0f * kap[j]);
+        const float dlv = 1.0f / kap[j] - 2.0f * e / (-expm1f(-2.0f * kap[j])) + (cosv[j] - 1.0f);
+        g[5 * j + 3] = kcap[j] ? 0.0f : gq * r * dlv * sig[j];
+        g[5 * j + 4] = gq * (r - w[j]);
+        float gm[3];
+        for (int c = 0; c < 3; c++) gm[c] = gq * r * kap[j] * d[c];
+        const float p = gm[0] * mu[j][0] + gm[1] * mu[j][1] + gm[2] * mu[j][2];
+        for (int c = 0; c < 3; c++) g[5 * j + c] = (gm[c] - p * mu[j][c]) / nm[j];
+    }
+    return logf(q);
+}
+
+__global__ void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict__ total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float v = i < n ? c[i] : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v != 0.0f) atomicAdd(total, v);
+}
+
+// per row: the clamped, loss-scaled output gradient (FP16, 48 columns) and the loss term
+__global__ void k_mixture_grad(const float* __restrict__ out, const float* __restrict__ dirs,
+                               const float* __restrict__ c, const float* __restrict__ csum, uint32_t n,
+                               uint32_t n_pad, _Float16* __restrict__ dz, float* __restrict__ loss) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    float lterm = 0.0f;
+    if (r < n_pad) {
+        float g[N_OUT_PAD];
+        for (int k = 0; k < N_OUT_PAD; k++) g[k] = 0.0f;
+        if (r < n) {
+            const float total = *csum;
+            const float cmean = total / (float)n;
+            const float wr = cmean > 0.0f ? fminf(c[r] / cmean, 64.0f) : 0.0f;
+            if (wr > 0.0f) {
+                const float lq = mixture_grad(out + (size_t)r * N_OUT_PAD, dirs + (size_t)r * 3, wr, g);
+                lterm = -(c[r] / total) * lq;
+            }
+        }
+        for (int k = 0; k < N_OUT_PAD; k++)
+            dz[(size_t)r * N_OUT_PAD + k] = (_Float16)fminf(fmaxf(g[k], -30000.0f), 30000.0f);
+    }
+    for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off);
+    if ((threadIdx.x & 63) == 0 && lterm != 0.0f) atomicAdd(loss, lterm);
+}
+
+// dZ_{l-1} = (dZ_l W_l) masked by the ReLU of layer l's input
+__global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restrict__ w16, int l,
+                                                       const _Float16* __restrict__ dz_in, int in_stride,
+                                                       const _Float16* __restrict__ acts, uint32_t n_pad,
+                                                       _Float16* __restrict__ dz_out) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    const int K = layer_out(l);     // reduction over layer l's outputs
+    const int N = layer_in(l);      // 64
+    const _Float16* W = w16 + layer_w_offset(l);
+    const uint32_t nwaves = gridDim.x * 4;
+    for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
+        const uint32_t r0 = chunk * 16;
+        f4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int s = 0; s < K / 16; s++) {
+            const h4 a = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                h4 b;
+#pragma unroll
+                for (int i = 0; i < 4; i++) b[i] = W[(16 * s + 4 * lg + i) * N + 16 * t + li];
+                acc[t] = mfma16(a, b, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t r = r0 + 4 * lg + j;
+                const _Float16 a = acts[(size_t)r * ACT_STRIDE + act_offset(l) + 16 * t + li];
+                const float v = (float)a > 0.0f ? fminf(fmaxf(acc[t][j], -30000.0f), 30000.0f) : 0.0f;
+                dz_out[(size_t)r * HIDDEN + 16 * t + li] = (_Float16)v;
+            }
+    }
+}
+
+// grad(W_l)[n][k] += sum_r dZ_l[r][n] * in_l[r][k]; grad(b_l)[n] += sum_r dZ_l[r][n]
+// block: one 16-row slice of n (blockIdx.y) over a chunk of rows (blockIdx.x); the
+// waves share the k tiles; the last k tile is the bias (a column of ones).
+__global__ __launch_bounds__(256) void k_backward_weight(const _Float16* __restrict__ dz, int dz_stride, int l,
+                                                         const _Float16* __restrict__ acts, uint32_t n_pad,
+                                                         uint32_t rows_per_block, float* __restrict__ grad) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    const int K = layer_in(l);
+    const int ktiles = K / 16 + 1;
+    const int tn = blockIdx.y;
+    const uint32_t rbeg = blockIdx.x * rows_per_block;
+    const uint32_t rend = rbeg + rows_per_block < n_pad ? rbeg + rows_per_block : n_pad;
+    f4 acc[4];  // up to 4 k tiles per wave (ktiles <= 15)
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t r0 = rbeg; r0 < rend; r0 += 16) {
+        h4 a;
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = dz[(size_t)(r0 + 4 * lg + i) * dz_stride + 16 * tn + li];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int tk = wave + 4 * q;
+            if (tk >= ktiles) break;
+            h4 b;
+            if (tk == ktiles - 1) {
+                const _Float16 one = (_Float16)(li == 0 ? 1.0f : 0.0f);
+                b[0] = b[1] = b[2] = b[3] = one;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) b[i] = acts[(size_t)(r0 + 4 * lg + i) * ACT_STRIDE + act_offset(l) + 16 * tk + li];
+            }
+            acc[q] = mfma16(a, b, acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int tk = wave + 4 * q;
+        if (tk >= ktiles) break;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int nrow = 16 * tn + 4 * lg + j;
+            if (tk == ktiles - 1) {
+                if (li == 0) atomicAdd(&grad[layer_b_offset(l) + nrow], acc[q][j]);
+            } else {
+                atomicAdd(&grad[layer_w_offset(l) + nrow * K + 16 * tk + li], acc[q][j]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ bool live_param(uint32_t p) {
+    // padded input columns of layer 0 and padded outputs of the last layer stay zero
+    if (p < (uint32_t)layer_b_offset(0)) return (p % N_FEAT_PAD) < N_FEAT;
+    const uint32_t w5 = layer_w_offset(N_LAYERS - 1), b5 = layer_b_offset(N_LAYERS - 1);
+    if (p >= w5 && p < b5) return (p - w5) / HIDDEN < N_OUT;
+    if (p >= b5) return p - b5 < N_OUT;
+    return true;
+}
+
+__global__ void k_adam(float* __restrict__ w32, _Float16* __restrict__ w16, float* __restrict__ m,
+                       float* __restrict__ v, float* __restrict__ grad, float inv_n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, uint32_t apply) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)N_PARAMS) return;
+    if (!apply) return;
+    if (!live_param(p)) {
+        grad[p] = 0.0f;
+        return;
+    }
+    const float g = grad[p] * inv_n;
+    const float mm = b1 * m[p] + (1.0f - b1) * g;
+    const float vv = b2 * v[p] + (1.0f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float w = w32[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    w32[p] = w;
+    w16[p] = (_Float16)w;
+    grad[p] = 0.0f;
+}
+
+__global__ void k_to_half(const float* __restrict__ w32, _Float16* __restrict__ w16) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (uint32_t)N_PARAMS) w16[p] = (_Float16)w32[p];
+}
+
+// ------------------------------------------------------------------ host
+
+void model_free(model& M) {
+    for (void* p : {(void*)M.w16, (void*)M.w32, (void*)M.m, (void*)M.v, (void*)M.grad, (void*)M.loss,
+                    (void*)M.acts, (void*)M.dz, (void*)M.out})
+        if (p) (void)hipFree(p);
+    M = model();
+}
+
+void model_set(model& M, const float* host, hipStream_t s) {
+    GCHECK(hipMemcpyAsync(M.w32, host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_to_half, dim3((N_PARAMS + 255) / 256), dim3(256), 0, s, M.w32, M.w16);
+    GCHECK(hipMemsetAsync(M.m, 0, sizeof(float) * N_PARAMS, s));
+    GCHECK(hipMemsetAsync(M.v, 0, sizeof(float) * N_PARAMS, s));
+    GCHECK(hipMemsetAsync(M.grad, 0, sizeof(float) * N_PARAMS, s));
+    M.step = 0;
+}
+
+void model_get(model& M, float* host, hipStream_t s) {
+    GCHECK(hipMemcpyAsync(host, M.w32, sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, s));
+    GCHECK(hipStreamSynchronize(s));
+}
+
+void model_init(model& M, uint32_t seed, hipStream_t s) {
+    if (!M.w32) {
+        GCHECK(hipMalloc(&M.w16, sizeof(_Float16) * N_PARAMS));
+        GCHECK(hipMalloc(&M.w32, sizeof(float) * N_PARAMS));
+        GCHECK(hipMalloc(&M.m, sizeof(float) * N_PARAMS));
+        GCHECK(hipMalloc(&M.v, sizeof(float) * N_PARAMS));
+        GCHECK(hipMalloc(&M.grad, sizeof(float) * N_PARAMS));
+        GCHECK(hipMalloc(&M.loss, sizeof(float) * 4));
+    }
+    // He-uniform, zero biases; the last layer at half scale
+    std::vector<float> p(N_PARAMS, 0.0f);
+    pcg32 g;
+    g.seed(seed);
+    for (int l = 0; l < N_LAYERS; l++) {
+        const int fan_in = l == 0 ? N_FEAT : HIDDEN;
+        const int outs = l == N_LAYERS - 1 ? N_OUT : HIDDEN;
+        const float lim = std::sqrt(6.0f / fan_in) * (l == N_LAYERS - 1 ? 0.5f : 1.0f);
+        for (int o = 0; o < outs; o++)
+            for (int i = 0; i < (l == 0 ? N_FEAT : HIDDEN); i++)
+                p[layer_w_offset(l) + o * layer_in(l) + i] = (2.0f * g.uniform() - 1.0f) * lim;
+    }
+    model_set(M, p.data(), s);
+    GCHECK(hipStreamSynchronize(s));
+}
+
+void model_ensure(model& M, uint32_t rows) {
+    const uint32_t pad = (rows + 15) & ~15u;
+    if (pad <= M.cap && M.acts) return;
+    for (void* q : {(void*)M.acts, (void*)M.dz, (void*)M.out})
+        if (q) (void)hipFree(q);
+    GCHECK(hipMalloc(&M.acts, sizeof(_Float16) * (size_t)pad * ACT_STRIDE));
+    GCHECK(hipMalloc(&M.dz, sizeof(_Float16) * (size_t)pad * (2 * HIDDEN + N_OUT_PAD)));
+    GCHECK(hipMalloc(&M.out, sizeof(float) * (size_t)pad * N_OUT_PAD));
+    M.cap = pad;
+}
+
+static uint32_t grid_waves(uint32_t n_pad) {
+    const uint32_t chunks = n_pad / 16;
+    const uint32_t blocks = (chunks + 3) / 4;
+    return blocks < 2048 ? (blocks ? blocks : 1) : 2048;
+}
+
+void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t n_pad = (n + 15) & ~15u;
+    hipLaunchKernelGGL(k_forward<false>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
+                       out, nullptr);
+    GCHECK(hipGetLastError());
+}
+
+void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
+                const adam_opts& opt, bool apply, hipStream_t s) {
+    if (n == 0) return;
+    model_ensure(M, n);
+    const uint32_t n_pad = (n + 15) & ~15u;
+    _Float16* dz5 = M.dz + (size_t)n_pad * 2 * HIDDEN;
+    _Float16* dzA = M.dz;
+    _Float16* dzB = M.dz + (size_t)n_pad * HIDDEN;
+    GCHECK(hipMemsetAsync(M.loss, 0, sizeof(float) * 4, s));
+    hipLaunchKernelGGL(k_forward<true>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
+                       M.out, M.acts);
+    hipLaunchKernelGGL(k_sum, dim3((n + 255) / 256), dim3(256), 0, s, weights, n, M.loss + 1);
+    hipLaunchKernelGGL(k_mixture_grad, dim3((n_pad + 255) / 256), dim3(256), 0, s, M.out, dirs, weights,
+                       M.loss + 1, n, n_pad, dz5, M.loss);
+    const uint32_t rows_per_block = 2048;
+    const uint32_t wblocks = (n_pad + rows_per_block - 1) / rows_per_block;
+    const _Float16* dz = dz5;
+    int dz_stride = N_OUT_PAD;
+    for (int l = N_LAYERS - 1; l >= 0; l--) {
+        hipLaunchKernelGGL(k_backward_weight, dim3(wblocks, layer_out(l) / 16), dim3(256), 0, s, dz, dz_stride, l,
+                           M.acts, n_pad, rows_per_block, M.grad);
+        if (l > 0) {
+            _Float16* nxt = (dz == dzA) ? dzB : dzA;
+            hipLaunchKernelGGL(k_backward_data, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, l, dz, dz_stride,
+                               M.acts, n_pad, nxt);
+            dz = nxt;
+            dz_stride = HIDDEN;
+        }
+    }
+    if (apply) {
+        M.step++;
+        const float c1 = 1.0f - std::pow(opt.b1, (float)M.step);
+        const float c2 = 1.0f - std::pow(opt.b2, (float)M.step);
+        adam_apply(M, n, opt, c1, c2, s);
+    }
+    GCHECK(hipGetLastError());
+}
+
+void adam_apply(model& M, uint32_t n, const adam_opts& opt, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, dim3((N_PARAMS + 255) / 256), dim3(256), 0, s, M.w32, M.w16, M.m, M.v, M.grad,
+                       1.0f / (float)n, opt.lr, opt.b1, opt.b2, opt.eps, c1, c2, 1u);
+    GCHECK(hipGetLastError());
+}
+
+}  // namespace guide
+}  // namespace rbw

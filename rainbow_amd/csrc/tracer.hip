@@ -25,6 +25,7 @@
 #include "../../include/rbw.h"
 #include "bvh.hpp"
 #include "device.hpp"
+#include "guiding.hpp"
 #include "shading.hpp"
 
 using namespace rbw;
@@ -185,7 +186,7 @@ enum fidx : uint32_t {
     F_CBX, F_CBY, F_CBZ,                         // beta at the vertex
     F_COUNT
 };
-enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_COUNT };
+enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_COUNT };
 
 constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
 
@@ -212,6 +213,34 @@ struct dcamera {
     float lens_radius, focus;
 };
 
+// guided continuation: per guide-queue row the surface record, MLP query and
+// head outputs; per (slot, bounce) the training record of the sampled direction
+enum gsurf : uint32_t { G_PX, G_PY, G_PZ, G_NX, G_NY, G_NZ, G_WX, G_WY, G_WZ, G_XX, G_XY, G_XZ, G_YX, G_YY, G_YZ,
+                        G_ZX, G_ZY, G_ZZ, G_COUNT };
+enum grec : uint32_t { R_Q0 = 0, R_DX = 9, R_DY, R_DZ, R_PDF, R_BX, R_BY, R_BZ, R_MX, R_MY, R_MZ, R_COUNT };
+
+struct dguide {
+    uint32_t enable;
+    float alpha;                 // probability of sampling the guide
+    uint32_t train;
+    f3 smin, sinv;               // position normalisation: (p - smin) * sinv
+    float* surf;                 // G_COUNT * cap (SoA)
+    uint32_t* surf_mat;          // material of the row
+    uint32_t* gq;                // guide queue: slot of each row
+    uint32_t* gq_count;
+    float* query;                // 9 per row
+    float* mix;                  // 48 per row (head outputs)
+    float* rec;                  // R_COUNT * (cap * max_depth) (SoA)
+    uint32_t* rec_flag;          // 0 none, 1 sampled, 2 closed
+    uint32_t cap;
+    // training batch
+    float* t_in;                 // 9 per sample
+    float* t_dir;                // 3 per sample
+    float* t_c;                  // radiance / pdf
+    uint32_t* t_count;
+    uint32_t t_cap;
+};
+
 struct dparams {
     uint32_t spp, max_depth;
     float rr_threshold;
@@ -222,6 +251,7 @@ struct dparams {
     const uint64_t* replay_offsets;
     const float* replay_values;
     uint32_t* debug;  // device debug hit buffer or null
+    dguide g;
 };
 
 // glm mat4 * vec4 (xyz): (m0*x + m1*y) + (m2*z + m3*w)
@@ -331,6 +361,7 @@ __global__ void k_raygen(dscene S, dpaths P, dparams prm, dcamera cam, uint32_t 
     P.U(U_PIXEL, i) = pixel;
     P.U(U_SAMPLE, i) = s;
     P.U(U_BOUNCE, i) = 0;
+    P.U(U_PENDING, i) = RBW_NONE;
     P.rng[i] = smp.state;
     queue[i] = i;
     if (i == 0) *queue_count = n;
@@ -384,24 +415,42 @@ __device__ __forceinline__ surf hit_surface(const dscene& S, uint32_t prim, f3 o
 }
 
 // sample_surface_interaction (integrator.cpp:329-404) up to the light and BSDF
-// samples of next-event estimation, then path_integrator.cpp:35-43
+// samples of next-event estimation, then path_integrator.cpp:35-43.  GUIDED:
+// vertices with a BSDF go to the guide queue instead; k_continue draws their
+// continuation after the MLP has run.
+template <bool GUIDED>
 __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, const uint32_t* qin_count,
                         uint32_t* qout, uint32_t* qout_count, uint32_t* cq, uint32_t* cq_count) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *qin_count;
-    bool alive = false, need_connect = false;
-    uint32_t slot = 0;
+    bool alive = false, need_connect = false, to_guide = false;
+    uint32_t slot = 0, guide_mat = 0;
+    surf gsi;
     if (q < count) {
         slot = qin[q];
         const uint32_t prim = P.U(U_HPRIM, slot);
         int bounce = (int)P.U(U_BOUNCE, slot);
+        f3 value = P.F3(F_VX, slot);
+        if (GUIDED) {
+            // the radiance carried back along the previous vertex's sampled direction is
+            // everything the path gathers from here on
+            const uint32_t pend = P.U(U_PENDING, slot);
+            if (pend != RBW_NONE) {
+                const size_t ri = (size_t)slot * prm.max_depth + pend;
+                const size_t rc = (size_t)P.cap * prm.max_depth;
+                prm.g.rec[R_MX * rc + ri] = value.x;
+                prm.g.rec[R_MY * rc + ri] = value.y;
+                prm.g.rec[R_MZ * rc + ri] = value.z;
+                prm.g.rec_flag[ri] = 2;
+                P.U(U_PENDING, slot) = RBW_NONE;
+            }
+        }
         if (prim != RBW_MISS) {
             const f3 o_w = P.F3(F_OX, slot), d_w = P.F3(F_DX, slot);
             uint32_t eid;
             const surf si = hit_surface(S, prim, o_w, d_w, eid);
             const dentity E = S.ents[eid];
             f3 beta = P.F3(F_BX, slot);
-            f3 value = P.F3(F_VX, slot);
             sampler_state smp = load_sampler(P, prm, slot);
             bool cont = true;
             // emission at the first vertex (integrator.cpp:350-351; no specular lobes exist)
@@ -467,34 +516,45 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
                         need_connect = true;
                     }
                 }
-                // continuation sample (integrator.cpp:380-397)
-                f3 wi, f;
-                float pdf = 0.0f;
-                const bool ok = fn.sample(si, smp.next2(), wi, f, pdf);
-                if (!ok || is_black(f) || pdf == 0.0f) {
-                    cont = false;
+                if (GUIDED && fn.kind != LOBE_NONE) {
+                    to_guide = true;
+                    gsi = si;
+                    guide_mat = (S.tri_material && prim < S.num_tris && S.tri_material[prim] != RBW_NONE)
+                                    ? S.tri_material[prim] : E.material;
+                } else if (GUIDED) {
+                    cont = false;  // an empty lobe list ends the path (integrator.cpp:380-382)
                 } else {
-                    beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
-                    f3 no, nd;
-                    spawn_ray(si.point, si.normal, wi, no, nd);
-                    P.setF3(F_OX, slot, no);
-                    P.setF3(F_DX, slot, nd);
+                    // continuation sample (integrator.cpp:380-397)
+                    f3 wi, f;
+                    float pdf = 0.0f;
+                    const bool ok = fn.sample(si, smp.next2(), wi, f, pdf);
+                    if (!ok || is_black(f) || pdf == 0.0f) {
+                        cont = false;
+                    } else {
+                        beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
+                        f3 no, nd;
+                        spawn_ray(si.point, si.normal, wi, no, nd);
+                        P.setF3(F_OX, slot, no);
+                        P.setF3(F_DX, slot, nd);
+                    }
                 }
             }
-            if (cont) {
-                // Russian roulette (path_integrator.cpp:35-43)
-                const float mc = spectrum_max(beta * 1.0f);
-                if (mc < prm.rr_threshold && bounce > 3) {
-                    const float qq = gmax(0.05f, 1.0f - mc);
-                    if (smp.next() < qq) cont = false;
-                    else beta = beta / (1.0f - qq);
+            if (!to_guide) {
+                if (cont) {
+                    // Russian roulette (path_integrator.cpp:35-43)
+                    const float mc = spectrum_max(beta * 1.0f);
+                    if (mc < prm.rr_threshold && bounce > 3) {
+                        const float qq = gmax(0.05f, 1.0f - mc);
+                        if (smp.next() < qq) cont = false;
+                        else beta = beta / (1.0f - qq);
+                    }
                 }
+                bounce += 1;
+                alive = cont && (uint32_t)bounce < prm.max_depth;
+                P.U(U_BOUNCE, slot) = (uint32_t)bounce;
             }
-            bounce += 1;
-            alive = cont && (uint32_t)bounce < prm.max_depth;
             P.setF3(F_BX, slot, beta);
             P.setF3(F_VX, slot, value);
-            P.U(U_BOUNCE, slot) = (uint32_t)bounce;
             P.rng[slot] = smp.state;
         }
     }
@@ -502,6 +562,194 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
     if (alive) qout[a] = slot;
     const uint32_t c = wave_append(cq_count, need_connect);
     if (need_connect) cq[c] = slot;
+    if (GUIDED) {
+        const uint32_t g = wave_append(prm.g.gq_count, to_guide);
+        if (to_guide) {
+            const dguide& G = prm.g;
+            const size_t cap = G.cap;
+            const float v[G_COUNT] = {gsi.point.x, gsi.point.y, gsi.point.z, gsi.normal.x, gsi.normal.y, gsi.normal.z,
+                                      gsi.wo.x, gsi.wo.y, gsi.wo.z, gsi.sh.x.x, gsi.sh.x.y, gsi.sh.x.z,
+                                      gsi.sh.y.x, gsi.sh.y.y, gsi.sh.y.z, gsi.sh.z.x, gsi.sh.z.y, gsi.sh.z.z};
+#pragma unroll
+            for (int k = 0; k < G_COUNT; k++) G.surf[k * cap + g] = v[k];
+            G.surf_mat[g] = guide_mat;
+            G.gq[g] = slot;
+            // MLP query: position in the scene box, shading normal and wo mapped to [0,1]
+            const f3 pn = (gsi.point - G.smin) * G.sinv;
+            const float qv[9] = {gclamp(pn.x, 0.0f, 1.0f), gclamp(pn.y, 0.0f, 1.0f), gclamp(pn.z, 0.0f, 1.0f),
+                                 0.5f * (gsi.sh.z.x + 1.0f), 0.5f * (gsi.sh.z.y + 1.0f), 0.5f * (gsi.sh.z.z + 1.0f),
+                                 0.5f * (gsi.wo.x + 1.0f), 0.5f * (gsi.wo.y + 1.0f), 0.5f * (gsi.wo.z + 1.0f)};
+#pragma unroll
+            for (int k = 0; k < 9; k++) G.query[(size_t)g * 9 + k] = qv[k];
+        }
+    }
+}
+
+// the vMF-mixture head of the guiding MLP (same parameterisation as the training loss)
+struct vmf_mixture {
+    f3 mu[guide::N_LOBES];
+    float kappa[guide::N_LOBES], w[guide::N_LOBES], norm[guide::N_LOBES];
+    __device__ void load(const float* o) {
+        float amax = -3.4e38f;
+        for (int j = 0; j < guide::N_LOBES; j++) amax = fmaxf(amax, o[5 * j + 4]);
+        float ws = 0.0f;
+        for (int j = 0; j < guide::N_LOBES; j++) {
+            w[j] = expf(o[5 * j + 4] - amax);
+            ws += w[j];
+            const f3 m = f3{o[5 * j], o[5 * j + 1], o[5 * j + 2]};
+            const float nm = fmaxf(sqrtf(m.x * m.x + m.y * m.y + m.z * m.z), 1e-12f);
+            mu[j] = f3{m.x / nm, m.y / nm, m.z / nm};
+            const float x = o[5 * j + 3];
+            const float sp = x > 20.0f ? x : log1pf(expf(x));
+            kappa[j] = fminf(sp + guide::KAPPA_MIN, guide::KAPPA_MAX);
+            norm[j] = kappa[j] / (2.0f * 3.14159265f * (-expm1f(-2.0f * kappa[j])));
+        }
+        for (int j = 0; j < guide::N_LOBES; j++) w[j] /= ws;
+    }
+    __device__ float pdf(f3 d) const {
+        float q = 0.0f;
+        for (int j = 0; j < guide::N_LOBES; j++)
+            q += w[j] * norm[j] * expf(kappa[j] * ((mu[j].x * d.x + mu[j].y * d.y + mu[j].z * d.z) - 1.0f));
+        return q;
+    }
+    // pick a lobe with u.x, then Jakob's vMF sampler around its mean
+    __device__ f3 sample(f2 u) const {
+        int j = 0;
+        float acc = 0.0f;
+        for (; j < guide::N_LOBES - 1; j++) {
+            if (u.x < acc + w[j]) break;
+            acc += w[j];
+        }
+        const float ux = fminf(fmaxf((u.x - acc) / fmaxf(w[j], 1e-20f), 0.0f), 0.99999994f);
+        const float k = kappa[j];
+        const float W = 1.0f + logf(ux + (1.0f - ux) * expf(-2.0f * k)) / k;
+        const float sr = sqrtf(fmaxf(0.0f, 1.0f - W * W));
+        const float phi = 2.0f * 3.14159265f * u.y;
+        const frame fr = frame_from_normal(mu[j]);
+        return local_to_world(fr, f3{sr * cosf(phi), sr * sinf(phi), W});
+    }
+};
+
+// guided continuation: one-sample MIS of the vMF mixture and the BSDF, then
+// Russian roulette (path_integrator.cpp:35-43); the sampled direction becomes a
+// training record closed when the next vertex is reached
+__global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint32_t* qout_count) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const dguide& G = prm.g;
+    const uint32_t count = *G.gq_count;
+    bool alive = false;
+    uint32_t slot = 0;
+    if (g < count) {
+        slot = G.gq[g];
+        const size_t cap = G.cap;
+        float v[G_COUNT];
+#pragma unroll
+        for (int k = 0; k < G_COUNT; k++) v[k] = G.surf[k * cap + g];
+        surf si;
+        si.point = f3{v[G_PX], v[G_PY], v[G_PZ]};
+        si.normal = f3{v[G_NX], v[G_NY], v[G_NZ]};
+        si.wo = f3{v[G_WX], v[G_WY], v[G_WZ]};
+        si.sh.x = f3{v[G_XX], v[G_XY], v[G_XZ]};
+        si.sh.y = f3{v[G_YX], v[G_YY], v[G_YZ]};
+        si.sh.z = f3{v[G_ZX], v[G_ZY], v[G_ZZ]};
+        const bsdf fn = make_bsdf(S, G.surf_mat[g], RBW_NONE);
+        vmf_mixture mix;
+        mix.load(G.mix + (size_t)g * guide::N_OUT_PAD);
+        sampler_state smp = load_sampler(P, prm, slot);
+        int bounce = (int)P.U(U_BOUNCE, slot);
+        f3 beta = P.F3(F_BX, slot);
+        bool cont = true;
+        const float sel = smp.next();
+        const f2 u = smp.next2();
+        f3 wi;
+        if (sel < G.alpha) {
+            wi = mix.sample(u);
+        } else {
+            f3 f0;
+            float p0;
+            if (!fn.sample(si, u, wi, f0, p0)) cont = false;
+        }
+        float pdf = 0.0f;
+        if (cont) {
+            const f3 wo_s = world_to_local(si.sh, si.wo), wi_s = world_to_local(si.sh, wi);
+            const f3 f = fn.evaluate(wo_s, wi_s);
+            pdf = G.alpha * mix.pdf(wi) + (1.0f - G.alpha) * fn.pdf(wo_s, wi_s);
+            if (is_black(f) || !(pdf > 0.0f)) {
+                cont = false;
+            } else {
+                beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
+                f3 no, nd;
+                spawn_ray(si.point, si.normal, wi, no, nd);
+                P.setF3(F_OX, slot, no);
+                P.setF3(F_DX, slot, nd);
+            }
+        }
+        if (cont) {
+            const float mc = spectrum_max(beta * 1.0f);
+            if (mc < prm.rr_threshold && bounce > 3) {
+                const float qq = gmax(0.05f, 1.0f - mc);
+                if (smp.next() < qq) cont = false;
+                else beta = beta / (1.0f - qq);
+            }
+        }
+        const int b0 = bounce;
+        bounce += 1;
+        alive = cont && (uint32_t)bounce < prm.max_depth;
+        if (alive && G.train) {
+            const size_t ri = (size_t)slot * prm.max_depth + (uint32_t)b0;
+            const size_t rc = (size_t)P.cap * prm.max_depth;
+            for (int k = 0; k < 9; k++) G.rec[(R_Q0 + k) * rc + ri] = G.query[(size_t)g * 9 + k];
+            G.rec[R_DX * rc + ri] = wi.x;
+            G.rec[R_DY * rc + ri] = wi.y;
+            G.rec[R_DZ * rc + ri] = wi.z;
+            G.rec[R_PDF * rc + ri] = pdf;
+            G.rec[R_BX * rc + ri] = beta.x;
+            G.rec[R_BY * rc + ri] = beta.y;
+            G.rec[R_BZ * rc + ri] = beta.z;
+            G.rec_flag[ri] = 1;
+            P.U(U_PENDING, slot) = (uint32_t)b0;
+        }
+        P.setF3(F_BX, slot, beta);
+        P.U(U_BOUNCE, slot) = (uint32_t)bounce;
+        P.rng[slot] = smp.state;
+    }
+    const uint32_t a = wave_append(qout_count, alive);
+    if (alive) qout[a] = slot;
+}
+
+// after a pass: closed training records -> (query, direction, radiance / pdf)
+__global__ void k_collect(dpaths P, dparams prm, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const dguide& G = prm.g;
+    const size_t rc = (size_t)P.cap * prm.max_depth;
+    bool take = false;
+    float c = 0.0f;
+    size_t ri = 0;
+    if (i < n * prm.max_depth) {
+        const uint32_t slot = i / prm.max_depth;
+        ri = i;
+        if (G.rec_flag[ri] == 2) {
+            const f3 vf = P.F3(F_VX, slot);
+            const f3 m = f3{G.rec[R_MX * rc + ri], G.rec[R_MY * rc + ri], G.rec[R_MZ * rc + ri]};
+            const f3 b = f3{G.rec[R_BX * rc + ri], G.rec[R_BY * rc + ri], G.rec[R_BZ * rc + ri]};
+            const f3 d = vf - m;
+            const float Lx = b.x > 0.0f ? d.x / b.x : 0.0f, Ly = b.y > 0.0f ? d.y / b.y : 0.0f,
+                        Lz = b.z > 0.0f ? d.z / b.z : 0.0f;
+            const float lum = 0.2126f * Lx + 0.7152f * Ly + 0.0722f * Lz;
+            const float pdf = G.rec[R_PDF * rc + ri];
+            c = pdf > 0.0f ? lum / pdf : 0.0f;
+            take = c > 0.0f && isfinite(c);
+        }
+        G.rec_flag[ri] = 0;
+    }
+    const uint32_t k = wave_append(G.t_count, take);
+    if (take && k < G.t_cap) {
+        for (int j = 0; j < 9; j++) G.t_in[(size_t)k * 9 + j] = G.rec[(R_Q0 + j) * rc + ri];
+        G.t_dir[(size_t)k * 3 + 0] = G.rec[R_DX * rc + ri];
+        G.t_dir[(size_t)k * 3 + 1] = G.rec[R_DY * rc + ri];
+        G.t_dir[(size_t)k * 3 + 2] = G.rec[R_DZ * rc + ri];
+        G.t_c[k] = c;
+    }
 }
 
 // shadow ray, MIS ray and light pdf of next-event estimation; value += beta * L
@@ -787,6 +1035,18 @@ struct rbw_ctx {
     uint32_t path_cap = 0;
     uint32_t* host_count = nullptr;  // pinned
     kernel_timer timer;
+    // guiding
+    f3 scene_min{0, 0, 0}, scene_max{0, 0, 0};
+    guide::model gm;
+    bool guide_ready = false;
+    dbuf g_surf, g_mat, g_q, g_query, g_mix, g_rec, g_flag, g_tin, g_tdir, g_tc, g_io_in, g_io_out, g_io_dir, g_io_c;
+    uint32_t g_cap = 0, g_rec_cap = 0, g_t_cap = 0;
+    void ensure_guide() {
+        if (!guide_ready) {
+            guide::model_init(gm, 0x5eed, stream);
+            guide_ready = true;
+        }
+    }
 
     dscene scene_view() const {
         dscene S;
@@ -815,8 +1075,11 @@ struct rbw_ctx {
         (void)hipSetDevice(device);
         for (dbuf* b : {&nodes, &prim_a, &prim_b, &prim_c, &pos, &nrm, &uv, &idx, &tri_entity, &tri_material, &spheres,
                         &sphere_entity, &d_ents, &d_mats, &d_emitters, &pf, &pu, &prng, &q0, &q1, &cq, &counts,
-                        &film_sum, &film_w, &rgb, &dbg, &replay_off, &replay_val, &counters})
+                        &film_sum, &film_w, &rgb, &dbg, &replay_off, &replay_val, &counters, &g_surf, &g_mat, &g_q,
+                        &g_query, &g_mix, &g_rec, &g_flag, &g_tin, &g_tdir, &g_tc, &g_io_in, &g_io_out, &g_io_dir,
+                        &g_io_c})
             b->release();
+        guide::model_free(gm);
         if (host_count) (void)hipHostFree(host_count);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -937,6 +1200,13 @@ void upload_scene(rbw_ctx* ctx, const rbw_scene_desc* d) {
             b.hi[a] = expand_hi(d->spheres[4 * s + a] + r);
         }
     }
+    f3 smin{3.4e38f, 3.4e38f, 3.4e38f}, smax{-3.4e38f, -3.4e38f, -3.4e38f};
+    for (const aabb& b : boxes) {
+        smin = f3{std::min(smin.x, b.lo[0]), std::min(smin.y, b.lo[1]), std::min(smin.z, b.lo[2])};
+        smax = f3{std::max(smax.x, b.hi[0]), std::max(smax.y, b.hi[1]), std::max(smax.z, b.hi[2])};
+    }
+    ctx->scene_min = smin;
+    ctx->scene_max = smax;
     bvh_build_result bvh;
     build_bvh(boxes, bvh, 4);
     require(bvh.depth < RBW_STACK, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
@@ -1057,8 +1327,18 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     const uint64_t npix = uint64_t(bw) * bh;
     const uint64_t nsamples_total = uint64_t(W) * H * pr.spp;
     require(npix <= (1u << 28), RBW_ERR_UNSUPPORTED, "film too large");
+    const rbw_guiding_opts* go = pr.guiding;
+    const bool guided = go && go->enable;
+    if (guided) {
+        require(go->mis_fraction >= 0.0f && go->mis_fraction <= 1.0f, RBW_ERR_INVALID_ARGUMENT,
+                "guiding mis_fraction must be in [0, 1]");
+        require(!go->train || (go->batch > 0 && go->batch <= (1u << 20)), RBW_ERR_INVALID_ARGUMENT,
+                "guiding batch must be in [1, 2^20]");
+    }
     uint32_t k = pr.samples_per_pass;
-    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pr.spp, (1u << 21) / npix));
+    // guided renders train between passes: smaller passes give the network more updates
+    const uint64_t per_pass = guided ? (1u << 18) : (1u << 21);
+    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pr.spp, per_pass / npix));
     k = std::min(k, pr.spp);
     while (uint64_t(k) * npix > (1ull << 30) && k > 1) k--;
     const uint32_t cap = uint32_t(uint64_t(k) * npix);
@@ -1073,6 +1353,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         ctx->path_cap = cap;
     }
     ctx->counts.ensure(16 * sizeof(uint32_t));
+    HIP_CHECK(hipMemsetAsync(ctx->counts.p, 0, 16 * sizeof(uint32_t), s));
     ctx->counters.ensure(16 * sizeof(unsigned long long));
     ctx->film_sum.ensure(size_t(W) * H * 3 * sizeof(float));
     ctx->film_w.ensure(size_t(W) * H * sizeof(float));
@@ -1108,6 +1389,60 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         ctx->dbg.ensure(ndbg * sizeof(uint32_t));
         HIP_CHECK(hipMemsetAsync(ctx->dbg.p, 0xFF, ndbg * sizeof(uint32_t), s));
         dp.debug = ctx->dbg.as<uint32_t>();
+    }
+    std::memset(&dp.g, 0, sizeof(dp.g));
+    if (guided) {
+        ctx->ensure_guide();
+        const uint32_t gcap = ctx->path_cap;
+        if (ctx->g_cap < gcap) {
+            ctx->g_surf.ensure(size_t(G_COUNT) * gcap * sizeof(float));
+            ctx->g_mat.ensure(size_t(gcap) * 4);
+            ctx->g_q.ensure(size_t(gcap) * 4);
+            ctx->g_query.ensure(size_t(gcap) * 9 * sizeof(float));
+            ctx->g_mix.ensure(size_t(gcap + 16) * guide::N_OUT_PAD * sizeof(float));
+            ctx->g_cap = gcap;
+        }
+        dp.g.enable = 1;
+        dp.g.alpha = go->mis_fraction;
+        dp.g.train = go->train ? 1u : 0u;
+        f3 smin = ctx->scene_min, smax = ctx->scene_max;
+        if (go->scene_min[0] != 0.0f || go->scene_min[1] != 0.0f || go->scene_min[2] != 0.0f ||
+            go->scene_max[0] != 0.0f || go->scene_max[1] != 0.0f || go->scene_max[2] != 0.0f) {
+            smin = f3{go->scene_min[0], go->scene_min[1], go->scene_min[2]};
+            smax = f3{go->scene_max[0], go->scene_max[1], go->scene_max[2]};
+        }
+        dp.g.smin = smin;
+        dp.g.sinv = f3{1.0f / std::max(smax.x - smin.x, 1e-6f), 1.0f / std::max(smax.y - smin.y, 1e-6f),
+                       1.0f / std::max(smax.z - smin.z, 1e-6f)};
+        dp.g.surf = ctx->g_surf.as<float>();
+        dp.g.surf_mat = ctx->g_mat.as<uint32_t>();
+        dp.g.gq = ctx->g_q.as<uint32_t>();
+        dp.g.gq_count = ctx->counts.as<uint32_t>() + 3;
+        dp.g.query = ctx->g_query.as<float>();
+        dp.g.mix = ctx->g_mix.as<float>();
+        dp.g.cap = ctx->path_cap;
+        if (go->train) {
+            const size_t nrec = size_t(ctx->path_cap) * pr.max_depth;
+            if (ctx->g_rec_cap < nrec) {
+                ctx->g_rec.ensure(size_t(R_COUNT) * nrec * sizeof(float));
+                ctx->g_flag.ensure(nrec * 4);
+                ctx->g_rec_cap = uint32_t(nrec);
+            }
+            HIP_CHECK(hipMemsetAsync(ctx->g_flag.p, 0, nrec * 4, s));
+            if (ctx->g_t_cap < go->batch) {
+                ctx->g_tin.ensure(size_t(go->batch) * 9 * sizeof(float));
+                ctx->g_tdir.ensure(size_t(go->batch) * 3 * sizeof(float));
+                ctx->g_tc.ensure(size_t(go->batch) * sizeof(float));
+                ctx->g_t_cap = go->batch;
+            }
+            dp.g.rec = ctx->g_rec.as<float>();
+            dp.g.rec_flag = ctx->g_flag.as<uint32_t>();
+            dp.g.t_in = ctx->g_tin.as<float>();
+            dp.g.t_dir = ctx->g_tdir.as<float>();
+            dp.g.t_c = ctx->g_tc.as<float>();
+            dp.g.t_count = ctx->counts.as<uint32_t>() + 4;
+            dp.g.t_cap = go->batch;
+        }
     }
     const dscene S = ctx->scene_view();
     dpaths P;
@@ -1157,9 +1492,29 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                                    counts + cin, counters);
             if (timed) ctx->timer.end(s);
             if (timed) ctx->timer.begin(RBW_KERNEL_SHADE, s);
-            hipLaunchKernelGGL(k_shade, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin, counts + cin, qout,
-                               counts + cout, cqp, counts + 2);
+            if (guided) {
+                HIP_CHECK(hipMemsetAsync(counts + 3, 0, 4, s));
+                hipLaunchKernelGGL(k_shade<true>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin, counts + cin,
+                                   qout, counts + cout, cqp, counts + 2);
+            } else {
+                hipLaunchKernelGGL(k_shade<false>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin,
+                                   counts + cin, qout, counts + cout, cqp, counts + 2);
+            }
             if (timed) ctx->timer.end(s);
+            if (guided) {
+                HIP_CHECK(hipMemcpyAsync(ctx->host_count + 1, counts + 3, 4, hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                const uint32_t ng = ctx->host_count[1];
+                if (ng > 0) {
+                    if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_INFER, s);
+                    guide::forward(ctx->gm, dp.g.query, ng, dp.g.mix, s);
+                    if (timed) ctx->timer.end(s);
+                    if (timed) ctx->timer.begin(RBW_KERNEL_CONTINUE, s);
+                    hipLaunchKernelGGL(k_continue, dim3(blocks(ng, BS)), dim3(BS), 0, s, S, P, dp, qout, counts + cout);
+                    if (timed) ctx->timer.end(s);
+                    HIP_CHECK(hipGetLastError());
+                }
+            }
             if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
             if (count)
                 hipLaunchKernelGGL(k_connect<true>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, cqp,
@@ -1174,6 +1529,22 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             active = *ctx->host_count;
             std::swap(qin, qout);
             std::swap(cin, cout);
+        }
+        if (guided && go->train) {
+            if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
+            HIP_CHECK(hipMemsetAsync(counts + 4, 0, 4, s));
+            hipLaunchKernelGGL(k_collect, dim3(blocks(uint64_t(n) * pr.max_depth, BS)), dim3(BS), 0, s, P, dp, n);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(ctx->host_count + 2, counts + 4, 4, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            const uint32_t nt = std::min(ctx->host_count[2], go->batch);
+            if (nt >= 256) {
+                guide::adam_opts ao;
+                ao.lr = go->learning_rate;
+                for (uint32_t st = 0; st < go->steps_per_pass; st++)
+                    guide::train_step(ctx->gm, dp.g.t_in, dp.g.t_dir, dp.g.t_c, nt, ao, true, s);
+            }
+            if (timed) ctx->timer.end(s);
         }
         if (timed) ctx->timer.begin(RBW_KERNEL_FILM, s);
         hipLaunchKernelGGL(k_film_own, dim3(blocks(npix, BS)), dim3(BS), 0, s, P, dc, kk, ctx->film_sum.as<float>(),
@@ -1292,6 +1663,91 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const 
             throw;
         }
         o.release(); d.release(); t.release(); h.release();
+    });
+}
+
+rbw_status rbw_guide_reset(rbw_ctx* ctx, uint32_t seed) {
+    return guarded([&] {
+        require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        guide::model_init(ctx->gm, seed, ctx->stream);
+        ctx->guide_ready = true;
+    });
+}
+
+rbw_status rbw_guide_get_params(rbw_ctx* ctx, float* params) {
+    return guarded([&] {
+        require(ctx != nullptr && params != nullptr, RBW_ERR_INVALID_ARGUMENT, "null argument");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->ensure_guide();
+        guide::model_get(ctx->gm, params, ctx->stream);
+    });
+}
+
+rbw_status rbw_guide_set_params(rbw_ctx* ctx, const float* params) {
+    return guarded([&] {
+        require(ctx != nullptr && params != nullptr, RBW_ERR_INVALID_ARGUMENT, "null argument");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->ensure_guide();
+        guide::model_set(ctx->gm, params, ctx->stream);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+rbw_status rbw_guide_forward(rbw_ctx* ctx, uint32_t n, const float* inputs, float* out) {
+    return guarded([&] {
+        require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
+        require(n == 0 || (inputs && out), RBW_ERR_INVALID_ARGUMENT, "null arrays");
+        if (n == 0) return;
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->ensure_guide();
+        hipStream_t s = ctx->stream;
+        ctx->g_io_in.upload(inputs, size_t(n) * 9, s);
+        const uint32_t n_pad = (n + 15) & ~15u;
+        ctx->g_io_out.ensure(size_t(n_pad) * guide::N_OUT_PAD * sizeof(float));
+        guide::forward(ctx->gm, ctx->g_io_in.as<float>(), n, ctx->g_io_out.as<float>(), s);
+        HIP_CHECK(hipMemcpyAsync(out, ctx->g_io_out.p, size_t(n) * guide::N_OUT_PAD * sizeof(float),
+                                 hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const float* dirs, const float* c,
+                           float learning_rate, uint32_t apply, float* loss, float* grad) {
+    return guarded([&] {
+        require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
+        require(n > 0 && inputs && dirs && c, RBW_ERR_INVALID_ARGUMENT, "null arrays");
+        require(n <= (1u << 22), RBW_ERR_INVALID_ARGUMENT, "batch too large");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->ensure_guide();
+        hipStream_t s = ctx->stream;
+        ctx->g_io_in.upload(inputs, size_t(n) * 9, s);
+        ctx->g_io_dir.upload(dirs, size_t(n) * 3, s);
+        ctx->g_io_c.upload(c, n, s);
+        HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
+        guide::adam_opts ao;
+        ao.lr = learning_rate;
+        guide::train_step(ctx->gm, ctx->g_io_in.as<float>(), ctx->g_io_dir.as<float>(), ctx->g_io_c.as<float>(), n,
+                          ao, false, s);
+        if (grad || loss) {
+            std::vector<float> g(guide::N_PARAMS);
+            float l = 0.0f;
+            HIP_CHECK(hipMemcpyAsync(g.data(), ctx->gm.grad, sizeof(float) * guide::N_PARAMS, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(&l, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (grad)
+                for (int i = 0; i < guide::N_PARAMS; i++) grad[i] = g[i] / (float)n;
+            if (loss) *loss = l;
+        }
+        if (apply) {
+            // the Adam step of train_step(apply) on the gradient just computed
+            ctx->gm.step++;
+            const float c1 = 1.0f - std::pow(ao.b1, (float)ctx->gm.step);
+            const float c2 = 1.0f - std::pow(ao.b2, (float)ctx->gm.step);
+            guide::adam_apply(ctx->gm, n, ao, c1, c2, s);
+        }
+        HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
+        HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

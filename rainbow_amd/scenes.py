@@ -400,3 +400,45 @@ def displaced_icosphere_box(level=8, width=1024, height=1024, seed=1):
     sc.add_mesh(P, F, tm[0], normals=N, tri_material=tm)
     cam = perspective_camera(width, height, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
     return sc, cam
+
+
+def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0, seed=1, mesh=True):
+    """A closed room lit only through a gap x gap hole in its +x wall by a quad light outside
+    (radiance 50); the displaced icosphere with per-triangle materials inside; camera inside."""
+    sc = Scene()
+    white = sc.matte((0.73, 0.73, 0.73))
+    black = sc.matte((0.0, 0.0, 0.0))
+    walls = [((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1)),     # floor, n=+y
+             ((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1)),         # ceiling, n=-y
+             ((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)),     # back z=-1, n=+z
+             ((1, -1, 1), (-1, -1, 1), (-1, 1, 1), (1, 1, 1)),         # front z=+1, n=-z
+             ((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1))]     # left x=-1, n=+x
+    for q in walls:
+        P, I, N = _quad(*q)
+        sc.add_mesh(P, I, white, normals=N)
+    # +x wall with a hole centred at (y, z) = (cy, 0)
+    cy, h = 0.2, gap / 2
+    y0, y1, z0, z1 = cy - h, cy + h, -h, h
+    rects = [(-1, y0, -1, 1), (y1, 1, -1, 1), (y0, y1, -1, z0), (y0, y1, z1, 1)]  # (ylo, yhi, zlo, zhi)
+    Ps, Is, Ns = [], [], []
+    for k, (ya, yb, za, zb) in enumerate(rects):
+        P, I, N = _quad((1, ya, za), (1, ya, zb), (1, yb, zb), (1, yb, za))   # n=-x
+        Ps.append(P)
+        Is.append(I + 4 * k)
+        Ns.append(N)
+    sc.add_mesh(np.concatenate(Ps), np.concatenate(Is), white, normals=np.concatenate(Ns))
+    # the light outside, facing the hole (n=-x)
+    s = 0.25
+    P, I, N = _quad((1.5, cy - s, -s), (1.5, cy - s, s), (1.5, cy + s, s), (1.5, cy + s, -s))
+    sc.add_mesh(P, I, black, normals=N, radiance=(radiance,) * 3)
+    if mesh:
+        P, F, Nn = fbm_displaced_icosphere(level=level, seed=seed, center=(-0.2, -0.45, -0.3), radius=0.45)
+        tm = per_triangle_materials(sc, len(F), seed)
+        sc.add_mesh(P, F, tm[0], normals=Nn, tri_material=tm)
+    cam = perspective_camera(width, height, 65.0, eye=(0.6, 0.3, 0.95), target=(-0.3, -0.3, -0.6))
+    return sc, cam
+
+
+def rel_mse(img, ref):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)"""
+    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 0.01)))

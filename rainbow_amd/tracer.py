@@ -90,6 +90,43 @@ class Tracer:
         return out
 
 
+    # -- guiding network (no counterpart in the reference) -------------------
+    def guide_reset(self, seed=0):
+        _check(self.lib.rbw_guide_reset(self.h, seed))
+
+    def guide_params(self):
+        out = np.zeros(abi.RBW_GUIDE_PARAMS, np.float32)
+        _check(self.lib.rbw_guide_get_params(self.h, abi.ptr(out, abi.f32p)))
+        return out
+
+    def guide_set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        assert flat.size == abi.RBW_GUIDE_PARAMS
+        _check(self.lib.rbw_guide_set_params(self.h, abi.ptr(flat, abi.f32p)))
+
+    def guide_forward(self, inputs):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        out = np.zeros((len(x), 48), np.float32)
+        _check(self.lib.rbw_guide_forward(self.h, len(x), abi.ptr(x, abi.f32p), abi.ptr(out, abi.f32p)))
+        return out
+
+    def guide_train(self, inputs, dirs, c, lr=1e-2, apply=True, want_grad=False):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(c, np.float32).reshape(-1)
+        loss = np.zeros(1, np.float32)
+        grad = np.zeros(abi.RBW_GUIDE_PARAMS, np.float32) if want_grad else None
+        _check(self.lib.rbw_guide_train(self.h, len(x), abi.ptr(x, abi.f32p), abi.ptr(d, abi.f32p),
+                                        abi.ptr(w, abi.f32p), float(lr), int(apply), abi.ptr(loss, abi.f32p),
+                                        abi.ptr(grad, abi.f32p)))
+        return float(loss[0]), grad
+
+
+def guiding_opts(mis_fraction=0.5, train=True, learning_rate=1e-2, batch=1 << 18, steps_per_pass=4):
+    return abi.rbw_guiding_opts(enable=1, mis_fraction=mis_fraction, train=int(train), learning_rate=learning_rate,
+                                batch=batch, steps_per_pass=steps_per_pass)
+
+
 def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
     p = abi.rbw_render_params(spp=spp, max_depth=max_depth, rr_threshold=rr_threshold, seed=seed)
     for k, v in kw.items():

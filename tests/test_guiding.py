@@ -1,0 +1,207 @@
+"""Path guiding: the float64 oracle pinned by finite differences, closed forms
+(vMF normalisation, Adam's first step) and sampling histograms; the HIP
+FP16-MFMA kernels (inference, gradients, Adam) checked against the oracle;
+the guided estimator's unbiasedness checked on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import guiding_ref as G
+from rainbow_amd import abi, scenes
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 9))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = rng.random(n) + 0.1
+    return x, d, c
+
+
+def test_oracle_parameter_count():
+    assert G.n_params() == 33128          # 216*64+64 + 4*(64*64+64) + 64*40+40
+    assert G.to_padded(*G.init_params(0)).size == abi.RBW_GUIDE_PARAMS
+
+
+def test_oracle_gradients_match_finite_differences():
+    Ws, bs = G.init_params(1)
+    x, d, c = _batch(48, 0)
+    X = G.encode(x)
+    o, acts, pre = G.forward(Ws, bs, X)
+    _, dout = G.loss_and_grad_out(o, d, c)
+    dWs, dbs = G.backward(Ws, bs, acts, pre, dout)
+    eps = 1e-6
+    rng = np.random.default_rng(3)
+    for l in range(6):
+        for _ in range(4):
+            a = rng.integers(Ws[l].shape[0])
+            b = rng.integers(G.N_FEAT if l == 0 else Ws[l].shape[1])
+            Wp = [w.copy() for w in Ws]
+            Wm = [w.copy() for w in Ws]
+            Wp[l][a, b] += eps
+            Wm[l][a, b] -= eps
+            fd = (G.loss_fn(Wp, bs, X, d, c) - G.loss_fn(Wm, bs, X, d, c)) / (2 * eps)
+            assert abs(fd - dWs[l][a, b]) <= 1e-6 + 1e-4 * abs(fd), (l, a, b, fd, dWs[l][a, b])
+        j = rng.integers(len(bs[l]))
+        bp = [v.copy() for v in bs]
+        bm = [v.copy() for v in bs]
+        bp[l][j] += eps
+        bm[l][j] -= eps
+        fd = (G.loss_fn(Ws, bp, X, d, c) - G.loss_fn(Ws, bm, X, d, c)) / (2 * eps)
+        assert abs(fd - dbs[l][j]) <= 1e-6 + 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("kappa", [0.05, 1.0, 30.0, 400.0])
+def test_vmf_normalised(kappa):
+    n = 400
+    th = (np.arange(n) + 0.5) / n * math.pi
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * math.pi
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dA = (np.sin(T) * (math.pi / n) * (math.pi / n)).reshape(-1)
+    mu = np.tile([0.0, 0.0, 1.0], (len(d), 1))
+    pdf = G.vmf_pdf(d, mu, np.full(len(d), kappa))
+    assert abs((pdf * dA).sum() - 1.0) < 2e-3
+
+
+def test_vmf_sampler_matches_pdf():
+    rng = np.random.default_rng(5)
+    mu = np.array([0.3, -0.5, 0.8])
+    mu /= np.linalg.norm(mu)
+    kappa = 12.0
+    cos = np.array([G.vmf_sample(mu, kappa, *rng.random(2)) @ mu for _ in range(40000)])
+    # the cosine to the mean has density kappa e^{kappa (t-1)} / (1 - e^{-2 kappa}) on [-1, 1]
+    edges = np.linspace(0.5, 1.0, 11)
+    hist, _ = np.histogram(cos, edges)
+    cdf = lambda t: (np.exp(kappa * (t - 1)) - np.exp(-2 * kappa)) / (1 - np.exp(-2 * kappa))
+    expect = (cdf(edges[1:]) - cdf(edges[:-1])) * len(cos)
+    assert np.all(np.abs(hist - expect) < 5 * np.sqrt(expect) + 5)
+
+
+def test_adam_first_step_closed_form():
+    p = [np.array([1.0, -2.0, 0.5])]
+    g = [np.array([0.3, -1e-3, 4.0])]
+    opt = G.Adam(p, lr=0.01)
+    new = opt.step(p, g)[0]
+    # first bias-corrected step: m_hat = g, v_hat = g^2 -> p - lr * g / (|g| + eps)
+    assert np.allclose(new, p[0] - 0.01 * g[0] / (np.abs(g[0]) + 1e-8))
+
+
+# ------------------------------------------------------------------ GPU kernels
+
+@pytest.mark.gpu
+def test_gpu_guide_forward_matches_oracle(tracer):
+    Ws, bs = G.init_params(7)
+    tracer.guide_set_params(G.to_padded(Ws, bs))
+    x, _, _ = _batch(1000, 1)
+    got = tracer.guide_forward(x)[:, :40]
+    ref, _, _ = G.forward(Ws, bs, G.encode(x))
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    print("forward max error / output scale", err)
+    assert err < 2e-2            # FP16 operands, FP32 accumulation
+    assert np.corrcoef(got.ravel(), ref.ravel())[0, 1] > 0.9995
+
+
+@pytest.mark.gpu
+def test_gpu_guide_gradients_match_oracle(tracer):
+    Ws, bs = G.init_params(11)
+    tracer.guide_set_params(G.to_padded(Ws, bs))
+    x, d, c = _batch(4096, 2)
+    loss, grad = tracer.guide_train(x, d, c, apply=False, want_grad=True)
+    X = G.encode(x)
+    o, acts, pre = G.forward(Ws, bs, X)
+    ref_loss, dout = G.loss_and_grad_out(o, d, c)
+    dWs, dbs = G.backward(Ws, bs, acts, pre, dout)
+    gW, gb = G.from_padded(grad)
+    print("loss", loss, ref_loss)
+    assert abs(loss - ref_loss) < 1e-2 * abs(ref_loss) + 1e-3
+    for l in range(6):
+        for a, b, name in ((gW[l], dWs[l], "W"), (gb[l], dbs[l], "b")):
+            if l == 0 and name == "W":
+                a, b = a[:, :G.N_FEAT], b[:, :G.N_FEAT]
+            cos = (a * b).sum() / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            print(f"layer {l} {name}: cosine {cos:.5f} relative error {rel:.4f}")
+            assert cos > 0.995 and rel < 0.1, (l, name, cos, rel)
+
+
+@pytest.mark.gpu
+def test_gpu_adam_step(tracer):
+    Ws, bs = G.init_params(3)
+    p0 = G.to_padded(Ws, bs)
+    tracer.guide_set_params(p0)
+    x, d, c = _batch(2048, 4)
+    _, grad = tracer.guide_train(x, d, c, apply=False, want_grad=True)
+    tracer.guide_set_params(p0)          # clears the Adam state
+    tracer.guide_train(x, d, c, lr=1e-3, apply=True)
+    p1 = tracer.guide_params()
+    live = np.abs(grad) > 1e-5
+    # first step: p1 = p0 - lr * g / (|g| + eps)
+    assert np.allclose((p1 - p0)[live], -1e-3 * np.sign(grad[live]), atol=2e-5)
+
+
+@pytest.mark.gpu
+def test_gpu_training_learns_a_direction(tracer):
+    """samples whose radiance comes from one direction (depending on the input) are learned"""
+    tracer.guide_reset(5)
+    rng = np.random.default_rng(9)
+    target = lambda x: np.stack([np.cos(4 * x[:, 0]), np.sin(4 * x[:, 0]), 0.3 + 0 * x[:, 0]], 1)
+    first = last = None
+    for it in range(150):
+        x = rng.random((8192, 9))
+        t = target(x)
+        t /= np.linalg.norm(t, axis=1, keepdims=True)
+        d = rng.normal(size=(8192, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        # radiance around the target direction, sampled uniformly on the sphere (pdf 1/4pi)
+        L = np.exp(20.0 * ((d * t).sum(1) - 1.0))
+        loss, _ = tracer.guide_train(x, d, L * 4 * math.pi, lr=5e-3)
+        first = loss if first is None else first
+        last = loss
+    x = rng.random((2000, 9))
+    t = target(x)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    q_t = G.mixture_pdf(tracer.guide_forward(x)[:, :40].astype(np.float64), t)
+    print("loss", first, "->", last, "median q(target)", np.median(q_t))
+    assert last < first - 1.0
+    assert np.median(q_t) > 1.0          # uniform density is 1/(4 pi) = 0.08
+
+
+@pytest.mark.gpu
+def test_gpu_guided_render_unbiased(tracer):
+    """the guided estimator's expectation equals the plain one (Cornell box, mean radiance)"""
+    from rainbow_amd.tracer import guiding_opts, render_params
+    sc, cam = scenes.cornell_box()
+    tracer.upload(sc)
+    tracer.guide_reset(1)
+    plain = tracer.render(cam, render_params(spp=64, max_depth=5, seed=21))["image"]
+    go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 16)
+    guided = tracer.render(cam, render_params(spp=64, max_depth=5, seed=22, guiding=C_ref(go)))
+    gi = guided["image"]
+    assert np.isfinite(gi).all()
+    mp, mg = plain.mean(axis=(0, 1)), gi.mean(axis=(0, 1))
+    print("plain", mp, "guided", mg, guided["stats"]["kernel_ms"])
+    assert np.all(np.abs(mg - mp) < 0.01 * mp + 1e-4)
+
+
+@pytest.mark.gpu
+def test_gpu_guided_furnace(tracer):
+    """closed emissive box: E[L] = Le * sum_k rho^k also when the continuation is guided"""
+    from rainbow_amd.tracer import guiding_opts, render_params
+    from tests.test_oracle import closed_box, furnace_expected, FURNACE_DEPTH
+    sc, cam = closed_box(0.5, 1.0)
+    tracer.upload(sc)
+    tracer.guide_reset(2)
+    go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 14)
+    r = tracer.render(cam, render_params(spp=256, max_depth=FURNACE_DEPTH, seed=4, guiding=C_ref(go)))
+    mean = float(r["image"].mean())
+    expect = furnace_expected(0.5, 1.0, FURNACE_DEPTH)
+    assert abs(mean - expect) < 0.01 * expect, (mean, expect)
+
+
+def C_ref(opts):
+    import ctypes as C
+    return C.pointer(opts)
