@@ -50,6 +50,77 @@ def cpu_baseline(sc, cam, spp, depth, target_s=15.0):
     }
 
 
+def _time_to(curve, threshold):
+    """first crossing of relMSE <= threshold on a (ms, relMSE) curve, log-log interpolated"""
+    import math
+    for (t0, e0), (t1, e1) in zip(curve, curve[1:]):
+        if e1 <= threshold < e0:
+            f = (math.log(e0) - math.log(threshold)) / (math.log(e0) - math.log(e1))
+            return t0 + f * (t1 - t0)
+    if curve and curve[0][1] <= threshold:
+        return curve[0][0]
+    return None
+
+
+def guided_leg(full, threshold=1e-3):
+    """Time to relMSE <= threshold, plain vs guided, on the narrow-aperture room.
+
+    The room is lit only through a 0.3x0.3 gap in its +x wall by a 0.5x0.5 quad light
+    (radiance 50) outside, aimed 45 degrees upward through the gap, so the beam lands on
+    the ceiling and the camera's view is lit indirectly.  The judge is a plain render of
+    judge_spp samples per pixel; time is the sum of the render calls' wall time (all
+    kernels: tracing, MLP inference, training), relMSE computed between calls.
+    """
+    import numpy as np
+    from rainbow_amd import scenes
+    from rainbow_amd.tracer import Tracer, guiding_opts, render_params
+    res, judge_spp = (512, 65536) if full else (256, 16384)
+    sc, cam = scenes.narrow_aperture_room(level=8, width=res, height=res, light_tilt=45.0)
+    tr = Tracer(0)
+    tr.upload(sc)
+    depth, chunk, max_spp, train_calls = 8, 8, 4096, 32
+    t0 = time.perf_counter()
+    ref = tr.render(cam, render_params(spp=judge_spp, max_depth=depth, seed=1 << 20))["image"]
+    judge_s = time.perf_counter() - t0
+    print(f"# guided leg: judge {judge_spp} spp in {judge_s:.1f} s", file=sys.stderr, flush=True)
+
+    def run(guided):
+        acc = np.zeros_like(ref)
+        elapsed, curve, n = 0.0, [], 0
+        if guided:
+            tr.guide_reset(7)
+        while n * chunk < max_spp:
+            kw = {}
+            if guided:
+                go = guiding_opts(train=n < train_calls, steps_per_pass=4, batch=1 << 18)
+                kw["guiding"] = C.pointer(go)
+            t = time.perf_counter()
+            img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=(2 if guided else 1) * 100000 + n,
+                                               **kw), stats=False)["image"]
+            elapsed += time.perf_counter() - t
+            acc += img
+            n += 1
+            e = scenes.rel_mse(acc / n, ref)
+            curve.append((elapsed * 1e3, e))
+            if e <= threshold * 0.8:
+                break
+        return curve, n * chunk
+
+    pc, pspp = run(False)
+    gc, gspp = run(True)
+    tp, tg = _time_to(pc, threshold), _time_to(gc, threshold)
+    return {
+        "workload": f"narrow-aperture room, displaced icosphere level 8, {res}x{res}, depth 8, "
+                    f"beam through a 0.3x0.3 gap onto the ceiling",
+        "judge_spp": judge_spp, "threshold": threshold,
+        "plain_ms_to_threshold": None if tp is None else round(tp, 1),
+        "guided_ms_to_threshold": None if tg is None else round(tg, 1),
+        "speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
+        "plain_spp_run": pspp, "guided_spp_run": gspp,
+        "guided_training": f"online, first {train_calls} calls of {chunk} spp, 4 Adam steps of 2^18 per pass",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +129,9 @@ def main():
     ap.add_argument("--workload", default="icosphere", choices=["icosphere", "cornell"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
+    ap.add_argument("--guided", default="reduced", choices=["reduced", "full", "off"],
+                    help="guided-vs-plain time-to-relMSE leg: 256^2 with a 16384-spp judge (reduced), "
+                         "512^2 with a 65536-spp judge (full), or skipped")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -140,6 +214,8 @@ def main():
         "mrays_per_s": round(rays / wall / 1e6, 2),
         "roofline": roof,
     }
+    if args.guided != "off":
+        out["guided"] = guided_leg(args.guided == "full")
     if not args.no_cpu:
         cb = cpu_baseline(sc, cam, spp, depth)
         out["cpu_baseline"] = cb

@@ -176,7 +176,7 @@ enum {
     RBW_KERNEL_CONNECT = 3,
     RBW_KERNEL_FILM = 4,
     RBW_KERNEL_GUIDE_INFER = 5,      /* MLP inference over the guided vertices */
-    RBW_KERNEL_CONTINUE = 6,         /* guided continuation sample */
+    RBW_KERNEL_SHADE_GUIDED = 6,     /* guided vertex: light, MIS and continuation samples */
     RBW_KERNEL_GUIDE_TRAIN = 7,      /* training-sample collection and Adam steps */
     RBW_KERNEL_COUNT = 8
 };
@@ -197,6 +197,9 @@ typedef struct rbw_stats {
     uint64_t extend_prim_tests;
     double kernel_ms[RBW_KERNEL_COUNT];
     uint64_t kernel_launches[RBW_KERNEL_COUNT];
+    uint64_t guide_samples;          /* guided + train: training samples collected */
+    uint64_t guide_steps;            /* guided + train: Adam steps taken */
+    double guide_loss;               /* guided + train: loss of the last step, before its update */
 } rbw_stats;
 
 typedef struct rbw_device_opts {

@@ -32,7 +32,7 @@ RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL = 1, 2
 RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY = 0, 1
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
-RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "continue", "guide_train"]
+RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "shade_guided", "guide_train"]
 RBW_GUIDE_PARAMS = 34160
 RBW_KERNEL_COUNT = 8
 
@@ -89,7 +89,8 @@ class rbw_stats(C.Structure):
                 ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64),
                 ("extend_node_visits", C.c_uint64), ("extend_prim_tests", C.c_uint64),
                 ("kernel_ms", C.c_double * RBW_KERNEL_COUNT),
-                ("kernel_launches", C.c_uint64 * RBW_KERNEL_COUNT)]
+                ("kernel_launches", C.c_uint64 * RBW_KERNEL_COUNT),
+                ("guide_samples", C.c_uint64), ("guide_steps", C.c_uint64), ("guide_loss", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("kernel_ms", "kernel_launches")}

@@ -73,7 +73,7 @@ __device__ __forceinline__ h4 load_h4(const _Float16* p) { return *reinterpret_c
 template <bool STORE>
 __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w16, const float* __restrict__ w32,
                                                  const float* __restrict__ inputs, uint32_t n, uint32_t n_pad,
-                                                 float* __restrict__ out, _Float16* __restrict__ acts) {
+                                                 float* __restrict__ out, _Float16* __restrict__ acts, sample_map map) {
     __shared__ _Float16 tile[4][16][TILE_PAD];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w1
     for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
         const uint32_t r0 = chunk * 16;
         const uint32_t arow = r0 + li;  // the row this lane feeds as A operand
-        const uint32_t src = arow < n ? arow : n - 1;
+        const uint32_t src = map(arow < n ? arow : n - 1);
         float x[N_IN];
 #pragma unroll
         for (int i = 0; i < N_IN; i++) x[i] = inputs[(size_t)src * N_IN + i];
@@ -190,9 +190,9 @@ __device__ float mixture_grad(const float* o, const float* d, float scale, float
     return logf(q);
 }
 
-__global__ void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict__ total) {
+__global__ void k_sum(const float* __restrict__ c, uint32_t n, sample_map map, float* __restrict__ total) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float v = i < n ? c[i] : 0.0f;
+    float v = i < n ? c[map(i)] : 0.0f;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if ((threadIdx.x & 63) == 0 && v != 0.0f) atomicAdd(total, v);
 }
@@ -200,7 +200,7 @@ __global__ void k_sum(const float* __restrict__ c, uint32_t n, float* __restrict
 // per row: the clamped, loss-scaled output gradient (FP16, 48 columns) and the loss term
 __global__ void k_mixture_grad(const float* __restrict__ out, const float* __restrict__ dirs,
                                const float* __restrict__ c, const float* __restrict__ csum, uint32_t n,
-                               uint32_t n_pad, _Float16* __restrict__ dz, float* __restrict__ loss) {
+                               uint32_t n_pad, sample_map map, _Float16* __restrict__ dz, float* __restrict__ loss) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     float lterm = 0.0f;
     if (r < n_pad) {
@@ -209,10 +209,11 @@ __global__ void k_mixture_grad(const float* __restrict__ out, const float* __res
         if (r < n) {
             const float total = *csum;
             const float cmean = total / (float)n;
-            const float wr = cmean > 0.0f ? fminf(c[r] / cmean, 64.0f) : 0.0f;
+            const uint32_t src = map(r);
+            const float wr = cmean > 0.0f ? fminf(c[src] / cmean, 64.0f) : 0.0f;
             if (wr > 0.0f) {
-                const float lq = mixture_grad(out + (size_t)r * N_OUT_PAD, dirs + (size_t)r * 3, wr, g);
-                lterm = -(c[r] / total) * lq;
+                const float lq = mixture_grad(out + (size_t)r * N_OUT_PAD, dirs + (size_t)src * 3, wr, g);
+                lterm = -(c[src] / total) * lq;
             }
         }
         for (int k = 0; k < N_OUT_PAD; k++)
@@ -414,14 +415,39 @@ static uint32_t grid_waves(uint32_t n_pad) {
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s) {
     if (n == 0) return;
     const uint32_t n_pad = (n + 15) & ~15u;
+    sample_map id;
+    id.n_src = n;
     hipLaunchKernelGGL(k_forward<false>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
-                       out, nullptr);
+                       out, nullptr, id);
     GCHECK(hipGetLastError());
 }
 
+sample_map make_sample_map(uint32_t n_src, uint64_t seed) {
+    sample_map m;
+    m.n_src = n_src;
+    if (n_src <= 1) return m;
+    pcg32 g;
+    g.seed(seed);
+    auto gcd = [](uint64_t x, uint64_t y) {
+        while (y) {
+            const uint64_t t = x % y;
+            x = y;
+            y = t;
+        }
+        return x;
+    };
+    uint64_t a = (uint64_t(g.next_u32()) % n_src) | 1u;
+    while (gcd(a, n_src) != 1) a = (a + 2) % n_src;
+    m.a = uint32_t(a);
+    m.b = g.next_u32() % n_src;
+    return m;
+}
+
 void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
-                const adam_opts& opt, bool apply, hipStream_t s) {
+                const adam_opts& opt, bool apply, hipStream_t s, sample_map map) {
     if (n == 0) return;
+    if (map.n_src == 0) map.n_src = n;
+    if (map.n_src < n) throw std::runtime_error("training batch larger than the sample set");
     model_ensure(M, n);
     const uint32_t n_pad = (n + 15) & ~15u;
     _Float16* dz5 = M.dz + (size_t)n_pad * 2 * HIDDEN;
@@ -429,10 +455,10 @@ void train_step(model& M, const float* inputs, const float* dirs, const float* w
     _Float16* dzB = M.dz + (size_t)n_pad * HIDDEN;
     GCHECK(hipMemsetAsync(M.loss, 0, sizeof(float) * 4, s));
     hipLaunchKernelGGL(k_forward<true>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
-                       M.out, M.acts);
-    hipLaunchKernelGGL(k_sum, dim3((n + 255) / 256), dim3(256), 0, s, weights, n, M.loss + 1);
+                       M.out, M.acts, map);
+    hipLaunchKernelGGL(k_sum, dim3((n + 255) / 256), dim3(256), 0, s, weights, n, map, M.loss + 1);
     hipLaunchKernelGGL(k_mixture_grad, dim3((n_pad + 255) / 256), dim3(256), 0, s, M.out, dirs, weights,
-                       M.loss + 1, n, n_pad, dz5, M.loss);
+                       M.loss + 1, n, n_pad, map, dz5, M.loss);
     const uint32_t rows_per_block = 2048;
     const uint32_t wblocks = (n_pad + rows_per_block - 1) / rows_per_block;
     const _Float16* dz = dz5;

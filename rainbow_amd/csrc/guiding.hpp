@@ -60,12 +60,22 @@ void model_ensure(model& M, uint32_t rows);                   // training scratc
 void model_get(model& M, float* host_params, hipStream_t s);  // N_PARAMS floats (padded layout)
 void model_set(model& M, const float* host_params, hipStream_t s);
 
+// training rows r = 0..n-1 read sample (a * r + b) mod n_src of the collected set; with
+// gcd(a, n_src) = 1 and n <= n_src the rows are distinct samples
+struct sample_map {
+    uint32_t n_src = 0, a = 1, b = 0;
+    __host__ __device__ uint32_t operator()(uint32_t r) const {
+        return (uint32_t)(((uint64_t)a * r + b) % n_src);
+    }
+};
+sample_map make_sample_map(uint32_t n_src, uint64_t seed);
+
 // inputs: n x 9 floats in [0,1] (device); out: n x N_OUT_PAD raw outputs (device)
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s);
 // one training step on n samples: inputs n x 9, dirs n x 3, weights n (radiance / pdf), all device.
 // Leaves the gradient in M.grad (scaled to the mean loss) and, when apply, takes an Adam step.
 void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
-                const adam_opts& opt, bool apply, hipStream_t s);
+                const adam_opts& opt, bool apply, hipStream_t s, sample_map map = sample_map());
 // Adam step on the accumulated gradient of n samples (bias corrections c1, c2)
 void adam_apply(model& M, uint32_t n, const adam_opts& opt, float c1, float c2, hipStream_t s);
 

@@ -416,8 +416,8 @@ __device__ __forceinline__ surf hit_surface(const dscene& S, uint32_t prim, f3 o
 
 // sample_surface_interaction (integrator.cpp:329-404) up to the light and BSDF
 // samples of next-event estimation, then path_integrator.cpp:35-43.  GUIDED:
-// vertices with a BSDF go to the guide queue instead; k_continue draws their
-// continuation after the MLP has run.
+// vertices with a BSDF go to the guide queue instead; k_shade_guided samples
+// them once the MLP has produced their mixtures.
 template <bool GUIDED>
 __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, const uint32_t* qin_count,
                         uint32_t* qout, uint32_t* qout_count, uint32_t* cq, uint32_t* cq_count) {
@@ -455,6 +455,15 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
             bool cont = true;
             // emission at the first vertex (integrator.cpp:350-351; no specular lobes exist)
             if (bounce == 0 && (E.flags & RBW_ENTITY_EMITTER)) value = value + beta * light_evaluate(E, si.normal, -d_w);
+            if (GUIDED && bounce > 0 && (E.flags & RBW_ENTITY_EMITTER)) {
+                // the guided continuation is the second technique of next-event estimation
+                const f3 Le = light_evaluate(E, si.normal, -d_w);
+                if (!is_black(Le)) {
+                    const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
+                    const float lp = light_pdf(S, E, mp, mn, mw) / (float)S.num_emitters;
+                    value = value + beta * Le * power_heuristic(P.F(F_MPDF, slot), lp);
+                }
+            }
             if (E.material == RBW_NONE) {
                 // invisible entity: straight through, the bounce does not count (integrator.cpp:355-363)
                 f3 no, nd;
@@ -465,7 +474,7 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
             } else {
                 const bsdf fn = make_bsdf(S, E.material, prim);
                 uint32_t cflags = 0;
-                if (fn.kind != LOBE_NONE) {
+                if (!GUIDED && fn.kind != LOBE_NONE) {
                     // uniform_sample_one_emitter (integrator.cpp:101-161): light sample
                     const uint32_t ne = S.num_emitters;
                     const float u1 = smp.next();
@@ -630,14 +639,36 @@ struct vmf_mixture {
     }
 };
 
-// guided continuation: one-sample MIS of the vMF mixture and the BSDF, then
-// Russian roulette (path_integrator.cpp:35-43); the sampled direction becomes a
-// training record closed when the next vertex is reached
-__global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint32_t* qout_count) {
+// sample from the guided technique: the vMF mixture with probability alpha, else
+// the BSDF; returns false when no direction results
+__device__ __forceinline__ bool guided_sample(const bsdf& fn, const surf& si, const vmf_mixture& mix, float alpha,
+                                              float sel, f2 u, f3& wi) {
+    if (sel < alpha) {
+        wi = mix.sample(u);
+        return true;
+    }
+    f3 f0;
+    float p0;
+    return fn.sample(si, u, wi, f0, p0);
+}
+
+// Guided vertex: next-event estimation whose second technique is the path's own
+// continuation, drawn from the guided density p_G = alpha q + (1 - alpha) p_bsdf.
+// The light sample is weighted by the power heuristic against p_G (the structure
+// of integrator.cpp:101-161 with the BSDF technique replaced); the continuation
+// ray, when it reaches an emitter, adds the emission weighted against the light
+// pdf (k_shade, bounce > 0).  At the last vertex there is no continuation and the
+// light sample takes weight 1.  Russian roulette as path_integrator.cpp:35-43.
+// The continuation direction becomes a training record, closed at the next vertex:
+// its target is everything the continuation brings back (emission weighted by its
+// MIS weight, plus all later bounces), so the guide learns the radiance that light
+// sampling does not already cover.
+__global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, uint32_t* qout_count, uint32_t* cq,
+                               uint32_t* cq_count) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const dguide& G = prm.g;
     const uint32_t count = *G.gq_count;
-    bool alive = false;
+    bool alive = false, need_connect = false;
     uint32_t slot = 0;
     if (g < count) {
         slot = G.gq[g];
@@ -658,22 +689,52 @@ __global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint
         sampler_state smp = load_sampler(P, prm, slot);
         int bounce = (int)P.U(U_BOUNCE, slot);
         f3 beta = P.F3(F_BX, slot);
-        bool cont = true;
-        const float sel = smp.next();
-        const f2 u = smp.next2();
-        f3 wi;
-        if (sel < G.alpha) {
-            wi = mix.sample(u);
-        } else {
-            f3 f0;
-            float p0;
-            if (!fn.sample(si, u, wi, f0, p0)) cont = false;
+        const f3 wo_s = world_to_local(si.sh, si.wo);
+        const float a = G.alpha;
+        const bool last = (uint32_t)bounce + 1 >= prm.max_depth;
+        auto pdf_g = [&](f3 w, f3 w_s) { return a * mix.pdf(w) + (1.0f - a) * fn.pdf(wo_s, w_s); };
+        // light sample
+        const uint32_t ne = S.num_emitters;
+        const float u1 = smp.next();
+        uint32_t which = (uint32_t)floorf(u1 * (float)ne);
+        if (which > ne - 1) which = ne - 1;
+        const float pick = 1.0f / (float)ne;
+        const uint32_t lid = S.emitters[which];
+        const dentity L = S.ents[lid];
+        light_sample ls = sample_light(S, L, si.point, si.normal, smp.next2());
+        ls.pdf = ls.pdf * pick;
+        if (!is_black(ls.L) && ls.pdf > 0.0f) {
+            const f3 wi = world_to_local(si.sh, ls.wi);
+            f3 f = fn.evaluate(wo_s, wi);
+            f = f * fabsf(dot(ls.wi, si.sh.z));
+            if (!is_black(f)) {
+                f3 so, sd;
+                float slen;
+                spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
+                const float w = last ? 1.0f : power_heuristic(ls.pdf, pdf_g(ls.wi, wi));
+                P.setF3(F_SOX, slot, so);
+                P.setF3(F_SDX, slot, sd);
+                P.F(F_SLEN, slot) = slen;
+                P.setF3(F_SFX, slot, f * ls.L * w / ls.pdf);
+                P.U(U_SENT, slot) = lid;
+                P.setF3(F_CBX, slot, beta);
+                P.U(U_CFLAGS, slot) = C_SHADOW | ((uint32_t)bounce << 8);
+                need_connect = true;
+            }
         }
+        // continuation from the guided technique
+        bool cont = !last;
+        f3 wi = f3{0.0f, 0.0f, 0.0f};
         float pdf = 0.0f;
         if (cont) {
-            const f3 wo_s = world_to_local(si.sh, si.wo), wi_s = world_to_local(si.sh, wi);
+            const float sel = smp.next();
+            const f2 u = smp.next2();
+            cont = guided_sample(fn, si, mix, a, sel, u, wi);
+        }
+        if (cont) {
+            const f3 wi_s = world_to_local(si.sh, wi);
             const f3 f = fn.evaluate(wo_s, wi_s);
-            pdf = G.alpha * mix.pdf(wi) + (1.0f - G.alpha) * fn.pdf(wo_s, wi_s);
+            pdf = pdf_g(wi, wi_s);
             if (is_black(f) || !(pdf > 0.0f)) {
                 cont = false;
             } else {
@@ -682,6 +743,11 @@ __global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint
                 spawn_ray(si.point, si.normal, wi, no, nd);
                 P.setF3(F_OX, slot, no);
                 P.setF3(F_DX, slot, nd);
+                // what the next vertex needs to weight emission it finds against light sampling
+                P.setF3(F_MPX, slot, si.point);
+                P.setF3(F_MNX, slot, si.normal);
+                P.setF3(F_MWX, slot, wi);
+                P.F(F_MPDF, slot) = pdf;
             }
         }
         if (cont) {
@@ -694,7 +760,7 @@ __global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint
         }
         const int b0 = bounce;
         bounce += 1;
-        alive = cont && (uint32_t)bounce < prm.max_depth;
+        alive = cont;
         if (alive && G.train) {
             const size_t ri = (size_t)slot * prm.max_depth + (uint32_t)b0;
             const size_t rc = (size_t)P.cap * prm.max_depth;
@@ -713,8 +779,10 @@ __global__ void k_continue(dscene S, dpaths P, dparams prm, uint32_t* qout, uint
         P.U(U_BOUNCE, slot) = (uint32_t)bounce;
         P.rng[slot] = smp.state;
     }
-    const uint32_t a = wave_append(qout_count, alive);
-    if (alive) qout[a] = slot;
+    const uint32_t ai = wave_append(qout_count, alive);
+    if (alive) qout[ai] = slot;
+    const uint32_t ci = wave_append(cq_count, need_connect);
+    if (need_connect) cq[ci] = slot;
 }
 
 // after a pass: closed training records -> (query, direction, radiance / pdf)
@@ -981,6 +1049,7 @@ struct dbuf {
 
 constexpr uint32_t k_instanced_threshold = 1u << 17;  // scene.cpp:43
 constexpr uint32_t k_max_light_triangles = 4096;
+constexpr uint32_t k_train_pool = 8;  // training samples kept per pass, in batches
 
 struct kernel_timer {
     struct rec {
@@ -1334,6 +1403,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 "guiding mis_fraction must be in [0, 1]");
         require(!go->train || (go->batch > 0 && go->batch <= (1u << 20)), RBW_ERR_INVALID_ARGUMENT,
                 "guiding batch must be in [1, 2^20]");
+        require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG, RBW_ERR_INVALID_ARGUMENT,
+                "guiding draws a different number of samples per vertex than the replayed reference stream");
     }
     uint32_t k = pr.samples_per_pass;
     // guided renders train between passes: smaller passes give the network more updates
@@ -1429,11 +1500,14 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 ctx->g_rec_cap = uint32_t(nrec);
             }
             HIP_CHECK(hipMemsetAsync(ctx->g_flag.p, 0, nrec * 4, s));
-            if (ctx->g_t_cap < go->batch) {
-                ctx->g_tin.ensure(size_t(go->batch) * 9 * sizeof(float));
-                ctx->g_tdir.ensure(size_t(go->batch) * 3 * sizeof(float));
-                ctx->g_tc.ensure(size_t(go->batch) * sizeof(float));
-                ctx->g_t_cap = go->batch;
+            // a pass closes up to one continuation record per path vertex; it keeps up to
+            // k_train_pool batches of them and each step draws a batch from that pool
+            const uint32_t tcap = go->batch * k_train_pool;
+            if (ctx->g_t_cap < tcap) {
+                ctx->g_tin.ensure(size_t(tcap) * 9 * sizeof(float));
+                ctx->g_tdir.ensure(size_t(tcap) * 3 * sizeof(float));
+                ctx->g_tc.ensure(size_t(tcap) * sizeof(float));
+                ctx->g_t_cap = tcap;
             }
             dp.g.rec = ctx->g_rec.as<float>();
             dp.g.rec_flag = ctx->g_flag.as<uint32_t>();
@@ -1441,7 +1515,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             dp.g.t_dir = ctx->g_tdir.as<float>();
             dp.g.t_c = ctx->g_tc.as<float>();
             dp.g.t_count = ctx->counts.as<uint32_t>() + 4;
-            dp.g.t_cap = go->batch;
+            dp.g.t_cap = tcap;
         }
     }
     const dscene S = ctx->scene_view();
@@ -1458,7 +1532,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     ctx->timer.reset();
     hipEvent_t ev0 = ctx->timer.get(), ev1 = ctx->timer.get();
     HIP_CHECK(hipEventRecord(ev0, s));
-    uint64_t ext_rays = 0;
+    uint64_t ext_rays = 0, guide_samples = 0, guide_steps = 0;
     const uint32_t BS = 256;
     uint32_t* qa = ctx->q0.as<uint32_t>();
     uint32_t* qb = ctx->q1.as<uint32_t>();
@@ -1466,6 +1540,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     for (uint32_t s0 = 0; s0 < pr.spp; s0 += k) {
         const uint32_t kk = std::min(k, pr.spp - s0);
         const uint32_t n = uint32_t(uint64_t(kk) * npix);
+        // the pass's training samples are its closed continuation records
+        if (guided && go->train) HIP_CHECK(hipMemsetAsync(counts + 4, 0, 4, s));
         if (timed) ctx->timer.begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_raygen, dim3(blocks(n, BS)), dim3(BS), 0, s, S, P, dp, dc, n, s0, qa, counts + 0);
         if (timed) ctx->timer.end(s);
@@ -1509,8 +1585,9 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                     if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_INFER, s);
                     guide::forward(ctx->gm, dp.g.query, ng, dp.g.mix, s);
                     if (timed) ctx->timer.end(s);
-                    if (timed) ctx->timer.begin(RBW_KERNEL_CONTINUE, s);
-                    hipLaunchKernelGGL(k_continue, dim3(blocks(ng, BS)), dim3(BS), 0, s, S, P, dp, qout, counts + cout);
+                    if (timed) ctx->timer.begin(RBW_KERNEL_SHADE_GUIDED, s);
+                    hipLaunchKernelGGL(k_shade_guided, dim3(blocks(ng, BS)), dim3(BS), 0, s, S, P, dp, qout,
+                                       counts + cout, cqp, counts + 2);
                     if (timed) ctx->timer.end(s);
                     HIP_CHECK(hipGetLastError());
                 }
@@ -1532,17 +1609,22 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         }
         if (guided && go->train) {
             if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
-            HIP_CHECK(hipMemsetAsync(counts + 4, 0, 4, s));
             hipLaunchKernelGGL(k_collect, dim3(blocks(uint64_t(n) * pr.max_depth, BS)), dim3(BS), 0, s, P, dp, n);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(ctx->host_count + 2, counts + 4, 4, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
-            const uint32_t nt = std::min(ctx->host_count[2], go->batch);
+            const uint32_t pool = std::min(ctx->host_count[2], dp.g.t_cap);
+            const uint32_t nt = std::min(pool, go->batch);
+            guide_samples += ctx->host_count[2];
             if (nt >= 256) {
+                guide_steps += go->steps_per_pass;
                 guide::adam_opts ao;
                 ao.lr = go->learning_rate;
-                for (uint32_t st = 0; st < go->steps_per_pass; st++)
-                    guide::train_step(ctx->gm, dp.g.t_in, dp.g.t_dir, dp.g.t_c, nt, ao, true, s);
+                for (uint32_t st = 0; st < go->steps_per_pass; st++) {
+                    const guide::sample_map map = guide::make_sample_map(pool, (uint64_t(pr.seed) << 32) ^
+                                                                                   (uint64_t(s0) << 8) ^ st);
+                    guide::train_step(ctx->gm, dp.g.t_in, dp.g.t_dir, dp.g.t_c, nt, ao, true, s, map);
+                }
             }
             if (timed) ctx->timer.end(s);
         }
@@ -1563,6 +1645,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     if (ndbg) HIP_CHECK(hipMemcpyAsync(pr.debug_hits, ctx->dbg.p, ndbg * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     unsigned long long hc[16];
     HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    float gloss = 0.0f;
+    if (guide_steps) HIP_CHECK(hipMemcpyAsync(&gloss, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -1585,6 +1669,9 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             stats->kernel_ms[r.kind] += kms;
             stats->kernel_launches[r.kind] += 1;
         }
+        stats->guide_samples = guide_samples;
+        stats->guide_steps = guide_steps;
+        stats->guide_loss = gloss;
         const double sec = ms * 1e-3;
         stats->msamples_per_s = sec > 0 ? stats->samples / sec * 1e-6 : 0.0;
         const uint64_t rays = stats->extension_rays + stats->shadow_rays + stats->mis_rays;

@@ -402,9 +402,12 @@ def displaced_icosphere_box(level=8, width=1024, height=1024, seed=1):
     return sc, cam
 
 
-def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0, seed=1, mesh=True):
+def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0, seed=1, mesh=True,
+                         light_dist=0.5, light_size=0.5, light_tilt=0.0):
     """A closed room lit only through a gap x gap hole in its +x wall by a quad light outside
-    (radiance 50); the displaced icosphere with per-triangle materials inside; camera inside."""
+    (radiance 50, light_size square, light_dist from the hole centre along its normal, which
+    points through the hole tilted light_tilt degrees upward from -x); the displaced
+    icosphere with per-triangle materials inside; camera inside."""
     sc = Scene()
     white = sc.matte((0.73, 0.73, 0.73))
     black = sc.matte((0.0, 0.0, 0.0))
@@ -427,9 +430,17 @@ def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0,
         Is.append(I + 4 * k)
         Ns.append(N)
     sc.add_mesh(np.concatenate(Ps), np.concatenate(Is), white, normals=np.concatenate(Ns))
-    # the light outside, facing the hole (n=-x)
-    s = 0.25
-    P, I, N = _quad((1.5, cy - s, -s), (1.5, cy - s, s), (1.5, cy + s, s), (1.5, cy + s, -s))
+    # the light outside, its normal through the hole centre
+    th = math.radians(light_tilt)
+    n = np.array([-math.cos(th), math.sin(th), 0.0])
+    u = np.array([math.sin(th), math.cos(th), 0.0])
+    v = np.array([0.0, 0.0, 1.0])
+    c = np.array([1.0, cy, 0.0]) - light_dist * n
+    s = 0.5 * light_size
+    corners = [c - s * u - s * v, c - s * u + s * v, c + s * u + s * v, c + s * u - s * v]
+    if np.dot(np.cross(corners[1] - corners[0], corners[2] - corners[0]), n) < 0:
+        corners = corners[::-1]
+    P, I, N = _quad(*corners)
     sc.add_mesh(P, I, black, normals=N, radiance=(radiance,) * 3)
     if mesh:
         P, F, Nn = fbm_displaced_icosphere(level=level, seed=seed, center=(-0.2, -0.45, -0.3), radius=0.45)

@@ -202,6 +202,28 @@ def test_gpu_guided_furnace(tracer):
     assert abs(mean - expect) < 0.01 * expect, (mean, expect)
 
 
+@pytest.mark.gpu
+def test_gpu_trained_guide_reduces_error(tracer):
+    """What the trained network gains: on the narrow-aperture room (beam through the gap onto
+    the ceiling, the view lit indirectly) the guided estimator, after online training,
+    has lower relMSE than the plain one at equal spp, with the same expectation."""
+    from rainbow_amd.tracer import guiding_opts, render_params
+    sc, cam = scenes.narrow_aperture_room(level=5, width=96, height=96, light_tilt=45.0)
+    tracer.upload(sc)
+    ref = tracer.render(cam, render_params(spp=16384, max_depth=8, seed=100))["image"]
+    plain = tracer.render(cam, render_params(spp=256, max_depth=8, seed=200))["image"]
+    tracer.guide_reset(3)
+    for i in range(16):
+        go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 16)
+        tracer.render(cam, render_params(spp=16, max_depth=8, seed=300 + i, guiding=C_ref(go)))
+    go = guiding_opts(train=False)
+    guided = tracer.render(cam, render_params(spp=256, max_depth=8, seed=400, guiding=C_ref(go)))["image"]
+    ep, eg = scenes.rel_mse(plain, ref), scenes.rel_mse(guided, ref)
+    print("relMSE plain", ep, "guided", eg, "ratio", eg / ep)
+    assert abs(guided.mean() - ref.mean()) < 0.01 * ref.mean()
+    assert eg < 0.97 * ep
+
+
 def C_ref(opts):
     import ctypes as C
     return C.pointer(opts)
