@@ -262,6 +262,69 @@ rbw_status rbw_guide_forward(rbw_ctx* ctx, uint32_t n, const float* inputs, floa
 rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const float* dirs, const float* c,
                            float learning_rate, uint32_t apply, float* loss, float* grad);
 
+/* ---------------------------------------------------------------- batches
+ * Many render jobs of uneven size in one wavefront (no counterpart in the
+ * reference, whose integrator::render, integrators/integrator.hpp:64-66,
+ * renders one camera of one scene per call).  Scenes are added to the
+ * context's scene pool once; a job names its scene, camera and sampling
+ * parameters.  All jobs of a submission share one pool of path slots: a path
+ * that ends is replaced by the next camera sample of the job queue, and the
+ * bounce loop (regenerate, extend, shade, connect, retire) runs as a captured
+ * hipGraph.  Each job's image equals the image rbw_render gives for the same
+ * scene, camera and parameters (per-sample pcg32 stream, same seed). */
+
+/* adds a scene to the pool (validated and BVH-built; the caller keeps the arrays) */
+rbw_status rbw_add_scene(rbw_ctx* ctx, const rbw_scene_desc* scene, uint32_t* scene_id);
+/* frees a pooled scene's host copy; its id stays invalid afterwards */
+rbw_status rbw_remove_scene(rbw_ctx* ctx, uint32_t scene_id);
+
+typedef struct rbw_job {
+    uint32_t scene;                  /* id from rbw_add_scene */
+    uint32_t spp;
+    uint32_t max_depth;
+    float rr_threshold;              /* path_integrator.cpp:35-43 */
+    uint32_t seed;                   /* per-sample pcg32 stream seed (RBW_SAMPLER_PER_SAMPLE_PCG) */
+    rbw_camera_desc camera;
+    float* out_rgb;                  /* width*height*3 host floats, or NULL */
+} rbw_job;
+
+typedef struct rbw_batch_opts {
+    uint32_t block_size;             /* threads per block of the wavefront kernels; 0 = default 256 */
+    uint32_t pool_size;              /* path slots shared by all jobs; 0 = default 2^21 */
+    uint32_t bvh_width;              /* 0 or 2: binary BVH (the only width built) */
+    uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
+    uint32_t autotune;               /* 1: time candidate block/pool sizes on a 1-spp probe first */
+    uint32_t triangle_test;          /* RBW_TRI_* */
+    uint32_t shadow_mode;            /* RBW_SHADOW_* */
+} rbw_batch_opts;
+
+typedef struct rbw_job_stats {
+    uint64_t samples;
+    double active_ms;                /* from the job's first camera sample to its last retired path */
+    double msamples_per_s;           /* samples / active_ms */
+    uint32_t first_iteration, last_iteration;
+} rbw_job_stats;
+
+typedef struct rbw_batch_stats {
+    double render_ms;                /* HIP-event time of the whole batch (films resolved) */
+    double msamples_per_s;           /* all jobs' samples / render_ms */
+    double mrays_per_s;
+    double autotune_ms;              /* probe time when autotune = 1 */
+    uint64_t samples, extension_rays, shadow_rays, mis_rays;
+    uint32_t iterations;             /* bounce-loop iterations */
+    uint32_t graph_launches;
+    uint32_t block_size, pool_size, bvh_width;  /* the configuration that ran */
+    uint32_t scene_triangles;        /* triangles of the merged scene */
+} rbw_batch_stats;
+
+/* renders every job; job_stats (nullable) has n entries */
+rbw_status rbw_submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, const rbw_batch_opts* opts,
+                           rbw_job_stats* job_stats, rbw_batch_stats* stats);
+/* the autotuner alone: fills block_size/pool_size/bvh_width of *opts with the
+ * fastest candidate on a 1-spp probe of the jobs */
+rbw_status rbw_autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, rbw_batch_opts* opts,
+                             double* probe_ms);
+
 #ifdef __cplusplus
 }
 #endif

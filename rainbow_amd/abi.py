@@ -112,6 +112,36 @@ HIT_DTYPE = np.dtype([("prim", np.uint32), ("entity", np.uint32), ("t", np.float
                       ("b2", np.float32)])
 
 
+class rbw_job(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_threshold", C.c_float),
+                ("seed", C.c_uint32), ("camera", rbw_camera_desc), ("out_rgb", C.POINTER(C.c_float))]
+
+
+class rbw_batch_opts(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("pool_size", C.c_uint32), ("bvh_width", C.c_uint32),
+                ("use_graph", C.c_uint32), ("autotune", C.c_uint32), ("triangle_test", C.c_uint32),
+                ("shadow_mode", C.c_uint32)]
+
+
+class rbw_job_stats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("active_ms", C.c_double), ("msamples_per_s", C.c_double),
+                ("first_iteration", C.c_uint32), ("last_iteration", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class rbw_batch_stats(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("msamples_per_s", C.c_double), ("mrays_per_s", C.c_double),
+                ("autotune_ms", C.c_double), ("samples", C.c_uint64), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("mis_rays", C.c_uint64), ("iterations", C.c_uint32),
+                ("graph_launches", C.c_uint32), ("block_size", C.c_uint32), ("pool_size", C.c_uint32),
+                ("bvh_width", C.c_uint32), ("scene_triangles", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def ptr(a, t):
     """ctypes pointer to a contiguous numpy array (None -> NULL)."""
     if a is None:
@@ -135,6 +165,12 @@ RBW_SIGNATURES = {
     "rbw_guide_set_params": (C.c_int32, [C.c_void_p, f32p]),
     "rbw_guide_forward": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p]),
     "rbw_guide_train": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_uint32, f32p, f32p]),
+    "rbw_add_scene": (C.c_int32, [C.c_void_p, C.POINTER(rbw_scene_desc), C.POINTER(C.c_uint32)]),
+    "rbw_remove_scene": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "rbw_submit_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),
+                                    C.POINTER(rbw_job_stats), C.POINTER(rbw_batch_stats)]),
+    "rbw_autotune_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),
+                                      C.POINTER(C.c_double)]),
 }
 
 

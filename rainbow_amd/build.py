@@ -42,7 +42,7 @@ def build_host(verbose=False):
 
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, "tracer.hip"), os.path.join(CSRC, "guiding.hip"), os.path.join(CSRC, "bvh.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("tracer.hip", "batch.hip", "scene.hip", "guiding.hip", "bvh.cpp")]
     tmp = OUT + ".tmp"
     cmd = [hipcc, *HIPCC_FLAGS, "-shared", "-o", tmp, *srcs]
     if verbose:

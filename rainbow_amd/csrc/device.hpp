@@ -51,6 +51,7 @@ struct dscene {
     uint32_t num_emitters;
     uint32_t num_tris;
     uint32_t num_spheres;
+    uint32_t root;           // BVH root node (a part's root inside a merged scene)
 };
 
 struct hitrec {

@@ -1,0 +1,671 @@
+// Device-side path state and the per-path stages shared by the single-render
+// wavefront (tracer.hip) and the batched multi-job wavefront (batch.hip):
+// BVH traversal, the SoA path state, samplers, and the surface lookup of an
+// extension hit.  The arithmetic follows rainbow-cpu's path integrator
+// (integrators/path_integrator.cpp:15-47, integrators/integrator.cpp:101-404).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rbw.h"
+#include "bvh.hpp"
+#include "device.hpp"
+#include "guiding.hpp"
+#include "shading.hpp"
+
+namespace rbw {
+
+#define RBW_STACK 64
+
+// ------------------------------------------------------------------ traversal
+
+// acceptance bound of the next candidate: inside the same instanced mesh the
+// reference compares against the mesh-local length directly (mesh.cpp:431);
+// any other candidate sees the world length mapped into its local space.
+__device__ __forceinline__ float thr_from(uint32_t best_inst, uint32_t cand_inst, float best_t, float len_w, float L0) {
+    if (cand_inst != RBW_NONE && cand_inst == best_inst) return best_t;
+    return len_w * L0;
+}
+
+
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
+                                           uint32_t target_entity, trace_counters& cnt) {
+    const local_ray lr = make_local_ray(o_w, d_w);
+    wt_setup wt;
+    if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+    hitrec best;
+    best.prim = RBW_MISS;
+    best.t = 0.0f;
+    best.b1 = best.b2 = 0.0f;
+    uint32_t best_inst_entity = RBW_NONE;  // entity of the best hit when it belongs to an instanced mesh
+    float thr_local = len_w * lr.L0;       // current acceptance bound in local units
+    uint32_t stack[RBW_STACK];
+    int sp = 0;
+    int32_t cur = (int32_t)S.root;
+    const f3 o = lr.o;
+    const f3 inv = lr.inv;
+    const f3 oi = f3{o.x * inv.x, o.y * inv.y, o.z * inv.z};
+    for (;;) {
+        if (cur >= 0) {
+            const float4* n = S.nodes + 4 * (size_t)cur;
+            const float4 a = n[0], b = n[1], c = n[2];
+            const int4 ch = *reinterpret_cast<const int4*>(n + 3);
+            if (COUNT) cnt.nodes++;
+            const float tmax = thr_local * 1.0000002f;
+            // left
+            // slab distances (lo - o) / d as fma(lo, 1/d, -o/d); the bounds below are widened
+            // by a few ulps so that rounding never culls a box that holds an accepted hit
+            const float lx0 = __builtin_fmaf(a.x, inv.x, -oi.x), lx1 = __builtin_fmaf(a.y, inv.x, -oi.x);
+            const float ly0 = __builtin_fmaf(a.z, inv.y, -oi.y), ly1 = __builtin_fmaf(a.w, inv.y, -oi.y);
+            const float lz0 = __builtin_fmaf(c.x, inv.z, -oi.z), lz1 = __builtin_fmaf(c.y, inv.z, -oi.z);
+            const float rx0 = __builtin_fmaf(b.x, inv.x, -oi.x), rx1 = __builtin_fmaf(b.y, inv.x, -oi.x);
+            const float ry0 = __builtin_fmaf(b.z, inv.y, -oi.y), ry1 = __builtin_fmaf(b.w, inv.y, -oi.y);
+            const float rz0 = __builtin_fmaf(c.z, inv.z, -oi.z), rz1 = __builtin_fmaf(c.w, inv.z, -oi.z);
+            const float lmin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), 0.0f));
+            const float lmax = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax));
+            const float rmin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), 0.0f));
+            const float rmax = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax));
+            const bool hl = lmin <= lmax * 1.0000004f;
+            const bool hr = rmin <= rmax * 1.0000004f;
+            if (hl && hr) {
+                const bool lfirst = lmin <= rmin;
+                cur = lfirst ? ch.x : ch.y;
+                stack[sp++] = (uint32_t)(lfirst ? ch.y : ch.x);
+                continue;
+            } else if (hl) {
+                cur = ch.x;
+                continue;
+            } else if (hr) {
+                cur = ch.y;
+                continue;
+            }
+        } else if (cur != k_empty_child) {
+            const uint32_t first = (uint32_t)cur & 0x07FFFFFFu;
+            const uint32_t count = (((uint32_t)cur >> 27) & 15u) + 1u;
+            for (uint32_t k = 0; k < count; k++) {
+                const uint32_t slot = first + k;
+                const float4 pa = S.prim_a[slot];
+                const uint32_t word = __float_as_uint(pa.w);
+                if (VISIBLE_ONLY && (word & k_prim_invisible)) continue;
+                if (COUNT) cnt.prims++;
+                const uint32_t id = word & k_prim_id_mask;
+                float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
+                bool hit;
+                uint32_t inst_entity = RBW_NONE;
+                if (word & k_prim_sphere) {
+                    const float4 pb = S.prim_b[slot];
+                    const float thr = thr_from(best_inst_entity, RBW_NONE, best.t, len_w, lr.L0);
+                    const f3 ol = translate(o_w, f3{-pa.x, -pa.y, -pa.z});
+                    hit = sphere_test(pb.x, ol, lr.d, thr, t);
+                } else {
+                    const float4 pb = S.prim_b[slot], pc = S.prim_c[slot];
+                    if (word & k_prim_instanced) inst_entity = S.tri_entity[id];
+                    const float thr = thr_from(best_inst_entity, inst_entity, best.t, len_w, lr.L0);
+                    const f3 p0 = f3{pa.x, pa.y, pa.z}, p1 = f3{pb.x, pb.y, pb.z}, p2 = f3{pc.x, pc.y, pc.z};
+                    if (tri_test == RBW_TRI_WATERTIGHT)
+                        hit = wt_test(p0, p1, p2, o, lr.d, thr, wt.kz, wt.Sx, wt.Sy, wt.Sz, t, b1, b2);
+                    else
+                        hit = mt_test(p0, p1, p2, o, lr.d, thr, t, b1, b2);
+                }
+                if (hit) {
+                    if (ANY_HIT) {
+                        const uint32_t ent =
+                            (word & k_prim_sphere) ? S.sphere_entity[id - S.num_tris] : S.tri_entity[id];
+                        if (ent != target_entity) {
+                            best.prim = id;
+                            best.t = t;
+                            return best;
+                        }
+                        continue;
+                    }
+                    best.prim = id;
+                    best.t = t;
+                    best.b1 = b1;
+                    best.b2 = b2;
+                    best_inst_entity = inst_entity;
+                    len_w = t * lr.L1;  // entity.cpp:384
+                    thr_local = (inst_entity != RBW_NONE) ? t : len_w * lr.L0;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = (int32_t)stack[--sp];
+    }
+    return best;
+}
+
+// ------------------------------------------------------------------ wave helpers
+
+// slot in an append queue for every lane with `pred` set; call with the whole wave converged
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t prefix = __popcll(mask & lt);
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + prefix;
+}
+
+__device__ __forceinline__ void wave_add64(unsigned long long* counter, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(counter, x);
+}
+
+// ------------------------------------------------------------------ path state
+
+enum fidx : uint32_t {
+    F_OX, F_OY, F_OZ, F_DX, F_DY, F_DZ,          // path ray (world)
+    F_BX, F_BY, F_BZ, F_VX, F_VY, F_VZ,          // beta, value
+    F_PX, F_PY,                                  // raster position of the camera sample
+    F_HT,                                        // extension hit t (local)
+    F_SOX, F_SOY, F_SOZ, F_SDX, F_SDY, F_SDZ, F_SLEN,  // shadow ray
+    F_SFX, F_SFY, F_SFZ,                         // f * Le * w / pdf of the light sample
+    F_MPX, F_MPY, F_MPZ, F_MNX, F_MNY, F_MNZ,    // vertex point and normal (MIS ray origin)
+    F_MWX, F_MWY, F_MWZ, F_MFX, F_MFY, F_MFZ, F_MPDF,  // BSDF sample direction, f*|cos|, pdf
+    F_CBX, F_CBY, F_CBZ,                         // beta at the vertex
+    F_COUNT
+};
+enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_COUNT };
+
+constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
+
+struct dpaths {
+    float* f;      // F_COUNT * cap
+    uint32_t* u;   // U_COUNT * cap
+    uint64_t* rng; // pcg32 state, or the replay cursor
+    uint32_t cap;
+    __device__ float& F(uint32_t k, uint32_t i) const { return f[(size_t)k * cap + i]; }
+    __device__ uint32_t& U(uint32_t k, uint32_t i) const { return u[(size_t)k * cap + i]; }
+    __device__ f3 F3(uint32_t k, uint32_t i) const { return f3{F(k, i), F(k + 1, i), F(k + 2, i)}; }
+    __device__ void setF3(uint32_t k, uint32_t i, f3 v) const {
+        F(k, i) = v.x;
+        F(k + 1, i) = v.y;
+        F(k + 2, i) = v.z;
+    }
+};
+
+struct dcamera {
+    float r2c[16];
+    float c2w[16];
+    uint32_t width, height;
+    int32_t bx0, by0, bx1, by1;
+    float lens_radius, focus;
+};
+
+// guided continuation: per guide-queue row the surface record, MLP query and
+// head outputs; per (slot, bounce) the training record of the sampled direction
+enum gsurf : uint32_t { G_PX, G_PY, G_PZ, G_NX, G_NY, G_NZ, G_WX, G_WY, G_WZ, G_XX, G_XY, G_XZ, G_YX, G_YY, G_YZ,
+                        G_ZX, G_ZY, G_ZZ, G_COUNT };
+enum grec : uint32_t { R_Q0 = 0, R_DX = 9, R_DY, R_DZ, R_PDF, R_BX, R_BY, R_BZ, R_MX, R_MY, R_MZ, R_COUNT };
+
+struct dguide {
+    uint32_t enable;
+    float alpha;                 // probability of sampling the guide
+    uint32_t train;
+    f3 smin, sinv;               // position normalisation: (p - smin) * sinv
+    float* surf;                 // G_COUNT * cap (SoA)
+    uint32_t* surf_mat;          // material of the row
+    uint32_t* gq;                // guide queue: slot of each row
+    uint32_t* gq_count;
+    float* query;                // 9 per row
+    float* mix;                  // 48 per row (head outputs)
+    float* rec;                  // R_COUNT * (cap * max_depth) (SoA)
+    uint32_t* rec_flag;          // 0 none, 1 sampled, 2 closed
+    uint32_t cap;
+    // training batch
+    float* t_in;                 // 9 per sample
+    float* t_dir;                // 3 per sample
+    float* t_c;                  // radiance / pdf
+    uint32_t* t_count;
+    uint32_t t_cap;
+};
+
+struct dparams {
+    uint32_t spp, max_depth;
+    float rr_threshold;
+    uint32_t seed;
+    uint32_t sampler;
+    uint32_t tri_test;
+    uint32_t shadow_mode;
+    const uint64_t* replay_offsets;
+    const float* replay_values;
+    uint32_t* debug;  // device debug hit buffer or null
+    dguide g;
+};
+
+// glm mat4 * vec4 (xyz): (m0*x + m1*y) + (m2*z + m3*w)
+__device__ __forceinline__ f3 mat_mul(const float* m, f3 v, float w) {
+    f3 r;
+    r.x = (m[0] * v.x + m[4] * v.y) + (m[8] * v.z + m[12] * w);
+    r.y = (m[1] * v.x + m[5] * v.y) + (m[9] * v.z + m[13] * w);
+    r.z = (m[2] * v.x + m[6] * v.y) + (m[10] * v.z + m[14] * w);
+    return r;
+}
+
+struct sampler_state {
+    uint64_t state;
+    uint64_t inc;
+    uint64_t end;
+    const float* replay;
+    __device__ float next() {
+        if (replay) return state < end ? replay[state++] : 0.0f;
+        const uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        const uint32_t r = (xs >> rot) | (xs << ((~rot + 1u) & 31));
+        const float x = (float)r * 2.3283064365386963e-10f;
+        return (x < k_one_minus_epsilon) ? x : k_one_minus_epsilon;
+    }
+    __device__ f2 next2() {
+        const float a = next();
+        const float b = next();
+        return f2{a, b};
+    }
+};
+
+__device__ __forceinline__ sampler_state load_sampler(const dpaths& P, const dparams& prm, uint32_t slot) {
+    sampler_state s;
+    const uint64_t sid = (uint64_t)P.U(U_PIXEL, slot) * prm.spp + P.U(U_SAMPLE, slot);
+    s.state = P.rng[slot];
+    if (prm.sampler == RBW_SAMPLER_REPLAY) {
+        s.replay = prm.replay_values;
+        s.end = prm.replay_offsets[sid + 1];
+        s.inc = 0;
+    } else {
+        s.replay = nullptr;
+        s.end = 0;
+        s.inc = (sample_stream(prm.seed, sid) << 1u) | 1u;
+    }
+    return s;
+}
+
+__device__ __forceinline__ uint32_t* debug_slot(const dpaths& P, const dparams& prm, uint32_t slot, int bounce) {
+    if (!prm.debug || bounce < 0 || (uint32_t)bounce >= prm.max_depth) return nullptr;
+    const uint64_t sid = (uint64_t)P.U(U_PIXEL, slot) * prm.spp + P.U(U_SAMPLE, slot);
+    return prm.debug + (sid * prm.max_depth + (uint32_t)bounce) * 3;
+}
+
+// surface interaction at the extension hit: entity::intersect's local ray and
+// local -> world mapping (entity.cpp:18-58) for a mesh (identity) or sphere (translation)
+__device__ __forceinline__ surf hit_surface(const dscene& S, uint32_t prim, f3 o_w, f3 d_w, uint32_t& entity) {
+    const local_ray lr = make_local_ray(o_w, d_w);
+    if (prim < S.num_tris) {
+        entity = S.tri_entity[prim];
+        const dentity E = S.ents[entity];
+        const uint32_t i0 = S.idx[3 * prim], i1 = S.idx[3 * prim + 1], i2 = S.idx[3 * prim + 2];
+        const f3 p0 = load3(S.pos, i0), p1 = load3(S.pos, i1), p2 = load3(S.pos, i2);
+        float t, b1, b2;
+        mt_test(p0, p1, p2, lr.o, lr.d, __builtin_inff(), t, b1, b2);
+        const float b0 = 1.0f - b1 - b2;
+        const surf l = triangle_interaction(S, E, prim, lr.d, b0, b1, b2);
+        return surf_to_world(l, false, f3{0, 0, 0});
+    }
+    const uint32_t sph = prim - S.num_tris;
+    entity = S.sphere_entity[sph];
+    const dentity E = S.ents[entity];
+    const float4 sp = S.spheres[sph];
+    const f3 c = f3{sp.x, sp.y, sp.z};
+    const f3 ol = translate(o_w, f3{-c.x, -c.y, -c.z});
+    float t = 0.0f;
+    sphere_test(sp.w, ol, lr.d, __builtin_inff(), t);
+    const surf l = sphere_interaction(sp.w, (E.flags & RBW_ENTITY_REVERSE_ORIENTATION) != 0, ol, lr.d, t);
+    return surf_to_world(l, true, c);
+}
+
+// sampler_integrator.cpp:98-127 + perspective_camera.cpp:78-99: the camera sample s of
+// pixel (x, y) into path slot `slot`
+__device__ __forceinline__ void raygen_path(const dcamera& cam, const dparams& prm, const dpaths& P, uint32_t slot,
+                                            int32_t x, int32_t y, uint32_t s) {
+    const uint32_t pixel = (uint32_t)y * cam.width + (uint32_t)x;
+    const uint64_t sid = (uint64_t)pixel * prm.spp + s;
+    sampler_state smp;
+    if (prm.sampler == RBW_SAMPLER_REPLAY) {
+        smp.replay = prm.replay_values;
+        smp.state = prm.replay_offsets[sid];
+        smp.end = prm.replay_offsets[sid + 1];
+        smp.inc = 0;
+    } else {
+        pcg32 g;
+        g.seed(sample_stream(prm.seed, sid));
+        smp.state = g.state;
+        smp.inc = g.inc;
+        smp.replay = nullptr;
+        smp.end = 0;
+    }
+    const f2 j = smp.next2();
+    const float px = (float)x + j.x, py = (float)y + j.y;
+    const f2 lu = smp.next2();
+    // perspective_camera::sample(position, sample)
+    const f3 target = mat_mul(cam.r2c, f3{px, py, 0.0f}, 1.0f);
+    f3 d = normalize(normalize(target));
+    f3 o = f3{0.0f, 0.0f, 0.0f};
+    if (cam.lens_radius > 0.0f) {
+        const f3 focus = o + d * (cam.focus / d.z);
+        const f2 disk = concentric_sample_disk(lu);
+        o = f3{disk.x * cam.lens_radius, disk.y * cam.lens_radius, 0.0f};
+        d = normalize(focus - o);
+    }
+    // transform::operator()(ray) with CameraToWorld (transform.cpp:94-105)
+    const f3 dv = mat_mul(cam.c2w, d, 0.0f);
+    const f3 dw = normalize(normalize(dv));
+    const f3 ow = mat_mul(cam.c2w, o, 1.0f);
+    P.setF3(F_OX, slot, ow);
+    P.setF3(F_DX, slot, dw);
+    P.setF3(F_BX, slot, f3{1.0f, 1.0f, 1.0f});
+    P.setF3(F_VX, slot, f3{0.0f, 0.0f, 0.0f});
+    P.F(F_PX, slot) = px;
+    P.F(F_PY, slot) = py;
+    P.U(U_PIXEL, slot) = pixel;
+    P.U(U_SAMPLE, slot) = s;
+    P.U(U_BOUNCE, slot) = 0;
+    P.U(U_PENDING, slot) = RBW_NONE;
+    P.rng[slot] = smp.state;
+}
+
+// closest hit of the path ray (scene::intersect, scene.cpp:82-95)
+template <bool COUNT>
+__device__ __forceinline__ void extend_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
+                                            trace_counters& cnt) {
+    const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
+    const hitrec h = traverse<false, false, COUNT>(S, o, d, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+    P.U(U_HPRIM, slot) = h.prim;
+    P.F(F_HT, slot) = h.t;
+    if (uint32_t* dbg = debug_slot(P, prm, slot, (int)P.U(U_BOUNCE, slot))) dbg[0] = h.prim;
+}
+
+// sample_surface_interaction (integrator.cpp:329-404) up to the light and BSDF
+// samples of next-event estimation, then path_integrator.cpp:35-43, for the path in
+// `slot`.  GUIDED: vertices with a BSDF are handed back in gsi/guide_mat (to_guide)
+// for the guided vertex stage instead.
+template <bool GUIDED>
+__device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
+                                           bool& alive, bool& need_connect, bool& to_guide, surf& gsi,
+                                           uint32_t& guide_mat) {
+    const uint32_t prim = P.U(U_HPRIM, slot);
+    int bounce = (int)P.U(U_BOUNCE, slot);
+    f3 value = P.F3(F_VX, slot);
+    if (GUIDED) {
+        // the radiance carried back along the previous vertex's sampled direction is
+        // everything the path gathers from here on
+        const uint32_t pend = P.U(U_PENDING, slot);
+        if (pend != RBW_NONE) {
+            const size_t ri = (size_t)slot * prm.max_depth + pend;
+            const size_t rc = (size_t)P.cap * prm.max_depth;
+            prm.g.rec[R_MX * rc + ri] = value.x;
+            prm.g.rec[R_MY * rc + ri] = value.y;
+            prm.g.rec[R_MZ * rc + ri] = value.z;
+            prm.g.rec_flag[ri] = 2;
+            P.U(U_PENDING, slot) = RBW_NONE;
+        }
+    }
+    if (prim != RBW_MISS) {
+        const f3 o_w = P.F3(F_OX, slot), d_w = P.F3(F_DX, slot);
+        uint32_t eid;
+        const surf si = hit_surface(S, prim, o_w, d_w, eid);
+        const dentity E = S.ents[eid];
+        f3 beta = P.F3(F_BX, slot);
+        sampler_state smp = load_sampler(P, prm, slot);
+        bool cont = true;
+        // emission at the first vertex (integrator.cpp:350-351; no specular lobes exist)
+        if (bounce == 0 && (E.flags & RBW_ENTITY_EMITTER)) value = value + beta * light_evaluate(E, si.normal, -d_w);
+        if (GUIDED && bounce > 0 && (E.flags & RBW_ENTITY_EMITTER)) {
+            // the guided continuation is the second technique of next-event estimation
+            const f3 Le = light_evaluate(E, si.normal, -d_w);
+            if (!is_black(Le)) {
+                const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
+                const float lp = light_pdf(S, E, mp, mn, mw) / (float)S.num_emitters;
+                value = value + beta * Le * power_heuristic(P.F(F_MPDF, slot), lp);
+            }
+        }
+        if (E.material == RBW_NONE) {
+            // invisible entity: straight through, the bounce does not count (integrator.cpp:355-363)
+            f3 no, nd;
+            spawn_ray(si.point, si.normal, d_w, no, nd);
+            P.setF3(F_OX, slot, no);
+            P.setF3(F_DX, slot, nd);
+            bounce -= 1;
+        } else {
+            const bsdf fn = make_bsdf(S, E.material, prim);
+            uint32_t cflags = 0;
+            if (!GUIDED && fn.kind != LOBE_NONE) {
+                // uniform_sample_one_emitter (integrator.cpp:101-161): light sample
+                const uint32_t ne = S.num_emitters;
+                const float u1 = smp.next();
+                uint32_t which = (uint32_t)floorf(u1 * (float)ne);
+                if (which > ne - 1) which = ne - 1;
+                const float pick = 1.0f / (float)ne;
+                const uint32_t lid = S.emitters[which];
+                const dentity L = S.ents[lid];
+                light_sample ls = sample_light(S, L, si.point, si.normal, smp.next2());
+                ls.pdf = ls.pdf * pick;
+                if (!is_black(ls.L) && ls.pdf > 0.0f) {
+                    const f3 wi = world_to_local(si.sh, ls.wi);
+                    const f3 wo = world_to_local(si.sh, si.wo);
+                    f3 f = fn.evaluate(wo, wi);
+                    const float fp = fn.pdf(wo, wi);
+                    f = f * fabsf(dot(ls.wi, si.sh.z));
+                    if (!is_black(f) && fp > 0.0f) {
+                        f3 so, sd;
+                        float slen;
+                        spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
+                        const float w = power_heuristic(ls.pdf, fp);
+                        const f3 contrib = f * ls.L * w / ls.pdf;
+                        P.setF3(F_SOX, slot, so);
+                        P.setF3(F_SDX, slot, sd);
+                        P.F(F_SLEN, slot) = slen;
+                        P.setF3(F_SFX, slot, contrib);
+                        P.U(U_SENT, slot) = lid;
+                        cflags |= C_SHADOW;
+                    }
+                }
+                // BSDF sample for MIS (integrator.cpp:163-198)
+                f3 mwi, mf;
+                float mpdf = 0.0f;
+                if (fn.sample(si, smp.next2(), mwi, mf, mpdf)) {
+                    mf = mf * fabsf(dot(mwi, si.sh.z));
+                    if (!is_black(mf) && mpdf > 0.0f) {
+                        P.setF3(F_MPX, slot, si.point);
+                        P.setF3(F_MNX, slot, si.normal);
+                        P.setF3(F_MWX, slot, mwi);
+                        P.setF3(F_MFX, slot, mf);
+                        P.F(F_MPDF, slot) = mpdf;
+                        cflags |= C_MIS;
+                    }
+                }
+                if (cflags) {
+                    P.setF3(F_CBX, slot, beta);
+                    P.U(U_CFLAGS, slot) = cflags | ((uint32_t)bounce << 8);
+                    need_connect = true;
+                }
+            }
+            if (GUIDED && fn.kind != LOBE_NONE) {
+                to_guide = true;
+                gsi = si;
+                guide_mat = (S.tri_material && prim < S.num_tris && S.tri_material[prim] != RBW_NONE)
+                                ? S.tri_material[prim] : E.material;
+            } else if (GUIDED) {
+                cont = false;  // an empty lobe list ends the path (integrator.cpp:380-382)
+            } else {
+                // continuation sample (integrator.cpp:380-397)
+                f3 wi, f;
+                float pdf = 0.0f;
+                const bool ok = fn.sample(si, smp.next2(), wi, f, pdf);
+                if (!ok || is_black(f) || pdf == 0.0f) {
+                    cont = false;
+                } else {
+                    beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
+                    f3 no, nd;
+                    spawn_ray(si.point, si.normal, wi, no, nd);
+                    P.setF3(F_OX, slot, no);
+                    P.setF3(F_DX, slot, nd);
+                }
+            }
+        }
+        if (!to_guide) {
+            if (cont) {
+                // Russian roulette (path_integrator.cpp:35-43)
+                const float mc = spectrum_max(beta * 1.0f);
+                if (mc < prm.rr_threshold && bounce > 3) {
+                    const float qq = gmax(0.05f, 1.0f - mc);
+                    if (smp.next() < qq) cont = false;
+                    else beta = beta / (1.0f - qq);
+                }
+            }
+            bounce += 1;
+            alive = cont && (uint32_t)bounce < prm.max_depth;
+            P.U(U_BOUNCE, slot) = (uint32_t)bounce;
+        }
+        P.setF3(F_BX, slot, beta);
+        P.setF3(F_VX, slot, value);
+        P.rng[slot] = smp.state;
+    }
+}
+
+// shadow ray, MIS ray and light pdf of next-event estimation for the path in `slot`;
+// value += beta * L (integrator.cpp:101-201)
+template <bool COUNT>
+__device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
+                                             trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
+                                             uint32_t& nlight) {
+    const uint32_t flags = P.U(U_CFLAGS, slot);
+    const int bounce = (int)(flags >> 8);
+    uint32_t* dbg = debug_slot(P, prm, slot, bounce);
+    f3 L = f3{0.0f, 0.0f, 0.0f};
+    if (flags & C_SHADOW) {
+        nshadow = 1;
+        const uint32_t lid = P.U(U_SENT, slot);
+        const f3 so = P.F3(F_SOX, slot), sd = P.F3(F_SDX, slot);
+        const float slen = P.F(F_SLEN, slot);
+        hitrec h;
+        if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
+            h = traverse<true, true, COUNT>(S, so, sd, slen, prm.tri_test, lid, cnt);
+        else
+            h = traverse<true, false, COUNT>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
+        if (dbg) dbg[1] = h.prim;
+        bool unoccluded = h.prim == RBW_MISS;
+        if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT) {
+            const uint32_t he = h.prim < S.num_tris ? S.tri_entity[h.prim] : S.sphere_entity[h.prim - S.num_tris];
+            unoccluded = he == lid;
+        }
+        if (unoccluded) L = L + P.F3(F_SFX, slot);
+    }
+    if (flags & C_MIS) {
+        nmis = 1;
+        // find_emitter (integrator.cpp:49-82)
+        const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
+        f3 eo, ed;
+        spawn_ray(mp, mn, mw, eo, ed);
+        const hitrec h = traverse<true, false, COUNT>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+        if (dbg) dbg[2] = h.prim;
+        if (h.prim != RBW_MISS) {
+            uint32_t he;
+            const surf hs = hit_surface(S, h.prim, eo, ed, he);
+            const dentity E = S.ents[he];
+            if (E.flags & RBW_ENTITY_EMITTER) {
+                nlight = 1;
+                const float pick = 1.0f / (float)S.num_emitters;
+                const f3 Le = light_evaluate(E, hs.normal, -mw);
+                const float lp = light_pdf(S, E, mp, mn, mw) * pick;
+                if (!is_black(Le) && lp > 0.0f) {
+                    const float mpdf = P.F(F_MPDF, slot);
+                    const float w = power_heuristic(mpdf, lp);
+                    L = L + P.F3(F_MFX, slot) * Le * w / mpdf;
+                }
+            }
+        }
+    }
+    const f3 beta = P.F3(F_CBX, slot);
+    P.setF3(F_VX, slot, P.F3(F_VX, slot) + beta * L);
+}
+
+// ------------------------------------------------------------------ film
+
+// Film sums are 64-bit fixed point (2^-32 radiance units): integer sums do not
+// depend on the order in which samples arrive, so a render is reproducible bit
+// for bit whether its samples are accumulated in sample order (single render)
+// or as paths retire (batched jobs).  |value| is clamped to 2^30; a non-finite
+// sample marks its channel, which then resolves to NaN (the reference's film
+// sum turns NaN there, film.cpp:138).
+struct dfilm {
+    unsigned long long* sum;  // 3 per pixel
+    uint32_t* weight;         // samples per pixel (box filter weight 1)
+    uint32_t* nan;            // per pixel: channels that saw a non-finite sample
+};
+
+__device__ __forceinline__ long long film_fix(float v) {
+    const double x = fmin(fmax((double)v, -1073741824.0), 1073741824.0) * 4294967296.0;
+    return __double2ll_rn(x);
+}
+
+// film_tile::add_sample footprint (film.cpp:57-92, box filter radius 0.5) clamped to the pixel bounds
+__device__ __forceinline__ void footprint(const dcamera& cam, float px, float py, int32_t& x0, int32_t& y0,
+                                          int32_t& x1, int32_t& y1) {
+    const float dx = px - 0.5f, dy = py - 0.5f;
+    x0 = (int32_t)ceilf(dx - 0.5f);
+    y0 = (int32_t)ceilf(dy - 0.5f);
+    x1 = (int32_t)floorf(dx + 0.5f) + 1;
+    y1 = (int32_t)floorf(dy + 0.5f) + 1;
+    x0 = x0 > cam.bx0 ? x0 : cam.bx0;
+    y0 = y0 > cam.by0 ? y0 : cam.by0;
+    x1 = x1 < cam.bx1 ? x1 : cam.bx1;
+    y1 = y1 < cam.by1 ? y1 : cam.by1;
+}
+
+// adds a sample to every pixel of [x0,x1) x [y0,y1) with atomics
+__device__ __forceinline__ void film_add_span(const dcamera& cam, const dfilm& film, int32_t x0, int32_t y0, int32_t x1,
+                                              int32_t y1, f3 v) {
+    const float c[3] = {v.x, v.y, v.z};
+    long long q[3];
+    uint32_t bad = 0;
+    for (int ch = 0; ch < 3; ch++) {
+        q[ch] = isfinite(c[ch]) ? film_fix(c[ch]) : 0;
+        if (!isfinite(c[ch])) bad |= 1u << ch;
+    }
+    for (int32_t yy = y0; yy < y1; yy++)
+        for (int32_t xx = x0; xx < x1; xx++) {
+            const size_t p = (size_t)yy * cam.width + (size_t)xx;
+            for (int ch = 0; ch < 3; ch++)
+                if (q[ch]) atomicAdd(&film.sum[3 * p + ch], (unsigned long long)q[ch]);
+            atomicAdd(&film.weight[p], 1u);
+            if (bad) atomicOr(&film.nan[p], bad);
+        }
+}
+
+// pixel::spectrum (film.cpp:20-25): sum / weight; pixels outside the bounds are 0
+__device__ __forceinline__ void film_resolve_pixel(const dcamera& cam, const dfilm& film, uint32_t p, float* rgb) {
+    const int32_t x = (int32_t)(p % cam.width), y = (int32_t)(p / cam.width);
+    const uint32_t w = film.weight[p];
+    const bool in = x >= cam.bx0 && x < cam.bx1 && y >= cam.by0 && y < cam.by1 && w != 0;
+    const uint32_t bad = in ? film.nan[p] : 0u;
+    for (int ch = 0; ch < 3; ch++) {
+        float v = 0.0f;
+        if (in) v = (float)((double)(long long)film.sum[3 * p + ch] * 2.3283064365386963e-10 / (double)w);
+        if (bad & (1u << ch)) v = __builtin_nanf("");
+        rgb[3 * p + ch] = v;
+    }
+}
+
+// host: the device camera record of a camera descriptor
+inline dcamera make_dcamera(const rbw_camera_desc* c) {
+    dcamera d;
+    std::memcpy(d.r2c, c->raster_to_camera, sizeof(d.r2c));
+    std::memcpy(d.c2w, c->camera_to_world, sizeof(d.c2w));
+    d.width = c->width;
+    d.height = c->height;
+    d.bx0 = c->bound_min[0];
+    d.by0 = c->bound_min[1];
+    d.bx1 = c->bound_max[0];
+    d.by1 = c->bound_max[1];
+    d.lens_radius = c->lens_radius;
+    d.focus = c->focus_distance;
+    return d;
+}
+
+}  // namespace rbw

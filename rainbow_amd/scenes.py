@@ -402,6 +402,68 @@ def displaced_icosphere_box(level=8, width=1024, height=1024, seed=1):
     return sc, cam
 
 
+def fbm_displaced_grid_sphere(ntri, seed=1, center=(0.05, -0.35, -0.3), radius=0.42, amplitude=0.1, octaves=6):
+    """latitude-longitude grid sphere of about `ntri` triangles (pole fans plus quad rings),
+    vertices displaced along the normal by seeded fBm; smooth vertex normals."""
+    nr = max(2, int(round((ntri / 4.0) ** 0.5)) + 1)       # rings of latitude (including poles)
+    ns = 2 * (nr - 1)                                       # segments of longitude
+    th = np.linspace(0.0, math.pi, nr + 1)[1:-1]            # interior rings
+    ph = np.linspace(0.0, 2 * math.pi, ns, endpoint=False)
+    ring = np.stack([np.outer(np.sin(th), np.cos(ph)), np.repeat(np.cos(th)[:, None], ns, 1),
+                     np.outer(np.sin(th), np.sin(ph))], -1).reshape(-1, 3)
+    V = np.concatenate([[(0.0, 1.0, 0.0)], ring, [(0.0, -1.0, 0.0)]])
+    south = len(V) - 1
+    F = []
+    j = np.arange(ns)
+    jn = (j + 1) % ns
+    F.append(np.stack([np.zeros(ns, np.int64), 1 + jn, 1 + j], 1))
+    for r in range(nr - 2):
+        a, b = 1 + r * ns, 1 + (r + 1) * ns
+        F.append(np.stack([a + j, a + jn, b + jn], 1))
+        F.append(np.stack([a + j, b + jn, b + j], 1))
+    last = 1 + (nr - 2) * ns
+    F.append(np.stack([last + j, last + jn, np.full(ns, south)], 1))
+    F = np.concatenate(F).astype(np.int64)
+    table = _pcg32_stream(seed, 256)
+    n = np.zeros(len(V))
+    freq, amp = 3.0, 1.0
+    for _ in range(octaves):
+        n += amp * (_value_noise3(V * freq + 17.0, table) - 0.5)
+        freq *= 2.0
+        amp *= 0.5
+    P = V * (radius * (1.0 + amplitude * n))[:, None] + np.asarray(center)
+    P32 = P.astype(np.float32)
+    Pd = P32.astype(np.float64)
+    fn = np.cross(Pd[F[:, 1]] - Pd[F[:, 0]], Pd[F[:, 2]] - Pd[F[:, 0]])
+    N = np.zeros_like(Pd)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    N /= np.maximum(np.linalg.norm(N, axis=1, keepdims=True), 1e-30)
+    return P32, F.astype(np.uint32), N.astype(np.float32)
+
+
+def batch_workload(n_jobs=64, seed=2, tris=(1e4, 2e6), res=(128, 1024), spp=(4, 256), depth=(3, 12)):
+    """Jobs of uneven size: per job a seeded fBm-displaced grid sphere with per-triangle
+    Lambert/GGX materials inside the Cornell box; triangle count, square resolution and
+    spp log-uniform in their ranges, max depth uniform in [depth[0], depth[1]].
+    Returns a list of dicts {scene, camera, spp, max_depth, seed, triangles}."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        t = int(round(math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1])))))
+        w = int(round(math.exp(rng.uniform(math.log(res[0]), math.log(res[1])))))
+        s = int(round(math.exp(rng.uniform(math.log(spp[0]), math.log(spp[1])))))
+        d = int(rng.integers(depth[0], depth[1] + 1))
+        sc = Scene()
+        add_cornell_box(sc, spheres=False)
+        P, F, N = fbm_displaced_grid_sphere(t, seed=1000 + i)
+        tm = per_triangle_materials(sc, len(F), 1000 + i)
+        sc.add_mesh(P, F, tm[0], normals=N, tri_material=tm)
+        cam = perspective_camera(w, w, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
+        jobs.append({"scene": sc, "camera": cam, "spp": s, "max_depth": d, "seed": 7 + i, "triangles": len(F) + 10})
+    return jobs
+
+
 def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0, seed=1, mesh=True,
                          light_dist=0.5, light_size=0.5, light_tilt=0.0):
     """A closed room lit only through a gap x gap hole in its +x wall by a quad light outside

@@ -122,6 +122,65 @@ class Tracer:
         return float(loss[0]), grad
 
 
+    # -- batched jobs (no counterpart in the reference) ----------------------
+    def add_scene(self, scene):
+        """adds a scene to the context's pool; returns its id"""
+        desc = scene.desc()
+        sid = C.c_uint32()
+        _check(self.lib.rbw_add_scene(self.h, C.byref(desc), C.byref(sid)))
+        return sid.value
+
+    def remove_scene(self, sid):
+        _check(self.lib.rbw_remove_scene(self.h, sid))
+
+    def submit_jobs(self, jobs, opts=None, images=True):
+        """jobs: list of dicts {scene, camera, spp, max_depth, rr_threshold=1, seed=0}.
+        Returns (images or None, per-job stats, batch stats)."""
+        n = len(jobs)
+        arr = (abi.rbw_job * n)()
+        outs = []
+        for i, j in enumerate(jobs):
+            cam = j["camera"]
+            arr[i].scene = j["scene"]
+            arr[i].spp = j["spp"]
+            arr[i].max_depth = j["max_depth"]
+            arr[i].rr_threshold = j.get("rr_threshold", 1.0)
+            arr[i].seed = j.get("seed", 0)
+            arr[i].camera = cam
+            if images:
+                img = np.zeros((cam.height, cam.width, 3), np.float32)
+                outs.append(img)
+                arr[i].out_rgb = abi.ptr(img, abi.f32p)
+        js = (abi.rbw_job_stats * n)()
+        bs = abi.rbw_batch_stats()
+        o = opts if opts is not None else batch_opts()
+        _check(self.lib.rbw_submit_jobs(self.h, arr, n, C.byref(o), js, C.byref(bs)))
+        return (outs if images else None), [js[i].as_dict() for i in range(n)], bs.as_dict()
+
+    def autotune_jobs(self, jobs, opts=None):
+        n = len(jobs)
+        arr = (abi.rbw_job * n)()
+        for i, j in enumerate(jobs):
+            arr[i].scene = j["scene"]
+            arr[i].spp = j["spp"]
+            arr[i].max_depth = j["max_depth"]
+            arr[i].rr_threshold = j.get("rr_threshold", 1.0)
+            arr[i].seed = j.get("seed", 0)
+            arr[i].camera = j["camera"]
+        o = opts if opts is not None else batch_opts()
+        ms = C.c_double()
+        _check(self.lib.rbw_autotune_jobs(self.h, arr, n, C.byref(o), C.byref(ms)))
+        return o, ms.value
+
+
+def batch_opts(block_size=0, pool_size=0, use_graph=True, autotune=False, **kw):
+    o = abi.rbw_batch_opts(block_size=block_size, pool_size=pool_size, bvh_width=0, use_graph=int(use_graph),
+                           autotune=int(autotune))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
 def guiding_opts(mis_fraction=0.5, train=True, learning_rate=1e-2, batch=1 << 18, steps_per_pass=4):
     return abi.rbw_guiding_opts(enable=1, mis_fraction=mis_fraction, train=int(train), learning_rate=learning_rate,
                                 batch=batch, steps_per_pass=steps_per_pass)
