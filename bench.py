@@ -121,6 +121,55 @@ def guided_leg(full, threshold=1e-3):
     }
 
 
+def batch_leg(n_jobs=64, autotune=True):
+    """The batched-jobs workload: n_jobs seeded jobs of uneven size (grid-sphere meshes of
+    1e4-2e6 triangles log-uniform in the Cornell box, 128^2-1024^2, 4-256 spp, depth 3-12)
+    submitted as one batch.  Reports aggregate and per-job Msamples/s and, for comparison,
+    the same jobs rendered one rbw_render call at a time."""
+    from rainbow_amd import scenes
+    from rainbow_amd.tracer import Tracer, batch_opts, render_params
+    t0 = time.perf_counter()
+    jobs = scenes.batch_workload(n_jobs=n_jobs, seed=2)
+    tr = Tracer(0)
+    ids = [tr.add_scene(j["scene"]) for j in jobs]
+    build_s = time.perf_counter() - t0
+    print(f"# batch leg: {n_jobs} jobs generated and BVH-built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    js = [{"scene": ids[i], "camera": j["camera"], "spp": j["spp"], "max_depth": j["max_depth"], "seed": j["seed"]}
+          for i, j in enumerate(jobs)]
+    opts = batch_opts(autotune=autotune)
+    tuned_ms = 0.0
+    if autotune:
+        opts, tuned_ms = tr.autotune_jobs(js, opts)
+        opts.autotune = 0
+    tr.submit_jobs(js, opts, images=False)  # warm: merged scene upload, pool allocation
+    _, jst, bst = tr.submit_jobs(js, opts, images=False)
+    print(f"# batch: {bst['msamples_per_s']:.1f} Msamples/s in {bst['render_ms']:.0f} ms, "
+          f"{bst['iterations']} iterations", file=sys.stderr, flush=True)
+    # the same jobs one render call at a time (scene upload excluded)
+    single_ms = 0.0
+    for j in jobs:
+        tr.upload(j["scene"])
+        st = tr.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"]),
+                       stats=True)["stats"]
+        single_ms += st["render_ms"]
+    samples = bst["samples"]
+    per_job = [round(x["msamples_per_s"], 1) for x in jst]
+    return {
+        "workload": f"{n_jobs} jobs, seed 2: grid-sphere mesh 1e4-2e6 tris (log-uniform) in the Cornell box, "
+                    "128^2-1024^2, 4-256 spp, depth 3-12, per-triangle Lambert/GGX",
+        "jobs": n_jobs, "samples": samples, "triangles": sum(j["triangles"] for j in jobs),
+        "batch_ms": round(bst["render_ms"], 1), "aggregate_msamples_per_s": round(bst["msamples_per_s"], 2),
+        "mrays_per_s": round(bst["mrays_per_s"], 1),
+        "one_at_a_time_ms": round(single_ms, 1),
+        "one_at_a_time_msamples_per_s": round(samples / single_ms * 1e-3, 2),
+        "per_job_msamples_per_s": {"min": min(per_job), "median": sorted(per_job)[len(per_job) // 2],
+                                   "max": max(per_job)},
+        "iterations": bst["iterations"], "graph_launches": bst["graph_launches"],
+        "block_size": bst["block_size"], "pool_size": bst["pool_size"], "bvh_width": bst["bvh_width"],
+        "autotune_ms": round(tuned_ms, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +178,8 @@ def main():
     ap.add_argument("--workload", default="icosphere", choices=["icosphere", "cornell"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
+    ap.add_argument("--batch", action=argparse.BooleanOptionalAction, default=True,
+                    help="batched-jobs leg (64 jobs of uneven size in one submission)")
     ap.add_argument("--guided", default="reduced", choices=["reduced", "full", "off"],
                     help="guided-vs-plain time-to-relMSE leg: 256^2 with a 16384-spp judge (reduced), "
                          "512^2 with a 65536-spp judge (full), or skipped")
@@ -214,6 +265,8 @@ def main():
         "mrays_per_s": round(rays / wall / 1e6, 2),
         "roofline": roof,
     }
+    if args.batch:
+        out["batch"] = batch_leg()
     if args.guided != "off":
         out["guided"] = guided_leg(args.guided == "full")
     if not args.no_cpu:

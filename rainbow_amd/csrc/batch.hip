@@ -14,7 +14,8 @@
 //   retire   finished paths into their job's film, slots back to the free stack
 // Two iterations (the live queue alternates) are captured as one hipGraph and
 // replayed until the job queue is empty and every slot is free.  Grids are
-// fixed at capture; each kernel reads its queue length from device memory.
+// fixed at capture (one thread per pool slot); each kernel reads its queue
+// length from device memory and blocks past it leave at once.
 //
 // A job's image is bit-identical to rbw_render's: the same per-sample pcg32
 // stream, the same BVH (built by the same builder from the same scene), and
@@ -35,6 +36,10 @@ using namespace rbw::host;
 
 namespace {
 
+// largest block size of the wavefront kernels (the autotuner's candidates are 64..512);
+// bounding it lets the register allocator give each lane more than 128 VGPRs
+constexpr int k_batch_max_block = 512;
+
 struct djob {
     dcamera cam;
     dfilm film;
@@ -47,29 +52,33 @@ struct djob {
 };
 
 struct dctl {
-    unsigned long long cursor;    // next sample of the job queue
-    unsigned long long total;     // samples of all jobs
-    unsigned long long gen_base;  // first sample regenerated this iteration
-    uint32_t free_top;            // free-slot stack height
-    uint32_t n_new;               // slots regenerated this iteration
-    uint32_t slot_base;           // free_stack[slot_base, slot_base + n_new) are taken
+    unsigned long long cursor;  // next sample of the job queue
+    unsigned long long total;   // samples of all jobs
+    unsigned long long live;    // slots holding a path after this iteration's regeneration
     uint32_t iteration;
-    uint32_t count[4];            // live queue A, live queue B, connect queue, retire queue
+    uint32_t pad;
 };
 
 struct dbatch {
     dctl* ctl;
     const djob* jobs;
     uint32_t njobs;
-    uint32_t* free_stack;
-    uint32_t* q[2];
-    uint32_t* cq;
-    uint32_t* retire;
-    unsigned long long* retired;   // per job
-    uint32_t* first_iter;          // per job
-    uint32_t* last_iter;           // per job
-    unsigned long long* counters;  // extension, shadow, MIS, light casts
+    uint32_t pool;
+    uint32_t nblocks;               // blocks of the per-slot kernels
+    uint32_t* block_free;           // per block: slots free at the next regeneration
+    uint32_t* block_n;              // per block: camera samples granted this iteration
+    unsigned long long* block_base; // per block: first granted sample
+    uint32_t* wave_job;             // per wave of slots: a job that retired a path in the last regeneration
+    uint32_t nwaves;
+    uint32_t* first_iter;           // per job
+    uint32_t* last_iter;            // per job
+    unsigned long long* counters;   // extension, shadow, MIS, light casts
 };
+
+// slot state (U_STATE): 0 = free
+constexpr uint32_t ST_ALIVE = 1u;    // trace an extension ray this iteration
+constexpr uint32_t ST_CONNECT = 2u;  // next-event estimation rays pending
+constexpr uint32_t ST_DYING = 4u;    // finished: into the film at the next regeneration
 
 __device__ __forceinline__ uint32_t find_job(const djob* jobs, uint32_t n, unsigned long long g) {
     uint32_t lo = 0, hi = n;  // jobs[lo].base <= g < jobs[hi].base
@@ -94,188 +103,213 @@ __device__ __forceinline__ void job_view(const dscene& S, const dparams& base, c
     pj.seed = J.seed;
 }
 
-__global__ void k_b_init(dbatch B, uint32_t pool, unsigned long long total) {
+// number of lanes of the block with `pred`, and the rank of this lane among them
+// (slot order); every thread of the block must call it
+__device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t& total, uint32_t* lds) {
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (uint32_t w = 0; w < nwaves; w++) {
+        const uint32_t c = lds[w];
+        off += w < wave ? c : 0u;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return off + (uint32_t)__popcll(m & below);
+}
+
+__global__ void k_b_init(dbatch B, dpaths P, unsigned long long total, uint32_t bs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < pool) B.free_stack[i] = i;
+    if (i < B.pool) P.U(U_STATE, i) = 0u;
+    if (i < B.nblocks) {
+        const uint32_t first = i * bs;
+        B.block_free[i] = B.pool - first < bs ? B.pool - first : bs;
+    }
+    if (i < B.nwaves) B.wave_job[i] = RBW_NONE;
     if (i < B.njobs) {
-        B.retired[i] = 0;
         B.first_iter[i] = 0xFFFFFFFFu;
-        B.last_iter[i] = 0xFFFFFFFFu;
+        B.last_iter[i] = 0u;
     }
     if (i < 4) B.counters[i] = 0;
     if (i == 0) {
         dctl c;
         c.cursor = 0;
         c.total = total;
-        c.gen_base = 0;
-        c.free_top = pool;
-        c.n_new = 0;
-        c.slot_base = 0;
+        c.live = 0;
         c.iteration = 0;
-        c.count[0] = c.count[1] = c.count[2] = c.count[3] = 0;
+        c.pad = 0;
         *B.ctl = c;
     }
 }
 
-__global__ void k_b_plan(dbatch B) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    dctl& c = *B.ctl;
-    const unsigned long long left = c.total - c.cursor;
-    const uint32_t n = left < (unsigned long long)c.free_top ? (uint32_t)left : c.free_top;
-    c.n_new = n;
-    c.slot_base = c.free_top - n;
-    c.gen_base = c.cursor;
-    c.cursor += n;
-    c.free_top -= n;
-    c.iteration += 1;
-}
-
-__global__ void k_b_regen(dscene S, dpaths P, dparams base, dbatch B, int cur) {
-    const uint32_t n = B.ctl->n_new;
-    const uint32_t sbase = B.ctl->slot_base;
-    const unsigned long long g0 = B.ctl->gen_base;
-    const uint32_t iter = B.ctl->iteration;
-    // grid-stride over [0, n) in whole blocks, so every lane of a wave takes the same trips
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        const bool act = i < n;
-        uint32_t slot = 0;
-        if (act) {
-            slot = B.free_stack[sbase + i];
-            const unsigned long long g = g0 + i;
-            const uint32_t j = find_job(B.jobs, B.njobs, g);
-            const djob& J = B.jobs[j];
-            const unsigned long long l = g - J.base;
-            const uint32_t s = (uint32_t)(l / J.npix), li = (uint32_t)(l % J.npix);
-            dscene Sj;
-            dparams pj;
-            job_view(S, base, J, Sj, pj);
-            raygen_path(J.cam, pj, P, slot, J.cam.bx0 + (int32_t)(li % J.bw), J.cam.by0 + (int32_t)(li / J.bw), s);
-            P.U(U_JOB, slot) = j;
-            if (l == 0) B.first_iter[j] = iter;
+// Folds the last regeneration's per-wave retire records into the jobs' last
+// iterations (plain stores: every record of one regeneration carries the same
+// iteration), with the threads of one block.
+__device__ __forceinline__ void fold_retire_records(const dbatch& B, uint32_t iter) {
+    for (uint32_t w = threadIdx.x; w < B.nwaves; w += blockDim.x) {
+        const uint32_t j = B.wave_job[w];
+        if (j != RBW_NONE) {
+            B.last_iter[j] = iter;
+            B.wave_job[w] = RBW_NONE;
         }
-        const uint32_t k = wave_append(&B.ctl->count[cur], act);
-        if (act) B.q[cur][k] = slot;
     }
 }
 
-__global__ void k_b_extend(dscene S, dpaths P, dparams base, dbatch B, int cur) {
-    const uint32_t n = B.ctl->count[cur];
+__global__ __launch_bounds__(1024) void k_b_fold(dbatch B) { fold_retire_records(B, B.ctl->iteration); }
+
+// Hands out the next camera samples of the job queue: block b's free slots get
+// the range [block_base[b], block_base[b] + block_n[b]), blocks in order.  One
+// block of 1024 threads scans the per-block free counts.
+__global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x, n = B.nblocks;
+    fold_retire_records(B, B.ctl->iteration);
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += B.block_free[b];
+    part[t] = sum;
+    __syncthreads();
+    // inclusive scan of the 1024 partial sums (Hillis-Steele)
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const uint32_t total_free = part[1023];
+    const unsigned long long cursor = B.ctl->cursor, total = B.ctl->total;
+    const unsigned long long left = total > cursor ? total - cursor : 0ull;
+    uint32_t prefix = part[t] - sum;
+    for (uint32_t b = b0; b < b1; b++) {
+        const uint32_t f = B.block_free[b];
+        const unsigned long long avail = left > prefix ? left - prefix : 0ull;
+        B.block_base[b] = cursor + prefix;
+        B.block_n[b] = avail < f ? (uint32_t)avail : f;
+        prefix += f;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const unsigned long long granted = left < total_free ? left : total_free;
+        B.ctl->cursor = cursor + granted;
+        B.ctl->live = (unsigned long long)B.pool - (total_free - granted);
+        B.ctl->iteration += 1;
+    }
+}
+
+// Retires finished paths into their job's film, then regenerates the free slots
+// in place with the block's granted samples, in slot order, so consecutive slots
+// get consecutive samples (neighbouring pixels of one job).
+__global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths P, dparams base, dbatch B) {
+    __shared__ uint32_t lds[k_batch_max_block / 64];
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = slot < B.pool;
+    const uint32_t iter = B.ctl->iteration;
+    uint32_t st = in ? P.U(U_STATE, slot) : ST_ALIVE;
+    const bool dying = (st & ST_DYING) != 0;
+    uint32_t j = 0xFFFFFFFFu;
+    if (dying) {
+        j = P.U(U_JOB, slot);
+        const djob& J = B.jobs[j];
+        const uint32_t pixel = P.U(U_PIXEL, slot);
+        film_add_sample(J.cam, J.film, (int32_t)(pixel % J.cam.width), (int32_t)(pixel / J.cam.width),
+                        P.F(F_PX, slot), P.F(F_PY, slot), P.F3(F_VX, slot), false);
+        st = 0u;
+    }
+    // the iteration in which a job's last path retired: the wave records its first
+    // retiring job (folded by the next plan); lanes of another job, met only where
+    // the job queue moves on, use an atomic
+    {
+        const int leader = __ffsll((unsigned long long)__ballot(dying)) - 1;
+        const uint32_t jl = __shfl(j, leader < 0 ? 0 : leader);
+        if (leader >= 0 && (int)__lane_id() == leader) B.wave_job[slot >> 6] = jl;
+        if (dying && j != jl) atomicMax(&B.last_iter[j], iter);
+    }
+    uint32_t nfree = 0;
+    const bool free_slot = in && st == 0u;
+    const uint32_t r = block_rank(free_slot, nfree, lds);
+    const bool gen = free_slot && r < B.block_n[blockIdx.x];
+    if (gen) {
+        const unsigned long long g = B.block_base[blockIdx.x] + r;
+        const uint32_t jn = find_job(B.jobs, B.njobs, g);
+        const djob& J = B.jobs[jn];
+        const unsigned long long l = g - J.base;
+        const uint32_t s = (uint32_t)(l / J.npix), li = (uint32_t)(l % J.npix);
+        dscene Sj;
+        dparams pj;
+        job_view(S, base, J, Sj, pj);
+        raygen_path(J.cam, pj, P, slot, J.cam.bx0 + (int32_t)(li % J.bw), J.cam.by0 + (int32_t)(li / J.bw), s);
+        P.U(U_JOB, slot) = jn;
+        if (l == 0) B.first_iter[jn] = iter;
+        st = ST_ALIVE;
+    }
+    if (in && (gen || dying)) P.U(U_STATE, slot) = st;
+}
+
+__global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths P, dparams base, dbatch B) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
     uint32_t rays = 0;
-    // grid-stride over [0, n) in whole blocks, so every lane of a wave takes the same trips
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        const bool act = i < n;
-        if (act) {
-            const uint32_t slot = B.q[cur][i];
-            const djob& J = B.jobs[P.U(U_JOB, slot)];
-            dscene Sj;
-            dparams pj;
-            job_view(S, base, J, Sj, pj);
-            extend_path<false>(Sj, P, pj, slot, cnt);
-            rays++;
-        }
+    if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
+        dscene Sj;
+        dparams pj;
+        job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
+        extend_path<false>(Sj, P, pj, slot, cnt);
+        rays = 1;
     }
     wave_add64(B.counters + 0, rays);
 }
 
-__global__ void k_b_shade(dscene S, dpaths P, dparams base, dbatch B, int cur) {
-    const uint32_t n = B.ctl->count[cur];
-    const int nxt = cur ^ 1;
-    // grid-stride over [0, n) in whole blocks, so every lane of a wave takes the same trips
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        const bool act = i < n;
+__global__ __launch_bounds__(k_batch_max_block) void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
-        uint32_t slot = 0, gmat = 0;
+        uint32_t gmat = 0;
         surf gsi;
-        if (act) {
-            slot = B.q[cur][i];
-            const djob& J = B.jobs[P.U(U_JOB, slot)];
-            dscene Sj;
-            dparams pj;
-            job_view(S, base, J, Sj, pj);
-            shade_path<false>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
-        }
-        const uint32_t a = wave_append(&B.ctl->count[nxt], alive);
-        if (alive) B.q[nxt][a] = slot;
-        const uint32_t c = wave_append(&B.ctl->count[2], need_connect);
-        if (need_connect) B.cq[c] = slot;
-        const bool dead = act && !alive;
-        const uint32_t r = wave_append(&B.ctl->count[3], dead);
-        if (dead) B.retire[r] = slot;
+        dscene Sj;
+        dparams pj;
+        job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
+        shade_path<false>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
+        P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
     }
 }
 
-__global__ void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
-    const uint32_t n = B.ctl->count[2];
+// next-event estimation rays; then the block's count of slots the next
+// regeneration will find free (free or finished)
+__global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
+    __shared__ uint32_t lds[k_batch_max_block / 64];
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
     uint32_t nshadow = 0, nmis = 0, nlight = 0;
-    // grid-stride over [0, n) in whole blocks, so every lane of a wave takes the same trips
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        const bool act = i < n;
-        if (act) {
-            const uint32_t slot = B.cq[i];
-            const djob& J = B.jobs[P.U(U_JOB, slot)];
+    uint32_t st = ST_ALIVE;
+    if (slot < B.pool) {
+        st = P.U(U_STATE, slot);
+        if (st & ST_CONNECT) {
             dscene Sj;
             dparams pj;
-            job_view(S, base, J, Sj, pj);
+            job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
             connect_path<false>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
+            st &= ~ST_CONNECT;
+            P.U(U_STATE, slot) = st;
         }
     }
     wave_add64(B.counters + 1, nshadow);
     wave_add64(B.counters + 2, nmis);
     wave_add64(B.counters + 3, nlight);
+    uint32_t nfree = 0;
+    block_rank(slot < B.pool && (st == 0u || (st & ST_DYING)), nfree, lds);
+    if (threadIdx.x == 0) B.block_free[blockIdx.x] = nfree;
 }
 
-__global__ void k_b_retire(dpaths P, dbatch B) {
-    const uint32_t n = B.ctl->count[3];
-    const uint32_t iter = B.ctl->iteration;
-    // grid-stride over [0, n) in whole blocks, so every lane of a wave takes the same trips
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-        const uint32_t i = b0 + threadIdx.x;
-        const bool act = i < n;
-        uint32_t slot = 0, j = 0xFFFFFFFFu;
-        if (act) {
-            slot = B.retire[i];
-            j = P.U(U_JOB, slot);
-            const djob& J = B.jobs[j];
-            int32_t x0, y0, x1, y1;
-            footprint(J.cam, P.F(F_PX, slot), P.F(F_PY, slot), x0, y0, x1, y1);
-            film_add_span(J.cam, J.film, x0, y0, x1, y1, P.F3(F_VX, slot));
-        }
-        // per-job retired counts, one atomic per (wave, job)
-        const int leader = __ffsll((unsigned long long)__ballot(act)) - 1;
-        const uint32_t jl = __shfl(j, leader < 0 ? 0 : leader);
-        const bool same = act && j == jl;
-        const uint32_t nsame = (uint32_t)__popcll(__ballot(same));
-        if (leader >= 0 && (int)__lane_id() == leader) {
-            const unsigned long long before = atomicAdd(&B.retired[jl], (unsigned long long)nsame);
-            if (before + nsame == B.jobs[jl].count) B.last_iter[jl] = iter;
-        }
-        if (act && !same) {
-            const unsigned long long before = atomicAdd(&B.retired[j], 1ull);
-            if (before + 1 == B.jobs[j].count) B.last_iter[j] = iter;
-        }
-        const uint32_t k = wave_append(&B.ctl->free_top, act);
-        if (act) B.free_stack[k] = slot;
-    }
-}
-
-// the consumed live queue and the connect/retire queues start empty next iteration
-__global__ void k_b_reset(dbatch B, int cur) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    B.ctl->count[cur] = 0;
-    B.ctl->count[2] = 0;
-    B.ctl->count[3] = 0;
-}
-
-__global__ void k_b_resolve(dcamera cam, dfilm film, float* rgb) {
+// every pixel inside the bounds has spp own samples; the film's weights count the spills
+__global__ void k_b_resolve(dcamera cam, dfilm film, uint32_t spp, float* rgb) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= cam.width * cam.height) return;
-    film_resolve_pixel(cam, film, p, rgb);
+    film_resolve_pixel(cam, film, p, rgb, spp);
 }
 
 }  // namespace
@@ -287,10 +321,10 @@ struct rbw_batch_state {
     std::vector<uint32_t> roots, em_first, em_count;
     uint32_t merged_tris = 0;
     // path pool
-    dbuf pf, pu, prng, q0, q1, cq, retire, free_stack;
+    dbuf pf, pu, prng;
     uint32_t pool = 0;
     // per batch
-    dbuf ctl, jobs, retired, first_iter, last_iter, counters, films, rgb;
+    dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, films, rgb;
     dctl* host_ctl = nullptr;  // pinned
     // captured bounce loop
     hipGraph_t graph = nullptr;
@@ -349,8 +383,8 @@ batch_config config_of(const rbw_batch_opts* o) {
     c.graph = o->use_graph != 0;
     c.tri_test = o->triangle_test;
     c.shadow = o->shadow_mode;
-    require(c.block >= 64 && c.block <= 1024 && (c.block & 63) == 0, RBW_ERR_INVALID_ARGUMENT,
-            "block_size must be a multiple of 64 in [64, 1024]");
+    require(c.block >= 64 && c.block <= (uint32_t)k_batch_max_block && (c.block & 63) == 0,
+            RBW_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 512]");
     require(c.pool >= 1024 && c.pool <= (1u << 26), RBW_ERR_INVALID_ARGUMENT, "pool_size must be in [2^10, 2^26]");
     require(c.width == 2, RBW_ERR_UNSUPPORTED, "bvh_width: only the binary BVH is built");
     require(c.tri_test == RBW_TRI_MOLLER_TRUMBORE || c.tri_test == RBW_TRI_WATERTIGHT, RBW_ERR_INVALID_ARGUMENT,
@@ -392,11 +426,6 @@ void ensure_pool(rbw_batch_state& b, uint32_t pool) {
     b.pf.ensure(size_t(F_COUNT) * pool * sizeof(float));
     b.pu.ensure(size_t(U_COUNT) * pool * sizeof(uint32_t));
     b.prng.ensure(size_t(pool) * sizeof(uint64_t));
-    b.q0.ensure(size_t(pool) * 4);
-    b.q1.ensure(size_t(pool) * 4);
-    b.cq.ensure(size_t(pool) * 4);
-    b.retire.ensure(size_t(pool) * 4);
-    b.free_stack.ensure(size_t(pool) * 4);
     b.pool = pool;
 }
 
@@ -423,7 +452,13 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     ensure_pool(b, cfg.pool);
     b.ctl.ensure(sizeof(dctl));
     b.jobs.upload(hjobs.data(), hjobs.size(), s);
-    b.retired.ensure(size_t(nj) * 8);
+    const uint32_t bs = cfg.block;
+    const uint32_t nblocks = blocks(cfg.pool, bs);
+    b.block_free.ensure(size_t(nblocks) * 4);
+    b.block_n.ensure(size_t(nblocks) * 4);
+    b.block_base.ensure(size_t(nblocks) * 8);
+    const uint32_t nwaves = blocks(cfg.pool, 64);
+    b.wave_job.ensure(size_t(nwaves) * 4);
     b.first_iter.ensure(size_t(nj) * 4);
     b.last_iter.ensure(size_t(nj) * 4);
     b.counters.ensure(4 * 8);
@@ -433,12 +468,13 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     B.ctl = b.ctl.as<dctl>();
     B.jobs = b.jobs.as<djob>();
     B.njobs = nj;
-    B.free_stack = b.free_stack.as<uint32_t>();
-    B.q[0] = b.q0.as<uint32_t>();
-    B.q[1] = b.q1.as<uint32_t>();
-    B.cq = b.cq.as<uint32_t>();
-    B.retire = b.retire.as<uint32_t>();
-    B.retired = b.retired.as<unsigned long long>();
+    B.pool = cfg.pool;
+    B.nblocks = nblocks;
+    B.block_free = b.block_free.as<uint32_t>();
+    B.block_n = b.block_n.as<uint32_t>();
+    B.block_base = b.block_base.as<unsigned long long>();
+    B.wave_job = b.wave_job.as<uint32_t>();
+    B.nwaves = nwaves;
     B.first_iter = b.first_iter.as<uint32_t>();
     B.last_iter = b.last_iter.as<uint32_t>();
     B.counters = b.counters.as<unsigned long long>();
@@ -453,22 +489,19 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     base.tri_test = cfg.tri_test;
     base.shadow_mode = cfg.shadow;
     const dscene S = b.scene.view();
-    const uint32_t bs = cfg.block;
-    // grid: every slot of the pool, at most 16 blocks of 256 threads per CU's worth
-    const uint32_t grid = std::min(blocks(cfg.pool, bs), std::max(1u, (1u << 20) / bs));
+    // grid: one thread per slot of the pool
+    const uint32_t grid = nblocks;
 
     const uint32_t init_n = std::max(cfg.pool, nj);
-    hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, cfg.pool, total);
+    hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total, bs);
     HIP_CHECK(hipGetLastError());
 
-    auto iteration = [&](int cur) {
-        hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(64), 0, s, B);
-        hipLaunchKernelGGL(k_b_regen, dim3(grid), dim3(bs), 0, s, S, P, base, B, cur);
-        hipLaunchKernelGGL(k_b_extend, dim3(grid), dim3(bs), 0, s, S, P, base, B, cur);
-        hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B, cur);
+    auto iteration = [&]() {
+        hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
+        hipLaunchKernelGGL(k_b_regen, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(k_b_extend, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
         hipLaunchKernelGGL(k_b_connect, dim3(grid), dim3(bs), 0, s, S, P, base, B);
-        hipLaunchKernelGGL(k_b_retire, dim3(grid), dim3(bs), 0, s, P, B);
-        hipLaunchKernelGGL(k_b_reset, dim3(1), dim3(64), 0, s, B, cur);
     };
     if (b.exec) {
         (void)hipGraphExecDestroy(b.exec);
@@ -480,8 +513,8 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     }
     if (cfg.graph) {
         HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        iteration(0);
-        iteration(1);
+        iteration();
+        iteration();
         const hipError_t ce = hipStreamEndCapture(s, &b.graph);
         HIP_CHECK(ce);
         HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
@@ -503,8 +536,8 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     for (;;) {
         if (cfg.graph) HIP_CHECK(hipGraphLaunch(b.exec, s));
         else {
-            iteration(0);
-            iteration(1);
+            iteration();
+            iteration();
             HIP_CHECK(hipGetLastError());
         }
         launches++;
@@ -517,10 +550,12 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
         if (launches % check_every == 0) {
             HIP_CHECK(hipMemcpyAsync(b.host_ctl, B.ctl, sizeof(dctl), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
-            if (b.host_ctl->cursor == total && b.host_ctl->free_top == cfg.pool) break;
+            if (b.host_ctl->cursor >= total && b.host_ctl->live == 0) break;
             require(launches < max_launches, RBW_ERR_DEVICE, "batched bounce loop did not drain");
         }
     }
+    hipLaunchKernelGGL(k_b_fold, dim3(1), dim3(1024), 0, s, B);  // the final regeneration's records
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
     rr.launches = launches;
     rr.iterations = b.host_ctl->iteration;
@@ -656,7 +691,7 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     for (size_t i = 0; i < n; i++) {
         if (!jobs[i].out_rgb) continue;
         const size_t np = size_t(jobs[i].camera.width) * jobs[i].camera.height;
-        hipLaunchKernelGGL(k_b_resolve, dim3(blocks(np, 256)), dim3(256), 0, s, hj[i].cam, hj[i].film,
+        hipLaunchKernelGGL(k_b_resolve, dim3(blocks(np, 256)), dim3(256), 0, s, hj[i].cam, hj[i].film, hj[i].spp,
                            b.rgb.as<float>());
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(jobs[i].out_rgb, b.rgb.p, np * 3 * sizeof(float), hipMemcpyDeviceToHost, s));

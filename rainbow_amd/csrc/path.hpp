@@ -171,7 +171,7 @@ enum fidx : uint32_t {
     F_CBX, F_CBY, F_CBZ,                         // beta at the vertex
     F_COUNT
 };
-enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_COUNT };
+enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_COUNT };
 
 constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
 
@@ -638,10 +638,36 @@ __device__ __forceinline__ void film_add_span(const dcamera& cam, const dfilm& f
         }
 }
 
-// pixel::spectrum (film.cpp:20-25): sum / weight; pixels outside the bounds are 0
-__device__ __forceinline__ void film_resolve_pixel(const dcamera& cam, const dfilm& film, uint32_t p, float* rgb) {
+// One sample of own pixel (ox, oy) into the film with atomics.  Every sample's box
+// footprint holds its own pixel; with count_own = false that pixel's weight is
+// left out (the caller knows it: spp per pixel) and only spills into neighbours
+// are counted.
+__device__ __forceinline__ void film_add_sample(const dcamera& cam, const dfilm& film, int32_t ox, int32_t oy, float px,
+                                                float py, f3 v, bool count_own) {
+    int32_t x0, y0, x1, y1;
+    footprint(cam, px, py, x0, y0, x1, y1);
+    const float c[3] = {v.x, v.y, v.z};
+    long long q[3];
+    uint32_t bad = 0;
+    for (int ch = 0; ch < 3; ch++) {
+        q[ch] = isfinite(c[ch]) ? film_fix(c[ch]) : 0;
+        if (!isfinite(c[ch])) bad |= 1u << ch;
+    }
+    for (int32_t yy = y0; yy < y1; yy++)
+        for (int32_t xx = x0; xx < x1; xx++) {
+            const size_t p = (size_t)yy * cam.width + (size_t)xx;
+            for (int ch = 0; ch < 3; ch++)
+                if (q[ch]) atomicAdd(&film.sum[3 * p + ch], (unsigned long long)q[ch]);
+            if (count_own || xx != ox || yy != oy) atomicAdd(&film.weight[p], 1u);
+            if (bad) atomicOr(&film.nan[p], bad);
+        }
+}
+
+// pixel::spectrum (film.cpp:20-25): sum / (base_weight + weight); pixels outside the bounds are 0
+__device__ __forceinline__ void film_resolve_pixel(const dcamera& cam, const dfilm& film, uint32_t p, float* rgb,
+                                                   uint32_t base_weight = 0) {
     const int32_t x = (int32_t)(p % cam.width), y = (int32_t)(p / cam.width);
-    const uint32_t w = film.weight[p];
+    const uint32_t w = base_weight + film.weight[p];
     const bool in = x >= cam.bx0 && x < cam.bx1 && y >= cam.by0 && y < cam.by1 && w != 0;
     const uint32_t bad = in ? film.nan[p] : 0u;
     for (int ch = 0; ch < 3; ch++) {
