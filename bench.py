@@ -234,8 +234,12 @@ def main():
 
     roof = None
     if counted:
+        # per-kernel HIP-event times: one more render of the same workload with the bounce
+        # loop launched kernel by kernel (the timed steps replay it as a hipGraph)
+        tp = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, time_kernels=1)
+        timed = tr.render(cam, tp)["stats"]
         kb = roofline.kernel_bytes(counted)
-        last = step_stats[-1]
+        last = timed
         kms = {k: last["kernel_ms"][k] for k in ("extend", "shade", "connect")}
         dom = max(kms, key=kms.get)
         launches = last["kernel_launches"][dom]

@@ -167,7 +167,13 @@ typedef struct rbw_render_params {
     uint32_t* debug_hits;            /* NULL or width*height*spp*max_depth*3 entries */
     uint32_t count_visits;           /* nonzero: count BVH node visits and primitive tests */
     const rbw_guiding_opts* guiding; /* NULL = plain BSDF continuation (the reference's) */
+    uint32_t time_kernels;           /* nonzero: per-kernel HIP-event times in rbw_stats::kernel_ms */
 } rbw_render_params;
+/* Schedules: the plain estimator with the per-sample sampler (no replay, no debug
+ * hits, no visit counts, no guiding) runs on persistent path slots regenerated in
+ * place (the batched-jobs wavefront with one job, its bounce loop a hipGraph;
+ * time_kernels runs it without the graph).  Every other mode runs in passes of
+ * samples with compacted queues.  Both give the same image bit for bit. */
 
 enum {
     RBW_KERNEL_RAYGEN = 0,

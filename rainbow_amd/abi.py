@@ -79,7 +79,7 @@ class rbw_render_params(C.Structure):
                 ("shadow_mode", C.c_uint32), ("samples_per_pass", C.c_uint32),
                 ("replay_offsets", u64p), ("replay_values", f32p), ("replay_count", C.c_uint64),
                 ("debug_hits", u32p), ("count_visits", C.c_uint32),
-                ("guiding", C.POINTER(rbw_guiding_opts))]
+                ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):

@@ -348,6 +348,10 @@ void batch_free(rbw_ctx* ctx) {
 
 namespace {
 
+// the single render's schedule (measured best on the 1.3M-triangle icosphere at 1024^2)
+constexpr uint32_t k_render_block = 128;
+constexpr uint32_t k_render_pool = 1u << 22;
+
 struct batch_config {
     uint32_t block = 256, pool = 1u << 21, width = 2;
     bool graph = true;
@@ -438,8 +442,8 @@ struct run_result {
 };
 
 // one batch through the bounce loop; films must be cleared by the caller
-void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config& cfg, run_result& rr,
-               bool want_job_times) {
+void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, const batch_config& cfg, run_result& rr,
+               bool want_job_times, kernel_timer* timer = nullptr) {
     rbw_batch_state& b = *ctx->batch;
     hipStream_t s = ctx->stream;
     const uint32_t nj = uint32_t(hjobs.size());
@@ -488,7 +492,6 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     base.sampler = RBW_SAMPLER_PER_SAMPLE_PCG;
     base.tri_test = cfg.tri_test;
     base.shadow_mode = cfg.shadow;
-    const dscene S = b.scene.view();
     // grid: one thread per slot of the pool
     const uint32_t grid = nblocks;
 
@@ -496,12 +499,22 @@ void run_batch(rbw_ctx* ctx, const std::vector<djob>& hjobs, const batch_config&
     hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total, bs);
     HIP_CHECK(hipGetLastError());
 
+    // per-kernel HIP events only without graph capture (kernel_ms of rbw_stats)
+    kernel_timer* tm = cfg.graph ? nullptr : timer;
     auto iteration = [&]() {
+        if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
         hipLaunchKernelGGL(k_b_regen, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (tm) tm->end(s);
+        if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
         hipLaunchKernelGGL(k_b_extend, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (tm) tm->end(s);
+        if (tm) tm->begin(RBW_KERNEL_SHADE, s);
         hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (tm) tm->end(s);
+        if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
         hipLaunchKernelGGL(k_b_connect, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (tm) tm->end(s);
     };
     if (b.exec) {
         (void)hipGraphExecDestroy(b.exec);
@@ -639,7 +652,7 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
         c.pool = pool;
         std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 1);
         run_result rr;
-        run_batch(ctx, hj, c, rr, false);
+        run_batch(ctx, ctx->batch->scene.view(), hj, c, rr, false);
         return rr.ms;
     };
     uint32_t best_block = 256, best_pool = 1u << 21;
@@ -682,7 +695,7 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     rbw_batch_state& b = *ctx->batch;
     std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 0);
     run_result rr;
-    run_batch(ctx, hj, cfg, rr, job_stats != nullptr);
+    run_batch(ctx, b.scene.view(), hj, cfg, rr, job_stats != nullptr);
     // films -> caller
     hipStream_t s = ctx->stream;
     size_t max_px = 0;
@@ -731,6 +744,79 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
         stats->pool_size = cfg.pool;
         stats->bvh_width = cfg.width;
         stats->scene_triangles = b.merged_tris;
+    }
+}
+
+// rbw_render's plain estimator: the scene of rbw_upload_scene as a one-job batch
+void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params& pr, float* out_rgb,
+                      rbw_stats* stats) {
+    if (!ctx->batch) ctx->batch = new rbw_batch_state();
+    rbw_batch_state& b = *ctx->batch;
+    hipStream_t s = ctx->stream;
+    const size_t np = size_t(cam->width) * cam->height;
+    b.films.ensure(np * 32);
+    HIP_CHECK(hipMemsetAsync(b.films.p, 0, np * 32, s));
+    djob J;
+    std::memset(&J, 0, sizeof(J));
+    J.cam = make_dcamera(cam);
+    char* f = b.films.as<char>();
+    J.film.sum = reinterpret_cast<unsigned long long*>(f);
+    J.film.weight = reinterpret_cast<uint32_t*>(f + np * 24);
+    J.film.nan = reinterpret_cast<uint32_t*>(f + np * 28);
+    J.root = 0;
+    J.em_first = 0;
+    J.em_count = ctx->main.num_emitters;
+    J.bw = uint32_t(cam->bound_max[0] - cam->bound_min[0]);
+    J.npix = J.bw * uint32_t(cam->bound_max[1] - cam->bound_min[1]);
+    J.spp = pr.spp;
+    J.max_depth = pr.max_depth;
+    J.seed = pr.seed;
+    J.rr_threshold = pr.rr_threshold;
+    J.base = 0;
+    J.count = (unsigned long long)J.npix * pr.spp;
+    batch_config cfg;
+    cfg.block = k_render_block;
+    // no more slots than samples (rounded to whole blocks)
+    const unsigned long long want = (J.count + 1023) & ~1023ull;
+    cfg.pool = (uint32_t)std::min<unsigned long long>(k_render_pool, std::max<unsigned long long>(1024, want));
+    cfg.graph = !pr.time_kernels;
+    cfg.tri_test = pr.triangle_test;
+    cfg.shadow = pr.shadow_mode;
+    ctx->timer.reset();
+    run_result rr;
+    std::vector<djob> jobs(1, J);
+    run_batch(ctx, ctx->main.view(), jobs, cfg, rr, false, stats ? &ctx->timer : nullptr);
+    b.rgb.ensure(np * 3 * sizeof(float));
+    hipEvent_t ea = ctx->timer.get(), eb = ctx->timer.get();
+    HIP_CHECK(hipEventRecord(ea, s));
+    hipLaunchKernelGGL(k_b_resolve, dim3(blocks(np, 256)), dim3(256), 0, s, J.cam, J.film, J.spp, b.rgb.as<float>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(eb, s));
+    if (out_rgb) HIP_CHECK(hipMemcpyAsync(out_rgb, b.rgb.p, np * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float resolve_ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&resolve_ms, ea, eb));
+        stats->render_ms = rr.ms + resolve_ms;
+        stats->samples = J.count;
+        stats->camera_rays = J.count;
+        stats->extension_rays = rr.counters[0];
+        stats->shadow_rays = rr.counters[1];
+        stats->mis_rays = rr.counters[2];
+        stats->light_casts = rr.counters[3];
+        for (const auto& r : ctx->timer.recs) {
+            float kms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&kms, r.a, r.b));
+            stats->kernel_ms[r.kind] += kms;
+            stats->kernel_launches[r.kind] += 1;
+        }
+        stats->kernel_ms[RBW_KERNEL_FILM] += resolve_ms;
+        stats->kernel_launches[RBW_KERNEL_FILM] += 1;
+        const double sec = stats->render_ms * 1e-3;
+        stats->msamples_per_s = sec > 0 ? double(stats->samples) / sec * 1e-6 : 0.0;
+        const uint64_t rays = stats->extension_rays + stats->shadow_rays + stats->mis_rays;
+        stats->mrays_per_s = sec > 0 ? double(rays) / sec * 1e-6 : 0.0;
     }
 }
 

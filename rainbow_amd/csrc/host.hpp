@@ -180,5 +180,7 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
                  rbw_batch_stats* stats);
 void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* inout, double* ms);
 void batch_free(rbw_ctx* ctx);
+void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params& pr, float* out_rgb,
+                      rbw_stats* stats);
 }  // namespace host
 }  // namespace rbw

@@ -512,6 +512,11 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG, RBW_ERR_INVALID_ARGUMENT,
                 "guiding draws a different number of samples per vertex than the replayed reference stream");
     }
+    // the plain estimator runs on persistent, in-place regenerated path slots
+    if (pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG && !pr.debug_hits && !pr.count_visits && !guided) {
+        render_scheduled(ctx, cam, pr, out_rgb, stats);
+        return;
+    }
     uint32_t k = pr.samples_per_pass;
     // guided renders train between passes: smaller passes give the network more updates
     const uint64_t per_pass = guided ? (1u << 18) : (1u << 21);
