@@ -295,7 +295,7 @@ typedef struct rbw_job {
 } rbw_job;
 
 typedef struct rbw_batch_opts {
-    uint32_t block_size;             /* threads per block of the wavefront kernels; 0 = default 256 */
+    uint32_t block_size;             /* threads per block of the wavefront kernels: 64, 128, 256 or 512; 0 = 256 */
     uint32_t pool_size;              /* path slots shared by all jobs; 0 = default 2^21 */
     uint32_t bvh_width;              /* 0 or 2: binary BVH (the only width built) */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */

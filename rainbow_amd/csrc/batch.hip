@@ -39,6 +39,9 @@ namespace {
 // largest block size of the wavefront kernels (the autotuner's candidates are 64..512);
 // bounding it lets the register allocator give each lane more than 128 VGPRs
 constexpr int k_batch_max_block = 512;
+// regeneration hands out samples per group of this many consecutive slots (one
+// block of the regeneration kernel); block sizes of the other kernels divide it
+constexpr uint32_t k_group = 512;
 
 struct djob {
     dcamera cam;
@@ -64,10 +67,10 @@ struct dbatch {
     const djob* jobs;
     uint32_t njobs;
     uint32_t pool;
-    uint32_t nblocks;               // blocks of the per-slot kernels
-    uint32_t* block_free;           // per block: slots free at the next regeneration
-    uint32_t* block_n;              // per block: camera samples granted this iteration
-    unsigned long long* block_base; // per block: first granted sample
+    uint32_t ngroups;               // groups of k_group consecutive slots
+    uint32_t* block_free;           // per group: slots free at the next regeneration
+    uint32_t* block_n;              // per group: camera samples granted this iteration
+    unsigned long long* block_base; // per group: first granted sample
     uint32_t* wave_job;             // per wave of slots: a job that retired a path in the last regeneration
     uint32_t nwaves;
     uint32_t* first_iter;           // per job
@@ -122,12 +125,12 @@ __device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t& total, uint3
     return off + (uint32_t)__popcll(m & below);
 }
 
-__global__ void k_b_init(dbatch B, dpaths P, unsigned long long total, uint32_t bs) {
+__global__ void k_b_init(dbatch B, dpaths P, unsigned long long total) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < B.pool) P.U(U_STATE, i) = 0u;
-    if (i < B.nblocks) {
-        const uint32_t first = i * bs;
-        B.block_free[i] = B.pool - first < bs ? B.pool - first : bs;
+    if (i < B.ngroups) {
+        const uint32_t first = i * k_group;
+        B.block_free[i] = B.pool - first < k_group ? B.pool - first : k_group;
     }
     if (i < B.nwaves) B.wave_job[i] = RBW_NONE;
     if (i < B.njobs) {
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(1024) void k_b_fold(dbatch B) { fold_retire_records
 // block of 1024 threads scans the per-block free counts.
 __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x, n = B.nblocks;
+    const uint32_t t = threadIdx.x, n = B.ngroups;
     fold_retire_records(B, B.ctl->iteration);
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
@@ -187,6 +190,7 @@ __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     uint32_t prefix = part[t] - sum;
     for (uint32_t b = b0; b < b1; b++) {
         const uint32_t f = B.block_free[b];
+        B.block_free[b] = 0u;  // the connect kernel counts the next iteration's free slots
         const unsigned long long avail = left > prefix ? left - prefix : 0ull;
         B.block_base[b] = cursor + prefix;
         B.block_n[b] = avail < f ? (uint32_t)avail : f;
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpath
     wave_add64(B.counters + 3, nlight);
     uint32_t nfree = 0;
     block_rank(slot < B.pool && (st == 0u || (st & ST_DYING)), nfree, lds);
-    if (threadIdx.x == 0) B.block_free[blockIdx.x] = nfree;
+    if (threadIdx.x == 0 && nfree) atomicAdd(&B.block_free[blockIdx.x * blockDim.x / k_group], nfree);
 }
 
 // every pixel inside the bounds has spp own samples; the film's weights count the spills
@@ -387,8 +391,8 @@ batch_config config_of(const rbw_batch_opts* o) {
     c.graph = o->use_graph != 0;
     c.tri_test = o->triangle_test;
     c.shadow = o->shadow_mode;
-    require(c.block >= 64 && c.block <= (uint32_t)k_batch_max_block && (c.block & 63) == 0,
-            RBW_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 512]");
+    require(c.block == 64 || c.block == 128 || c.block == 256 || c.block == 512, RBW_ERR_INVALID_ARGUMENT,
+            "block_size must be 64, 128, 256 or 512");
     require(c.pool >= 1024 && c.pool <= (1u << 26), RBW_ERR_INVALID_ARGUMENT, "pool_size must be in [2^10, 2^26]");
     require(c.width == 2, RBW_ERR_UNSUPPORTED, "bvh_width: only the binary BVH is built");
     require(c.tri_test == RBW_TRI_MOLLER_TRUMBORE || c.tri_test == RBW_TRI_WATERTIGHT, RBW_ERR_INVALID_ARGUMENT,
@@ -458,9 +462,10 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     b.jobs.upload(hjobs.data(), hjobs.size(), s);
     const uint32_t bs = cfg.block;
     const uint32_t nblocks = blocks(cfg.pool, bs);
-    b.block_free.ensure(size_t(nblocks) * 4);
-    b.block_n.ensure(size_t(nblocks) * 4);
-    b.block_base.ensure(size_t(nblocks) * 8);
+    const uint32_t ngroups = blocks(cfg.pool, k_group);
+    b.block_free.ensure(size_t(ngroups) * 4);
+    b.block_n.ensure(size_t(ngroups) * 4);
+    b.block_base.ensure(size_t(ngroups) * 8);
     const uint32_t nwaves = blocks(cfg.pool, 64);
     b.wave_job.ensure(size_t(nwaves) * 4);
     b.first_iter.ensure(size_t(nj) * 4);
@@ -473,7 +478,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     B.jobs = b.jobs.as<djob>();
     B.njobs = nj;
     B.pool = cfg.pool;
-    B.nblocks = nblocks;
+    B.ngroups = ngroups;
     B.block_free = b.block_free.as<uint32_t>();
     B.block_n = b.block_n.as<uint32_t>();
     B.block_base = b.block_base.as<unsigned long long>();
@@ -496,7 +501,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     const uint32_t grid = nblocks;
 
     const uint32_t init_n = std::max(cfg.pool, nj);
-    hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total, bs);
+    hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total);
     HIP_CHECK(hipGetLastError());
 
     // per-kernel HIP events only without graph capture (kernel_ms of rbw_stats)
@@ -504,7 +509,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
-        hipLaunchKernelGGL(k_b_regen, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(k_b_regen, dim3(ngroups), dim3(k_group), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
         hipLaunchKernelGGL(k_b_extend, dim3(grid), dim3(bs), 0, s, S, P, base, B);
