@@ -168,6 +168,7 @@ typedef struct rbw_render_params {
     uint32_t count_visits;           /* nonzero: count BVH node visits and primitive tests */
     const rbw_guiding_opts* guiding; /* NULL = plain BSDF continuation (the reference's) */
     uint32_t time_kernels;           /* nonzero: per-kernel HIP-event times in rbw_stats::kernel_ms */
+    uint32_t bvh_width;              /* 0 (default: 2), 2 (binary BVH) or 8 (eight-wide BVH) */
 } rbw_render_params;
 /* Schedules: the plain estimator with the per-sample sampler (no replay, no debug
  * hits, no visit counts, no guiding) runs on persistent path slots regenerated in
@@ -246,7 +247,7 @@ rbw_status rbw_render(rbw_ctx* ctx, const rbw_camera_desc* camera,
 rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
                           const float* directions, const float* tmax,
                           uint32_t visible_only, uint32_t triangle_test,
-                          rbw_hit* out);
+                          uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
 
 /* ---------------------------------------------------------------- guiding
  * The guiding network of a context (no counterpart in the reference).
@@ -297,7 +298,7 @@ typedef struct rbw_job {
 typedef struct rbw_batch_opts {
     uint32_t block_size;             /* threads per block of the wavefront kernels: 64, 128, 256 or 512; 0 = 256 */
     uint32_t pool_size;              /* path slots shared by all jobs; 0 = default 2^21 */
-    uint32_t bvh_width;              /* 0 or 2: binary BVH (the only width built) */
+    uint32_t bvh_width;              /* 0 (default: 2), 2 or 8; the autotuner tries both */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
     uint32_t autotune;               /* 1: time candidate block/pool sizes on a 1-spp probe first */
     uint32_t triangle_test;          /* RBW_TRI_* */

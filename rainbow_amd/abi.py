@@ -79,7 +79,7 @@ class rbw_render_params(C.Structure):
                 ("shadow_mode", C.c_uint32), ("samples_per_pass", C.c_uint32),
                 ("replay_offsets", u64p), ("replay_values", f32p), ("replay_count", C.c_uint64),
                 ("debug_hits", u32p), ("count_visits", C.c_uint32),
-                ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32)]
+                ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):
@@ -158,7 +158,7 @@ RBW_SIGNATURES = {
     "rbw_upload_scene": (C.c_int32, [C.c_void_p, C.POINTER(rbw_scene_desc)]),
     "rbw_render": (C.c_int32, [C.c_void_p, C.POINTER(rbw_camera_desc), C.POINTER(rbw_render_params),
                                f32p, C.POINTER(rbw_stats)]),
-    "rbw_trace_rays": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_uint32,
+    "rbw_trace_rays": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_void_p]),
     "rbw_guide_reset": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "rbw_guide_get_params": (C.c_int32, [C.c_void_p, f32p]),

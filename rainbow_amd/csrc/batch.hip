@@ -51,7 +51,8 @@ struct djob {
     uint32_t root, em_first, em_count, spp;
     uint32_t max_depth, seed, npix, bw;
     float rr_threshold;
-    uint32_t pad[3];
+    uint32_t root8;
+    uint32_t pad[2];
 };
 
 struct dctl {
@@ -97,6 +98,7 @@ __device__ __forceinline__ uint32_t find_job(const djob* jobs, uint32_t n, unsig
 __device__ __forceinline__ void job_view(const dscene& S, const dparams& base, const djob& J, dscene& Sj, dparams& pj) {
     Sj = S;
     Sj.root = J.root;
+    Sj.root8 = J.root8;
     Sj.emitters = S.emitters + J.em_first;
     Sj.num_emitters = J.em_count;
     pj = base;
@@ -254,6 +256,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
     if (in && (gen || dying)) P.U(U_STATE, slot) = st;
 }
 
+template <int W>
 __global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths P, dparams base, dbatch B) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths
         dscene Sj;
         dparams pj;
         job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-        extend_path<false>(Sj, P, pj, slot, cnt);
+        extend_path<W, false>(Sj, P, pj, slot, cnt);
         rays = 1;
     }
     wave_add64(B.counters + 0, rays);
@@ -284,6 +287,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_shade(dscene S, dpaths 
 
 // next-event estimation rays; then the block's count of slots the next
 // regeneration will find free (free or finished)
+template <int W>
 __global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpath
             dscene Sj;
             dparams pj;
             job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-            connect_path<false>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
+            connect_path<W, false>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
             st &= ~ST_CONNECT;
             P.U(U_STATE, slot) = st;
         }
@@ -322,7 +326,7 @@ struct rbw_batch_state {
     // merged scene of the last submission
     std::vector<uint32_t> scene_ids;
     dev_scene scene;
-    std::vector<uint32_t> roots, em_first, em_count;
+    std::vector<uint32_t> roots, roots8, em_first, em_count;
     uint32_t merged_tris = 0;
     // path pool
     dbuf pf, pu, prng;
@@ -357,7 +361,7 @@ constexpr uint32_t k_render_block = 128;
 constexpr uint32_t k_render_pool = 1u << 22;
 
 struct batch_config {
-    uint32_t block = 256, pool = 1u << 21, width = 2;
+    uint32_t block = 256, pool = 1u << 21, width = k_default_width;
     bool graph = true;
     uint32_t tri_test = RBW_TRI_MOLLER_TRUMBORE, shadow = RBW_SHADOW_CLOSEST_VISIBLE;
 };
@@ -387,14 +391,13 @@ batch_config config_of(const rbw_batch_opts* o) {
     if (!o) return c;
     if (o->block_size) c.block = o->block_size;
     if (o->pool_size) c.pool = o->pool_size;
-    if (o->bvh_width) c.width = o->bvh_width;
+    c.width = resolve_width(o->bvh_width);
     c.graph = o->use_graph != 0;
     c.tri_test = o->triangle_test;
     c.shadow = o->shadow_mode;
     require(c.block == 64 || c.block == 128 || c.block == 256 || c.block == 512, RBW_ERR_INVALID_ARGUMENT,
             "block_size must be 64, 128, 256 or 512");
     require(c.pool >= 1024 && c.pool <= (1u << 26), RBW_ERR_INVALID_ARGUMENT, "pool_size must be in [2^10, 2^26]");
-    require(c.width == 2, RBW_ERR_UNSUPPORTED, "bvh_width: only the binary BVH is built");
     require(c.tri_test == RBW_TRI_MOLLER_TRUMBORE || c.tri_test == RBW_TRI_WATERTIGHT, RBW_ERR_INVALID_ARGUMENT,
             "unknown triangle test");
     require(c.shadow == RBW_SHADOW_CLOSEST_VISIBLE || c.shadow == RBW_SHADOW_ANY_HIT, RBW_ERR_INVALID_ARGUMENT,
@@ -420,7 +423,7 @@ void prepare_scene(rbw_ctx* ctx, const rbw_job* jobs, size_t n, std::vector<uint
     std::vector<const host_scene*> parts;
     for (uint32_t id : ids) parts.push_back(ctx->pool[id].get());
     host_scene merged;
-    merge_scenes(parts, merged, b.roots, b.em_first, b.em_count);
+    merge_scenes(parts, merged, b.roots, b.roots8, b.em_first, b.em_count);
     b.scene.upload(merged, ctx->stream);
     b.scene_ids = ids;
     b.merged_tris = merged.nt;
@@ -512,13 +515,14 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         hipLaunchKernelGGL(k_b_regen, dim3(ngroups), dim3(k_group), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
-        hipLaunchKernelGGL(k_b_extend, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(S.width == 8 ? k_b_extend<8> : k_b_extend<2>, dim3(grid), dim3(bs), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_SHADE, s);
         hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
-        hipLaunchKernelGGL(k_b_connect, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(S.width == 8 ? k_b_connect<8> : k_b_connect<2>, dim3(grid), dim3(bs), 0, s, S, P, base,
+                           B);
         if (tm) tm->end(s);
     };
     if (b.exec) {
@@ -624,6 +628,7 @@ std::vector<djob> make_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const s
         d.film.nan = reinterpret_cast<uint32_t*>(f + np * 28);
         const uint32_t p = part_of_job[i];
         d.root = b.roots[p];
+        d.root8 = b.roots8[p];
         d.em_first = b.em_first[p];
         d.em_count = b.em_count[p];
         d.bw = uint32_t(J.camera.bound_max[0] - J.camera.bound_min[0]);
@@ -651,26 +656,35 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
     batch_config cfg = config_of(inout);
     const auto t0 = std::chrono::steady_clock::now();
     // probe: every job at 1 spp; coordinate search over block size, then pool size
-    auto probe = [&](uint32_t block, uint32_t pool) {
+    auto probe = [&](uint32_t block, uint32_t pool, uint32_t width) {
         batch_config c = cfg;
         c.block = block;
         c.pool = pool;
+        c.width = width;
         std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 1);
         run_result rr;
-        run_batch(ctx, ctx->batch->scene.view(), hj, c, rr, false);
+        run_batch(ctx, ctx->batch->scene.view(c.width), hj, c, rr, false);
         return rr.ms;
     };
-    uint32_t best_block = 256, best_pool = 1u << 21;
+    // coordinate search: BVH width, then block size, then pool size
+    uint32_t best_block = 256, best_pool = 1u << 21, best_width = k_default_width;
     double best = 1e300;
-    for (uint32_t block : {64u, 128u, 256u, 512u}) {
-        const double t = probe(block, best_pool);
+    for (uint32_t width : {2u, 8u}) {
+        const double t = probe(best_block, best_pool, width);
+        if (t < best) {
+            best = t;
+            best_width = width;
+        }
+    }
+    for (uint32_t block : {64u, 128u, 512u}) {
+        const double t = probe(block, best_pool, best_width);
         if (t < best) {
             best = t;
             best_block = block;
         }
     }
-    for (uint32_t pool : {1u << 19, 1u << 20, 1u << 22}) {
-        const double t = probe(best_block, pool);
+    for (uint32_t pool : {1u << 20, 1u << 22}) {
+        const double t = probe(best_block, pool, best_width);
         if (t < best) {
             best = t;
             best_pool = pool;
@@ -678,7 +692,7 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
     }
     inout->block_size = best_block;
     inout->pool_size = best_pool;
-    inout->bvh_width = 2;  // the only width built
+    inout->bvh_width = best_width;
     if (ms_out)
         *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -700,7 +714,7 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     rbw_batch_state& b = *ctx->batch;
     std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 0);
     run_result rr;
-    run_batch(ctx, b.scene.view(), hj, cfg, rr, job_stats != nullptr);
+    run_batch(ctx, b.scene.view(cfg.width), hj, cfg, rr, job_stats != nullptr);
     // films -> caller
     hipStream_t s = ctx->stream;
     size_t max_px = 0;
@@ -769,6 +783,7 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
     J.film.weight = reinterpret_cast<uint32_t*>(f + np * 24);
     J.film.nan = reinterpret_cast<uint32_t*>(f + np * 28);
     J.root = 0;
+    J.root8 = 0;
     J.em_first = 0;
     J.em_count = ctx->main.num_emitters;
     J.bw = uint32_t(cam->bound_max[0] - cam->bound_min[0]);
@@ -787,10 +802,11 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
     cfg.graph = !pr.time_kernels;
     cfg.tri_test = pr.triangle_test;
     cfg.shadow = pr.shadow_mode;
+    cfg.width = resolve_width(pr.bvh_width);
     ctx->timer.reset();
     run_result rr;
     std::vector<djob> jobs(1, J);
-    run_batch(ctx, ctx->main.view(), jobs, cfg, rr, false, stats ? &ctx->timer : nullptr);
+    run_batch(ctx, ctx->main.view(cfg.width), jobs, cfg, rr, false, stats ? &ctx->timer : nullptr);
     b.rgb.ensure(np * 3 * sizeof(float));
     hipEvent_t ea = ctx->timer.get(), eb = ctx->timer.get();
     HIP_CHECK(hipEventRecord(ea, s));

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 
 namespace rbw {
@@ -196,6 +197,155 @@ void build_bvh(const std::vector<aabb>& bounds, bvh_build_result& out, uint32_t 
         }
     }
     if (bounds.size() >= (1u << 27)) throw std::runtime_error("too many primitives for the bvh2 leaf encoding");
+}
+
+namespace {
+
+// exponent e with 2^e >= x (x > 0)
+inline int ceil_log2(double x) {
+    int k;
+    const double m = std::frexp(x, &k);  // x = m 2^k, m in [0.5, 1)
+    return m == 0.5 ? k - 1 : k;
+}
+
+inline float step_of(uint8_t e) {
+    const uint32_t bits = uint32_t(e) << 23;
+    float s;
+    std::memcpy(&s, &bits, 4);
+    return s;
+}
+
+}  // namespace
+
+void build_bvh8(const bvh_build_result& bin, bvh8_result& out) {
+    const std::vector<build_node>& t = bin.tree;
+    out = bvh8_result();
+    if (t.empty()) throw std::runtime_error("empty tree");
+    // primitive range of every binary subtree (the builder partitions in place, so a
+    // subtree's primitives are consecutive slots); subtrees of at most k_leaf8 primitives
+    // become one wide-tree leaf
+    constexpr uint32_t k_leaf8 = 2;
+    std::vector<uint32_t> first(t.size()), count(t.size());
+    for (size_t k = t.size(); k-- > 0;) {
+        if (t[k].left < 0) {
+            first[k] = t[k].first;
+            count[k] = t[k].count;
+        } else {
+            first[k] = std::min(first[size_t(t[k].left)], first[size_t(t[k].right)]);
+            count[k] = count[size_t(t[k].left)] + count[size_t(t[k].right)];
+        }
+    }
+    auto is_leaf = [&](int32_t b) { return t[b].left < 0 || count[size_t(b)] <= k_leaf8; };
+    out.order.reserve(bin.order.size());
+    struct item {
+        int32_t bnode;
+        uint32_t node8, level;
+    };
+    std::vector<item> queue;
+    queue.push_back({0, 0, 1});
+    out.nodes.push_back(bvh8_node{});
+    for (size_t qi = 0; qi < queue.size(); qi++) {
+        const item it = queue[qi];
+        out.depth = std::max(out.depth, it.level);
+        // children: open the largest inner child until there are eight
+        std::vector<int32_t> ch;
+        if (is_leaf(it.bnode)) {
+            ch.push_back(it.bnode);
+        } else {
+            ch.push_back(t[it.bnode].left);
+            ch.push_back(t[it.bnode].right);
+            while (ch.size() < 8) {
+                // the largest subtree too big for a leaf; when none is left, split the
+                // largest small subtree into finer leaves while slots are free
+                int best = -1;
+                float best_area = -1.0f;
+                for (int pass = 0; pass < 2 && best < 0; pass++)
+                    for (size_t k = 0; k < ch.size(); k++) {
+                        const bool open = pass == 0 ? !is_leaf(ch[k]) : t[ch[k]].left >= 0;
+                        if (open && area(t[ch[k]].box) > best_area) {
+                            best_area = area(t[ch[k]].box);
+                            best = int(k);
+                        }
+                    }
+                if (best < 0) break;
+                const int32_t b = ch[size_t(best)];
+                ch[size_t(best)] = t[b].left;
+                ch.push_back(t[b].right);
+            }
+        }
+        aabb nb = empty_box();
+        for (int32_t c : ch) grow(nb, t[c].box);
+        // slots by octant of the child centre; larger children choose first
+        std::vector<int32_t> order_by_area(ch.begin(), ch.end());
+        std::stable_sort(order_by_area.begin(), order_by_area.end(),
+                         [&](int32_t a, int32_t b) { return area(t[a].box) > area(t[b].box); });
+        int32_t child_in_slot[8];
+        for (int i = 0; i < 8; i++) child_in_slot[i] = -1;
+        for (int32_t c : order_by_area) {
+            int oct = 0;
+            for (int a = 0; a < 3; a++) {
+                const float cc = 0.5f * (t[c].box.lo[a] + t[c].box.hi[a]);
+                const float nc = 0.5f * (nb.lo[a] + nb.hi[a]);
+                if (cc > nc) oct |= 4 >> a;
+            }
+            int pick = -1, pick_d = 99;
+            for (int s = 0; s < 8; s++) {
+                if (child_in_slot[s] >= 0) continue;
+                const int d = __builtin_popcount(unsigned(s ^ oct));
+                if (d < pick_d) {
+                    pick_d = d;
+                    pick = s;
+                }
+            }
+            child_in_slot[pick] = c;
+        }
+        bvh8_node n;
+        std::memset(&n, 0, sizeof(n));
+        // quantisation grid: p = node minimum, step 2^e with 255 steps covering the box
+        for (int a = 0; a < 3; a++) {
+            n.p[a] = nb.lo[a];
+            const double range = double(nb.hi[a]) - double(nb.lo[a]);
+            int e = range > 0.0 ? ceil_log2(range * (1.0 + 1.0 / (1 << 18)) / 255.0) : -126;
+            e = std::max(-126, std::min(127, e));
+            n.e[a] = uint8_t(e + 127);
+        }
+        // inner children first (consecutive node indices in slot order), then leaves
+        n.child_base = uint32_t(out.nodes.size());
+        n.prim_base = uint32_t(out.order.size());
+        for (int s = 0; s < 8; s++) {
+            const int32_t c = child_in_slot[s];
+            if (c < 0 || is_leaf(c)) continue;
+            n.imask |= uint8_t(1u << s);
+            queue.push_back({c, uint32_t(out.nodes.size()), it.level + 1});
+            out.nodes.push_back(bvh8_node{});
+        }
+        for (int s = 0; s < 8; s++) {
+            const int32_t c = child_in_slot[s];
+            if (c < 0) continue;
+            if (is_leaf(c)) {
+                const uint32_t off = uint32_t(out.order.size()) - n.prim_base;
+                const uint32_t cnt = count[size_t(c)];
+                if (cnt < 1 || cnt > 4 || off > 31) throw std::runtime_error("bvh8 leaf does not fit its meta byte");
+                n.meta[s] = uint8_t(k_meta_leaf | ((cnt - 1) << 5) | off);
+                for (uint32_t k = 0; k < cnt; k++) out.order.push_back(bin.order[first[size_t(c)] + k]);
+            }
+            for (int a = 0; a < 3; a++) {
+                const float st = step_of(n.e[a]);
+                double ql = std::floor((double(t[c].box.lo[a]) - double(n.p[a])) / double(st));
+                double qh = std::ceil((double(t[c].box.hi[a]) - double(n.p[a])) / double(st));
+                int lo = int(std::max(0.0, std::min(255.0, ql)));
+                int hi = int(std::max(0.0, std::min(255.0, qh)));
+                // conservative in the traversal's float arithmetic: p + q s rounded once
+                while (lo > 0 && n.p[a] + float(lo) * st > t[c].box.lo[a]) lo--;
+                while (hi < 255 && n.p[a] + float(hi) * st < t[c].box.hi[a]) hi++;
+                if (n.p[a] + float(lo) * st > t[c].box.lo[a] || n.p[a] + float(hi) * st < t[c].box.hi[a])
+                    throw std::runtime_error("bvh8 quantisation is not conservative");
+                n.qlo[a][s] = uint8_t(lo);
+                n.qhi[a][s] = uint8_t(hi);
+            }
+        }
+        out.nodes[it.node8] = n;
+    }
 }
 
 }  // namespace rbw

@@ -32,11 +32,21 @@ constexpr int32_t k_leaf_bit = int32_t(0x80000000u);
 constexpr uint32_t k_leaf_max = 16;
 constexpr int32_t k_empty_child = int32_t(0xFFFFFFFFu);  // leaf flag with count 16 at slot 2^27-1: never hit
 
-// Wide BVH with 8-bit quantised child boxes (80 bytes per node), the layout of
-// Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
-// Through Compressed Wide BVHs", HPG 2017:
-//   p (float3 origin), e (3 x uint8 exponent), imask (inner-child mask),
-//   child_base, prim_base, meta[8], qlo_x[8], qlo_y[8], qlo_z[8], qhi_x[8], qhi_y[8], qhi_z[8]
+// Eight-wide BVH with 8-bit quantised child boxes, 80 bytes per node (five
+// 16-byte loads), after Ylitie, Karras and Laine, "Efficient Incoherent Ray
+// Traversal on GPUs Through Compressed Wide BVHs", HPG 2017; the field coding
+// below is this build's own:
+//   p          node box minimum (float3)
+//   e          per axis the biased exponent of the quantisation step s = 2^(e-127)
+//   imask      bit i: child slot i is an inner node
+//   child_base inner children are nodes child_base + (number of inner slots below i)
+//   prim_base  leaf children hold primitive slots prim_base + offset ...
+//   meta[i]    leaf child: 0x80 | (count - 1) << 5 | offset  (count 1..4, offset 0..31)
+//              inner child or empty slot: 0
+//   qlo/qhi    child box = p + q * s per axis (rounded once), conservative
+// Children sit in slots by the octant of their centre relative to the node
+// centre, so visiting slots in the order j ^ octant(ray direction) is roughly
+// front to back.
 struct bvh8_node {
     float p[3];
     uint8_t e[3];
@@ -48,6 +58,25 @@ struct bvh8_node {
     uint8_t qhi[3][8];
 };
 static_assert(sizeof(bvh8_node) == 80, "bvh8 node is 80 bytes");
+
+constexpr uint8_t k_meta_leaf = 0x80;
+
+// child box of slot i as the traversal sees it
+inline void bvh8_child_box(const bvh8_node& n, int i, float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; a++) {
+        const uint32_t bits = uint32_t(n.e[a]) << 23;
+        float s;
+        __builtin_memcpy(&s, &bits, 4);
+        lo[a] = n.p[a] + float(n.qlo[a][i]) * s;
+        hi[a] = n.p[a] + float(n.qhi[a][i]) * s;
+    }
+}
+
+struct bvh8_result {
+    std::vector<bvh8_node> nodes;   // root at index 0
+    std::vector<uint32_t> order;    // leaf slot -> primitive index
+    uint32_t depth = 0;             // levels of nodes
+};
 
 // Binary SAH tree in builder form (children indices, node boxes)
 struct build_node {
@@ -67,9 +96,9 @@ struct bvh_build_result {
 // bounds: conservative primitive boxes.  Builds the binned-SAH tree and the bvh2 layout.
 void build_bvh(const std::vector<aabb>& bounds, bvh_build_result& out, uint32_t max_leaf = 4);
 
-// Collapses the binary tree into the compressed 8-wide layout.  Leaves hold at
-// most 3 primitives in consecutive slots of `order8`.
-void build_bvh8(const std::vector<aabb>& bounds, const bvh_build_result& bin, std::vector<bvh8_node>& nodes8,
-                std::vector<uint32_t>& order8);
+// Collapses the binary SAH tree into the eight-wide layout: each wide node
+// takes a binary node's children and keeps opening its largest inner child
+// until it has eight.  Binary leaves (1..4 primitives) stay leaves.
+void build_bvh8(const bvh_build_result& bin, bvh8_result& out);
 
 }  // namespace rbw

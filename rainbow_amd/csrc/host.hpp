@@ -110,6 +110,8 @@ struct kernel_timer {
 struct host_scene {
     std::vector<float4> nodes;      // 4 per bvh2 node
     std::vector<float4> pa, pb, pc; // leaf slots
+    std::vector<float4> nodes8;     // 5 per bvh8 node
+    std::vector<float4> pa8, pb8, pc8;  // leaf slots of the bvh8
     std::vector<float> pos, nrm, uv;
     std::vector<uint32_t> idx, tri_entity, tri_material, sphere_entity, emitters;
     std::vector<float4> spheres;
@@ -117,7 +119,7 @@ struct host_scene {
     std::vector<dmaterial> mats;
     uint32_t nt = 0, ns = 0, nv = 0;
     bool has_nrm = false, has_uv = false, has_tri_material = false;
-    uint32_t depth = 0;
+    uint32_t depth = 0, depth8 = 0;
     f3 smin{0, 0, 0}, smax{0, 0, 0};
     size_t bytes() const;
 };
@@ -126,19 +128,24 @@ struct host_scene {
 void build_host_scene(const rbw_scene_desc* d, host_scene& out);
 
 // One scene holding several: every index rebased (nodes, slots, primitives,
-// vertices, entities, materials), each part's BVH kept intact under its own
-// root.  roots/em_first/em_count describe part i inside the union.
+// vertices, entities, materials), each part's BVHs kept intact under their own
+// roots.  roots/roots8/em_first/em_count describe part i inside the union.
 void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& out, std::vector<uint32_t>& roots,
-                  std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count);
+                  std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count);
+
+constexpr uint32_t k_default_width = 2;  // BVH width of a render that does not name one (the faster on the icosphere)
 
 struct dev_scene {
-    dbuf nodes, prim_a, prim_b, prim_c, pos, nrm, uv, idx, tri_entity, tri_material, spheres, sphere_entity, ents,
-        mats, emitters;
+    dbuf nodes, prim_a, prim_b, prim_c, nodes8, prim8_a, prim8_b, prim8_c, pos, nrm, uv, idx, tri_entity,
+        tri_material, spheres, sphere_entity, ents, mats, emitters;
     uint32_t nt = 0, ns = 0, num_emitters = 0;
     bool has_uv = false, has_tri_material = false, valid = false;
     void upload(const host_scene& h, hipStream_t s);
-    dscene view() const;
+    dscene view(uint32_t width = k_default_width) const;
 };
+
+// 0 -> the default width; otherwise 2 or 8
+uint32_t resolve_width(uint32_t requested);
 
 }  // namespace host
 }  // namespace rbw

@@ -28,18 +28,83 @@ __device__ __forceinline__ float thr_from(uint32_t best_inst, uint32_t cand_inst
 }
 
 
+// per-ray state of a closest-hit walk
+struct trav_state {
+    hitrec best;
+    uint32_t best_inst_entity;  // entity of the best hit when it belongs to an instanced mesh
+    float len_w;                // world length of the ray (shrinks with each accepted hit)
+    float thr_local;            // current acceptance bound in local units
+};
+
+// the primitives of one leaf, slots [first, first + count) of (pa, pb, pc);
+// returns true when an any-hit query may stop
 template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
-__device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
-                                           uint32_t target_entity, trace_counters& cnt) {
+__device__ __forceinline__ bool test_leaf(const dscene& S, const float4* __restrict__ pa_, const float4* __restrict__ pb_,
+                                          const float4* __restrict__ pc_, uint32_t first, uint32_t count,
+                                          const local_ray& lr, f3 o_w, const wt_setup& wt, uint32_t tri_test,
+                                          uint32_t target_entity, trace_counters& cnt, trav_state& st) {
+    for (uint32_t k = 0; k < count; k++) {
+        const uint32_t slot = first + k;
+        const float4 pa = pa_[slot];
+        const uint32_t word = __float_as_uint(pa.w);
+        if (VISIBLE_ONLY && (word & k_prim_invisible)) continue;
+        if (COUNT) cnt.prims++;
+        const uint32_t id = word & k_prim_id_mask;
+        float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
+        bool hit;
+        uint32_t inst_entity = RBW_NONE;
+        if (word & k_prim_sphere) {
+            const float4 pb = pb_[slot];
+            const float thr = thr_from(st.best_inst_entity, RBW_NONE, st.best.t, st.len_w, lr.L0);
+            const f3 ol = translate(o_w, f3{-pa.x, -pa.y, -pa.z});
+            hit = sphere_test(pb.x, ol, lr.d, thr, t);
+        } else {
+            const float4 pb = pb_[slot], pc = pc_[slot];
+            if (word & k_prim_instanced) inst_entity = S.tri_entity[id];
+            const float thr = thr_from(st.best_inst_entity, inst_entity, st.best.t, st.len_w, lr.L0);
+            const f3 p0 = f3{pa.x, pa.y, pa.z}, p1 = f3{pb.x, pb.y, pb.z}, p2 = f3{pc.x, pc.y, pc.z};
+            if (tri_test == RBW_TRI_WATERTIGHT)
+                hit = wt_test(p0, p1, p2, lr.o, lr.d, thr, wt.kz, wt.Sx, wt.Sy, wt.Sz, t, b1, b2);
+            else
+                hit = mt_test(p0, p1, p2, lr.o, lr.d, thr, t, b1, b2);
+        }
+        if (hit) {
+            if (ANY_HIT) {
+                const uint32_t ent = (word & k_prim_sphere) ? S.sphere_entity[id - S.num_tris] : S.tri_entity[id];
+                if (ent != target_entity) {
+                    st.best.prim = id;
+                    st.best.t = t;
+                    return true;
+                }
+                continue;
+            }
+            st.best.prim = id;
+            st.best.t = t;
+            st.best.b1 = b1;
+            st.best.b2 = b2;
+            st.best_inst_entity = inst_entity;
+            st.len_w = t * lr.L1;  // entity.cpp:384
+            st.thr_local = (inst_entity != RBW_NONE) ? t : st.len_w * lr.L0;
+        }
+    }
+    return false;
+}
+
+// closest hit (or closest visible hit) through the binary BVH: near child first,
+// the far child on a private stack
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
+                                            uint32_t target_entity, trace_counters& cnt) {
     const local_ray lr = make_local_ray(o_w, d_w);
     wt_setup wt;
     if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
-    hitrec best;
-    best.prim = RBW_MISS;
-    best.t = 0.0f;
-    best.b1 = best.b2 = 0.0f;
-    uint32_t best_inst_entity = RBW_NONE;  // entity of the best hit when it belongs to an instanced mesh
-    float thr_local = len_w * lr.L0;       // current acceptance bound in local units
+    trav_state st;
+    st.best.prim = RBW_MISS;
+    st.best.t = 0.0f;
+    st.best.b1 = st.best.b2 = 0.0f;
+    st.best_inst_entity = RBW_NONE;
+    st.len_w = len_w;
+    st.thr_local = len_w * lr.L0;
     uint32_t stack[RBW_STACK];
     int sp = 0;
     int32_t cur = (int32_t)S.root;
@@ -52,8 +117,7 @@ __device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, floa
             const float4 a = n[0], b = n[1], c = n[2];
             const int4 ch = *reinterpret_cast<const int4*>(n + 3);
             if (COUNT) cnt.nodes++;
-            const float tmax = thr_local * 1.0000002f;
-            // left
+            const float tmax = st.thr_local * 1.0000002f;
             // slab distances (lo - o) / d as fma(lo, 1/d, -o/d); the bounds below are widened
             // by a few ulps so that rounding never culls a box that holds an accepted hit
             const float lx0 = __builtin_fmaf(a.x, inv.x, -oi.x), lx1 = __builtin_fmaf(a.y, inv.x, -oi.x);
@@ -83,56 +147,130 @@ __device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, floa
         } else if (cur != k_empty_child) {
             const uint32_t first = (uint32_t)cur & 0x07FFFFFFu;
             const uint32_t count = (((uint32_t)cur >> 27) & 15u) + 1u;
-            for (uint32_t k = 0; k < count; k++) {
-                const uint32_t slot = first + k;
-                const float4 pa = S.prim_a[slot];
-                const uint32_t word = __float_as_uint(pa.w);
-                if (VISIBLE_ONLY && (word & k_prim_invisible)) continue;
-                if (COUNT) cnt.prims++;
-                const uint32_t id = word & k_prim_id_mask;
-                float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
-                bool hit;
-                uint32_t inst_entity = RBW_NONE;
-                if (word & k_prim_sphere) {
-                    const float4 pb = S.prim_b[slot];
-                    const float thr = thr_from(best_inst_entity, RBW_NONE, best.t, len_w, lr.L0);
-                    const f3 ol = translate(o_w, f3{-pa.x, -pa.y, -pa.z});
-                    hit = sphere_test(pb.x, ol, lr.d, thr, t);
-                } else {
-                    const float4 pb = S.prim_b[slot], pc = S.prim_c[slot];
-                    if (word & k_prim_instanced) inst_entity = S.tri_entity[id];
-                    const float thr = thr_from(best_inst_entity, inst_entity, best.t, len_w, lr.L0);
-                    const f3 p0 = f3{pa.x, pa.y, pa.z}, p1 = f3{pb.x, pb.y, pb.z}, p2 = f3{pc.x, pc.y, pc.z};
-                    if (tri_test == RBW_TRI_WATERTIGHT)
-                        hit = wt_test(p0, p1, p2, o, lr.d, thr, wt.kz, wt.Sx, wt.Sy, wt.Sz, t, b1, b2);
-                    else
-                        hit = mt_test(p0, p1, p2, o, lr.d, thr, t, b1, b2);
-                }
-                if (hit) {
-                    if (ANY_HIT) {
-                        const uint32_t ent =
-                            (word & k_prim_sphere) ? S.sphere_entity[id - S.num_tris] : S.tri_entity[id];
-                        if (ent != target_entity) {
-                            best.prim = id;
-                            best.t = t;
-                            return best;
-                        }
-                        continue;
-                    }
-                    best.prim = id;
-                    best.t = t;
-                    best.b1 = b1;
-                    best.b2 = b2;
-                    best_inst_entity = inst_entity;
-                    len_w = t * lr.L1;  // entity.cpp:384
-                    thr_local = (inst_entity != RBW_NONE) ? t : len_w * lr.L0;
-                }
-            }
+            if (test_leaf<VISIBLE_ONLY, ANY_HIT, COUNT>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt,
+                                                        tri_test, target_entity, cnt, st))
+                return st.best;
         }
         if (sp == 0) break;
         cur = (int32_t)stack[--sp];
     }
-    return best;
+    return st.best;
+}
+
+// closest hit (or closest visible hit) through the eight-wide BVH.  The walk keeps
+// a group (a node's inner children still to visit, as a bit mask in visiting
+// order j, child slot j ^ octant); a visited node tests its eight child boxes,
+// tests hit leaves at once in visiting order and makes its hit inner children
+// the next group, the rest of the current group going onto the stack.
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
+                                            uint32_t target_entity, trace_counters& cnt) {
+    const local_ray lr = make_local_ray(o_w, d_w);
+    wt_setup wt;
+    if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+    trav_state st;
+    st.best.prim = RBW_MISS;
+    st.best.t = 0.0f;
+    st.best.b1 = st.best.b2 = 0.0f;
+    st.best_inst_entity = RBW_NONE;
+    st.len_w = len_w;
+    st.thr_local = len_w * lr.L0;
+    const f3 o = lr.o;
+    const f3 inv = lr.inv;
+    const f3 oi = f3{o.x * inv.x, o.y * inv.y, o.z * inv.z};
+    const uint32_t oct = (inv.x < 0.0f ? 4u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 1u : 0u);
+    uint64_t stack[RBW_STACK];
+    int sp = 0;
+    // the root as a group of one inner child
+    uint32_t g_base = S.root8, g_imask = 1u << oct, g_pending = 1u;
+    for (;;) {
+        if (g_pending == 0u) {
+            if (sp == 0) break;
+            const uint64_t e = stack[--sp];
+            g_base = (uint32_t)e;
+            g_imask = (uint32_t)(e >> 32) & 0xFFu;
+            g_pending = (uint32_t)(e >> 40) & 0xFFu;
+            continue;
+        }
+        const uint32_t j = (uint32_t)__builtin_ctz(g_pending);
+        g_pending &= g_pending - 1u;
+        const uint32_t slot = j ^ oct;
+        const uint32_t node = g_base + (uint32_t)__builtin_popcount(g_imask & ((1u << slot) - 1u));
+        if (g_pending) stack[sp++] = (uint64_t)g_base | ((uint64_t)g_imask << 32) | ((uint64_t)g_pending << 40);
+        const float4* n = S.nodes8 + 5 * (size_t)node;
+        const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3], n4 = n[4];
+        if (COUNT) cnt.nodes++;
+        const uint32_t ew = __float_as_uint(n0.w);
+        const float sx = __uint_as_float((ew & 0xFFu) << 23), sy = __uint_as_float(((ew >> 8) & 0xFFu) << 23),
+                    sz = __uint_as_float(((ew >> 16) & 0xFFu) << 23);
+        const uint32_t imask = ew >> 24;
+        const uint32_t child_base = __float_as_uint(n1.x), prim_base = __float_as_uint(n1.y);
+        const uint32_t meta0 = __float_as_uint(n1.z), meta1 = __float_as_uint(n1.w);
+        const uint32_t qlx[2] = {__float_as_uint(n2.x), __float_as_uint(n2.y)};
+        const uint32_t qly[2] = {__float_as_uint(n2.z), __float_as_uint(n2.w)};
+        const uint32_t qlz[2] = {__float_as_uint(n3.x), __float_as_uint(n3.y)};
+        const uint32_t qhx[2] = {__float_as_uint(n3.z), __float_as_uint(n3.w)};
+        const uint32_t qhy[2] = {__float_as_uint(n4.x), __float_as_uint(n4.y)};
+        const uint32_t qhz[2] = {__float_as_uint(n4.z), __float_as_uint(n4.w)};
+        const float tmax = st.thr_local * 1.0000002f;
+        uint32_t inner_hits = 0u, leaf_hits = 0u;
+        float tnear[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int w = c >> 2, sh = (c & 3) * 8;
+            // child box p + q s (q s is exact), then the slab distances as in the binary walk
+            const float bx0 = __builtin_fmaf((float)((qlx[w] >> sh) & 0xFFu), sx, n0.x);
+            const float bx1 = __builtin_fmaf((float)((qhx[w] >> sh) & 0xFFu), sx, n0.x);
+            const float by0 = __builtin_fmaf((float)((qly[w] >> sh) & 0xFFu), sy, n0.y);
+            const float by1 = __builtin_fmaf((float)((qhy[w] >> sh) & 0xFFu), sy, n0.y);
+            const float bz0 = __builtin_fmaf((float)((qlz[w] >> sh) & 0xFFu), sz, n0.z);
+            const float bz1 = __builtin_fmaf((float)((qhz[w] >> sh) & 0xFFu), sz, n0.z);
+            const float tx0 = __builtin_fmaf(bx0, inv.x, -oi.x), tx1 = __builtin_fmaf(bx1, inv.x, -oi.x);
+            const float ty0 = __builtin_fmaf(by0, inv.y, -oi.y), ty1 = __builtin_fmaf(by1, inv.y, -oi.y);
+            const float tz0 = __builtin_fmaf(bz0, inv.z, -oi.z), tz1 = __builtin_fmaf(bz1, inv.z, -oi.z);
+            const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+            const bool hit = tmin <= tfar * 1.0000004f;
+            tnear[c] = tmin;
+            const bool inner = (imask >> c) & 1u;
+            const bool leaf = (((w ? meta1 : meta0) >> sh) & (uint32_t)k_meta_leaf) != 0u;
+            const uint32_t jbit = 1u << ((uint32_t)c ^ oct);
+            inner_hits |= (hit && inner) ? jbit : 0u;
+            leaf_hits |= (hit && leaf) ? jbit : 0u;
+        }
+        // hit leaves nearest box first; a hit in one may already be nearer than the next boxes
+        while (leaf_hits) {
+            uint32_t ls = 0u;
+            float tl = 3.4e38f;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const bool cand = (leaf_hits >> ((uint32_t)c ^ oct)) & 1u;
+                if (cand && tnear[c] < tl) {
+                    tl = tnear[c];
+                    ls = (uint32_t)c;
+                }
+            }
+            leaf_hits &= ~(1u << (ls ^ oct));
+            if (tl > st.thr_local * 1.0000004f) break;
+            const uint32_t meta = ((ls & 4u ? meta1 : meta0) >> ((ls & 3u) * 8u)) & 0xFFu;
+            const uint32_t first = prim_base + (meta & 31u), count = ((meta >> 5) & 3u) + 1u;
+            if (test_leaf<VISIBLE_ONLY, ANY_HIT, COUNT>(S, S.prim8_a, S.prim8_b, S.prim8_c, first, count, lr, o_w, wt,
+                                                        tri_test, target_entity, cnt, st))
+                return st.best;
+        }
+        g_base = child_base;
+        g_imask = imask;
+        g_pending = inner_hits;
+    }
+    return st.best;
+}
+
+// W: the BVH width the kernel is instantiated for (each kernel carries one walk)
+template <int W, bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
+                                           uint32_t target_entity, trace_counters& cnt) {
+    if (W == 8) return traverse8<VISIBLE_ONLY, ANY_HIT, COUNT>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
+    return traverse2<VISIBLE_ONLY, ANY_HIT, COUNT>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
 }
 
 // ------------------------------------------------------------------ wave helpers
@@ -370,11 +508,11 @@ __device__ __forceinline__ void raygen_path(const dcamera& cam, const dparams& p
 }
 
 // closest hit of the path ray (scene::intersect, scene.cpp:82-95)
-template <bool COUNT>
+template <int W, bool COUNT>
 __device__ __forceinline__ void extend_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                             trace_counters& cnt) {
     const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
-    const hitrec h = traverse<false, false, COUNT>(S, o, d, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+    const hitrec h = traverse<W, false, false, COUNT>(S, o, d, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
     P.U(U_HPRIM, slot) = h.prim;
     P.F(F_HT, slot) = h.t;
     if (uint32_t* dbg = debug_slot(P, prm, slot, (int)P.U(U_BOUNCE, slot))) dbg[0] = h.prim;
@@ -530,7 +668,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
 
 // shadow ray, MIS ray and light pdf of next-event estimation for the path in `slot`;
 // value += beta * L (integrator.cpp:101-201)
-template <bool COUNT>
+template <int W, bool COUNT>
 __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                              trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
                                              uint32_t& nlight) {
@@ -545,9 +683,9 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const float slen = P.F(F_SLEN, slot);
         hitrec h;
         if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
-            h = traverse<true, true, COUNT>(S, so, sd, slen, prm.tri_test, lid, cnt);
+            h = traverse<W, true, true, COUNT>(S, so, sd, slen, prm.tri_test, lid, cnt);
         else
-            h = traverse<true, false, COUNT>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
+            h = traverse<W, true, false, COUNT>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
         if (dbg) dbg[1] = h.prim;
         bool unoccluded = h.prim == RBW_MISS;
         if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT) {
@@ -562,7 +700,7 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
         f3 eo, ed;
         spawn_ray(mp, mn, mw, eo, ed);
-        const hitrec h = traverse<true, false, COUNT>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+        const hitrec h = traverse<W, true, false, COUNT>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
         if (dbg) dbg[2] = h.prim;
         if (h.prim != RBW_MISS) {
             uint32_t he;

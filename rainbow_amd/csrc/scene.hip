@@ -33,10 +33,50 @@ inline uint32_t word_of(const float4& v) {
     return w;
 }
 
+// leaf slots in BVH order: (p0 | sphere centre, id word), (p1 | radius), p2
+void fill_slots(const rbw_scene_desc* d, const host_scene& h, const std::vector<uint32_t>& order,
+                std::vector<float4>& pa, std::vector<float4>& pb, std::vector<float4>& pc) {
+    const uint32_t nt = d->num_triangles;
+    const size_t nslots = order.size();
+    pa.resize(nslots);
+    pb.resize(nslots);
+    pc.resize(nslots);
+    for (size_t sl = 0; sl < nslots; sl++) {
+        const uint32_t prim = order[sl];
+        uint32_t word = prim;
+        if (prim < nt) {
+            const dentity& E = h.ents[d->triangle_entity[prim]];
+            if (E.material == RBW_NONE) word |= k_prim_invisible;
+            if (E.count > k_instanced_threshold) word |= k_prim_instanced;
+            const float* p0 = d->positions + 3 * size_t(d->indices[3 * prim]);
+            const float* p1 = d->positions + 3 * size_t(d->indices[3 * prim + 1]);
+            const float* p2 = d->positions + 3 * size_t(d->indices[3 * prim + 2]);
+            pa[sl] = with_word(p0[0], p0[1], p0[2], word);
+            pb[sl] = make_float4(p1[0], p1[1], p1[2], 0.0f);
+            pc[sl] = make_float4(p2[0], p2[1], p2[2], 0.0f);
+        } else {
+            const uint32_t s = prim - nt;
+            const dentity& E = h.ents[d->sphere_entity[s]];
+            word |= k_prim_sphere;
+            if (E.material == RBW_NONE) word |= k_prim_invisible;
+            pa[sl] = with_word(d->spheres[4 * s], d->spheres[4 * s + 1], d->spheres[4 * s + 2], word);
+            pb[sl] = make_float4(d->spheres[4 * s + 3], 0.0f, 0.0f, 0.0f);
+            pc[sl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
 }  // namespace
 
+uint32_t resolve_width(uint32_t requested) {
+    if (requested == 0) return k_default_width;
+    require(requested == 2 || requested == 8, RBW_ERR_INVALID_ARGUMENT, "bvh_width must be 0, 2 or 8");
+    return requested;
+}
+
 size_t host_scene::bytes() const {
-    return nodes.size() * sizeof(float4) + 3 * pa.size() * sizeof(float4) + (pos.size() + nrm.size() + uv.size()) * 4 +
+    return (nodes.size() + nodes8.size()) * sizeof(float4) + 3 * (pa.size() + pa8.size()) * sizeof(float4) +
+           (pos.size() + nrm.size() + uv.size()) * 4 +
            (idx.size() + tri_entity.size() + tri_material.size() + sphere_entity.size() + emitters.size()) * 4 +
            spheres.size() * sizeof(float4) + ents.size() * sizeof(dentity) + mats.size() * sizeof(dmaterial);
 }
@@ -161,35 +201,16 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
     bvh_build_result bvh;
     build_bvh(boxes, bvh, 4);
     require(bvh.depth < k_stack_depth, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    const size_t nslots = bvh.order.size();
-    h.pa.resize(nslots);
-    h.pb.resize(nslots);
-    h.pc.resize(nslots);
-    for (size_t sl = 0; sl < nslots; sl++) {
-        const uint32_t prim = bvh.order[sl];
-        uint32_t word = prim;
-        if (prim < nt) {
-            const dentity& E = h.ents[d->triangle_entity[prim]];
-            if (E.material == RBW_NONE) word |= k_prim_invisible;
-            if (E.count > k_instanced_threshold) word |= k_prim_instanced;
-            const float* p0 = d->positions + 3 * size_t(d->indices[3 * prim]);
-            const float* p1 = d->positions + 3 * size_t(d->indices[3 * prim + 1]);
-            const float* p2 = d->positions + 3 * size_t(d->indices[3 * prim + 2]);
-            h.pa[sl] = with_word(p0[0], p0[1], p0[2], word);
-            h.pb[sl] = make_float4(p1[0], p1[1], p1[2], 0.0f);
-            h.pc[sl] = make_float4(p2[0], p2[1], p2[2], 0.0f);
-        } else {
-            const uint32_t s = prim - nt;
-            const dentity& E = h.ents[d->sphere_entity[s]];
-            word |= k_prim_sphere;
-            if (E.material == RBW_NONE) word |= k_prim_invisible;
-            h.pa[sl] = with_word(d->spheres[4 * s], d->spheres[4 * s + 1], d->spheres[4 * s + 2], word);
-            h.pb[sl] = make_float4(d->spheres[4 * s + 3], 0.0f, 0.0f, 0.0f);
-            h.pc[sl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-    }
+    bvh8_result bvh8;
+    build_bvh8(bvh, bvh8);
+    require(bvh8.depth < k_stack_depth, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    fill_slots(d, h, bvh.order, h.pa, h.pb, h.pc);
+    fill_slots(d, h, bvh8.order, h.pa8, h.pb8, h.pc8);
     h.nodes.resize(bvh.nodes2.size() * 4);
     std::memcpy(h.nodes.data(), bvh.nodes2.data(), bvh.nodes2.size() * sizeof(bvh2_node));
+    h.nodes8.resize(bvh8.nodes.size() * 5);
+    std::memcpy(h.nodes8.data(), bvh8.nodes.data(), bvh8.nodes.size() * sizeof(bvh8_node));
+    h.depth8 = bvh8.depth;
     h.mats.resize(d->num_materials);
     for (uint32_t i = 0; i < d->num_materials; i++) {
         const rbw_material& m = d->materials[i];
@@ -223,9 +244,9 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
 }
 
 void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, std::vector<uint32_t>& roots,
-                  std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count) {
+                  std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count) {
     u = host_scene();
-    uint64_t T = 0, Sp = 0, V = 0, N = 0, L = 0, E = 0, M = 0;
+    uint64_t T = 0, Sp = 0, V = 0, N = 0, L = 0, E = 0, M = 0, N8 = 0, L8 = 0;
     bool any_nrm = false, any_uv = false, any_tm = false;
     for (const host_scene* p : parts) {
         T += p->nt;
@@ -233,6 +254,8 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         V += p->nv;
         N += p->nodes.size() / 4;
         L += p->pa.size();
+        N8 += p->nodes8.size() / 5;
+        L8 += p->pa8.size();
         E += p->ents.size();
         M += p->mats.size();
         any_nrm |= p->has_nrm;
@@ -241,8 +264,9 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
     }
     require(T + Sp < (1ull << 29), RBW_ERR_UNSUPPORTED, "merged scene has too many primitives");
     require(L < (1ull << 27), RBW_ERR_UNSUPPORTED, "merged scene has too many leaf slots");
-    require(N < (1ull << 31) && V < (1ull << 32) && E < (1ull << 31) && M < (1ull << 31), RBW_ERR_UNSUPPORTED,
-            "merged scene too large");
+    require(N < (1ull << 31) && N8 < (1ull << 32) && L8 < (1ull << 32) && V < (1ull << 32) && E < (1ull << 31) &&
+                M < (1ull << 31),
+            RBW_ERR_UNSUPPORTED, "merged scene too large");
     u.nt = uint32_t(T);
     u.ns = uint32_t(Sp);
     u.nv = uint32_t(V);
@@ -263,14 +287,20 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
     u.sphere_entity.reserve(Sp);
     u.ents.reserve(E);
     u.mats.reserve(M);
+    u.nodes8.reserve(N8 * 5);
+    u.pa8.reserve(L8);
+    u.pb8.reserve(L8);
+    u.pc8.reserve(L8);
     roots.clear();
+    roots8.clear();
     em_first.clear();
     em_count.clear();
-    uint32_t tb = 0, sb = 0, vb = 0, nb = 0, lb = 0, eb = 0, mb = 0;
+    uint32_t tb = 0, sb = 0, vb = 0, nb = 0, lb = 0, eb = 0, mb = 0, n8b = 0, l8b = 0;
     u.smin = f3{3.4e38f, 3.4e38f, 3.4e38f};
     u.smax = f3{-3.4e38f, -3.4e38f, -3.4e38f};
     for (const host_scene* p : parts) {
         roots.push_back(nb);
+        roots8.push_back(n8b);
         em_first.push_back(uint32_t(u.emitters.size()));
         em_count.push_back(uint32_t(p->emitters.size()));
         // nodes: inner children + nb, leaves' first slot + lb
@@ -292,16 +322,32 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
             std::memcpy(q, &nd, sizeof(nd));
             for (int k = 0; k < 4; k++) u.nodes.push_back(q[k]);
         }
-        // leaf slots: primitive ids into the merged id space (triangles first, then spheres)
-        for (size_t sl = 0; sl < p->pa.size(); sl++) {
-            const uint32_t w = word_of(p->pa[sl]);
-            const uint32_t flags = w & ~k_prim_id_mask;
-            const uint32_t id = w & k_prim_id_mask;
-            const uint32_t nid = (w & k_prim_sphere) ? uint32_t(T) + sb + (id - p->nt) : tb + id;
-            u.pa.push_back(with_word(p->pa[sl].x, p->pa[sl].y, p->pa[sl].z, flags | nid));
-            u.pb.push_back(p->pb[sl]);
-            u.pc.push_back(p->pc[sl]);
+        // wide nodes: first inner child + n8b, first leaf slot + l8b
+        for (size_t i = 0; i < p->nodes8.size(); i += 5) {
+            bvh8_node nd;
+            std::memcpy(&nd, &p->nodes8[i], sizeof(nd));
+            nd.child_base += n8b;
+            nd.prim_base += l8b;
+            float4 q[5];
+            std::memcpy(q, &nd, sizeof(nd));
+            for (int k = 0; k < 5; k++) u.nodes8.push_back(q[k]);
         }
+        // leaf slots: primitive ids into the merged id space (triangles first, then spheres)
+        auto rebase_slots = [&](const std::vector<float4>& pa, const std::vector<float4>& pb,
+                                const std::vector<float4>& pc, std::vector<float4>& oa, std::vector<float4>& ob,
+                                std::vector<float4>& oc) {
+            for (size_t sl = 0; sl < pa.size(); sl++) {
+                const uint32_t w = word_of(pa[sl]);
+                const uint32_t flags = w & ~k_prim_id_mask;
+                const uint32_t id = w & k_prim_id_mask;
+                const uint32_t nid = (w & k_prim_sphere) ? uint32_t(T) + sb + (id - p->nt) : tb + id;
+                oa.push_back(with_word(pa[sl].x, pa[sl].y, pa[sl].z, flags | nid));
+                ob.push_back(pb[sl]);
+                oc.push_back(pc[sl]);
+            }
+        };
+        rebase_slots(p->pa, p->pb, p->pc, u.pa, u.pb, u.pc);
+        rebase_slots(p->pa8, p->pb8, p->pc8, u.pa8, u.pb8, u.pc8);
         u.pos.insert(u.pos.end(), p->pos.begin(), p->pos.end());
         if (any_nrm) {
             if (p->has_nrm) u.nrm.insert(u.nrm.end(), p->nrm.begin(), p->nrm.end());
@@ -330,6 +376,7 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         u.mats.insert(u.mats.end(), p->mats.begin(), p->mats.end());
         for (uint32_t e : p->emitters) u.emitters.push_back(e + eb);
         u.depth = std::max(u.depth, p->depth);
+        u.depth8 = std::max(u.depth8, p->depth8);
         u.smin = f3{std::min(u.smin.x, p->smin.x), std::min(u.smin.y, p->smin.y), std::min(u.smin.z, p->smin.z)};
         u.smax = f3{std::max(u.smax.x, p->smax.x), std::max(u.smax.y, p->smax.y), std::max(u.smax.z, p->smax.z)};
         tb += p->nt;
@@ -337,6 +384,8 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         vb += p->nv;
         nb += uint32_t(p->nodes.size() / 4);
         lb += uint32_t(p->pa.size());
+        n8b += uint32_t(p->nodes8.size() / 5);
+        l8b += uint32_t(p->pa8.size());
         eb += uint32_t(p->ents.size());
         mb += uint32_t(p->mats.size());
     }
@@ -348,6 +397,10 @@ void dev_scene::upload(const host_scene& h, hipStream_t s) {
     prim_a.upload(h.pa.data(), h.pa.size(), s);
     prim_b.upload(h.pb.data(), h.pb.size(), s);
     prim_c.upload(h.pc.data(), h.pc.size(), s);
+    nodes8.upload(h.nodes8.data(), h.nodes8.size(), s);
+    prim8_a.upload(h.pa8.data(), h.pa8.size(), s);
+    prim8_b.upload(h.pb8.data(), h.pb8.size(), s);
+    prim8_c.upload(h.pc8.data(), h.pc8.size(), s);
     pos.upload(h.pos.data(), h.pos.size(), s);
     if (h.has_nrm) nrm.upload(h.nrm.data(), h.nrm.size(), s);
     else nrm.ensure(4);
@@ -369,7 +422,7 @@ void dev_scene::upload(const host_scene& h, hipStream_t s) {
     valid = true;
 }
 
-dscene dev_scene::view() const {
+dscene dev_scene::view(uint32_t width) const {
     dscene S;
     S.nodes = nodes.as<float4>();
     S.prim_a = prim_a.as<float4>();
@@ -390,6 +443,12 @@ dscene dev_scene::view() const {
     S.num_tris = nt;
     S.num_spheres = ns;
     S.root = 0;
+    S.nodes8 = nodes8.as<float4>();
+    S.prim8_a = prim8_a.as<float4>();
+    S.prim8_b = prim8_b.as<float4>();
+    S.prim8_c = prim8_c.as<float4>();
+    S.root8 = 0;
+    S.width = width;
     return S;
 }
 

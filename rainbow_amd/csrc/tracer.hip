@@ -47,13 +47,13 @@ __global__ void k_raygen(dscene S, dpaths P, dparams prm, dcamera cam, uint32_t 
     if (i == 0) *queue_count = n;
 }
 
-template <bool COUNT>
+template <int W, bool COUNT>
 __global__ void k_extend(dscene S, dpaths P, dparams prm, const uint32_t* queue, const uint32_t* queue_count,
                          unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *queue_count;
     trace_counters cnt{0, 0};
-    if (q < count) extend_path<COUNT>(S, P, prm, queue[q], cnt);
+    if (q < count) extend_path<W, COUNT>(S, P, prm, queue[q], cnt);
     if (COUNT) {
         wave_add64(counters + 0, cnt.nodes);
         wave_add64(counters + 1, cnt.prims);
@@ -330,7 +330,7 @@ __global__ void k_collect(dpaths P, dparams prm, uint32_t n) {
 }
 
 // shadow ray, MIS ray and light pdf of next-event estimation; value += beta * L
-template <bool COUNT>
+template <int W, bool COUNT>
 __global__ void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq, const uint32_t* cq_count,
                           unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,7 +338,7 @@ __global__ void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq, c
     trace_counters cnt{0, 0};
     uint32_t nshadow = 0, nmis = 0, nlight = 0;
     if (q < count) {
-        connect_path<COUNT>(S, P, prm, cq[q], cnt, nshadow, nmis, nlight);
+        connect_path<W, COUNT>(S, P, prm, cq[q], cnt, nshadow, nmis, nlight);
     }
     wave_add64(counters + 2, nshadow);
     wave_add64(counters + 3, nmis);
@@ -402,6 +402,7 @@ __global__ void k_film_resolve(dcamera cam, dfilm film, float* rgb) {
 }
 
 // scene::intersect / intersect_with_shadow_ray for caller-given rays
+template <int W>
 __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* d, const float* tmax, uint32_t visible,
                              uint32_t tri_test, rbw_hit* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,8 +410,8 @@ __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* 
     const f3 ro = f3{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
     const f3 rd = f3{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
     trace_counters cnt{0, 0};
-    const hitrec h = visible ? traverse<true, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt)
-                             : traverse<false, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt);
+    const hitrec h = visible ? traverse<W, true, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt)
+                             : traverse<W, false, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt);
     rbw_hit r;
     r.prim = h.prim;
     r.entity = h.prim == RBW_MISS ? RBW_NONE
@@ -635,7 +636,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             dp.g.t_cap = tcap;
         }
     }
-    const dscene S = ctx->main.view();
+    const dscene S = ctx->main.view(resolve_width(pr.bvh_width));
     dpaths P;
     P.f = ctx->pf.as<float>();
     P.u = ctx->pu.as<uint32_t>();
@@ -677,12 +678,12 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             HIP_CHECK(hipMemsetAsync(counts + cout, 0, 4, s));
             HIP_CHECK(hipMemsetAsync(counts + 2, 0, 4, s));
             if (timed) ctx->timer.begin(RBW_KERNEL_EXTEND, s);
-            if (count)
-                hipLaunchKernelGGL(k_extend<true>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin,
-                                   counts + cin, counters);
-            else
-                hipLaunchKernelGGL(k_extend<false>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin,
-                                   counts + cin, counters);
+            {
+                auto kx = S.width == 8 ? (count ? k_extend<8, true> : k_extend<8, false>)
+                                       : (count ? k_extend<2, true> : k_extend<2, false>);
+                hipLaunchKernelGGL(kx, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin, counts + cin,
+                                   counters);
+            }
             if (timed) ctx->timer.end(s);
             if (timed) ctx->timer.begin(RBW_KERNEL_SHADE, s);
             if (guided) {
@@ -710,12 +711,12 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 }
             }
             if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
-            if (count)
-                hipLaunchKernelGGL(k_connect<true>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, cqp,
-                                   counts + 2, counters);
-            else
-                hipLaunchKernelGGL(k_connect<false>, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, cqp,
-                                   counts + 2, counters);
+            {
+                auto kc = S.width == 8 ? (count ? k_connect<8, true> : k_connect<8, false>)
+                                       : (count ? k_connect<2, true> : k_connect<2, false>);
+                hipLaunchKernelGGL(kc, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, cqp, counts + 2,
+                                   counters);
+            }
             if (timed) ctx->timer.end(s);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(ctx->host_count, counts + cout, 4, hipMemcpyDeviceToHost, s));
@@ -871,13 +872,14 @@ rbw_status rbw_render(rbw_ctx* ctx, const rbw_camera_desc* camera, const rbw_ren
 }
 
 rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const float* directions, const float* tmax,
-                          uint32_t visible_only, uint32_t triangle_test, rbw_hit* out) {
+                          uint32_t visible_only, uint32_t triangle_test, uint32_t bvh_width, rbw_hit* out) {
     return guarded([&] {
         require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
         require(ctx->has_scene, RBW_ERR_NO_SCENE, "no scene uploaded");
         require(n == 0 || (origins && directions && tmax && out), RBW_ERR_INVALID_ARGUMENT, "null ray arrays");
         require(triangle_test == RBW_TRI_MOLLER_TRUMBORE || triangle_test == RBW_TRI_WATERTIGHT,
                 RBW_ERR_INVALID_ARGUMENT, "unknown triangle test");
+        const uint32_t width = resolve_width(bvh_width);
         if (n == 0) return;
         HIP_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
@@ -887,7 +889,8 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const 
             d.upload(directions, size_t(n) * 3, s);
             t.upload(tmax, n, s);
             h.ensure(size_t(n) * sizeof(rbw_hit));
-            hipLaunchKernelGGL(k_trace_rays, dim3(blocks(n, 256)), dim3(256), 0, s, ctx->main.view(), n, o.as<float>(),
+            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256), 0, s,
+                               ctx->main.view(width), n, o.as<float>(),
                                d.as<float>(), t.as<float>(), visible_only, triangle_test, h.as<rbw_hit>());
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(out, h.p, size_t(n) * sizeof(rbw_hit), hipMemcpyDeviceToHost, s));

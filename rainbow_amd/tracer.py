@@ -79,13 +79,14 @@ class Tracer:
             res["debug_hits"] = dbg.reshape(H * W * params.spp, params.max_depth, 3)
         return res
 
-    def trace(self, origins, directions, tmax, visible_only=False, triangle_test=abi.RBW_TRI_MOLLER_TRUMBORE):
+    def trace(self, origins, directions, tmax, visible_only=False, triangle_test=abi.RBW_TRI_MOLLER_TRUMBORE,
+              bvh_width=0):
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tmax, np.float32).reshape(-1)
         out = np.zeros(len(o), abi.HIT_DTYPE)
         _check(self.lib.rbw_trace_rays(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p),
-                                       abi.ptr(t, abi.f32p), int(visible_only), int(triangle_test),
+                                       abi.ptr(t, abi.f32p), int(visible_only), int(triangle_test), int(bvh_width),
                                        out.ctypes.data_as(C.c_void_p)))
         return out
 

@@ -47,12 +47,13 @@ def cornell(tracer):
     return sc, cam, orc.OracleScene(sc)
 
 
+@pytest.mark.parametrize("width", [2, 8])
 @pytest.mark.parametrize("visible", [False, True])
-def test_trace_rays_cornell(tracer, cornell, visible):
+def test_trace_rays_cornell(tracer, cornell, visible, width):
     sc, cam, os_ = cornell
     o, d = random_rays(200_000, 1 + visible)
     t = np.full(len(o), np.inf, np.float32)
-    g = tracer.trace(o, d, t, visible_only=visible)
+    g = tracer.trace(o, d, t, visible_only=visible, bvh_width=width)
     c = os_.trace(o, d, t, visible_only=visible)
     mism = np.count_nonzero(g["prim"] != c["prim"])
     print("trace mismatches", mism, "of", len(o))
@@ -104,12 +105,13 @@ def test_render_cornell_replay(tracer, cornell):
     assert frac < 1e-3
 
 
-def test_render_cornell_per_sample_pcg(tracer, cornell):
+@pytest.mark.parametrize("width", [2, 8])
+def test_render_cornell_per_sample_pcg(tracer, cornell, width):
     sc, cam, os_ = cornell
     orc = _oracle()
     p = orc.render_params(spp=16, max_depth=5, seed=3)
     c = os_.render(cam, p, sampler=orc.ORC_SAMPLER_PER_SAMPLE, debug=True)
-    g = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3), debug=True)
+    g = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3, bvh_width=width), debug=True)
     frac, idf = _compare_render(g, c, "per-sample pcg")
     assert idf < 1e-5
     assert frac < 1e-3
@@ -132,6 +134,17 @@ def icosphere(tracer):
     tracer.upload(sc)
     orc = _oracle()
     return sc, cam, orc.OracleScene(sc)
+
+
+def test_render_icosphere_widths_agree(tracer, icosphere):
+    """the persistent-slot render with the binary and the eight-wide BVH: the same image
+    up to the rare ties of equal-distance hits (shared edges) that the walk order breaks"""
+    sc, cam, os_ = icosphere
+    a = tracer.render(cam, render_params(spp=4, max_depth=8, seed=11, bvh_width=2))["image"]
+    b = tracer.render(cam, render_params(spp=4, max_depth=8, seed=11, bvh_width=8))["image"]
+    frac = pixel_mismatch(a, b).mean()
+    print("pixels differing between widths", frac)
+    assert frac < 1e-3
 
 
 def test_render_icosphere_window(tracer, icosphere):
