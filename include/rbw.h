@@ -137,9 +137,13 @@ enum {
 /* Learned path guiding (no counterpart in the reference): the continuation
  * sample of every vertex with a BSDF is drawn from the MLP's vMF mixture with
  * probability mis_fraction, else from the BSDF, and weighted by the one-sample
- * MIS density mis_fraction * q + (1 - mis_fraction) * p_bsdf (unbiased).  With
- * train set, the context's network is trained online between passes on the
- * radiance the render's own paths carried back along their sampled directions. */
+ * MIS density p_G = mis_fraction * q + (1 - mis_fraction) * p_bsdf (unbiased).
+ * The continuation doubles as the second technique of next-event estimation:
+ * the light sample is weighted by the power heuristic against p_G and emission
+ * the continuation reaches by the power heuristic against the light pdf (the
+ * reference's separate MIS ray is not traced).  With train set, the context's
+ * network is trained online between passes on the radiance the render's own
+ * paths carried back along their continuation directions. */
 typedef struct rbw_guiding_opts {
     uint32_t enable;
     float mis_fraction;              /* probability of sampling the guide, e.g. 0.5 */

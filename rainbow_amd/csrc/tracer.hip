@@ -1,14 +1,17 @@
 // Wavefront path tracer for MI355X (gfx950) behind the C ABI of include/rbw.h.
 //
-// One render = passes of (pixel, sample) paths held in SoA state in HBM.
-// Each bounce runs three kernels over compacted queues:
+// This file holds the pass schedule and the C ABI.  A render in passes holds
+// (pixel, sample) paths in SoA state in HBM; each bounce runs three kernels
+// over compacted queues:
 //   extend  : closest-hit BVH traversal of the path rays
 //   shade   : surface interaction, emission, material, light and BSDF samples
 //             for next-event estimation, continuation sample, Russian roulette
 //   connect : the shadow ray and the MIS BSDF ray of next-event estimation,
 //             the light's own pdf cast, and the MIS-weighted contribution
-// then film kernels accumulate the pass into the film in sample order.
-// The arithmetic follows rainbow-cpu's path integrator
+// then film kernels accumulate the pass into the film.  Replay of recorded
+// draws, debug hits, visit counting and guided rendering use it; the plain
+// estimator runs on the persistent-slot schedule of batch.hip.  The per-path
+// stages live in path.hpp and follow rainbow-cpu's path integrator
 // (integrators/path_integrator.cpp:15-47, integrators/integrator.cpp:101-404).
 #include <hip/hip_runtime.h>
 

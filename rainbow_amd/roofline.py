@@ -3,7 +3,8 @@
 Per unit of work each kernel must move at least these bytes between HBM and
 the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
 
-  node visit      64 B   one bvh2 node (two child boxes + child refs)
+  node visit      64 B   one bvh2 node (two child boxes + child refs); 80 B for a
+                         bvh8 node (eight 8-bit quantised child boxes)
   primitive test  48 B   one leaf slot (three float4: p0|id, p1, p2)
   extend ray      36 B   queue index 4 + ray origin/direction 24 (read)
                          + hit prim/t 8 (write)
@@ -19,6 +20,7 @@ the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
   light cast      96 B   two light triangles' vertex positions and indices
 """
 S_NODE = 64
+S_NODE8 = 80
 S_PRIM = 48
 B_EXTEND_RAY = 36
 B_SHADE_PATH = 264
@@ -28,8 +30,10 @@ B_LIGHT_CAST = 96
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 
-def kernel_bytes(stats):
-    """Algorithmic bytes of a render per kernel kind, from an instrumented run's counters."""
+def kernel_bytes(stats, width=2):
+    """Algorithmic bytes of a render per kernel kind, from an instrumented run's counters
+    (width: the BVH the instrumented run walked)."""
+    node = S_NODE8 if width == 8 else S_NODE
     ext_nodes = stats["extend_node_visits"]
     ext_prims = stats["extend_prim_tests"]
     con_nodes = stats["node_visits"] - ext_nodes
@@ -37,8 +41,8 @@ def kernel_bytes(stats):
     ext_rays = stats["extension_rays"]
     records = max(stats["shadow_rays"], stats["mis_rays"])
     return {
-        "extend": ext_rays * B_EXTEND_RAY + ext_nodes * S_NODE + ext_prims * S_PRIM,
+        "extend": ext_rays * B_EXTEND_RAY + ext_nodes * node + ext_prims * S_PRIM,
         "shade": ext_rays * B_SHADE_PATH,
-        "connect": records * B_CONNECT_REC + con_nodes * S_NODE + con_prims * S_PRIM
+        "connect": records * B_CONNECT_REC + con_nodes * node + con_prims * S_PRIM
         + stats["light_casts"] * B_LIGHT_CAST,
     }
