@@ -152,6 +152,8 @@ typedef struct rbw_guiding_opts {
     uint32_t batch;                  /* training samples per Adam step (<= 2^20), e.g. 2^18 */
     uint32_t steps_per_pass;         /* Adam steps after each pass */
     float scene_min[3], scene_max[3];/* position normalisation; all zero = scene bounds */
+    uint32_t light_records;          /* also train on unoccluded light samples (their MIS-weighted
+                                        radiance), so the guide learns all incident light */
 } rbw_guiding_opts;
 
 typedef struct rbw_render_params {

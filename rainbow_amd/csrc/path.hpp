@@ -362,6 +362,7 @@ struct dguide {
     float* t_c;                  // radiance / pdf
     uint32_t* t_count;
     uint32_t t_cap;
+    uint32_t light_records;      // also train on unoccluded light samples
 };
 
 struct dparams {
@@ -671,7 +672,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
 template <int W, bool COUNT>
 __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                              trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
-                                             uint32_t& nlight) {
+                                             uint32_t& nlight, bool* lit = nullptr) {
     const uint32_t flags = P.U(U_CFLAGS, slot);
     const int bounce = (int)(flags >> 8);
     uint32_t* dbg = debug_slot(P, prm, slot, bounce);
@@ -693,6 +694,7 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
             unoccluded = he == lid;
         }
         if (unoccluded) L = L + P.F3(F_SFX, slot);
+        if (lit) *lit = unoccluded;
     }
     if (flags & C_MIS) {
         nmis = 1;

@@ -228,6 +228,11 @@ __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, 
                 P.setF3(F_SDX, slot, sd);
                 P.F(F_SLEN, slot) = slen;
                 P.setF3(F_SFX, slot, f * ls.L * w / ls.pdf);
+                if (G.train && G.light_records) {
+                    // training record of the light sample: its MIS-weighted radiance / pdf
+                    P.F(F_MFX, slot) = w * (0.2126f * ls.L.x + 0.7152f * ls.L.y + 0.0722f * ls.L.z) / ls.pdf;
+                    P.F(F_MFY, slot) = __uint_as_float(g);
+                }
                 P.U(U_SENT, slot) = lid;
                 P.setF3(F_CBX, slot, beta);
                 P.U(U_CFLAGS, slot) = C_SHADOW | ((uint32_t)bounce << 8);
@@ -340,8 +345,26 @@ __global__ void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq, c
     const uint32_t count = *cq_count;
     trace_counters cnt{0, 0};
     uint32_t nshadow = 0, nmis = 0, nlight = 0;
+    bool lit = false;
+    uint32_t slot = 0;
     if (q < count) {
-        connect_path<W, COUNT>(S, P, prm, cq[q], cnt, nshadow, nmis, nlight);
+        slot = cq[q];
+        connect_path<W, COUNT>(S, P, prm, slot, cnt, nshadow, nmis, nlight, &lit);
+    }
+    if (prm.g.train && prm.g.light_records) {
+        // unoccluded guided light samples become training records (query row, direction, weight)
+        const float c = lit ? P.F(F_MFX, slot) : 0.0f;
+        const bool take = lit && c > 0.0f && isfinite(c);
+        const uint32_t k = wave_append(prm.g.t_count, take);
+        if (take && k < prm.g.t_cap) {
+            const uint32_t g = __float_as_uint(P.F(F_MFY, slot));
+            for (int j = 0; j < 9; j++) prm.g.t_in[(size_t)k * 9 + j] = prm.g.query[(size_t)g * 9 + j];
+            const f3 d = P.F3(F_SDX, slot);
+            prm.g.t_dir[(size_t)k * 3 + 0] = d.x;
+            prm.g.t_dir[(size_t)k * 3 + 1] = d.y;
+            prm.g.t_dir[(size_t)k * 3 + 2] = d.z;
+            prm.g.t_c[k] = c;
+        }
     }
     wave_add64(counters + 2, nshadow);
     wave_add64(counters + 3, nmis);
@@ -597,6 +620,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         dp.g.enable = 1;
         dp.g.alpha = go->mis_fraction;
         dp.g.train = go->train ? 1u : 0u;
+        dp.g.light_records = go->light_records ? 1u : 0u;
         f3 smin = ctx->scene_min, smax = ctx->scene_max;
         if (go->scene_min[0] != 0.0f || go->scene_min[1] != 0.0f || go->scene_min[2] != 0.0f ||
             go->scene_max[0] != 0.0f || go->scene_max[1] != 0.0f || go->scene_max[2] != 0.0f) {
