@@ -68,17 +68,48 @@ __device__ __forceinline__ h4 encode4(const float* x, int f0) {
 
 __device__ __forceinline__ h4 load_h4(const _Float16* p) { return *reinterpret_cast<const h4*>(p); }
 
-// Forward pass of 16-row chunks.  STORE writes every layer's input (training) for
-// all rows up to the padded count; rows >= n use the last valid input row.
+// LDS copy of the half weights: rows padded so that the 16 rows one MFMA B
+// fragment reads fall into distinct banks (row pitch of 114 dwords for layer 0,
+// 34 dwords for the others)
+constexpr int W0_PITCH = N_FEAT_PAD + 4;  // halfs
+constexpr int WH_PITCH = HIDDEN + 4;      // halfs
+__host__ __device__ constexpr int lds_w_offset(int l) {
+    return l == 0 ? 0 : HIDDEN * W0_PITCH + (l - 1) * HIDDEN * WH_PITCH;
+}
+constexpr int LDS_W_HALFS = lds_w_offset(N_LAYERS - 1) + N_OUT_PAD * WH_PITCH;  // 34,944 halfs
+constexpr int FWD_WAVES = 4;
+
+// Forward pass.  A block stages every layer's half weights in LDS once, then its
+// four waves run 16-row tiles through the network (one wave per tile), persistent
+// over the rows: the B fragments of all MFMAs come from LDS.  STORE writes every
+// layer's input (training) for all rows up to the padded count; rows >= n use the
+// last valid input row.
 template <bool STORE>
-__global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w16, const float* __restrict__ w32,
-                                                 const float* __restrict__ inputs, uint32_t n, uint32_t n_pad,
-                                                 float* __restrict__ out, _Float16* __restrict__ acts, sample_map map) {
-    __shared__ _Float16 tile[4][16][TILE_PAD];
+__global__ __launch_bounds__(FWD_WAVES * 64) void k_forward(const _Float16* __restrict__ w16,
+                                                            const float* __restrict__ w32,
+                                                            const float* __restrict__ inputs, uint32_t n,
+                                                            uint32_t n_pad, float* __restrict__ out,
+                                                            _Float16* __restrict__ acts, sample_map map) {
+    extern __shared__ __align__(16) _Float16 lds[];
+    _Float16* wl = lds;                                  // LDS_W_HALFS
+    _Float16* tiles = lds + LDS_W_HALFS;                 // FWD_WAVES x 16 x TILE_PAD
+    // stage the weights: 4 halfs per thread and step, row by row
+    for (int l = 0; l < N_LAYERS; l++) {
+        const int in = layer_in(l), outs = layer_out(l), pitch = l == 0 ? W0_PITCH : WH_PITCH;
+        const _Float16* src = w16 + layer_w_offset(l);
+        _Float16* dst = wl + lds_w_offset(l);
+        const int quads = outs * in / 4;
+        for (int q = threadIdx.x; q < quads; q += blockDim.x) {
+            const int row = (q * 4) / in, col = (q * 4) % in;
+            *reinterpret_cast<h4*>(dst + row * pitch + col) = load_h4(src + row * in + col);
+        }
+    }
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const uint32_t nwaves = gridDim.x * 4;
-    for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
+    _Float16* tile = tiles + wave * 16 * TILE_PAD;
+    const uint32_t nwaves = gridDim.x * FWD_WAVES;
+    for (uint32_t chunk = blockIdx.x * FWD_WAVES + wave; chunk * 16 < n_pad; chunk += nwaves) {
         const uint32_t r0 = chunk * 16;
         const uint32_t arow = r0 + li;  // the row this lane feeds as A operand
         const uint32_t src = map(arow < n ? arow : n - 1);
@@ -89,17 +120,17 @@ __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w1
         // ---- layer 0: encoded inputs (224) -> 64
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        const _Float16* W0 = w16 + layer_w_offset(0);
-        for (int s = 0; s < N_FEAT_PAD / 16; s++) {
-            const h4 a = encode4(x, 16 * s + 4 * lg);
-            if (STORE) *reinterpret_cast<h4*>(acts + (size_t)arow * ACT_STRIDE + 16 * s + 4 * lg) = a;
+        const _Float16* W0 = wl + lds_w_offset(0);
+        for (int s2 = 0; s2 < N_FEAT_PAD / 16; s2++) {
+            const h4 a = encode4(x, 16 * s2 + 4 * lg);
+            if (STORE) *reinterpret_cast<h4*>(acts + (size_t)arow * ACT_STRIDE + 16 * s2 + 4 * lg) = a;
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                acc[t] = mfma16(a, load_h4(W0 + (16 * t + li) * N_FEAT_PAD + 16 * s + 4 * lg), acc[t]);
+                acc[t] = mfma16(a, load_h4(W0 + (16 * t + li) * W0_PITCH + 16 * s2 + 4 * lg), acc[t]);
         }
         // ---- hidden layers 1..4 and the output layer 5
         for (int l = 1; l < N_LAYERS; l++) {
-            // bias + ReLU of layer l-1, to the LDS tile (and the activation store)
+            // bias + ReLU of layer l-1, to the wave's LDS tile (and the activation store)
             const float* bprev = w32 + layer_b_offset(l - 1);
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -108,7 +139,7 @@ __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w1
                 for (int j = 0; j < 4; j++) {
                     const float z = acc[t][j] + b;
                     const _Float16 h = (_Float16)(z > 0.0f ? z : 0.0f);
-                    tile[wave][4 * lg + j][16 * t + li] = h;
+                    tile[(4 * lg + j) * TILE_PAD + 16 * t + li] = h;
                     if (STORE) acts[(size_t)(r0 + 4 * lg + j) * ACT_STRIDE + act_offset(l) + 16 * t + li] = h;
                 }
             }
@@ -116,15 +147,15 @@ __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w1
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int nt = layer_out(l) / 16;
-            const _Float16* Wl = w16 + layer_w_offset(l);
+            const _Float16* Wl = wl + lds_w_offset(l);
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int s = 0; s < HIDDEN / 16; s++) {
-                const h4 a = load_h4(&tile[wave][li][16 * s + 4 * lg]);
+            for (int s2 = 0; s2 < HIDDEN / 16; s2++) {
+                const h4 a = load_h4(&tile[li * TILE_PAD + 16 * s2 + 4 * lg]);
 #pragma unroll
                 for (int t = 0; t < 4; t++)
-                    if (t < nt) acc[t] = mfma16(a, load_h4(Wl + (16 * t + li) * HIDDEN + 16 * s + 4 * lg), acc[t]);
+                    if (t < nt) acc[t] = mfma16(a, load_h4(Wl + (16 * t + li) * WH_PITCH + 16 * s2 + 4 * lg), acc[t]);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -143,6 +174,8 @@ __global__ __launch_bounds__(256) void k_forward(const _Float16* __restrict__ w1
         }
     }
 }
+
+constexpr size_t FWD_LDS_BYTES = (size_t(LDS_W_HALFS) + size_t(FWD_WAVES) * 16 * TILE_PAD) * sizeof(_Float16);
 
 __device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
 
@@ -406,6 +439,24 @@ void model_ensure(model& M, uint32_t rows) {
     M.cap = pad;
 }
 
+// the forward kernel's dynamic LDS (78 KiB) is above the default limit of a launch
+static void allow_forward_lds() {
+    static bool done = false;
+    if (done) return;
+    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forward<false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(FWD_LDS_BYTES)));
+    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forward<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(FWD_LDS_BYTES)));
+    done = true;
+}
+
+// persistent forward blocks: two fit a CU's 160 KiB of LDS
+static uint32_t forward_blocks(uint32_t n_pad) {
+    const uint32_t tiles = n_pad / 16;
+    const uint32_t want = (tiles + FWD_WAVES - 1) / FWD_WAVES;
+    return want < 512 ? (want ? want : 1) : 512;
+}
+
 static uint32_t grid_waves(uint32_t n_pad) {
     const uint32_t chunks = n_pad / 16;
     const uint32_t blocks = (chunks + 3) / 4;
@@ -417,7 +468,9 @@ void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t 
     const uint32_t n_pad = (n + 15) & ~15u;
     sample_map id;
     id.n_src = n;
-    hipLaunchKernelGGL(k_forward<false>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
+    allow_forward_lds();
+    hipLaunchKernelGGL(k_forward<false>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
+                       M.w32, inputs, n, n_pad,
                        out, nullptr, id);
     GCHECK(hipGetLastError());
 }
@@ -454,7 +507,9 @@ void train_step(model& M, const float* inputs, const float* dirs, const float* w
     _Float16* dzA = M.dz;
     _Float16* dzB = M.dz + (size_t)n_pad * HIDDEN;
     GCHECK(hipMemsetAsync(M.loss, 0, sizeof(float) * 4, s));
-    hipLaunchKernelGGL(k_forward<true>, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, M.w32, inputs, n, n_pad,
+    allow_forward_lds();
+    hipLaunchKernelGGL(k_forward<true>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
+                       M.w32, inputs, n, n_pad,
                        M.out, M.acts, map);
     hipLaunchKernelGGL(k_sum, dim3((n + 255) / 256), dim3(256), 0, s, weights, n, map, M.loss + 1);
     hipLaunchKernelGGL(k_mixture_grad, dim3((n_pad + 255) / 256), dim3(256), 0, s, M.out, dirs, weights,
