@@ -265,7 +265,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths
         dscene Sj;
         dparams pj;
         job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-        extend_path<W, false>(Sj, P, pj, slot, cnt);
+        extend_path<W, false, true>(Sj, P, pj, slot, cnt);
         rays = 1;
     }
     wave_add64(B.counters + 0, rays);
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpath
             dscene Sj;
             dparams pj;
             job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-            connect_path<W, false>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
+            connect_path<W, false, true>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
             st &= ~ST_CONNECT;
             P.U(U_STATE, slot) = st;
         }
@@ -507,6 +507,8 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total);
     HIP_CHECK(hipGetLastError());
 
+    // the walks' LDS stacks
+    const size_t trav_lds = lds_stack_bytes(bs);
     // per-kernel HIP events only without graph capture (kernel_ms of rbw_stats)
     kernel_timer* tm = cfg.graph ? nullptr : timer;
     auto iteration = [&]() {
@@ -515,14 +517,15 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         hipLaunchKernelGGL(k_b_regen, dim3(ngroups), dim3(k_group), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
-        hipLaunchKernelGGL(S.width == 8 ? k_b_extend<8> : k_b_extend<2>, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        hipLaunchKernelGGL(S.width == 8 ? k_b_extend<8> : k_b_extend<2>, dim3(grid), dim3(bs), trav_lds, s, S, P,
+                           base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_SHADE, s);
         hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
-        hipLaunchKernelGGL(S.width == 8 ? k_b_connect<8> : k_b_connect<2>, dim3(grid), dim3(bs), 0, s, S, P, base,
-                           B);
+        hipLaunchKernelGGL(S.width == 8 ? k_b_connect<8> : k_b_connect<2>, dim3(grid), dim3(bs), trav_lds, s, S, P,
+                           base, B);
         if (tm) tm->end(s);
     };
     if (b.exec) {

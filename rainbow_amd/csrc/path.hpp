@@ -17,6 +17,15 @@ namespace rbw {
 
 #define RBW_STACK 64
 
+// Both walks keep the top of each lane's stack in LDS, older entries in private
+// memory: RBW_LDS_STACK 32-bit entries of the binary walk, or RBW_LDS_STACK / 2
+// two-word groups of the wide walk.  Word e of lane t sits at
+// rbw_lds_stack[e * blockDim.x + t] (conflict-free).  A kernel instantiated with
+// LDS = true must be launched with lds_stack_bytes(blockDim.x) of dynamic LDS.
+constexpr int RBW_LDS_STACK = 16;
+extern __shared__ uint32_t rbw_lds_stack[];
+inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * block * sizeof(uint32_t); }
+
 // ------------------------------------------------------------------ traversal
 
 // acceptance bound of the next candidate: inside the same instanced mesh the
@@ -92,7 +101,7 @@ __device__ __forceinline__ bool test_leaf(const dscene& S, const float4* __restr
 
 // closest hit (or closest visible hit) through the binary BVH: near child first,
 // the far child on a private stack
-template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS>
 __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
                                             uint32_t target_entity, trace_counters& cnt) {
     const local_ray lr = make_local_ray(o_w, d_w);
@@ -107,6 +116,28 @@ __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, flo
     st.thr_local = len_w * lr.L0;
     uint32_t stack[RBW_STACK];
     int sp = 0;
+    uint32_t* ring = rbw_lds_stack + threadIdx.x;
+    const uint32_t ring_stride = blockDim.x;
+    auto push = [&](uint32_t v) {
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * ring_stride;
+            if (sp >= RBW_LDS_STACK) stack[sp - RBW_LDS_STACK] = *e;  // the oldest ring entry moves out
+            *e = v;
+        } else {
+            stack[sp] = v;
+        }
+        sp++;
+    };
+    auto pop = [&]() -> uint32_t {
+        sp--;
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * ring_stride;
+            const uint32_t v = *e;
+            if (sp >= RBW_LDS_STACK) *e = stack[sp - RBW_LDS_STACK];  // and comes back
+            return v;
+        }
+        return stack[sp];
+    };
     int32_t cur = (int32_t)S.root;
     const f3 o = lr.o;
     const f3 inv = lr.inv;
@@ -135,7 +166,7 @@ __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, flo
             if (hl && hr) {
                 const bool lfirst = lmin <= rmin;
                 cur = lfirst ? ch.x : ch.y;
-                stack[sp++] = (uint32_t)(lfirst ? ch.y : ch.x);
+                push((uint32_t)(lfirst ? ch.y : ch.x));
                 continue;
             } else if (hl) {
                 cur = ch.x;
@@ -152,7 +183,7 @@ __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, flo
                 return st.best;
         }
         if (sp == 0) break;
-        cur = (int32_t)stack[--sp];
+        cur = (int32_t)pop();
     }
     return st.best;
 }
@@ -162,7 +193,7 @@ __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, flo
 // order j, child slot j ^ octant); a visited node tests its eight child boxes,
 // tests hit leaves at once in visiting order and makes its hit inner children
 // the next group, the rest of the current group going onto the stack.
-template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS>
 __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
                                             uint32_t target_entity, trace_counters& cnt) {
     const local_ray lr = make_local_ray(o_w, d_w);
@@ -181,12 +212,40 @@ __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, flo
     const uint32_t oct = (inv.x < 0.0f ? 4u : 0u) | (inv.y < 0.0f ? 2u : 0u) | (inv.z < 0.0f ? 1u : 0u);
     uint64_t stack[RBW_STACK];
     int sp = 0;
+    constexpr int K8 = RBW_LDS_STACK / 2;
+    uint32_t* ring = rbw_lds_stack + threadIdx.x;
+    const uint32_t ring_stride = blockDim.x;
+    auto push = [&](uint64_t v) {
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (K8 - 1)) * 2u * ring_stride;
+            if (sp >= K8) stack[sp - K8] = (uint64_t)e[0] | ((uint64_t)e[ring_stride] << 32);
+            e[0] = (uint32_t)v;
+            e[ring_stride] = (uint32_t)(v >> 32);
+        } else {
+            stack[sp] = v;
+        }
+        sp++;
+    };
+    auto pop = [&]() -> uint64_t {
+        sp--;
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (K8 - 1)) * 2u * ring_stride;
+            const uint64_t v = (uint64_t)e[0] | ((uint64_t)e[ring_stride] << 32);
+            if (sp >= K8) {
+                const uint64_t o = stack[sp - K8];
+                e[0] = (uint32_t)o;
+                e[ring_stride] = (uint32_t)(o >> 32);
+            }
+            return v;
+        }
+        return stack[sp];
+    };
     // the root as a group of one inner child
     uint32_t g_base = S.root8, g_imask = 1u << oct, g_pending = 1u;
     for (;;) {
         if (g_pending == 0u) {
             if (sp == 0) break;
-            const uint64_t e = stack[--sp];
+            const uint64_t e = pop();
             g_base = (uint32_t)e;
             g_imask = (uint32_t)(e >> 32) & 0xFFu;
             g_pending = (uint32_t)(e >> 40) & 0xFFu;
@@ -196,7 +255,7 @@ __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, flo
         g_pending &= g_pending - 1u;
         const uint32_t slot = j ^ oct;
         const uint32_t node = g_base + (uint32_t)__builtin_popcount(g_imask & ((1u << slot) - 1u));
-        if (g_pending) stack[sp++] = (uint64_t)g_base | ((uint64_t)g_imask << 32) | ((uint64_t)g_pending << 40);
+        if (g_pending) push((uint64_t)g_base | ((uint64_t)g_imask << 32) | ((uint64_t)g_pending << 40));
         const float4* n = S.nodes8 + 5 * (size_t)node;
         const float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3], n4 = n[4];
         if (COUNT) cnt.nodes++;
@@ -266,11 +325,12 @@ __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, flo
 }
 
 // W: the BVH width the kernel is instantiated for (each kernel carries one walk)
-template <int W, bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT>
+// LDS: the kernel provides the walk's LDS stack (lds_stack_bytes of dynamic LDS)
+template <int W, bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS = false>
 __device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
                                            uint32_t target_entity, trace_counters& cnt) {
-    if (W == 8) return traverse8<VISIBLE_ONLY, ANY_HIT, COUNT>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
-    return traverse2<VISIBLE_ONLY, ANY_HIT, COUNT>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
+    if (W == 8) return traverse8<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
+    return traverse2<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
 }
 
 // ------------------------------------------------------------------ wave helpers
@@ -509,11 +569,11 @@ __device__ __forceinline__ void raygen_path(const dcamera& cam, const dparams& p
 }
 
 // closest hit of the path ray (scene::intersect, scene.cpp:82-95)
-template <int W, bool COUNT>
+template <int W, bool COUNT, bool LDS = false>
 __device__ __forceinline__ void extend_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                             trace_counters& cnt) {
     const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
-    const hitrec h = traverse<W, false, false, COUNT>(S, o, d, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+    const hitrec h = traverse<W, false, false, COUNT, LDS>(S, o, d, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
     P.U(U_HPRIM, slot) = h.prim;
     P.F(F_HT, slot) = h.t;
     if (uint32_t* dbg = debug_slot(P, prm, slot, (int)P.U(U_BOUNCE, slot))) dbg[0] = h.prim;
@@ -669,7 +729,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
 
 // shadow ray, MIS ray and light pdf of next-event estimation for the path in `slot`;
 // value += beta * L (integrator.cpp:101-201)
-template <int W, bool COUNT>
+template <int W, bool COUNT, bool LDS = false>
 __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                              trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
                                              uint32_t& nlight, bool* lit = nullptr) {
@@ -684,9 +744,9 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const float slen = P.F(F_SLEN, slot);
         hitrec h;
         if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
-            h = traverse<W, true, true, COUNT>(S, so, sd, slen, prm.tri_test, lid, cnt);
+            h = traverse<W, true, true, COUNT, LDS>(S, so, sd, slen, prm.tri_test, lid, cnt);
         else
-            h = traverse<W, true, false, COUNT>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
+            h = traverse<W, true, false, COUNT, LDS>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
         if (dbg) dbg[1] = h.prim;
         bool unoccluded = h.prim == RBW_MISS;
         if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT) {
@@ -702,7 +762,7 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
         f3 eo, ed;
         spawn_ray(mp, mn, mw, eo, ed);
-        const hitrec h = traverse<W, true, false, COUNT>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+        const hitrec h = traverse<W, true, false, COUNT, LDS>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
         if (dbg) dbg[2] = h.prim;
         if (h.prim != RBW_MISS) {
             uint32_t he;

@@ -56,7 +56,7 @@ __global__ void k_extend(dscene S, dpaths P, dparams prm, const uint32_t* queue,
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *queue_count;
     trace_counters cnt{0, 0};
-    if (q < count) extend_path<W, COUNT>(S, P, prm, queue[q], cnt);
+    if (q < count) extend_path<W, COUNT, true>(S, P, prm, queue[q], cnt);
     if (COUNT) {
         wave_add64(counters + 0, cnt.nodes);
         wave_add64(counters + 1, cnt.prims);
@@ -349,7 +349,7 @@ __global__ void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq, c
     uint32_t slot = 0;
     if (q < count) {
         slot = cq[q];
-        connect_path<W, COUNT>(S, P, prm, slot, cnt, nshadow, nmis, nlight, &lit);
+        connect_path<W, COUNT, true>(S, P, prm, slot, cnt, nshadow, nmis, nlight, &lit);
     }
     if (prm.g.train && prm.g.light_records) {
         // unoccluded guided light samples become training records (query row, direction, weight)
@@ -436,8 +436,8 @@ __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* 
     const f3 ro = f3{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
     const f3 rd = f3{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
     trace_counters cnt{0, 0};
-    const hitrec h = visible ? traverse<W, true, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt)
-                             : traverse<W, false, false, false>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt);
+    const hitrec h = visible ? traverse<W, true, false, false, true>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt)
+                             : traverse<W, false, false, false, true>(S, ro, rd, tmax[i], tri_test, RBW_NONE, cnt);
     rbw_hit r;
     r.prim = h.prim;
     r.entity = h.prim == RBW_MISS ? RBW_NONE
@@ -708,8 +708,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             {
                 auto kx = S.width == 8 ? (count ? k_extend<8, true> : k_extend<8, false>)
                                        : (count ? k_extend<2, true> : k_extend<2, false>);
-                hipLaunchKernelGGL(kx, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qin, counts + cin,
-                                   counters);
+                hipLaunchKernelGGL(kx, dim3(blocks(active, BS)), dim3(BS), lds_stack_bytes(BS), s, S,
+                                   P, dp, qin, counts + cin, counters);
             }
             if (timed) ctx->timer.end(s);
             if (timed) ctx->timer.begin(RBW_KERNEL_SHADE, s);
@@ -741,8 +741,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             {
                 auto kc = S.width == 8 ? (count ? k_connect<8, true> : k_connect<8, false>)
                                        : (count ? k_connect<2, true> : k_connect<2, false>);
-                hipLaunchKernelGGL(kc, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, cqp, counts + 2,
-                                   counters);
+                hipLaunchKernelGGL(kc, dim3(blocks(active, BS)), dim3(BS), lds_stack_bytes(BS), s, S,
+                                   P, dp, cqp, counts + 2, counters);
             }
             if (timed) ctx->timer.end(s);
             HIP_CHECK(hipGetLastError());
@@ -916,7 +916,7 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const 
             d.upload(directions, size_t(n) * 3, s);
             t.upload(tmax, n, s);
             h.ensure(size_t(n) * sizeof(rbw_hit));
-            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256), lds_stack_bytes(256), s,
                                ctx->main.view(width), n, o.as<float>(),
                                d.as<float>(), t.as<float>(), visible_only, triangle_test, h.as<rbw_hit>());
             HIP_CHECK(hipGetLastError());
