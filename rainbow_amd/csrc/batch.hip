@@ -288,7 +288,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_shade(dscene S, dpaths 
 // next-event estimation rays; then the block's count of slots the next
 // regeneration will find free (free or finished)
 template <int W>
-__global__ __launch_bounds__(k_batch_max_block) void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
+__global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};

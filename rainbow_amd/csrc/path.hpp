@@ -26,6 +26,10 @@ constexpr int RBW_LDS_STACK = 16;
 extern __shared__ uint32_t rbw_lds_stack[];
 inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * block * sizeof(uint32_t); }
 
+// The shadow/MIS kernels are latency bound on dependent node fetches: six waves per
+// SIMD (at most 80 VGPRs, a few spilled) beat the five their natural 95 VGPRs allow.
+#define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
+
 // ------------------------------------------------------------------ traversal
 
 // acceptance bound of the next candidate: inside the same instanced mesh the

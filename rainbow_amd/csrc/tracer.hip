@@ -339,8 +339,8 @@ __global__ void k_collect(dpaths P, dparams prm, uint32_t n) {
 
 // shadow ray, MIS ray and light pdf of next-event estimation; value += beta * L
 template <int W, bool COUNT>
-__global__ void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq, const uint32_t* cq_count,
-                          unsigned long long* counters) {
+__global__ RBW_CONNECT_OCCUPANCY void k_connect(dscene S, dpaths P, dparams prm, const uint32_t* cq,
+                                                const uint32_t* cq_count, unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *cq_count;
     trace_counters cnt{0, 0};
