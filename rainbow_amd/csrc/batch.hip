@@ -76,7 +76,7 @@ struct dbatch {
     uint32_t nwaves;
     uint32_t* first_iter;           // per job
     uint32_t* last_iter;            // per job
-    unsigned long long* counters;   // extension, shadow, MIS, light casts
+    unsigned long long* counters;   // extension, shadow, MIS, light casts (striped)
 };
 
 // slot state (U_STATE): 0 = free
@@ -139,7 +139,7 @@ __global__ void k_b_init(dbatch B, dpaths P, unsigned long long total) {
         B.first_iter[i] = 0xFFFFFFFFu;
         B.last_iter[i] = 0u;
     }
-    if (i < 4) B.counters[i] = 0;
+    if (i < RBW_STRIPES * RBW_STRIPE_WORDS) B.counters[i] = 0;
     if (i == 0) {
         dctl c;
         c.cursor = 0;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths
         extend_path<W, false, true>(Sj, P, pj, slot, cnt);
         rays = 1;
     }
-    wave_add64(B.counters + 0, rays);
+    stripe_add64(B.counters, 0, rays);
 }
 
 __global__ __launch_bounds__(k_batch_max_block) void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
@@ -305,9 +305,9 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_c
             P.U(U_STATE, slot) = st;
         }
     }
-    wave_add64(B.counters + 1, nshadow);
-    wave_add64(B.counters + 2, nmis);
-    wave_add64(B.counters + 3, nlight);
+    stripe_add64(B.counters, 1, nshadow);
+    stripe_add64(B.counters, 2, nmis);
+    stripe_add64(B.counters, 3, nlight);
     uint32_t nfree = 0;
     block_rank(slot < B.pool && (st == 0u || (st & ST_DYING)), nfree, lds);
     if (threadIdx.x == 0 && nfree) atomicAdd(&B.block_free[blockIdx.x * blockDim.x / k_group], nfree);
@@ -473,7 +473,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     b.wave_job.ensure(size_t(nwaves) * 4);
     b.first_iter.ensure(size_t(nj) * 4);
     b.last_iter.ensure(size_t(nj) * 4);
-    b.counters.ensure(4 * 8);
+    b.counters.ensure(RBW_STRIPES * RBW_STRIPE_WORDS * 8);
     if (!b.host_ctl) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.host_ctl), sizeof(dctl), hipHostMallocDefault));
 
     dbatch B;
@@ -503,7 +503,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     // grid: one thread per slot of the pool
     const uint32_t grid = nblocks;
 
-    const uint32_t init_n = std::max(cfg.pool, nj);
+    const uint32_t init_n = std::max(std::max(cfg.pool, nj), RBW_STRIPES * RBW_STRIPE_WORDS);
     hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total);
     HIP_CHECK(hipGetLastError());
 
@@ -587,7 +587,14 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     float ms = 0.0f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, b.events[launches]));
     rr.ms = ms;
-    HIP_CHECK(hipMemcpy(rr.counters, B.counters, sizeof(rr.counters), hipMemcpyDeviceToHost));
+    {
+        std::vector<unsigned long long> st(RBW_STRIPES * RBW_STRIPE_WORDS);
+        HIP_CHECK(hipMemcpy(st.data(), B.counters, st.size() * 8, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < 4; k++) {
+            rr.counters[k] = 0;
+            for (uint32_t t = 0; t < RBW_STRIPES; t++) rr.counters[k] += st[t * RBW_STRIPE_WORDS + k];
+        }
+    }
     if (want_job_times) {
         rr.launch_ms.resize(launches + 1);
         rr.launch_ms[0] = 0.0;

@@ -23,9 +23,12 @@ namespace guide {
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
-constexpr int ACT_STRIDE = N_FEAT_PAD + 5 * HIDDEN;  // 544 halfs per row: inputs of layers 0..5
-__host__ __device__ constexpr int act_offset(int l) { return l == 0 ? 0 : N_FEAT_PAD + (l - 1) * HIDDEN; }
+// stored activations: the inputs of layers 1..5 (layer 0's input, the encoding, is
+// recomputed from the 9 raw inputs where the backward pass needs it)
+constexpr int ACT_STRIDE = 5 * HIDDEN;  // 320 halfs per row
+__host__ __device__ constexpr int act_offset(int l) { return (l - 1) * HIDDEN; }
 constexpr int TILE_PAD = HIDDEN + 8;  // LDS row pitch (halfs)
 
 #define GCHECK(expr)                                                                                \
@@ -123,7 +126,6 @@ __global__ __launch_bounds__(FWD_WAVES * 64) void k_forward(const _Float16* __re
         const _Float16* W0 = wl + lds_w_offset(0);
         for (int s2 = 0; s2 < N_FEAT_PAD / 16; s2++) {
             const h4 a = encode4(x, 16 * s2 + 4 * lg);
-            if (STORE) *reinterpret_cast<h4*>(acts + (size_t)arow * ACT_STRIDE + 16 * s2 + 4 * lg) = a;
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 acc[t] = mfma16(a, load_h4(W0 + (16 * t + li) * W0_PITCH + 16 * s2 + 4 * lg), acc[t]);
@@ -140,12 +142,20 @@ __global__ __launch_bounds__(FWD_WAVES * 64) void k_forward(const _Float16* __re
                     const float z = acc[t][j] + b;
                     const _Float16 h = (_Float16)(z > 0.0f ? z : 0.0f);
                     tile[(4 * lg + j) * TILE_PAD + 16 * t + li] = h;
-                    if (STORE) acts[(size_t)(r0 + 4 * lg + j) * ACT_STRIDE + act_offset(l) + 16 * t + li] = h;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (STORE) {
+                // the tile's 16 rows x 64 halfs to the activation store, 16 bytes per lane and step
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int id = lane + 64 * h, row = id >> 3, c8 = id & 7;
+                    *reinterpret_cast<h8*>(acts + (size_t)(r0 + row) * ACT_STRIDE + act_offset(l) + 8 * c8) =
+                        *reinterpret_cast<const h8*>(&tile[row * TILE_PAD + 8 * c8]);
+                }
+            }
             const int nt = layer_out(l) / 16;
             const _Float16* Wl = wl + lds_w_offset(l);
 #pragma unroll
@@ -180,7 +190,7 @@ constexpr size_t FWD_LDS_BYTES = (size_t(LDS_W_HALFS) + size_t(FWD_WAVES) * 16 *
 __device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
 
 // d loss / d raw output of the vMF-mixture head for one row; returns log q
-__device__ float mixture_grad(const float* o, const float* d, float scale, float* g) {
+__device__ __forceinline__ float mixture_grad(const float* o, const float* d, float scale, float* g) {
     float w[N_LOBES], vv[N_LOBES], kap[N_LOBES], mu[N_LOBES][3], nm[N_LOBES], cosv[N_LOBES], sig[N_LOBES];
     bool kcap[N_LOBES];
     float amax = -3.4e38f;
@@ -223,21 +233,38 @@ __device__ float mixture_grad(const float* o, const float* d, float scale, float
     return logf(q);
 }
 
-__global__ void k_sum(const float* __restrict__ c, uint32_t n, sample_map map, float* __restrict__ total) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float v = i < n ? c[map(i)] : 0.0f;
+// sum of the batch's weights: SUM_PER_THREAD rows per thread, one atomic per block
+constexpr uint32_t SUM_PER_THREAD = 8;
+__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ c, uint32_t n, sample_map map,
+                                             float* __restrict__ total) {
+    __shared__ float part[4];
+    const uint32_t first = blockIdx.x * blockDim.x * SUM_PER_THREAD + threadIdx.x;
+    float v = 0.0f;
+#pragma unroll
+    for (uint32_t j = 0; j < SUM_PER_THREAD; j++) {
+        const uint32_t i = first + j * blockDim.x;
+        if (i < n) v += c[map(i)];
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0 && v != 0.0f) atomicAdd(total, v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float t = part[0] + part[1] + part[2] + part[3];
+        if (t != 0.0f) atomicAdd(total, t);
+    }
 }
 
 // per row: the clamped, loss-scaled output gradient (FP16, 48 columns) and the loss term
-__global__ void k_mixture_grad(const float* __restrict__ out, const float* __restrict__ dirs,
-                               const float* __restrict__ c, const float* __restrict__ csum, uint32_t n,
-                               uint32_t n_pad, sample_map map, _Float16* __restrict__ dz, float* __restrict__ loss) {
+__global__ __launch_bounds__(256) void k_mixture_grad(const float* __restrict__ out, const float* __restrict__ dirs,
+                                                      const float* __restrict__ c, const float* __restrict__ csum,
+                                                      uint32_t n, uint32_t n_pad, sample_map map,
+                                                      _Float16* __restrict__ dz, float* __restrict__ loss) {
+    __shared__ float part[4];
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     float lterm = 0.0f;
     if (r < n_pad) {
         float g[N_OUT_PAD];
+#pragma unroll
         for (int k = 0; k < N_OUT_PAD; k++) g[k] = 0.0f;
         if (r < n) {
             const float total = *csum;
@@ -245,15 +272,36 @@ __global__ void k_mixture_grad(const float* __restrict__ out, const float* __res
             const uint32_t src = map(r);
             const float wr = cmean > 0.0f ? fminf(c[src] / cmean, 64.0f) : 0.0f;
             if (wr > 0.0f) {
-                const float lq = mixture_grad(out + (size_t)r * N_OUT_PAD, dirs + (size_t)src * 3, wr, g);
+                float o[N_OUT_PAD];
+                const float4* orow = reinterpret_cast<const float4*>(out + (size_t)r * N_OUT_PAD);
+#pragma unroll
+                for (int k = 0; k < N_OUT_PAD / 4; k++) {
+                    const float4 v = orow[k];
+                    o[4 * k] = v.x;
+                    o[4 * k + 1] = v.y;
+                    o[4 * k + 2] = v.z;
+                    o[4 * k + 3] = v.w;
+                }
+                const float lq = mixture_grad(o, dirs + (size_t)src * 3, wr, g);
                 lterm = -(c[src] / total) * lq;
             }
         }
-        for (int k = 0; k < N_OUT_PAD; k++)
-            dz[(size_t)r * N_OUT_PAD + k] = (_Float16)fminf(fmaxf(g[k], -30000.0f), 30000.0f);
+        h8* drow = reinterpret_cast<h8*>(dz + (size_t)r * N_OUT_PAD);
+#pragma unroll
+        for (int k = 0; k < N_OUT_PAD / 8; k++) {
+            h8 v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = (_Float16)fminf(fmaxf(g[8 * k + e], -30000.0f), 30000.0f);
+            drow[k] = v;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off);
-    if ((threadIdx.x & 63) == 0 && lterm != 0.0f) atomicAdd(loss, lterm);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = lterm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float t = part[0] + part[1] + part[2] + part[3];
+        if (t != 0.0f) atomicAdd(loss, t);
+    }
 }
 
 // dZ_{l-1} = (dZ_l W_l) masked by the ReLU of layer l's input
@@ -266,20 +314,27 @@ __global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restric
     const int K = layer_out(l);     // reduction over layer l's outputs
     const int N = layer_in(l);      // 64
     const _Float16* W = w16 + layer_w_offset(l);
+    // the B fragments (W, at most 4 x 4 tiles) stay in registers for all of the wave's rows
+    h4 wb[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                wb[s][t][i] = s < K / 16 ? W[(16 * s + 4 * lg + i) * N + 16 * t + li] : (_Float16)0.0f;
     const uint32_t nwaves = gridDim.x * 4;
     for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
         const uint32_t r0 = chunk * 16;
         f4 acc[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int s = 0; s < K / 16; s++) {
-            const h4 a = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                h4 b;
+        for (int s = 0; s < 4; s++) {
+            if (s < K / 16) {
+                const h4 a = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
 #pragma unroll
-                for (int i = 0; i < 4; i++) b[i] = W[(16 * s + 4 * lg + i) * N + 16 * t + li];
-                acc[t] = mfma16(a, b, acc[t]);
+                for (int t = 0; t < 4; t++) acc[t] = mfma16(a, wb[s][t], acc[t]);
             }
         }
 #pragma unroll
@@ -294,55 +349,119 @@ __global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restric
     }
 }
 
-// grad(W_l)[n][k] += sum_r dZ_l[r][n] * in_l[r][k]; grad(b_l)[n] += sum_r dZ_l[r][n]
-// block: one 16-row slice of n (blockIdx.y) over a chunk of rows (blockIdx.x); the
-// waves share the k tiles; the last k tile is the bias (a column of ones).
-__global__ __launch_bounds__(256) void k_backward_weight(const _Float16* __restrict__ dz, int dz_stride, int l,
-                                                         const _Float16* __restrict__ acts, uint32_t n_pad,
-                                                         uint32_t rows_per_block, float* __restrict__ grad) {
+// Weight gradient of layer L over a block's rows: grad(W)[n][k] = sum_r dZ[r][n] in[r][k],
+// grad(b)[n] = sum_r dZ[r][n].  The rows are the MFMA reduction, so a lane's operand is
+// four consecutive rows of one column: the block stages 64-row slabs of dZ and of the
+// layer input in LDS transposed (row index fastest; pitch 36 dwords, so the 8-byte
+// operand reads of a half-wave hit distinct banks).  Layer 0's input, the frequency
+// encoding, is recomputed from the 9 raw inputs.  Output tiles (N/16 x (K/16 + 1), the
+// last k tile the bias as a column of ones) are dealt to the 4 waves; each block
+// writes its partial sums (weights then biases, the parameter layout of the layer)
+// to part[blockIdx.x], and k_reduce_partials adds them into grad.
+constexpr int BW_ROWS = 64;
+constexpr int BW_PITCH = BW_ROWS + 8;  // halfs
+constexpr uint32_t BW_MAX_BLOCKS = 512;
+__host__ __device__ constexpr int bw_count(int l) { return layer_out(l) * layer_in(l) + layer_out(l); }
+
+template <int L>
+__global__ __launch_bounds__(256) void k_backward_weight(const _Float16* __restrict__ dz,
+                                                         const _Float16* __restrict__ acts,
+                                                         const float* __restrict__ inputs, uint32_t n,
+                                                         uint32_t n_pad, uint32_t rows_per_block, sample_map map,
+                                                         float* __restrict__ part) {
+    constexpr int K = layer_in(L), N = layer_out(L);
+    constexpr int KT = K / 16 + 1, NT = N / 16, T = NT * KT, TPW = (T + 3) / 4;
+    __shared__ __align__(16) _Float16 actT[K * BW_PITCH];
+    __shared__ __align__(16) _Float16 dzT[N * BW_PITCH];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const int K = layer_in(l);
-    const int ktiles = K / 16 + 1;
-    const int tn = blockIdx.y;
+    f4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; j++) acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t rbeg = blockIdx.x * rows_per_block;
     const uint32_t rend = rbeg + rows_per_block < n_pad ? rbeg + rows_per_block : n_pad;
-    f4 acc[4];  // up to 4 k tiles per wave (ktiles <= 15)
+    const h8 zero8 = h8{0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t s0 = rbeg; s0 < rend; s0 += BW_ROWS) {
+        // stage the slab: lane = row, waves take turns over 8-column groups
+        const uint32_t r = s0 + (uint32_t)lane;
+        const bool valid = r < rend;  // rows past the end: zero dZ, no contribution
 #pragma unroll
-    for (int i = 0; i < 4; i++) acc[i] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t r0 = rbeg; r0 < rend; r0 += 16) {
-        h4 a;
+        for (int c8 = wave; c8 < N / 8; c8 += 4) {
+            const h8 v = valid ? *reinterpret_cast<const h8*>(dz + (size_t)r * N + 8 * c8) : zero8;
 #pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = dz[(size_t)(r0 + 4 * lg + i) * dz_stride + 16 * tn + li];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int tk = wave + 4 * q;
-            if (tk >= ktiles) break;
-            h4 b;
-            if (tk == ktiles - 1) {
-                const _Float16 one = (_Float16)(li == 0 ? 1.0f : 0.0f);
-                b[0] = b[1] = b[2] = b[3] = one;
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; i++) b[i] = acts[(size_t)(r0 + 4 * lg + i) * ACT_STRIDE + act_offset(l) + 16 * tk + li];
-            }
-            acc[q] = mfma16(a, b, acc[q]);
+            for (int e = 0; e < 8; e++) dzT[(8 * c8 + e) * BW_PITCH + lane] = v[e];
         }
-    }
+        if (L == 0) {
+            const uint32_t src = map(valid && r < n ? r : n - 1);
+            float x[N_IN];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int tk = wave + 4 * q;
-        if (tk >= ktiles) break;
+            for (int i = 0; i < N_IN; i++) x[i] = inputs[(size_t)src * N_IN + i];
+            for (int q = wave; q < K / 4; q += 4) {
+                const h4 a = encode4(x, 4 * q);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int nrow = 16 * tn + 4 * lg + j;
-            if (tk == ktiles - 1) {
-                if (li == 0) atomicAdd(&grad[layer_b_offset(l) + nrow], acc[q][j]);
-            } else {
-                atomicAdd(&grad[layer_w_offset(l) + nrow * K + 16 * tk + li], acc[q][j]);
+                for (int e = 0; e < 4; e++) actT[(4 * q + e) * BW_PITCH + lane] = a[e];
+            }
+        } else {
+#pragma unroll
+            for (int c8 = wave; c8 < K / 8; c8 += 4) {
+                const h8 v = valid ? *reinterpret_cast<const h8*>(acts + (size_t)r * ACT_STRIDE + act_offset(L) + 8 * c8)
+                                   : zero8;
+#pragma unroll
+                for (int e = 0; e < 8; e++) actT[(8 * c8 + e) * BW_PITCH + lane] = v[e];
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < BW_ROWS / 16; st++) {
+#pragma unroll
+            for (int j = 0; j < TPW; j++) {
+                const int ti = wave + 4 * j;
+                if (ti < T) {
+                    const int tn = ti / KT, tk = ti % KT;
+                    const h4 a = load_h4(&dzT[(16 * tn + li) * BW_PITCH + 16 * st + 4 * lg]);
+                    h4 b;
+                    if (tk == KT - 1) {
+                        const _Float16 one = (_Float16)(li == 0 ? 1.0f : 0.0f);
+                        b = h4{one, one, one, one};
+                    } else {
+                        b = load_h4(&actT[(16 * tk + li) * BW_PITCH + 16 * st + 4 * lg]);
+                    }
+                    acc[j] = mfma16(a, b, acc[j]);
+                }
+            }
+        }
+        __syncthreads();
     }
+    float* dst = part + (size_t)blockIdx.x * bw_count(L);
+#pragma unroll
+    for (int j = 0; j < TPW; j++) {
+        const int ti = wave + 4 * j;
+        if (ti >= T) break;
+        const int tn = ti / KT, tk = ti % KT;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int nrow = 16 * tn + 4 * lg + e;
+            if (tk == KT - 1) {
+                if (li == 0) dst[N * K + nrow] = acc[j][e];
+            } else {
+                dst[nrow * K + 16 * tk + li] = acc[j][e];
+            }
+        }
+    }
+}
+
+// grad[layer_w_offset(l) + p] += sum over blocks of part[b][p]; blockIdx.y takes a
+// group of BW_GROUP blocks
+constexpr uint32_t BW_GROUP = 16;
+__global__ void k_reduce_partials(const float* __restrict__ part, uint32_t nblocks, uint32_t count,
+                                  float* __restrict__ grad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    const uint32_t b0 = blockIdx.y * BW_GROUP;
+    const uint32_t b1 = b0 + BW_GROUP < nblocks ? b0 + BW_GROUP : nblocks;
+    float s = 0.0f;
+    for (uint32_t b = b0; b < b1; b++) s += part[(size_t)b * count + p];
+    atomicAdd(&grad[p], s);
 }
 
 __device__ __forceinline__ bool live_param(uint32_t p) {
@@ -384,7 +503,7 @@ __global__ void k_to_half(const float* __restrict__ w32, _Float16* __restrict__ 
 
 void model_free(model& M) {
     for (void* p : {(void*)M.w16, (void*)M.w32, (void*)M.m, (void*)M.v, (void*)M.grad, (void*)M.loss,
-                    (void*)M.acts, (void*)M.dz, (void*)M.out})
+                    (void*)M.acts, (void*)M.dz, (void*)M.out, (void*)M.part})
         if (p) (void)hipFree(p);
     M = model();
 }
@@ -436,6 +555,7 @@ void model_ensure(model& M, uint32_t rows) {
     GCHECK(hipMalloc(&M.acts, sizeof(_Float16) * (size_t)pad * ACT_STRIDE));
     GCHECK(hipMalloc(&M.dz, sizeof(_Float16) * (size_t)pad * (2 * HIDDEN + N_OUT_PAD)));
     GCHECK(hipMalloc(&M.out, sizeof(float) * (size_t)pad * N_OUT_PAD));
+    if (!M.part) GCHECK(hipMalloc(&M.part, sizeof(float) * (size_t)BW_MAX_BLOCKS * bw_count(0)));
     M.cap = pad;
 }
 
@@ -511,16 +631,46 @@ void train_step(model& M, const float* inputs, const float* dirs, const float* w
     hipLaunchKernelGGL(k_forward<true>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
                        M.w32, inputs, n, n_pad,
                        M.out, M.acts, map);
-    hipLaunchKernelGGL(k_sum, dim3((n + 255) / 256), dim3(256), 0, s, weights, n, map, M.loss + 1);
+    hipLaunchKernelGGL(k_sum, dim3((n + 256 * SUM_PER_THREAD - 1) / (256 * SUM_PER_THREAD)), dim3(256), 0, s, weights,
+                       n, map, M.loss + 1);
     hipLaunchKernelGGL(k_mixture_grad, dim3((n_pad + 255) / 256), dim3(256), 0, s, M.out, dirs, weights,
                        M.loss + 1, n, n_pad, map, dz5, M.loss);
-    const uint32_t rows_per_block = 2048;
+    // weight gradients: about BW_MAX_BLOCKS blocks of whole 64-row slabs
+    uint32_t rows_per_block = (n_pad + BW_MAX_BLOCKS - 1) / BW_MAX_BLOCKS;
+    rows_per_block = (rows_per_block + BW_ROWS - 1) / BW_ROWS * BW_ROWS;
     const uint32_t wblocks = (n_pad + rows_per_block - 1) / rows_per_block;
     const _Float16* dz = dz5;
     int dz_stride = N_OUT_PAD;
     for (int l = N_LAYERS - 1; l >= 0; l--) {
-        hipLaunchKernelGGL(k_backward_weight, dim3(wblocks, layer_out(l) / 16), dim3(256), 0, s, dz, dz_stride, l,
-                           M.acts, n_pad, rows_per_block, M.grad);
+        switch (l) {
+            case 0:
+                hipLaunchKernelGGL(k_backward_weight<0>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+            case 1:
+                hipLaunchKernelGGL(k_backward_weight<1>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+            case 2:
+                hipLaunchKernelGGL(k_backward_weight<2>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+            case 3:
+                hipLaunchKernelGGL(k_backward_weight<3>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+            case 4:
+                hipLaunchKernelGGL(k_backward_weight<4>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+            default:
+                hipLaunchKernelGGL(k_backward_weight<5>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
+                                   rows_per_block, map, M.part);
+                break;
+        }
+        const uint32_t count = (uint32_t)bw_count(l);
+        hipLaunchKernelGGL(k_reduce_partials, dim3((count + 255) / 256, (wblocks + BW_GROUP - 1) / BW_GROUP), dim3(256),
+                           0, s, M.part, wblocks, count, M.grad + layer_w_offset(l));
         if (l > 0) {
             _Float16* nxt = (dz == dzA) ? dzB : dzA;
             hipLaunchKernelGGL(k_backward_data, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, l, dz, dz_stride,

@@ -44,7 +44,8 @@ struct model {
     float* loss = nullptr;     // 1 float
     uint32_t step = 0;
     // activation scratch for training (capacity rows)
-    _Float16* acts = nullptr;  // layer inputs: [224 + 5*64] halfs per row
+    _Float16* acts = nullptr;  // inputs of layers 1..5: [5*64] halfs per row
+    float* part = nullptr;     // per-block partial weight gradients
     _Float16* dz = nullptr;    // gradients: 2 x [64] halfs per row (ping-pong) + output [48]
     float* out = nullptr;      // raw outputs, [48] floats per row
     uint32_t cap = 0;

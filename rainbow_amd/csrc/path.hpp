@@ -359,6 +359,50 @@ __device__ __forceinline__ void wave_add64(unsigned long long* counter, uint32_t
     if (__lane_id() == 0 && x) atomicAdd(counter, x);
 }
 
+// Statistics counters striped over RBW_STRIPES cache lines (RBW_STRIPE_WORDS words
+// apart, picked by block): atomics of many waves on one address serialise in one L2
+// channel.  The host sums the stripes.
+constexpr uint32_t RBW_STRIPES = 64;
+constexpr uint32_t RBW_STRIPE_WORDS = 16;
+__device__ __forceinline__ void stripe_add64(unsigned long long* counters, uint32_t idx, uint32_t v) {
+    wave_add64(counters + (blockIdx.x & (RBW_STRIPES - 1)) * RBW_STRIPE_WORDS + idx, v);
+}
+
+// Reserves n consecutive entries of *counter for this thread, with one atomic per
+// block (a queue's tail is a single address).  Every thread of the block must call
+// it; blockDim.x is a multiple of 64 and at most 1024.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t n) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    uint32_t x = n;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if ((int)lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < nwaves; w++) {
+            const uint32_t c = s_wave[w];
+            s_wave[w] = t;
+            t += c;
+        }
+        s_base = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = s_base + s_wave[wave] + x - n;
+    __syncthreads();  // the next call reuses s_wave and s_base
+    return r;
+}
+
+// block_reserve of one entry where pred holds
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred) {
+    return block_reserve(counter, pred ? 1u : 0u);
+}
+
 // ------------------------------------------------------------------ path state
 
 enum fidx : uint32_t {

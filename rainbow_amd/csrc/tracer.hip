@@ -58,8 +58,8 @@ __global__ void k_extend(dscene S, dpaths P, dparams prm, const uint32_t* queue,
     trace_counters cnt{0, 0};
     if (q < count) extend_path<W, COUNT, true>(S, P, prm, queue[q], cnt);
     if (COUNT) {
-        wave_add64(counters + 0, cnt.nodes);
-        wave_add64(counters + 1, cnt.prims);
+        stripe_add64(counters, 0, cnt.nodes);
+        stripe_add64(counters, 1, cnt.prims);
     }
 }
 
@@ -79,12 +79,12 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
         slot = qin[q];
         shade_path<GUIDED>(S, P, prm, slot, alive, need_connect, to_guide, gsi, guide_mat);
     }
-    const uint32_t a = wave_append(qout_count, alive);
+    const uint32_t a = block_append(qout_count, alive);
     if (alive) qout[a] = slot;
-    const uint32_t c = wave_append(cq_count, need_connect);
+    const uint32_t c = block_append(cq_count, need_connect);
     if (need_connect) cq[c] = slot;
     if (GUIDED) {
-        const uint32_t g = wave_append(prm.g.gq_count, to_guide);
+        const uint32_t g = block_append(prm.g.gq_count, to_guide);
         if (to_guide) {
             const dguide& G = prm.g;
             const size_t cap = G.cap;
@@ -296,24 +296,29 @@ __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, 
         P.U(U_BOUNCE, slot) = (uint32_t)bounce;
         P.rng[slot] = smp.state;
     }
-    const uint32_t ai = wave_append(qout_count, alive);
+    const uint32_t ai = block_append(qout_count, alive);
     if (alive) qout[ai] = slot;
-    const uint32_t ci = wave_append(cq_count, need_connect);
+    const uint32_t ci = block_append(cq_count, need_connect);
     if (need_connect) cq[ci] = slot;
 }
 
-// after a pass: closed training records -> (query, direction, radiance / pdf)
-__global__ void k_collect(dpaths P, dparams prm, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// after a pass: closed training records -> (query, direction, radiance / pdf).  A
+// block takes COLLECT_PER_THREAD records per thread (coalesced) and appends its
+// takes to the training pool with one reservation.
+constexpr uint32_t COLLECT_PER_THREAD = 8;
+__global__ __launch_bounds__(256) void k_collect(dpaths P, dparams prm, uint32_t n) {
     const dguide& G = prm.g;
     const size_t rc = (size_t)P.cap * prm.max_depth;
-    bool take = false;
-    float c = 0.0f;
-    size_t ri = 0;
-    if (i < n * prm.max_depth) {
-        const uint32_t slot = i / prm.max_depth;
-        ri = i;
-        if (G.rec_flag[ri] == 2) {
+    const size_t total = (size_t)n * prm.max_depth;
+    const size_t first = (size_t)blockIdx.x * blockDim.x * COLLECT_PER_THREAD + threadIdx.x;
+    float c[COLLECT_PER_THREAD];
+    uint32_t takes = 0, mask = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < COLLECT_PER_THREAD; j++) {
+        const size_t ri = first + (size_t)j * blockDim.x;
+        c[j] = 0.0f;
+        if (ri < total && G.rec_flag[ri] == 2) {
+            const uint32_t slot = (uint32_t)(ri / prm.max_depth);
             const f3 vf = P.F3(F_VX, slot);
             const f3 m = f3{G.rec[R_MX * rc + ri], G.rec[R_MY * rc + ri], G.rec[R_MZ * rc + ri]};
             const f3 b = f3{G.rec[R_BX * rc + ri], G.rec[R_BY * rc + ri], G.rec[R_BZ * rc + ri]};
@@ -322,18 +327,27 @@ __global__ void k_collect(dpaths P, dparams prm, uint32_t n) {
                         Lz = b.z > 0.0f ? d.z / b.z : 0.0f;
             const float lum = 0.2126f * Lx + 0.7152f * Ly + 0.0722f * Lz;
             const float pdf = G.rec[R_PDF * rc + ri];
-            c = pdf > 0.0f ? lum / pdf : 0.0f;
-            take = c > 0.0f && isfinite(c);
+            c[j] = pdf > 0.0f ? lum / pdf : 0.0f;
+            if (c[j] > 0.0f && isfinite(c[j])) {
+                mask |= 1u << j;
+                takes++;
+            }
         }
-        G.rec_flag[ri] = 0;
+        if (ri < total) G.rec_flag[ri] = 0;
     }
-    const uint32_t k = wave_append(G.t_count, take);
-    if (take && k < G.t_cap) {
-        for (int j = 0; j < 9; j++) G.t_in[(size_t)k * 9 + j] = G.rec[(R_Q0 + j) * rc + ri];
-        G.t_dir[(size_t)k * 3 + 0] = G.rec[R_DX * rc + ri];
-        G.t_dir[(size_t)k * 3 + 1] = G.rec[R_DY * rc + ri];
-        G.t_dir[(size_t)k * 3 + 2] = G.rec[R_DZ * rc + ri];
-        G.t_c[k] = c;
+    uint32_t k = block_reserve(G.t_count, takes);
+#pragma unroll
+    for (uint32_t j = 0; j < COLLECT_PER_THREAD; j++) {
+        if (!((mask >> j) & 1u)) continue;
+        const size_t ri = first + (size_t)j * blockDim.x;
+        if (k < G.t_cap) {
+            for (int q = 0; q < 9; q++) G.t_in[(size_t)k * 9 + q] = G.rec[(R_Q0 + q) * rc + ri];
+            G.t_dir[(size_t)k * 3 + 0] = G.rec[R_DX * rc + ri];
+            G.t_dir[(size_t)k * 3 + 1] = G.rec[R_DY * rc + ri];
+            G.t_dir[(size_t)k * 3 + 2] = G.rec[R_DZ * rc + ri];
+            G.t_c[k] = c[j];
+        }
+        k++;
     }
 }
 
@@ -355,7 +369,7 @@ __global__ RBW_CONNECT_OCCUPANCY void k_connect(dscene S, dpaths P, dparams prm,
         // unoccluded guided light samples become training records (query row, direction, weight)
         const float c = lit ? P.F(F_MFX, slot) : 0.0f;
         const bool take = lit && c > 0.0f && isfinite(c);
-        const uint32_t k = wave_append(prm.g.t_count, take);
+        const uint32_t k = block_append(prm.g.t_count, take);
         if (take && k < prm.g.t_cap) {
             const uint32_t g = __float_as_uint(P.F(F_MFY, slot));
             for (int j = 0; j < 9; j++) prm.g.t_in[(size_t)k * 9 + j] = prm.g.query[(size_t)g * 9 + j];
@@ -366,12 +380,12 @@ __global__ RBW_CONNECT_OCCUPANCY void k_connect(dscene S, dpaths P, dparams prm,
             prm.g.t_c[k] = c;
         }
     }
-    wave_add64(counters + 2, nshadow);
-    wave_add64(counters + 3, nmis);
-    wave_add64(counters + 4, nlight);
+    stripe_add64(counters, 2, nshadow);
+    stripe_add64(counters, 3, nmis);
+    stripe_add64(counters, 4, nlight);
     if (COUNT) {
-        wave_add64(counters + 5, cnt.nodes);
-        wave_add64(counters + 6, cnt.prims);
+        stripe_add64(counters, 5, cnt.nodes);
+        stripe_add64(counters, 6, cnt.prims);
     }
 }
 
@@ -545,8 +559,9 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         return;
     }
     uint32_t k = pr.samples_per_pass;
-    // guided renders train between passes: smaller passes give the network more updates
-    const uint64_t per_pass = guided ? (1u << 18) : (1u << 21);
+    // passes of ~2^21 paths keep every bounce's launches wide enough to hide the
+    // traversal's memory latency (guided renders train once per pass)
+    const uint64_t per_pass = 1u << 21;
     if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(pr.spp, per_pass / npix));
     k = std::min(k, pr.spp);
     while (uint64_t(k) * npix > (1ull << 30) && k > 1) k--;
@@ -563,7 +578,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     }
     ctx->counts.ensure(16 * sizeof(uint32_t));
     HIP_CHECK(hipMemsetAsync(ctx->counts.p, 0, 16 * sizeof(uint32_t), s));
-    ctx->counters.ensure(16 * sizeof(unsigned long long));
+    constexpr size_t n_counter_words = size_t(RBW_STRIPES) * RBW_STRIPE_WORDS;
+    ctx->counters.ensure(n_counter_words * sizeof(unsigned long long));
     ctx->film_sum.ensure(size_t(W) * H * 3 * sizeof(unsigned long long));
     ctx->film_w.ensure(size_t(W) * H * sizeof(uint32_t));
     ctx->film_nan.ensure(size_t(W) * H * sizeof(uint32_t));
@@ -575,7 +591,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     film.sum = ctx->film_sum.as<unsigned long long>();
     film.weight = ctx->film_w.as<uint32_t>();
     film.nan = ctx->film_nan.as<uint32_t>();
-    HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(unsigned long long), s));
+    HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, n_counter_words * sizeof(unsigned long long), s));
     dparams dp;
     dp.spp = pr.spp;
     dp.max_depth = pr.max_depth;
@@ -754,7 +770,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         }
         if (guided && go->train) {
             if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
-            hipLaunchKernelGGL(k_collect, dim3(blocks(uint64_t(n) * pr.max_depth, BS)), dim3(BS), 0, s, P, dp, n);
+            hipLaunchKernelGGL(k_collect, dim3(blocks(uint64_t(n) * pr.max_depth, 256 * COLLECT_PER_THREAD)), dim3(256), 0,
+                               s, P, dp, n);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(ctx->host_count + 2, counts + 4, 4, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -786,11 +803,15 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     if (out_rgb)
         HIP_CHECK(hipMemcpyAsync(out_rgb, ctx->rgb.p, size_t(W) * H * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     if (ndbg) HIP_CHECK(hipMemcpyAsync(pr.debug_hits, ctx->dbg.p, ndbg * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    unsigned long long hc[16];
-    HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> stripes(n_counter_words);
+    HIP_CHECK(hipMemcpyAsync(stripes.data(), counters, n_counter_words * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, s));
     float gloss = 0.0f;
     if (guide_steps) HIP_CHECK(hipMemcpyAsync(&gloss, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    unsigned long long hc[RBW_STRIPE_WORDS] = {};
+    for (uint32_t t = 0; t < RBW_STRIPES; t++)
+        for (uint32_t k = 0; k < RBW_STRIPE_WORDS; k++) hc[k] += stripes[t * RBW_STRIPE_WORDS + k];
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float ms = 0.0f;
