@@ -170,6 +170,28 @@ def batch_leg(n_jobs=64, autotune=True):
     }
 
 
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       "tracer_persistent_bvh2_icosphere_1024_64spp.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the icosphere workload, from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
+    reads for a copy kernel of known size); None when the summary has no such row."""
+    try:
+        lines = open(PROFILE).read().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            try:
+                return int(float(cells[4]) * 1e6)
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,7 +267,8 @@ def main():
         launches = last["kernel_launches"][dom]
         achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(f"k_b_{dom}<2>") if args.workload == "icosphere" else None,
                 "algorithmic_bytes_per_launch": int(kb[dom] / max(1, launches)),
                 "avg_launch_ms": kms[dom] / max(1, launches),
                 "kernel_ms": {k: round(v, 3) for k, v in kms.items()}}
