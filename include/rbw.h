@@ -338,6 +338,23 @@ rbw_status rbw_submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, const 
 rbw_status rbw_autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, rbw_batch_opts* opts,
                              double* probe_ms);
 
+/* ---- film output (host code; no context needed) ----
+ * The film's 8-bit image as film::write makes it (cameras/film.cpp:114-153):
+ * every byte 255, then for each pixel inside [bound_min, bound_max) (NULL: the whole
+ * film) and channel c: clamp(255 * gamma_correct(rgb * scale) + 0.5, 0, 255)
+ * truncated to a byte (gamma_correct: utilities.cpp:33-38), alpha 255.
+ * rgb: width*height*3 floats as rbw_render writes them; out_rgba: width*height*4. */
+rbw_status rbw_film_to_rgba8(const float* rgb, uint32_t width, uint32_t height, const int32_t* bound_min,
+                             const int32_t* bound_max, float scale, uint8_t* out_rgba);
+/* rbw_film_to_rgba8 written as an 8-bit RGBA PNG to `path` (film::write +
+ * file_system::write_image, rainbow-core/file_system.cpp:17-24, which appends
+ * ".png" to the name; this call takes the whole path) */
+rbw_status rbw_write_png(const char* path, const float* rgb, uint32_t width, uint32_t height,
+                         const int32_t* bound_min, const int32_t* bound_max, float scale);
+/* the film's float pixel values as a colour PFM (little-endian, rows bottom-up):
+ * the raw FP32 dump for error measurements; no counterpart in the reference */
+rbw_status rbw_write_pfm(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
 #ifdef __cplusplus
 }
 #endif

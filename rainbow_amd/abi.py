@@ -171,6 +171,11 @@ RBW_SIGNATURES = {
                                     C.POINTER(rbw_job_stats), C.POINTER(rbw_batch_stats)]),
     "rbw_autotune_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),
                                       C.POINTER(C.c_double)]),
+    "rbw_film_to_rgba8": (C.c_int32, [f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                      C.c_float, C.POINTER(C.c_uint8)]),
+    "rbw_write_png": (C.c_int32, [C.c_char_p, f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.c_float]),
+    "rbw_write_pfm": (C.c_int32, [C.c_char_p, f32p, C.c_uint32, C.c_uint32]),
 }
 
 

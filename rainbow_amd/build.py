@@ -17,6 +17,7 @@ HIPCC_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-cont
                "-Wno-unused-variable", "-Wno-unused-result"]
 
 
+SOURCES = ("tracer.hip", "batch.hip", "scene.hip", "guiding.hip", "bvh.cpp", "image.cpp")
 HOST_LIB = os.path.join(HERE, "librbw_host.so")
 TEST_BIN = os.path.join(HERE, "..", "tests", "cpp", "build", "cornell_integrator")
 
@@ -42,9 +43,9 @@ def build_host(verbose=False):
 
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, f) for f in ("tracer.hip", "batch.hip", "scene.hip", "guiding.hip", "bvh.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-shared", "-o", tmp, *srcs]
+    cmd = [hipcc, *HIPCC_FLAGS, "-shared", "-o", tmp, *srcs, "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

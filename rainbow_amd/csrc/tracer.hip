@@ -29,6 +29,7 @@
 #include "bvh.hpp"
 #include "device.hpp"
 #include "guiding.hpp"
+#include "image.hpp"
 #include "host.hpp"
 #include "path.hpp"
 #include "shading.hpp"
@@ -867,6 +868,35 @@ rbw_status rbw_create(const rbw_device_opts* opts, rbw_ctx** out) {
 void rbw_destroy(rbw_ctx* ctx) { delete ctx; }
 
 const char* rbw_last_error(void) { return g_last_error.c_str(); }
+
+rbw_status rbw_film_to_rgba8(const float* rgb, uint32_t width, uint32_t height, const int32_t* bound_min,
+                             const int32_t* bound_max, float scale, uint8_t* out_rgba) {
+    return guarded([&] {
+        require(rgb && out_rgba, RBW_ERR_INVALID_ARGUMENT, "rgb and out_rgba are required");
+        rbw::image::film_to_rgba8(rgb, width, height, bound_min, bound_max, scale, out_rgba);
+    });
+}
+
+rbw_status rbw_write_png(const char* path, const float* rgb, uint32_t width, uint32_t height,
+                         const int32_t* bound_min, const int32_t* bound_max, float scale) {
+    return guarded([&] {
+        require(path && rgb, RBW_ERR_INVALID_ARGUMENT, "path and rgb are required");
+        require(width > 0 && height > 0 && width <= (1u << 16) && height <= (1u << 16), RBW_ERR_INVALID_ARGUMENT,
+                "image size out of range");
+        std::vector<uint8_t> rgba(size_t(width) * height * 4);
+        rbw::image::film_to_rgba8(rgb, width, height, bound_min, bound_max, scale, rgba.data());
+        rbw::image::write_png(path, rgba.data(), width, height);
+    });
+}
+
+rbw_status rbw_write_pfm(const char* path, const float* rgb, uint32_t width, uint32_t height) {
+    return guarded([&] {
+        require(path && rgb, RBW_ERR_INVALID_ARGUMENT, "path and rgb are required");
+        require(width > 0 && height > 0 && width <= (1u << 16) && height <= (1u << 16), RBW_ERR_INVALID_ARGUMENT,
+                "image size out of range");
+        rbw::image::write_pfm(path, rgb, width, height);
+    });
+}
 
 rbw_status rbw_upload_scene(rbw_ctx* ctx, const rbw_scene_desc* scene) {
     return guarded([&] {

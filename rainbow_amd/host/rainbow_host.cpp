@@ -124,14 +124,24 @@ namespace cameras {
 
 film::film(const std::shared_ptr<filters::filter>& filter, const vector2i& resolution, const bound2& crop_window,
            real scale)
-    : mFilter(filter), mResolution(resolution) {
-    (void)scale;
+    : mFilter(filter), mResolution(resolution), mScale(scale) {
     // cameras/film.cpp:186-204
     mPixelsBound.min = vector2i{(int32_t)std::ceil(mResolution.x * crop_window.min.x),
                                 (int32_t)std::ceil(mResolution.y * crop_window.min.y)};
     mPixelsBound.max = vector2i{(int32_t)std::ceil(mResolution.x * crop_window.max.x),
                                 (int32_t)std::ceil(mResolution.y * crop_window.max.y)};
     mPixels.assign(size_t(resolution.x) * resolution.y, spectrum(0));
+}
+
+void film::write(const std::string& file_name) const {
+    std::vector<float> rgb(mPixels.size() * 3);
+    for (size_t i = 0; i < mPixels.size(); i++)
+        for (int c = 0; c < 3; c++) rgb[i * 3 + c] = mPixels[i][c];
+    const int32_t bmin[2] = {mPixelsBound.min.x, mPixelsBound.min.y};
+    const int32_t bmax[2] = {mPixelsBound.max.x, mPixelsBound.max.y};
+    if (rbw_write_png((file_name + ".png").c_str(), rgb.data(), (uint32_t)mResolution.x, (uint32_t)mResolution.y, bmin,
+                      bmax, mScale) != RBW_OK)
+        throw std::runtime_error(rbw_last_error());
 }
 
 perspective_camera::perspective_camera(const std::shared_ptr<cameras::film>& film, const camera_system& system,

@@ -259,6 +259,8 @@ public:
          real scale = 1);
     void set_pixel(size_t index, const spectrum& value) { mPixels[index] = value; }
     spectrum pixel(size_t index) const { return mPixels[index]; }
+    // cameras/film.cpp:114-153: the 8-bit image (scale, sRGB gamma, crop bound) as file_name + ".png"
+    void write(const std::string& file_name) const;
     vector2i resolution() const noexcept { return mResolution; }
     bound2i pixels_bound() const noexcept { return mPixelsBound; }
     std::shared_ptr<filters::filter> filter() const noexcept { return mFilter; }
@@ -267,6 +269,7 @@ private:
     vector2i mResolution;
     bound2i mPixelsBound;
     std::vector<spectrum> mPixels;
+    real mScale = 1;
 };
 
 enum class camera_system : uint32_t { right_hand = 0, left_hand = 1 };

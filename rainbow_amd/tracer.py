@@ -8,6 +8,7 @@ Python or CPU fallback -- a missing library or device is an error.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -193,3 +194,33 @@ def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def _bounds(bound_min, bound_max):
+    bmin = None if bound_min is None else (C.c_int32 * 2)(*bound_min)
+    bmax = None if bound_max is None else (C.c_int32 * 2)(*bound_max)
+    return bmin, bmax
+
+
+def film_to_rgba8(image, bound_min=None, bound_max=None, scale=1.0):
+    """The film's 8-bit RGBA image as film::write makes it (cameras/film.cpp:114-153)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 4), dtype=np.uint8)
+    bmin, bmax = _bounds(bound_min, bound_max)
+    _check(library().rbw_film_to_rgba8(abi.ptr(img, abi.f32p), w, h, bmin, bmax, scale,
+                                       out.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return out
+
+
+def write_png(path, image, bound_min=None, bound_max=None, scale=1.0):
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w = img.shape[:2]
+    bmin, bmax = _bounds(bound_min, bound_max)
+    _check(library().rbw_write_png(os.fsencode(path), abi.ptr(img, abi.f32p), w, h, bmin, bmax, scale))
+
+
+def write_pfm(path, image):
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w = img.shape[:2]
+    _check(library().rbw_write_pfm(os.fsencode(path), abi.ptr(img, abi.f32p), w, h))

@@ -25,7 +25,7 @@ static std::shared_ptr<materials::matte_material> matte(float r, float g, float 
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s flatten|render <out> [spp depth seed]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s flatten|render <out> [spp depth seed [png name]]\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -93,6 +93,7 @@ int main(int argc, char** argv) {
             std::fwrite(p.c.data(), 4, 3, out);
         }
         std::fprintf(stderr, "rendered %.1f Msamples/s\n", integ.last_stats.msamples_per_s);
+        if (argc > 6) film->write(argv[6]);  // argv[6] + ".png"
     }
     std::fclose(out);
     return 0;

@@ -71,8 +71,12 @@ def test_host_integrator_render_matches_c_abi(tracer, tmp_path):
     """wavefront_path_integrator::render(camera, scene) == rbw_render on the Python-built scene."""
     from rainbow_amd.tracer import render_params
     f = tmp_path / "film.bin"
-    subprocess.run([_binary(), "render", str(f), "16", "5", "3"], check=True)
+    subprocess.run([_binary(), "render", str(f), "16", "5", "3", str(tmp_path / "film")], check=True)
     img = np.fromfile(f, np.float32).reshape(256, 256, 3)
+    # film::write: the PNG holds the film's 8-bit conversion
+    from test_image import read_png
+    from rainbow_amd.tracer import film_to_rgba8
+    assert np.array_equal(read_png(tmp_path / "film.png"), film_to_rgba8(img))
     sc, cam = scenes.cornell_box()
     tracer.upload(sc)
     ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3))["image"]
