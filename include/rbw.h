@@ -118,7 +118,13 @@ enum {
      * integrator.cpp:118-165, 380, path_integrator.cpp:43) */
     RBW_SAMPLER_PER_SAMPLE_PCG = 0,
     /* replay of recorded per-sample draws (the reference's per-tile stream) */
-    RBW_SAMPLER_REPLAY = 1
+    RBW_SAMPLER_REPLAY = 1,
+    /* stratified (the contract of stratified_sampler.hpp:48-127: per pixel, each of the
+     * first dims_1d 1D and dims_2d 2D draws of the spp = strata_x*strata_y samples falls in
+     * a distinct stratum, strata shuffled independently per dimension; later draws from
+     * the per-sample pcg32 stream).  A design of this build: strata assigned by a
+     * hash-based permutation and jitter per (seed, pixel, dimension), no tables. */
+    RBW_SAMPLER_STRATIFIED = 2
 };
 
 enum { RBW_TRI_MOLLER_TRUMBORE = 0, RBW_TRI_WATERTIGHT = 1 };
@@ -175,8 +181,11 @@ typedef struct rbw_render_params {
     const rbw_guiding_opts* guiding; /* NULL = plain BSDF continuation (the reference's) */
     uint32_t time_kernels;           /* nonzero: per-kernel HIP-event times in rbw_stats::kernel_ms */
     uint32_t bvh_width;              /* 0 (default: 2), 2 (binary BVH) or 8 (eight-wide BVH) */
+    /* RBW_SAMPLER_STRATIFIED: strata_x * strata_y must equal spp; stratified 1D and 2D
+     * dimensions (at most 65535 each) */
+    uint32_t strata_x, strata_y, dims_1d, dims_2d;
 } rbw_render_params;
-/* Schedules: the plain estimator with the per-sample sampler (no replay, no debug
+/* Schedules: the plain estimator with the per-sample or stratified sampler (no replay, no debug
  * hits, no visit counts, no guiding) runs on persistent path slots regenerated in
  * place (the batched-jobs wavefront with one job, its bounce loop a hipGraph;
  * time_kernels runs it without the graph).  Every other mode runs in passes of
@@ -254,6 +263,13 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
                           const float* directions, const float* tmax,
                           uint32_t visible_only, uint32_t triangle_test,
                           uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
+
+/* Diagnostic: the draws the sampler of `params` (per-sample pcg32 or stratified)
+ * gives the spp samples of pixel p (y*width + x): per sample, n1 1D draws then n2 2D
+ * draws (x, y), out = spp * (n1 + 2*n2) floats.  Replaces sampler::next
+ * (samplers/samplers.hpp:31-33) called outside a render. */
+rbw_status rbw_sampler_draws(rbw_ctx* ctx, const rbw_render_params* params, uint32_t pixel, uint32_t n1,
+                             uint32_t n2, float* out);
 
 /* ---------------------------------------------------------------- guiding
  * The guiding network of a context (no counterpart in the reference).

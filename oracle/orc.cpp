@@ -795,21 +795,83 @@ inline bool scene_intersect_linear(const orc_scene& s, ray& r, bool visible_only
 
 // ---------------------------------------------------------------- sampling source
 
+// The reference's stratified samplers (samplers/detail/stratified_sampler.hpp): a pixel's
+// reset() fills, per stratified dimension, spp jittered stratum values (1D: n strata;
+// 2D: strata_x x strata_y, row by row, x then y jitter) and shuffles them (Fisher-Yates
+// with pcg32::uint32(0, n - i - 1)); the 1D sampler's dimensions first, then the 2D
+// sampler's (sampler_group::reset, integrator.cpp:29-33), all from the tile's shared
+// pcg32 (path_integrator.cpp:50-58).  Sample i's k-th 1D (2D) draw is table[k][i] while
+// k < dims, then random (one, or x then y, draws from the shared generator).
+struct stratified_tables {
+    uint32_t sx = 1, sy = 1, d1 = 0, d2 = 0;
+    std::vector<real> t1;   // d1 x n
+    std::vector<vec2> t2;   // d2 x n
+    uint32_t n() const { return sx * sy; }
+    void fill(pcg32& g) {
+        const uint32_t N = n();
+        const real one = k_one_minus_epsilon;
+        t1.resize(size_t(d1) * N);
+        t2.resize(size_t(d2) * N);
+        for (uint32_t d = 0; d < d1; d++) {  // stratified_sampler_t<1>::stratified_sample, :99-110
+            real* v = &t1[size_t(d) * N];
+            const real inv = static_cast<real>(1) / static_cast<real>(N);
+            for (uint32_t i = 0; i < N; i++) {
+                const real delta = g.uniform_real();
+                const real x = (static_cast<real>(i) + delta) * inv;
+                v[i] = (one < x) ? one : x;
+            }
+            for (uint32_t i = 0; i < N; i++) std::swap(v[i], v[i + g.range(0, N - i - 1)]);  // :112-119
+        }
+        for (uint32_t d = 0; d < d2; d++) {  // stratified_sampler_t<2>::stratified_sample, :121-141
+            vec2* v = &t2[size_t(d) * N];
+            const real dx = static_cast<real>(1) / static_cast<real>(sx);
+            const real dy = static_cast<real>(1) / static_cast<real>(sy);
+            uint32_t i = 0;
+            for (uint32_t y = 0; y < sy; y++)
+                for (uint32_t x = 0; x < sx; x++) {
+                    const real jx = g.uniform_real();
+                    const real jy = g.uniform_real();
+                    const real px = (static_cast<real>(x) + jx) * dx, py = (static_cast<real>(y) + jy) * dy;
+                    v[i++] = vec2((one < px) ? one : px, (one < py) ? one : py);
+                }
+            for (uint32_t k = 0; k < N; k++) std::swap(v[k], v[k + g.range(0, N - k - 1)]);
+        }
+    }
+};
+
 struct sample_stream {
     pcg32 gen;
     const float* replay = nullptr;
     uint64_t pos = 0, end = 0;
     std::vector<float>* record = nullptr;
-    real next() {
+    const stratified_tables* strat = nullptr;
+    uint32_t sample = 0, k1 = 0, k2 = 0;  // stratified: sample index in the pixel, dimensions drawn
+    real raw() {
         real u;
         if (replay) u = pos < end ? replay[pos++] : 0.0f;
         else u = gen.uniform_real();
         if (record) record->push_back(u);
         return u;
     }
-    vec2 next2() {  // random_sampler_t<2>::next: x then y
-        const real a = next();
-        const real b = next();
+    real next() {  // a sampler1d draw
+        if (strat && k1 < strat->d1) {
+            const real u = strat->t1[size_t(k1++) * strat->n() + sample];
+            if (record) record->push_back(u);
+            return u;
+        }
+        return raw();
+    }
+    vec2 next2() {  // a sampler2d draw; random_sampler_t<2>::next: x then y
+        if (strat && k2 < strat->d2) {
+            const vec2 v = strat->t2[size_t(k2++) * strat->n() + sample];
+            if (record) {
+                record->push_back(v.x);
+                record->push_back(v.y);
+            }
+            return v;
+        }
+        const real a = raw();
+        const real b = raw();
         return vec2(a, b);
     }
 };
@@ -1209,6 +1271,37 @@ static int trace_rays_impl(orc_scene* s, uint32_t n, const float* o, const float
     return RBW_OK;
 }
 
+void orc_pcg32_range(uint64_t seed, uint32_t n, const uint32_t* lo, const uint32_t* hi, uint32_t* out) {
+    pcg32 g(seed);
+    for (uint32_t i = 0; i < n; i++) out[i] = g.range(lo[i], hi[i]);
+}
+
+uint64_t orc_stratified_tables(uint64_t seed, uint64_t skip, uint32_t strata_x, uint32_t strata_y, uint32_t dims_1d,
+                               uint32_t dims_2d, float* out_1d, float* out_2d) {
+    pcg32 g(seed);
+    for (uint64_t i = 0; i < skip; i++) g.next_u32();
+    stratified_tables t;
+    t.sx = strata_x;
+    t.sy = strata_y;
+    t.d1 = dims_1d;
+    t.d2 = dims_2d;
+    const pcg32 before = g;
+    t.fill(g);
+    // count the draws by replaying the generator up to its new state
+    pcg32 c = before;
+    uint64_t draws = 0;
+    while (c.state != g.state) {
+        c.next_u32();
+        draws++;
+    }
+    for (size_t i = 0; i < t.t1.size(); i++) out_1d[i] = t.t1[i];
+    for (size_t i = 0; i < t.t2.size(); i++) {
+        out_2d[2 * i] = t.t2[i].x;
+        out_2d[2 * i + 1] = t.t2[i].y;
+    }
+    return draws;
+}
+
 int orc_trace_rays(orc_scene* s, uint32_t n, const float* o, const float* d, const float* tmax, uint32_t vis, rbw_hit* out) {
     return trace_rays_impl(s, n, o, d, tmax, vis, out, false);
 }
@@ -1222,6 +1315,9 @@ int orc_render(orc_scene* s, const rbw_camera_desc* cam, const rbw_render_params
                float* out_rgb, orc_stats* stats) {
     if (!s || !cam || !p || !opts) return RBW_ERR_INVALID_ARGUMENT;
     if (s->emitters.empty()) return RBW_ERR_INVALID_ARGUMENT;  // integrator.cpp:89,121 needs an emitter
+    if (opts->sampler == ORC_SAMPLER_TILE_STRATIFIED &&
+        (opts->strata_x == 0 || opts->strata_y == 0 || opts->strata_x * opts->strata_y != p->spp))
+        return RBW_ERR_INVALID_ARGUMENT;  // the stratified sampler's spp is strata_x * strata_y
     render_ctx ctx;
     ctx.s = s;
     ctx.cam = cam;
@@ -1265,13 +1361,25 @@ int orc_render(orc_scene* s, const rbw_camera_desc* cam, const rbw_render_params
             const int32_t fw = ft.fmax[0] - ft.fmin[0];
             ft.px.assign(static_cast<size_t>(fw) * (ft.fmax[1] - ft.fmin[1]), pixel_acc());
             sample_stream smp;
-            if (opts->sampler == ORC_SAMPLER_TILE) smp.gen = pcg32(static_cast<uint64_t>(tin.index));
+            const bool stratified = opts->sampler == ORC_SAMPLER_TILE_STRATIFIED;
+            if (opts->sampler == ORC_SAMPLER_TILE || stratified) smp.gen = pcg32(static_cast<uint64_t>(tin.index));
+            stratified_tables tables;
+            if (stratified) {
+                tables.sx = opts->strata_x;
+                tables.sy = opts->strata_y;
+                tables.d1 = opts->dims_1d;
+                tables.d2 = opts->dims_2d;
+                smp.strat = &tables;
+            }
             if (record) smp.record = &rec_vals[ti];
             counters& cnt = tile_cnt[ti];
             for (int32_t y = tin.y0; y < tin.y1; y++) {
                 for (int32_t x = tin.x0; x < tin.x1; x++) {
                     const uint64_t pixel = static_cast<uint64_t>(y) * W + x;
+                    if (stratified) tables.fill(smp.gen);  // trace_samplers.reset()
                     for (uint32_t si = 0; si < spp; si++) {
+                        smp.sample = si;  // next_sample(): dimension counters back to 0
+                        smp.k1 = smp.k2 = 0;
                         const uint64_t sid = pixel * spp + si;
                         if (opts->sampler == ORC_SAMPLER_PER_SAMPLE)
                             smp.gen = pcg32(static_cast<uint64_t>(p->seed) * 0x9E3779B97F4A7C15ull + sid);

@@ -20,7 +20,10 @@ extern "C" {
 enum {
     ORC_SAMPLER_TILE = 0,        /* the reference: one pcg32 per 16x16 tile, seeded with the tile index */
     ORC_SAMPLER_PER_SAMPLE = 1,  /* RBW_SAMPLER_PER_SAMPLE_PCG seeding */
-    ORC_SAMPLER_REPLAY = 2       /* consume rbw_render_params replay buffers */
+    ORC_SAMPLER_REPLAY = 2,      /* consume rbw_render_params replay buffers */
+    ORC_SAMPLER_TILE_STRATIFIED = 3  /* the reference's stratified samplers (1D and 2D) sharing the
+                                        per-tile pcg32: strata_x * strata_y = spp, dims_1d / dims_2d
+                                        stratified dimensions, random beyond */
 };
 
 typedef struct orc_render_opts {
@@ -31,6 +34,8 @@ typedef struct orc_render_opts {
     float* record_values;        /* NULL or record_capacity floats */
     uint64_t record_capacity;
     uint64_t record_count;       /* out: number of recorded draws */
+    uint32_t strata_x, strata_y; /* ORC_SAMPLER_TILE_STRATIFIED */
+    uint32_t dims_1d, dims_2d;
 } orc_render_opts;
 
 typedef struct orc_stats {
@@ -62,6 +67,13 @@ int orc_trace_rays_linear(orc_scene* scene, uint32_t n, const float* origins, co
 /* pcg32 of the reference: n uniform_real() draws of pcg32(seed) */
 void orc_pcg32_reals(uint64_t seed, uint32_t n, float* out);
 void orc_pcg32_u32(uint64_t seed, uint32_t n, uint32_t* out);
+/* pcg32(seed).uint32(lo[i], hi[i]) for i < n (bounded draws with rejection) */
+void orc_pcg32_range(uint64_t seed, uint32_t n, const uint32_t* lo, const uint32_t* hi, uint32_t* out);
+/* the tables one pixel's reset() fills (stratified_sampler.hpp:82-127) from pcg32(seed)
+ * advanced by `skip` draws: dims_1d x spp floats, then dims_2d x spp x 2 floats; returns
+ * the number of draws made */
+uint64_t orc_stratified_tables(uint64_t seed, uint64_t skip, uint32_t strata_x, uint32_t strata_y,
+                               uint32_t dims_1d, uint32_t dims_2d, float* out_1d, float* out_2d);
 
 /* warps and BSDF pieces, for closed-form tests */
 void orc_concentric_sample_disk(float u0, float u1, float* out2);

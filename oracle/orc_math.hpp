@@ -160,6 +160,16 @@ struct pcg32 {
         const real x = static_cast<real>(next_u32()) * 2.3283064365386963e-10f;
         return (x < k_one_minus_epsilon) ? x : k_one_minus_epsilon;
     }
+    // pcg32::uint32(value), random_generator.cpp:78-85: rejection below (2^32 - value) % value
+    uint32_t bounded(uint32_t value) {
+        const uint32_t threshold = (~value + 1u) % value;
+        for (;;) {
+            const uint32_t r = next_u32();
+            if (r >= threshold) return r % value;
+        }
+    }
+    // pcg32::uint32(min, max), random_generator.cpp:73-76
+    uint32_t range(uint32_t lo, uint32_t hi) { return bounded(hi - lo + 1u) + lo; }
 };
 
 }  // namespace orc

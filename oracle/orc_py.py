@@ -17,13 +17,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
 REF_PCG = os.path.join(HERE, "_ref", "libref_pcg.so")
 
-ORC_SAMPLER_TILE, ORC_SAMPLER_PER_SAMPLE, ORC_SAMPLER_REPLAY = 0, 1, 2
+ORC_SAMPLER_TILE, ORC_SAMPLER_PER_SAMPLE, ORC_SAMPLER_REPLAY, ORC_SAMPLER_TILE_STRATIFIED = 0, 1, 2, 3
 
 
 class orc_render_opts(C.Structure):
     _fields_ = [("sampler", C.c_uint32), ("threads", C.c_uint32), ("tile_stride", C.c_uint32),
                 ("record_offsets", abi.u64p), ("record_values", abi.f32p), ("record_capacity", C.c_uint64),
-                ("record_count", C.c_uint64)]
+                ("record_count", C.c_uint64), ("strata_x", C.c_uint32), ("strata_y", C.c_uint32),
+                ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32)]
 
 
 class orc_stats(C.Structure):
@@ -60,6 +61,10 @@ def lib():
             fn.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, abi.f32p, C.c_uint32, C.c_void_p]
         L.orc_pcg32_reals.argtypes = [C.c_uint64, C.c_uint32, abi.f32p]
         L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint32, abi.u32p]
+        L.orc_pcg32_range.argtypes = [C.c_uint64, C.c_uint32, abi.u32p, abi.u32p, abi.u32p]
+        L.orc_stratified_tables.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            abi.f32p, abi.f32p]
+        L.orc_stratified_tables.restype = C.c_uint64
         L.orc_concentric_sample_disk.argtypes = [C.c_float, C.c_float, abi.f32p]
         L.orc_cosine_sample_hemisphere.argtypes = [C.c_float, C.c_float, abi.f32p]
         L.orc_uniform_sample_triangle.argtypes = [C.c_float, C.c_float, abi.f32p]
@@ -113,10 +118,12 @@ class OracleScene:
         return out
 
     def render(self, cam, params, sampler=ORC_SAMPLER_TILE, threads=0, tile_stride=1, record=False,
-               debug=False):
+               debug=False, strata=(1, 1), dims=(0, 0)):
+        """strata/dims: ORC_SAMPLER_TILE_STRATIFIED's (strata_x, strata_y) and (dims_1d, dims_2d)"""
         W, H = cam.width, cam.height
         out = np.zeros((H, W, 3), np.float32)
-        opts = orc_render_opts(sampler=sampler, threads=threads, tile_stride=tile_stride)
+        opts = orc_render_opts(sampler=sampler, threads=threads, tile_stride=tile_stride, strata_x=strata[0],
+                               strata_y=strata[1], dims_1d=dims[0], dims_2d=dims[1])
         keep = []
         if record:
             n = W * H * params.spp
@@ -158,6 +165,24 @@ def pcg32_reals(seed, n):
     out = np.zeros(n, np.float32)
     lib().orc_pcg32_reals(seed, n, abi.ptr(out, abi.f32p))
     return out
+
+
+def pcg32_range(seed, lo, hi):
+    lo = np.ascontiguousarray(lo, np.uint32)
+    hi = np.ascontiguousarray(hi, np.uint32)
+    out = np.zeros(len(lo), np.uint32)
+    lib().orc_pcg32_range(seed, len(lo), abi.ptr(lo, abi.u32p), abi.ptr(hi, abi.u32p), abi.ptr(out, abi.u32p))
+    return out
+
+
+def stratified_tables(seed, strata, dims, skip=0):
+    """one pixel's reset(): (1D tables dims_1d x n, 2D tables dims_2d x n x 2, draws made)"""
+    n = strata[0] * strata[1]
+    t1 = np.zeros((dims[0], n), np.float32)
+    t2 = np.zeros((dims[1], n, 2), np.float32)
+    draws = lib().orc_stratified_tables(seed, skip, strata[0], strata[1], dims[0], dims[1], abi.ptr(t1, abi.f32p),
+                                        abi.ptr(t2, abi.f32p))
+    return t1, t2, int(draws)
 
 
 def pcg32_u32(seed, n):

@@ -12,3 +12,9 @@ extern "C" void ref_pcg32_u32(unsigned long long seed, unsigned n, unsigned* out
     rainbow::cpus::shared::pcg32 g(seed);
     for (unsigned i = 0; i < n; i++) out[i] = g.uniform_uint32();
 }
+
+extern "C" void ref_pcg32_range(unsigned long long seed, unsigned n, const unsigned* lo, const unsigned* hi,
+                                unsigned* out) {
+    rainbow::cpus::shared::pcg32 g(seed);
+    for (unsigned i = 0; i < n; i++) out[i] = g.uint32(lo[i], hi[i]);
+}

@@ -29,7 +29,7 @@ RBW_ENTITY_REVERSE_ORIENTATION = 2
 RBW_ENTITY_HAS_NORMALS = 4
 RBW_ENTITY_HAS_UVS = 8
 RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL = 1, 2
-RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY = 0, 1
+RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY, RBW_SAMPLER_STRATIFIED = 0, 1, 2
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
 RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "shade_guided", "guide_train"]
@@ -79,7 +79,8 @@ class rbw_render_params(C.Structure):
                 ("shadow_mode", C.c_uint32), ("samples_per_pass", C.c_uint32),
                 ("replay_offsets", u64p), ("replay_values", f32p), ("replay_count", C.c_uint64),
                 ("debug_hits", u32p), ("count_visits", C.c_uint32),
-                ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32)]
+                ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32),
+                ("strata_x", C.c_uint32), ("strata_y", C.c_uint32), ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):
@@ -171,6 +172,8 @@ RBW_SIGNATURES = {
                                     C.POINTER(rbw_job_stats), C.POINTER(rbw_batch_stats)]),
     "rbw_autotune_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),
                                       C.POINTER(C.c_double)]),
+    "rbw_sampler_draws": (C.c_int32, [C.c_void_p, C.POINTER(rbw_render_params), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      f32p]),
     "rbw_film_to_rgba8": (C.c_int32, [f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                       C.c_float, C.POINTER(C.c_uint8)]),
     "rbw_write_png": (C.c_int32, [C.c_char_p, f32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),

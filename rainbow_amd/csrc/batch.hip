@@ -364,6 +364,8 @@ struct batch_config {
     uint32_t block = 256, pool = 1u << 21, width = k_default_width;
     bool graph = true;
     uint32_t tri_test = RBW_TRI_MOLLER_TRUMBORE, shadow = RBW_SHADOW_CLOSEST_VISIBLE;
+    // RBW_SAMPLER_PER_SAMPLE_PCG or RBW_SAMPLER_STRATIFIED (a single render's sampler)
+    uint32_t sampler = RBW_SAMPLER_PER_SAMPLE_PCG, strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
 };
 
 void validate_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n) {
@@ -497,7 +499,11 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     P.cap = cfg.pool;
     dparams base;
     std::memset(&base, 0, sizeof(base));
-    base.sampler = RBW_SAMPLER_PER_SAMPLE_PCG;
+    base.sampler = cfg.sampler;
+    base.strata_x = cfg.strata_x;
+    base.strata_y = cfg.strata_y;
+    base.dims_1d = cfg.dims_1d;
+    base.dims_2d = cfg.dims_2d;
     base.tri_test = cfg.tri_test;
     base.shadow_mode = cfg.shadow;
     // grid: one thread per slot of the pool
@@ -813,6 +819,11 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
     cfg.tri_test = pr.triangle_test;
     cfg.shadow = pr.shadow_mode;
     cfg.width = resolve_width(pr.bvh_width);
+    cfg.sampler = pr.sampler;
+    cfg.strata_x = pr.strata_x;
+    cfg.strata_y = pr.strata_y;
+    cfg.dims_1d = pr.dims_1d;
+    cfg.dims_2d = pr.dims_2d;
     ctx->timer.reset();
     run_result rr;
     std::vector<djob> jobs(1, J);

@@ -417,7 +417,8 @@ enum fidx : uint32_t {
     F_CBX, F_CBY, F_CBZ,                         // beta at the vertex
     F_COUNT
 };
-enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_COUNT };
+// U_DIMS: the stratified sampler's draws made by the sample (1D | 2D << 16)
+enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_DIMS, U_COUNT };
 
 constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
 
@@ -483,8 +484,66 @@ struct dparams {
     const uint64_t* replay_offsets;
     const float* replay_values;
     uint32_t* debug;  // device debug hit buffer or null
+    uint32_t strata_x, strata_y, dims_1d, dims_2d;  // RBW_SAMPLER_STRATIFIED
     dguide g;
 };
+
+// ---- the stratified sampler's strata and jitter (no tables)
+// permute(i, l, p): a pseudo-random permutation of [0, l) keyed by p, and
+// randfloat(i, p): a hashed float in [0, 1); both from A. Kensler, "Correlated
+// Multi-Jittered Sampling", Pixar Technical Memo 13-01 (2013), section 3.
+__device__ __forceinline__ uint32_t cmj_permute(uint32_t i, uint32_t l, uint32_t p) {
+    uint32_t w = l - 1u;
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    w |= w >> 8;
+    w |= w >> 16;
+    do {  // cycle-walking: at most a few rounds, as w < 2l
+        i ^= p;
+        i *= 0xe170893du;
+        i ^= p >> 16;
+        i ^= (i & w) >> 4;
+        i ^= p >> 8;
+        i *= 0x0929eb3fu;
+        i ^= p >> 23;
+        i ^= (i & w) >> 1;
+        i *= 1u | p >> 27;
+        i *= 0x6935fa69u;
+        i ^= (i & w) >> 11;
+        i *= 0x74dcb303u;
+        i ^= (i & w) >> 2;
+        i *= 0x9e501cc3u;
+        i ^= (i & w) >> 2;
+        i *= 0xc860a3dfu;
+        i &= w;
+        i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+__device__ __forceinline__ float cmj_randfloat(uint32_t i, uint32_t p) {
+    i ^= p;
+    i ^= i >> 17;
+    i ^= i >> 10;
+    i *= 0xb36534e5u;
+    i ^= i >> 12;
+    i ^= i >> 21;
+    i *= 0x93fc4795u;
+    i ^= 0xdf6e307fu;
+    i ^= i >> 17;
+    i *= 1u | p >> 18;
+    return (float)i * (1.0f / 4294967808.0f);
+}
+
+// the pattern key of (seed, pixel, dimension): SplitMix64's finaliser over a mixed word
+__device__ __forceinline__ uint32_t stratum_key(uint32_t seed, uint32_t pixel, uint32_t dim) {
+    uint64_t z = (uint64_t)seed * 0x9E3779B97F4A7C15ull + (uint64_t)pixel * 0xD1B54A32D192ED03ull +
+                 (uint64_t)(dim + 1u) * 0x8CB92BA72F3D8DD7ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
 
 // glm mat4 * vec4 (xyz): (m0*x + m1*y) + (m2*z + m3*w)
 __device__ __forceinline__ f3 mat_mul(const float* m, f3 v, float w) {
@@ -500,7 +559,46 @@ struct sampler_state {
     uint64_t inc;
     uint64_t end;
     const float* replay;
+    // stratified: the first d1 1D and d2 2D draws (d1 = d2 = 0 otherwise)
+    uint32_t d1 = 0, d2 = 0, sx = 1, sy = 1, sseed = 0;
+    uint32_t pixel = 0, sample = 0, dims = 0;
+    __device__ void stratify(const dparams& prm, uint32_t px, uint32_t s, uint32_t drawn) {
+        d1 = prm.dims_1d;
+        d2 = prm.dims_2d;
+        sx = prm.strata_x;
+        sy = prm.strata_y;
+        sseed = prm.seed;
+        pixel = px;
+        sample = s;
+        dims = drawn;
+    }
     __device__ float next() {
+        if ((dims & 0xFFFFu) < d1) {
+            const uint32_t k = dims & 0xFFFFu;
+            dims += 1u;
+            const uint32_t n = sx * sy;
+            const uint32_t key = stratum_key(sseed, pixel, 2u * k);
+            const float x = ((float)cmj_permute(sample, n, key) + cmj_randfloat(sample, key * 0x68bc21ebu)) / (float)n;
+            return (x < k_one_minus_epsilon) ? x : k_one_minus_epsilon;
+        }
+        return uniform();
+    }
+    __device__ f2 next2() {
+        if ((dims >> 16) < d2) {
+            const uint32_t k = dims >> 16;
+            dims += 1u << 16;
+            const uint32_t key = stratum_key(sseed, pixel, 2u * k + 1u);
+            const uint32_t c = cmj_permute(sample, sx * sy, key);
+            const float x = ((float)(c % sx) + cmj_randfloat(sample, key * 0xa399d265u)) / (float)sx;
+            const float y = ((float)(c / sx) + cmj_randfloat(sample, key * 0x711ad6a5u)) / (float)sy;
+            return f2{(x < k_one_minus_epsilon) ? x : k_one_minus_epsilon, (y < k_one_minus_epsilon) ? y : k_one_minus_epsilon};
+        }
+        const float a = uniform();
+        const float b = uniform();
+        return f2{a, b};
+    }
+    // one draw of the replayed or per-sample pcg32 stream
+    __device__ float uniform() {
         if (replay) return state < end ? replay[state++] : 0.0f;
         const uint64_t old = state;
         state = old * 0x5851f42d4c957f2dULL + inc;
@@ -510,12 +608,12 @@ struct sampler_state {
         const float x = (float)r * 2.3283064365386963e-10f;
         return (x < k_one_minus_epsilon) ? x : k_one_minus_epsilon;
     }
-    __device__ f2 next2() {
-        const float a = next();
-        const float b = next();
-        return f2{a, b};
-    }
 };
+
+__device__ __forceinline__ void store_sampler(const dpaths& P, uint32_t slot, const sampler_state& smp) {
+    P.rng[slot] = smp.state;
+    if (smp.d1 | smp.d2) P.U(U_DIMS, slot) = smp.dims;
+}
 
 __device__ __forceinline__ sampler_state load_sampler(const dpaths& P, const dparams& prm, uint32_t slot) {
     sampler_state s;
@@ -529,6 +627,8 @@ __device__ __forceinline__ sampler_state load_sampler(const dpaths& P, const dpa
         s.replay = nullptr;
         s.end = 0;
         s.inc = (sample_stream(prm.seed, sid) << 1u) | 1u;
+        if (prm.sampler == RBW_SAMPLER_STRATIFIED)
+            s.stratify(prm, P.U(U_PIXEL, slot), P.U(U_SAMPLE, slot), P.U(U_DIMS, slot));
     }
     return s;
 }
@@ -585,6 +685,7 @@ __device__ __forceinline__ void raygen_path(const dcamera& cam, const dparams& p
         smp.inc = g.inc;
         smp.replay = nullptr;
         smp.end = 0;
+        if (prm.sampler == RBW_SAMPLER_STRATIFIED) smp.stratify(prm, pixel, s, 0u);
     }
     const f2 j = smp.next2();
     const float px = (float)x + j.x, py = (float)y + j.y;
@@ -613,7 +714,7 @@ __device__ __forceinline__ void raygen_path(const dcamera& cam, const dparams& p
     P.U(U_SAMPLE, slot) = s;
     P.U(U_BOUNCE, slot) = 0;
     P.U(U_PENDING, slot) = RBW_NONE;
-    P.rng[slot] = smp.state;
+    store_sampler(P, slot, smp);
 }
 
 // closest hit of the path ray (scene::intersect, scene.cpp:82-95)
@@ -771,7 +872,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
         }
         P.setF3(F_BX, slot, beta);
         P.setF3(F_VX, slot, value);
-        P.rng[slot] = smp.state;
+        store_sampler(P, slot, smp);
     }
 }
 

@@ -295,7 +295,7 @@ __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, 
         }
         P.setF3(F_BX, slot, beta);
         P.U(U_BOUNCE, slot) = (uint32_t)bounce;
-        P.rng[slot] = smp.state;
+        store_sampler(P, slot, smp);
     }
     const uint32_t ai = block_append(qout_count, alive);
     if (alive) qout[ai] = slot;
@@ -442,6 +442,27 @@ __global__ void k_film_resolve(dcamera cam, dfilm film, float* rgb) {
     film_resolve_pixel(cam, film, p, rgb);
 }
 
+// the sampler's draws for the samples of one pixel: per sample, n1 1D draws then n2 2D draws
+__global__ void k_sampler_draws(dparams prm, uint32_t pixel, uint32_t n1, uint32_t n2, float* out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= prm.spp) return;
+    pcg32 g;
+    g.seed(sample_stream(prm.seed, (uint64_t)pixel * prm.spp + s));
+    sampler_state smp;
+    smp.state = g.state;
+    smp.inc = g.inc;
+    smp.replay = nullptr;
+    smp.end = 0;
+    if (prm.sampler == RBW_SAMPLER_STRATIFIED) smp.stratify(prm, pixel, s, 0u);
+    float* o = out + (size_t)s * (n1 + 2 * n2);
+    for (uint32_t k = 0; k < n1; k++) o[k] = smp.next();
+    for (uint32_t k = 0; k < n2; k++) {
+        const f2 v = smp.next2();
+        o[n1 + 2 * k] = v.x;
+        o[n1 + 2 * k + 1] = v.y;
+    }
+}
+
 // scene::intersect / intersect_with_shadow_ray for caller-given rays
 template <int W>
 __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* d, const float* tmax, uint32_t visible,
@@ -532,8 +553,13 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 cam->bound_max[1] <= (int32_t)cam->height && cam->bound_min[0] < cam->bound_max[0] &&
                 cam->bound_min[1] < cam->bound_max[1],
             RBW_ERR_INVALID_ARGUMENT, "pixel bounds outside the film");
-    require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG || pr.sampler == RBW_SAMPLER_REPLAY, RBW_ERR_INVALID_ARGUMENT,
-            "unknown sampler");
+    require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG || pr.sampler == RBW_SAMPLER_REPLAY ||
+                pr.sampler == RBW_SAMPLER_STRATIFIED,
+            RBW_ERR_INVALID_ARGUMENT, "unknown sampler");
+    if (pr.sampler == RBW_SAMPLER_STRATIFIED)
+        require(pr.strata_x > 0 && pr.strata_y > 0 && uint64_t(pr.strata_x) * pr.strata_y == pr.spp &&
+                    pr.dims_1d <= 0xFFFFu && pr.dims_2d <= 0xFFFFu,
+                RBW_ERR_INVALID_ARGUMENT, "stratified sampler: strata_x * strata_y must equal spp, dims below 65536");
     require(pr.triangle_test == RBW_TRI_MOLLER_TRUMBORE || pr.triangle_test == RBW_TRI_WATERTIGHT,
             RBW_ERR_INVALID_ARGUMENT, "unknown triangle test");
     require(pr.shadow_mode == RBW_SHADOW_CLOSEST_VISIBLE || pr.shadow_mode == RBW_SHADOW_ANY_HIT,
@@ -551,11 +577,11 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 "guiding mis_fraction must be in [0, 1]");
         require(!go->train || (go->batch > 0 && go->batch <= (1u << 20)), RBW_ERR_INVALID_ARGUMENT,
                 "guiding batch must be in [1, 2^20]");
-        require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG, RBW_ERR_INVALID_ARGUMENT,
+        require(pr.sampler != RBW_SAMPLER_REPLAY, RBW_ERR_INVALID_ARGUMENT,
                 "guiding draws a different number of samples per vertex than the replayed reference stream");
     }
     // the plain estimator runs on persistent, in-place regenerated path slots
-    if (pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG && !pr.debug_hits && !pr.count_visits && !guided) {
+    if (pr.sampler != RBW_SAMPLER_REPLAY && !pr.debug_hits && !pr.count_visits && !guided) {
         render_scheduled(ctx, cam, pr, out_rgb, stats);
         return;
     }
@@ -599,6 +625,10 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     dp.rr_threshold = pr.rr_threshold;
     dp.seed = pr.seed;
     dp.sampler = pr.sampler;
+    dp.strata_x = pr.strata_x;
+    dp.strata_y = pr.strata_y;
+    dp.dims_1d = pr.dims_1d;
+    dp.dims_2d = pr.dims_2d;
     dp.tri_test = pr.triangle_test;
     dp.shadow_mode = pr.shadow_mode;
     dp.replay_offsets = nullptr;
@@ -868,6 +898,39 @@ rbw_status rbw_create(const rbw_device_opts* opts, rbw_ctx** out) {
 void rbw_destroy(rbw_ctx* ctx) { delete ctx; }
 
 const char* rbw_last_error(void) { return g_last_error.c_str(); }
+
+rbw_status rbw_sampler_draws(rbw_ctx* ctx, const rbw_render_params* params, uint32_t pixel, uint32_t n1,
+                             uint32_t n2, float* out) {
+    return guarded([&] {
+        require(ctx && params && out, RBW_ERR_INVALID_ARGUMENT, "ctx, params and out are required");
+        const rbw_render_params& pr = *params;
+        require(pr.sampler == RBW_SAMPLER_PER_SAMPLE_PCG || pr.sampler == RBW_SAMPLER_STRATIFIED,
+                RBW_ERR_INVALID_ARGUMENT, "sampler draws exist for the per-sample and stratified samplers");
+        require(pr.spp > 0 && pr.spp <= (1u << 20) && n1 <= 0xFFFFu && n2 <= 0xFFFFu, RBW_ERR_INVALID_ARGUMENT,
+                "spp or draw counts out of range");
+        if (pr.sampler == RBW_SAMPLER_STRATIFIED)
+            require(pr.strata_x > 0 && pr.strata_y > 0 && uint64_t(pr.strata_x) * pr.strata_y == pr.spp,
+                    RBW_ERR_INVALID_ARGUMENT, "stratified sampler: strata_x * strata_y must equal spp");
+        dparams dp;
+        std::memset(&dp, 0, sizeof(dp));
+        dp.spp = pr.spp;
+        dp.seed = pr.seed;
+        dp.sampler = pr.sampler;
+        dp.strata_x = pr.strata_x;
+        dp.strata_y = pr.strata_y;
+        dp.dims_1d = pr.dims_1d;
+        dp.dims_2d = pr.dims_2d;
+        const size_t n = size_t(pr.spp) * (n1 + 2 * size_t(n2));
+        dbuf d;
+        d.ensure(std::max<size_t>(n, 1) * sizeof(float));
+        hipStream_t s = ctx->stream;
+        hipLaunchKernelGGL(k_sampler_draws, dim3(blocks(pr.spp, 256)), dim3(256), 0, s, dp, pixel, n1, n2,
+                           d.as<float>());
+        HIP_CHECK(hipGetLastError());
+        if (n) HIP_CHECK(hipMemcpyAsync(out, d.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
 
 rbw_status rbw_film_to_rgba8(const float* rgb, uint32_t width, uint32_t height, const int32_t* bound_min,
                              const int32_t* bound_max, float scale, uint8_t* out_rgba) {

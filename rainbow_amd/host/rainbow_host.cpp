@@ -364,6 +364,16 @@ void wavefront_path_integrator::render(const std::shared_ptr<camera>& cam, const
     p.rr_threshold = mThreshold;
     p.seed = seed;
     p.sampler = RBW_SAMPLER_PER_SAMPLE_PCG;
+    if (mSampler2D->stratified()) {
+        // the stratified samplers' contract on the GPU: the 2D sampler's strata and
+        // dimensions, the 1D sampler's dimensions when it is stratified too
+        p.sampler = RBW_SAMPLER_STRATIFIED;
+        p.strata_x = uint32_t(mSampler2D->strata_x());
+        p.strata_y = uint32_t(mSampler2D->strata_y());
+        p.dims_2d = uint32_t(std::min<size_t>(mSampler2D->dimension(), 0xFFFF));
+        p.dims_1d = mSampler1D && mSampler1D->stratified() ? uint32_t(std::min<size_t>(mSampler1D->dimension(), 0xFFFF))
+                                                           : 0u;
+    }
     p.triangle_test = triangle_test;
     p.shadow_mode = shadow_mode;
     std::vector<float> rgb(size_t(cd.width) * cd.height * 3);

@@ -302,18 +302,50 @@ private:
 }  // namespace cameras
 
 namespace samplers {
+// samplers/samplers.hpp:13-45: what the GPU tracer needs of a sampler is its samples
+// per pixel and, for the stratified sampler, its strata and stratified dimensions
 template <size_t Dimension>
-class random_sampler_t : public interfaces::noncopyable {
+class sampler_t : public interfaces::noncopyable {
 public:
-    explicit random_sampler_t(size_t samples_per_pixel) : mSamplesPerPixel(samples_per_pixel) {}
+    explicit sampler_t(size_t samples_per_pixel) : mSamplesPerPixel(samples_per_pixel) {}
+    virtual ~sampler_t() = default;
     size_t samples_per_pixel() const noexcept { return mSamplesPerPixel; }
-private:
+    virtual bool stratified() const noexcept { return false; }
+    virtual size_t strata_x() const noexcept { return mSamplesPerPixel; }
+    virtual size_t strata_y() const noexcept { return 1; }
+    virtual size_t dimension() const noexcept { return 0; }
+protected:
     size_t mSamplesPerPixel;
 };
-using sampler1d = random_sampler_t<1>;
-using sampler2d = random_sampler_t<2>;
+
+// samplers/random_sampler.hpp
+template <size_t Dimension>
+class random_sampler_t final : public sampler_t<Dimension> {
+public:
+    explicit random_sampler_t(size_t samples_per_pixel) : sampler_t<Dimension>(samples_per_pixel) {}
+};
+
+// samplers/stratified_sampler.hpp:8-38 (constructor arguments as the reference's)
+template <size_t Dimension>
+class stratified_sampler_t final : public sampler_t<Dimension> {
+public:
+    stratified_sampler_t(size_t samples_per_pixel_x, size_t samples_per_pixel_y, size_t dimension)
+        : sampler_t<Dimension>(samples_per_pixel_x * samples_per_pixel_y), mX(samples_per_pixel_x),
+          mY(samples_per_pixel_y), mDimension(dimension) {}
+    bool stratified() const noexcept override { return true; }
+    size_t strata_x() const noexcept override { return mX; }
+    size_t strata_y() const noexcept override { return mY; }
+    size_t dimension() const noexcept override { return mDimension; }
+private:
+    size_t mX, mY, mDimension;
+};
+
+using sampler1d = sampler_t<1>;
+using sampler2d = sampler_t<2>;
 using random_sampler1d = random_sampler_t<1>;
 using random_sampler2d = random_sampler_t<2>;
+using stratified_sampler1d = stratified_sampler_t<1>;
+using stratified_sampler2d = stratified_sampler_t<2>;
 }  // namespace samplers
 
 namespace integrators {

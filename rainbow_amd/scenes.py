@@ -265,11 +265,11 @@ def add_cornell_box(sc: Scene, light_size=0.25, radiance=(17.0, 12.0, 4.0), sphe
     return light
 
 
-def cornell_box():
+def cornell_box(width=256, height=256):
     """Cornell-style box, 10 wall triangles, 2 spheres, quad light; returns (scene, camera)."""
     sc = Scene()
     add_cornell_box(sc)
-    cam = perspective_camera(256, 256, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
+    cam = perspective_camera(width, height, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
     return sc, cam
 
 

@@ -93,6 +93,12 @@ class Tracer:
 
 
     # -- guiding network (no counterpart in the reference) -------------------
+    def sampler_draws(self, params, pixel, n1, n2):
+        """per sample of `pixel`: n1 1D draws, then n2 2D draws -> (spp, n1 + 2*n2)"""
+        out = np.zeros((params.spp, n1 + 2 * n2), np.float32)
+        _check(self.lib.rbw_sampler_draws(self.h, C.byref(params), pixel, n1, n2, abi.ptr(out, abi.f32p)))
+        return out
+
     def guide_reset(self, seed=0):
         _check(self.lib.rbw_guide_reset(self.h, seed))
 

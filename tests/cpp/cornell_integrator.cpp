@@ -25,7 +25,7 @@ static std::shared_ptr<materials::matte_material> matte(float r, float g, float 
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s flatten|render <out> [spp depth seed [png name]]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s flatten|render|render_stratified <out> [spp depth seed [png name]]\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -84,8 +84,14 @@ int main(int argc, char** argv) {
     } else {
         const size_t spp = argc > 3 ? std::stoul(argv[3]) : 16;
         const size_t depth = argc > 4 ? std::stoul(argv[4]) : 5;
-        integrators::wavefront_path_integrator integ(std::make_shared<samplers::random_sampler2d>(spp),
-                                                     std::make_shared<samplers::random_sampler1d>(spp), depth, 1.0f);
+        // mode "render_stratified": stratified samplers of 4 x (spp / 4) strata, 8 1D and 12 2D dimensions
+        std::shared_ptr<samplers::sampler2d> s2 = std::make_shared<samplers::random_sampler2d>(spp);
+        std::shared_ptr<samplers::sampler1d> s1 = std::make_shared<samplers::random_sampler1d>(spp);
+        if (mode == "render_stratified") {
+            s2 = std::make_shared<samplers::stratified_sampler2d>(4, spp / 4, 12);
+            s1 = std::make_shared<samplers::stratified_sampler1d>(4, spp / 4, 8);
+        }
+        integrators::wavefront_path_integrator integ(s2, s1, depth, 1.0f);
         integ.seed = argc > 5 ? uint32_t(std::stoul(argv[5])) : 0;
         integ.render(cam, sc);
         for (size_t i = 0; i < 256 * 256; i++) {

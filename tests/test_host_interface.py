@@ -82,3 +82,19 @@ def test_host_integrator_render_matches_c_abi(tracer, tmp_path):
     ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3))["image"]
     bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
     assert bad.mean() < 1e-3, bad.sum()
+
+
+@pytest.mark.gpu
+def test_host_integrator_stratified_samplers(tracer, tmp_path):
+    """stratified_sampler2d(4, 4, 12) and stratified_sampler1d(4, 4, 8) passed to the
+    integrator select the GPU's stratified sampler with those strata and dimensions."""
+    from rainbow_amd.tracer import render_params
+    f = tmp_path / "film.bin"
+    subprocess.run([_binary(), "render_stratified", str(f), "16", "5", "3"], check=True)
+    img = np.fromfile(f, np.float32).reshape(256, 256, 3)
+    sc, cam = scenes.cornell_box()
+    tracer.upload(sc)
+    ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3, sampler=abi.RBW_SAMPLER_STRATIFIED,
+                                           strata_x=4, strata_y=4, dims_1d=8, dims_2d=12))["image"]
+    bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
+    assert bad.mean() < 1e-3, bad.sum()
