@@ -88,9 +88,9 @@ def f3(v):
 
 
 class OracleScene:
-    def __init__(self, scene):
+    def __init__(self, scene, desc=None):
         self.scene = scene
-        self.desc = scene.desc()
+        self.desc = scene.desc() if desc is None else desc
         self.h = C.c_void_p()
         rc = lib().orc_scene_create(C.byref(self.desc), C.byref(self.h))
         if rc != 0:
@@ -102,6 +102,11 @@ class OracleScene:
                 lib().orc_scene_destroy(self.h)
             except Exception:
                 pass
+
+    @classmethod
+    def from_desc(cls, desc):
+        """over a caller-built rbw_scene_desc (its arrays must outlive this call)"""
+        return cls(None, desc)
 
     def num_bvh_nodes(self):
         return lib().orc_scene_num_bvh_nodes(self.h)

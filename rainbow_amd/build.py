@@ -25,8 +25,8 @@ TEST_BIN = os.path.join(HERE, "..", "tests", "cpp", "build", "cornell_integrator
 def build_host(verbose=False):
     """The rainbow-style C++ host interface (librbw_host.so) and its C++ test driver."""
     cxx = os.environ.get("CXX", "g++")
-    host = os.path.join(HERE, "host", "rainbow_host.cpp")
-    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", host, "-L" + HERE, "-lrbw",
+    host = [os.path.join(HERE, "host", f) for f in ("rainbow_host.cpp", "scene_loader.cpp")]
+    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp", *host, "-L" + HERE, "-lrbw",
            "-Wl,-rpath,$ORIGIN"]
     if verbose:
         print(" ".join(cmd), flush=True)

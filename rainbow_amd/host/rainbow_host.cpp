@@ -67,6 +67,23 @@ transform translate(const vector3& v) {
     return transform(m, i);
 }
 
+transform rotate(real angle, const vector3& axis) {
+    // glm::rotate(mat4(1), angle, axis) in double, rounded; the inverse of a rotation is its transpose
+    const double len = std::sqrt(double(axis.x) * axis.x + double(axis.y) * axis.y + double(axis.z) * axis.z);
+    const double x = axis.x / len, y = axis.y / len, z = axis.z / len;
+    const double c = std::cos(double(angle)), s = std::sin(double(angle)), t = 1.0 - c;
+    const double r[3][3] = {{c + t * x * x, t * x * y + s * z, t * x * z - s * y},   // column 0
+                            {t * y * x - s * z, c + t * y * y, t * y * z + s * x},   // column 1
+                            {t * z * x + s * y, t * z * y - s * x, c + t * z * z}};  // column 2
+    matrix4x4 m = matrix4x4::identity(), i = matrix4x4::identity();
+    for (int col = 0; col < 3; col++)
+        for (int row = 0; row < 3; row++) {
+            m.m[col * 4 + row] = real(r[col][row]);
+            i.m[row * 4 + col] = real(r[col][row]);
+        }
+    return transform(m, i);
+}
+
 transform scale(const vector3& v) {
     matrix4x4 m = matrix4x4::identity(), i = matrix4x4::identity();
     m.m[0] = v.x; m.m[5] = v.y; m.m[10] = v.z;
@@ -190,7 +207,7 @@ rbw_scene_desc flat_scene::desc() const {
     d.num_vertices = uint32_t(positions.size() / 3);
     d.positions = positions.data();
     d.normals = has_normals ? normals.data() : nullptr;
-    d.uvs = nullptr;
+    d.uvs = has_uvs ? uvs.data() : nullptr;
     d.num_triangles = uint32_t(indices.size() / 3);
     d.indices = indices.data();
     d.triangle_entity = triangle_entity.data();
@@ -286,7 +303,16 @@ flat_scene flatten(const scene& s) {
             } else {
                 f.normals.insert(f.normals.end(), me->positions.size() * 3, 0.0f);
             }
-            require(me->uvs.empty(), "mesh uvs are not carried by this flattening");
+            if (!me->uvs.empty()) {
+                // per-vertex (u, v) (the reference's uvs are vector3 with z = 0); they set the
+                // triangle's dp/du, dp/dv and so its shading frame (mesh.cpp:328-442)
+                require(me->uvs.size() == me->positions.size(), "uvs must match positions");
+                f.has_uvs = true;
+                out.flags |= RBW_ENTITY_HAS_UVS;
+                for (const vector3& t : me->uvs) f.uvs.insert(f.uvs.end(), {t.x, t.y});
+            } else {
+                f.uvs.insert(f.uvs.end(), me->positions.size() * 2, 0.0f);
+            }
             if (sh->reverse_orientation()) out.flags |= RBW_ENTITY_REVERSE_ORIENTATION;
             out.shape = RBW_SHAPE_MESH;
             out.first = uint32_t(f.indices.size() / 3);
@@ -389,3 +415,50 @@ void wavefront_path_integrator::render(const std::shared_ptr<camera>& cam, const
 
 }  // namespace integrators
 }  // namespace rainbow::cpus
+
+namespace rainbow::cpus::shapes {
+
+// shapes/mesh.cpp:225-278: four vertices per face (-z, +z, +y, -y, -x, +x), two triangles each
+std::shared_ptr<mesh> mesh::create_box(real width, real height, real depth, bool reverse_orientation) {
+    const real w = 0.5f * width, h = 0.5f * height, d = 0.5f * depth;
+    // corners of each face in the reference's order, with the face normal
+    const struct {
+        real n[3];
+        real c[4][3];
+    } faces[6] = {
+        {{0, 0, -1}, {{-w, -h, -d}, {-w, +h, -d}, {+w, +h, -d}, {+w, -h, -d}}},
+        {{0, 0, +1}, {{-w, -h, +d}, {+w, -h, +d}, {+w, +h, +d}, {-w, +h, +d}}},
+        {{0, +1, 0}, {{-w, +h, -d}, {-w, +h, +d}, {+w, +h, +d}, {+w, +h, -d}}},
+        {{0, -1, 0}, {{-w, -h, -d}, {+w, -h, -d}, {+w, -h, +d}, {-w, -h, +d}}},
+        {{-1, 0, 0}, {{-w, -h, +d}, {-w, +h, +d}, {-w, +h, -d}, {-w, -h, -d}}},
+        {{+1, 0, 0}, {{+w, -h, -d}, {+w, +h, -d}, {+w, +h, +d}, {+w, -h, +d}}},
+    };
+    // the reference's per-face uv corners
+    const real uv[6][4][2] = {{{0, 1}, {0, 0}, {1, 0}, {1, 1}}, {{1, 1}, {0, 1}, {0, 0}, {1, 0}},
+                              {{0, 1}, {0, 0}, {1, 0}, {1, 1}}, {{1, 1}, {0, 1}, {0, 0}, {1, 0}},
+                              {{0, 1}, {0, 0}, {1, 0}, {1, 1}}, {{0, 1}, {0, 0}, {1, 0}, {1, 1}}};
+    std::vector<vector3> P, N, UV;
+    std::vector<unsigned> I;
+    for (unsigned f = 0; f < 6; f++) {
+        for (int k = 0; k < 4; k++) {
+            P.push_back(vector3{faces[f].c[k][0], faces[f].c[k][1], faces[f].c[k][2]});
+            N.push_back(vector3{faces[f].n[0], faces[f].n[1], faces[f].n[2]});
+            UV.push_back(vector3{uv[f][k][0], uv[f][k][1], 0});
+        }
+        const unsigned b = 4 * f;
+        for (unsigned k : {b, b + 1, b + 2, b, b + 2, b + 3}) I.push_back(k);
+    }
+    return std::make_shared<mesh>(P, std::vector<vector3>{}, N, UV, I, reverse_orientation);
+}
+
+// shapes/mesh.cpp:280-306
+std::shared_ptr<mesh> mesh::create_quad(real width, real height, bool reverse_orientation) {
+    const real w = 0.5f * width, h = 0.5f * height;
+    std::vector<vector3> P = {{-w, +h, 0}, {-w, -h, 0}, {+w, -h, 0}, {+w, +h, 0}};
+    std::vector<vector3> N(4, vector3{0, 0, 1});
+    std::vector<vector3> UV = {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}};
+    return std::make_shared<mesh>(P, std::vector<vector3>{}, N, UV, std::vector<unsigned>{0, 1, 2, 2, 3, 0},
+                                  reverse_orientation);
+}
+
+}  // namespace rainbow::cpus::shapes

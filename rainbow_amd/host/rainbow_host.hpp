@@ -108,6 +108,8 @@ private:
 
 transform translate(const vector3& v);
 transform scale(const vector3& v);
+// shared/transform.cpp:132-137: rotation by `angle` radians about `axis` (glm::rotate)
+transform rotate(real angle, const vector3& axis);
 transform perspective_right_hand(real fov, real width, real height, real near = 0.01f, real far = 1000.0f);
 transform look_at_right_hand(const vector3& origin, const vector3& target, const vector3& up);
 }  // namespace shared
@@ -144,6 +146,10 @@ public:
          bool reverse_orientation = false)
         : shape(reverse_orientation, indices.size() / 3), positions(positions), tangents(tangents), normals(normals),
           uvs(uvs), indices(indices) {}
+    // shapes/mesh.cpp:225-306: an axis-aligned box centred at the origin (24 vertices,
+    // outward normals) and a quad in the z = 0 plane facing +z
+    static std::shared_ptr<mesh> create_box(real width, real height, real depth, bool reverse_orientation = false);
+    static std::shared_ptr<mesh> create_quad(real width, real height, bool reverse_orientation = false);
     std::vector<vector3> positions, tangents, normals, uvs;
     std::vector<unsigned> indices;
 };
@@ -360,13 +366,13 @@ public:
 
 // Flat copy of a rainbow scene in the C ABI's form, owning its arrays.
 struct flat_scene {
-    std::vector<float> positions, normals;
+    std::vector<float> positions, normals, uvs;  // uvs: 2 per vertex (zero where a mesh has none)
     std::vector<uint32_t> indices, triangle_entity, triangle_material;
     std::vector<float> spheres;
     std::vector<uint32_t> sphere_entity;
     std::vector<rbw_entity> entities;
     std::vector<rbw_material> materials;
-    bool has_normals = false;
+    bool has_normals = false, has_uvs = false;
     rbw_scene_desc desc() const;
 };
 
@@ -400,5 +406,36 @@ private:
     uint64_t mUploadedGeneration = 0;
 };
 }  // namespace integrators
+
+namespace scenes {
+// A scene description file (JSON): materials, entities, camera and render settings.
+//   {"materials": {"white": {"matte": {"diffuse": [0.73, 0.73, 0.73], "sigma": 0}},
+//                  "gold": {"metal": {"eta": [...], "k": [...], "roughness": 0.1, "remap_roughness": false}}},
+//    "entities": [{"shape": {"box": [w, h, d]} | {"quad": [w, h]} | {"sphere": r} |
+//                           {"mesh": {"positions": [...], "indices": [...], "normals": [...]}},
+//                  "material": "white" | {...}, "emitter": {"radiance": [r, g, b]},
+//                  "transform": [{"translate": [x, y, z]}, {"scale": s | [x, y, z]},
+//                                {"rotate": {"axis": [x, y, z], "degrees": a}}],
+//                  "reverse_orientation": false}, ...],
+//    "camera": {"resolution": [w, h], "fov": 40, "eye": [...], "target": [...], "up": [0, 1, 0],
+//               "lens_radius": 0, "focus": 1e6, "crop": [x0, y0, x1, y1], "scale": 1},
+//    "render": {"spp": 16, "max_depth": 5, "threshold": 1, "seed": 0,
+//               "stratified": {"strata": [4, 4], "dims_1d": 8, "dims_2d": 12}, "output": "name"}}
+// Transforms compose left to right; the screen window is [-1, 1]^2 scaled by the aspect.
+struct scene_file {
+    std::shared_ptr<scenes::scene> scene;
+    std::shared_ptr<cameras::camera> camera;
+    size_t spp = 16, max_depth = 5;
+    real threshold = 1;
+    uint32_t seed = 0;
+    bool stratified = false;
+    size_t strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
+    std::string output;  // film::write name ("" = none)
+    // the integrator the render settings describe
+    std::shared_ptr<integrators::wavefront_path_integrator> integrator() const;
+};
+scene_file load_scene(const std::string& json_text);
+scene_file load_scene_file(const std::string& path);
+}  // namespace scenes
 
 }  // namespace rainbow::cpus

@@ -61,6 +61,10 @@ class Tracer:
         self._scene_desc = desc
         return self
 
+    def upload_desc(self, desc):
+        """uploads a caller-built rbw_scene_desc"""
+        _check(self.lib.rbw_upload_scene(self.h, C.byref(desc)))
+
     def render(self, cam, params, debug=False, stats=True):
         W, H = cam.width, cam.height
         out = np.zeros((H, W, 3), np.float32)

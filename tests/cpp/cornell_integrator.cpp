@@ -25,7 +25,7 @@ static std::shared_ptr<materials::matte_material> matte(float r, float g, float 
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s flatten|render|render_stratified <out> [spp depth seed [png name]]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s flatten|render|render_stratified <out> [spp depth seed [png name]] | load|render_file <json> <out>\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -63,11 +63,41 @@ int main(int argc, char** argv) {
         shared::perspective_right_hand(fov, 256, 256),
         shared::look_at_right_hand({0, 0, 3.5f}, {0, 0, 0}, {0, 1, 0}).inverse());
 
-    FILE* out = std::fopen(argv[2], "wb");
+    // a scene description file instead: "load <json> <out>" flattens it, "render_file <json> <out>"
+    // renders it with its own settings (and writes its "output" PNG, if it names one)
+    std::shared_ptr<cameras::camera> view = cam;
+    std::shared_ptr<scenes::scene> world = sc;
+    scenes::scene_file file;
+    const char* out_path = argv[2];
+    if (mode == "load" || mode == "render_file") {
+        if (argc < 4) return 2;
+        try {
+            file = scenes::load_scene_file(argv[2]);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 4;
+        }
+        view = file.camera;
+        world = file.scene;
+        out_path = argv[3];
+    }
+    FILE* out = std::fopen(out_path, "wb");
     if (!out) return 3;
-    if (mode == "flatten") {
-        const integrators::flat_scene fs = integrators::flatten(*sc);
-        const rbw_camera_desc cd = integrators::flatten(*cam);
+    if (mode == "render_file") {
+        auto integ = file.integrator();
+        integ->render(view, world);
+        auto f = view->film();
+        const size_t n = size_t(f->resolution().x) * f->resolution().y;
+        for (size_t i = 0; i < n; i++) std::fwrite(f->pixel(i).c.data(), 4, 3, out);
+        if (!file.output.empty()) f->write(file.output);
+        std::fclose(out);
+        return 0;
+    }
+    if (mode == "flatten" || mode == "load") {
+        auto* pc = dynamic_cast<cameras::perspective_camera*>(view.get());
+        if (!pc) return 5;
+        const integrators::flat_scene fs = integrators::flatten(*world);
+        const rbw_camera_desc cd = integrators::flatten(*pc);
         const uint32_t counts[5] = {uint32_t(fs.positions.size() / 3), uint32_t(fs.indices.size() / 3),
                                     uint32_t(fs.spheres.size() / 4), uint32_t(fs.entities.size()),
                                     uint32_t(fs.materials.size())};
@@ -81,6 +111,7 @@ int main(int argc, char** argv) {
         std::fwrite(fs.entities.data(), sizeof(rbw_entity), fs.entities.size(), out);
         std::fwrite(fs.materials.data(), sizeof(rbw_material), fs.materials.size(), out);
         std::fwrite(&cd, sizeof(cd), 1, out);
+        if (fs.has_uvs) std::fwrite(fs.uvs.data(), 4, fs.uvs.size(), out);  // (u, v) per vertex, when any
     } else {
         const size_t spp = argc > 3 ? std::stoul(argv[3]) : 16;
         const size_t depth = argc > 4 ? std::stoul(argv[4]) : 5;

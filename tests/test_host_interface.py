@@ -33,7 +33,35 @@ def _read_flat(path):
     out["M"] = np.frombuffer(b, MATERIAL_DTYPE, nm, off)
     off += nm * MATERIAL_DTYPE.itemsize
     out["CAM"] = abi.rbw_camera_desc.from_buffer_copy(b[off: off + C.sizeof(abi.rbw_camera_desc)])
+    off += C.sizeof(abi.rbw_camera_desc)
+    out["UV"] = take(np.float32, 2 * nv) if off < len(b) else None
     return out
+
+
+def flat_desc(f):
+    """an rbw_scene_desc over the arrays of a flattened scene (the arrays stay in f)"""
+    import ctypes as C
+    d = abi.rbw_scene_desc()
+    keep = {k: np.ascontiguousarray(v) for k, v in f.items() if isinstance(v, np.ndarray)}
+    f["_keep"] = keep
+    d.num_vertices = len(keep["P"]) // 3
+    d.positions = abi.ptr(keep["P"], abi.f32p)
+    d.normals = abi.ptr(keep["N"], abi.f32p)
+    d.uvs = abi.ptr(keep["UV"], abi.f32p) if "UV" in keep else abi.ptr(None, abi.f32p)
+    d.num_triangles = len(keep["I"]) // 3
+    d.indices = abi.ptr(keep["I"], abi.u32p)
+    d.triangle_entity = abi.ptr(keep["TE"], abi.u32p)
+    d.num_spheres = len(keep["S"]) // 4
+    d.spheres = abi.ptr(keep["S"], abi.f32p) if d.num_spheres else abi.ptr(None, abi.f32p)
+    d.sphere_entity = abi.ptr(keep["SE"], abi.u32p) if d.num_spheres else abi.ptr(None, abi.u32p)
+    ents = (abi.rbw_entity * (len(f["E"]) // C.sizeof(abi.rbw_entity))).from_buffer_copy(f["E"])
+    mats = (abi.rbw_material * len(keep["M"])).from_buffer_copy(keep["M"].tobytes())
+    f["_ents"], f["_mats"] = ents, mats
+    d.num_entities = len(ents)
+    d.entities = C.cast(ents, C.POINTER(abi.rbw_entity))
+    d.num_materials = len(mats)
+    d.materials = C.cast(mats, C.POINTER(abi.rbw_material))
+    return d
 
 
 def _binary():

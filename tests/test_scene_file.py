@@ -1,0 +1,157 @@
+"""Scene description files (rainbow_amd/host/scene_loader.cpp): JSON -> the mirror's
+scene and camera objects, with mesh::create_box / create_quad (shapes/mesh.cpp:225-306)
+and entity transforms.  The Cornell box written as a file must flatten to the same
+descriptors as the scene built in code; boxes and quads are checked by their geometry
+(corners, outward winding, normals) after the transforms."""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from rainbow_amd import abi, scenes
+from test_host_interface import _binary, _read_flat, flat_desc
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORNELL = os.path.join(HERE, "scenes", "cornell.json")
+
+
+def _load(tmp_path, scene):
+    src = tmp_path / "scene.json"
+    src.write_text(json.dumps(scene))
+    out = tmp_path / "flat.bin"
+    r = subprocess.run([_binary(), "load", str(src), str(out)], capture_output=True, text=True)
+    return r, out
+
+
+def test_cornell_file_flattens_like_the_built_scene(tmp_path):
+    a, b = tmp_path / "built.bin", tmp_path / "file.bin"
+    subprocess.run([_binary(), "flatten", str(a)], check=True)
+    subprocess.run([_binary(), "load", CORNELL, str(b)], check=True)
+    assert a.read_bytes() == b.read_bytes()
+
+
+def _tri_normals(P, I):
+    p = P.reshape(-1, 3)[I.reshape(-1, 3)]
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def test_box_and_quad_geometry_after_transforms(tmp_path):
+    w, h, d = 0.4, 0.6, 0.8
+    scene = {
+        "materials": {"m": {"matte": {"diffuse": [0.5, 0.5, 0.5]}}},
+        "entities": [
+            {"shape": {"box": [w, h, d]}, "material": "m",
+             "transform": [{"translate": [0.1, -0.2, 0.3]}, {"rotate": {"axis": [0, 1, 0], "degrees": 90}}]},
+            {"shape": {"quad": [2, 1]}, "emitter": {"radiance": [4, 5, 6]},
+             "transform": [{"scale": 0.5}]},
+        ],
+        "camera": {"resolution": [32, 16], "eye": [0, 0, 3], "target": [0, 0, 0]},
+    }
+    r, out = _load(tmp_path, scene)
+    assert r.returncode == 0, r.stderr
+    f = _read_flat(out)
+    P, N, I = f["P"].reshape(-1, 3), f["N"].reshape(-1, 3), f["I"].reshape(-1, 3)
+    assert len(P) == 24 + 4 and len(I) == 12 + 2
+    # the box: rotation by 90 degrees about y maps (x, y, z) -> (z, y, -x), then the translation
+    box = P[:24] - np.array([0.1, -0.2, 0.3], np.float32)
+    corners = sorted({tuple(float(x) for x in np.round(c.astype(np.float64), 5)) for c in box})
+    want = sorted({(round(sz * d / 2, 5), round(sy * h / 2, 5), round(-sx * w / 2, 5))
+                   for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)})
+    assert np.allclose(corners, want, atol=1e-5) and len(corners) == 8
+    # every triangle winds counter-clockwise around its outward normal, which is the
+    # stored (transformed) vertex normal, pointing away from the box centre
+    tn = _tri_normals(P[:24].reshape(-1), I[:12])
+    vn = N[:24][I[:12][:, 0]]
+    assert np.allclose(tn, vn, atol=1e-5)
+    centre = np.array([0.1, -0.2, 0.3])
+    mid = P[:24][I[:12]].mean(1)
+    assert (np.einsum("ij,ij->i", mid - centre, tn) > 0).all()
+    # the quad: z = 0 plane, scaled by 0.5, facing +z, two triangles wound around +z
+    q = P[24:]
+    assert np.allclose(sorted(map(tuple, q)), sorted([(-0.5, 0.25, 0), (-0.5, -0.25, 0), (0.5, -0.25, 0),
+                                                       (0.5, 0.25, 0)]), atol=1e-7)
+    assert np.allclose(_tri_normals(q.reshape(-1), I[12:] - 24), [[0, 0, 1], [0, 0, 1]], atol=1e-6)
+    # the quad is an emitter with the given radiance; the film is 32 x 16
+    import ctypes as C
+    ents = (abi.rbw_entity * 2).from_buffer_copy(f["E"])
+    assert ents[1].flags & abi.RBW_ENTITY_EMITTER and list(ents[1].radiance) == [4, 5, 6]
+    assert not ents[0].flags & abi.RBW_ENTITY_EMITTER
+    assert (f["CAM"].width, f["CAM"].height) == (32, 16)
+    want_cam = scenes.perspective_camera(32, 16, 40.0, eye=(0, 0, 3), target=(0, 0, 0))
+    assert list(f["CAM"].camera_to_world) == list(want_cam.camera_to_world)
+    assert np.allclose(np.array(f["CAM"].raster_to_camera), np.array(want_cam.raster_to_camera), rtol=2e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("text,needle", [
+    ('{"entities": [}', "offset"),
+    ('{"entities": [{"shape": {"sphere": 1}, "material": "nope"}], "camera": {}}', "unknown material"),
+    ('{"entities": [{"shape": {"cone": 1}, "material": {"matte": {"diffuse": 1}}}]}', "unknown shape"),
+    ('{"entities": [{"shape": {"mesh": {"positions": [0, 0, 0], "indices": [0, 0, 1]}}, "material": {"matte": {"diffuse": 1}}}]}',
+     "index out of range"),
+    ('{"entities": []}', 'missing "camera"'),
+])
+def test_bad_scene_files(tmp_path, text, needle):
+    src = tmp_path / "bad.json"
+    src.write_text(text)
+    r = subprocess.run([_binary(), "load", str(src), str(tmp_path / "x.bin")], capture_output=True, text=True)
+    assert r.returncode == 4 and needle in r.stderr, (r.returncode, r.stderr)
+
+
+@pytest.mark.gpu
+def test_gpu_render_file_matches_the_built_scene(tracer, tmp_path):
+    """The file's render settings (16 spp, depth 5, seed 3) through the integrator give the
+    render of the same scene built in Python; its "output" PNG holds that film."""
+    from rainbow_amd.tracer import film_to_rgba8, render_params
+    from test_image import read_png
+    scene = json.load(open(CORNELL))
+    scene["render"]["output"] = str(tmp_path / "cornell")
+    src = tmp_path / "scene.json"
+    src.write_text(json.dumps(scene))
+    out = tmp_path / "film.bin"
+    subprocess.run([_binary(), "render_file", str(src), str(out)], check=True)
+    img = np.fromfile(out, np.float32).reshape(256, 256, 3)
+    sc, cam = scenes.cornell_box()
+    tracer.upload(sc)
+    ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3))["image"]
+    bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
+    assert bad.mean() < 1e-3, bad.sum()
+    assert np.array_equal(read_png(tmp_path / "cornell.png"), film_to_rgba8(img))
+
+
+@pytest.mark.gpu
+def test_gpu_box_with_uvs_matches_oracle(tracer, tmp_path):
+    """create_box meshes carry their uvs, which set the triangles' shading frames; an
+    anisotropic GGX box and a Lambert quad light render as the oracle renders them."""
+    from oracle import orc_py
+    from rainbow_amd.tracer import render_params
+    scene = {
+        "materials": {"brushed": {"metal": {"eta": [0.2, 0.9, 1.1], "k": [3.9, 2.4, 1.6], "roughness": [0.05, 0.4],
+                                            "remap_roughness": False}},
+                      "floor": {"matte": {"diffuse": [0.6, 0.6, 0.6]}}},
+        "entities": [
+            {"shape": {"box": [0.8, 0.5, 0.6]}, "material": "brushed",
+             "transform": [{"rotate": {"axis": [0, 1, 0], "degrees": 35}}]},
+            {"shape": {"quad": [4, 4]}, "material": "floor",
+             "transform": [{"translate": [0, -0.25, 0]}, {"rotate": {"axis": [1, 0, 0], "degrees": -90}}]},
+            {"shape": {"quad": [1, 1]}, "emitter": {"radiance": [8, 8, 8]},
+             "transform": [{"translate": [0, 1.5, 0]}, {"rotate": {"axis": [1, 0, 0], "degrees": 90}}]},
+        ],
+        "camera": {"resolution": [64, 64], "fov": 45, "eye": [1.2, 1.0, 2.2], "target": [0, 0, 0]},
+    }
+    r, out = _load(tmp_path, scene)
+    assert r.returncode == 0, r.stderr
+    f = _read_flat(out)
+    assert f["UV"] is not None
+    d = flat_desc(f)
+    cam = f["CAM"]
+    tracer.upload_desc(d)
+    p = render_params(spp=32, max_depth=4, seed=5)
+    g = tracer.render(cam, p)["image"]
+    o = orc_py.OracleScene.from_desc(d).render(cam, orc_py.render_params(spp=32, max_depth=4, seed=5),
+                                               sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    assert g.mean() > 0
+    bad = (np.abs(g - o) > 1e-4 * np.maximum(np.abs(g), np.abs(o)) + 1e-6).any(-1)
+    assert bad.mean() < 2e-3, bad.sum()
