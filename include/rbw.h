@@ -52,7 +52,7 @@ enum {
 
 /* materials: matte (materials/matte_material.cpp:13-29, Lambert only: sigma must be 0)
  *            metal (materials/metal_material.cpp:17-37, GGX conductor, visible-normal sampling) */
-enum { RBW_MATERIAL_MATTE = 1, RBW_MATERIAL_METAL = 2 };
+enum { RBW_MATERIAL_MATTE = 1, RBW_MATERIAL_METAL = 2, RBW_MATERIAL_PLASTIC = 3 };
 
 typedef struct rbw_material {
     uint32_t type;
@@ -60,8 +60,12 @@ typedef struct rbw_material {
     float sigma;        /* matte Oren-Nayar sigma (only 0 is supported) */
     float eta[3];       /* metal: conductor eta */
     float k[3];         /* metal: conductor absorption */
-    float alpha_u;      /* metal: Trowbridge-Reitz alpha_x (after any roughness map) */
-    float alpha_v;      /* metal: Trowbridge-Reitz alpha_y */
+    float alpha_u;      /* metal, plastic: Trowbridge-Reitz alpha_x (after any roughness map) */
+    float alpha_v;      /* metal, plastic: Trowbridge-Reitz alpha_y (plastic: = alpha_u) */
+    /* plastic (plastic_material.cpp:30-56): a Lambert lobe of `diffuse` and a GGX lobe of
+     * reflectance `specular` with the dielectric Fresnel of eta_i = 1, eta_o = ior */
+    float specular[3];
+    float ior;
 } rbw_material;
 
 /* One rainbow entity (scenes/entity.hpp:45-70): shape + material + emitter.

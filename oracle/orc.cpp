@@ -311,97 +311,168 @@ struct ggx {
     }
 };
 
-enum lobe_kind : uint32_t { LOBE_NONE = 0, LOBE_LAMBERT = 1, LOBE_GGX = 2 };
+enum : uint32_t { LOBE_NONE = 0, LOBE_LAMBERT = 1, LOBE_GGX = 2 };
 
 // The scattering_function_collection of one surface: all configurations use
 // at most one lobe (matte -> Lambert, metal -> microfacet reflection).
+// fresnel_effect.cpp:5-33
+inline real fresnel_dielectric(real cos_i, real eta_i, real eta_o) {
+    cos_i = std::min(std::max(cos_i, static_cast<real>(-1)), static_cast<real>(1));
+    if (cos_i < 0.f) {
+        std::swap(eta_i, eta_o);
+        cos_i = std::fabs(cos_i);
+    }
+    const real sin_i = std::sqrt(std::max(static_cast<real>(0), 1 - cos_i * cos_i));
+    const real sin_o = sin_i * eta_i / eta_o;
+    const real cos_o = std::sqrt(std::max(static_cast<real>(0), 1 - sin_o * sin_o));
+    if (sin_o >= 1) return 1;
+    const real r0 = (eta_o * cos_i - eta_i * cos_o) / (eta_o * cos_i + eta_i * cos_o);
+    const real r1 = (eta_i * cos_i - eta_o * cos_o) / (eta_i * cos_i + eta_o * cos_o);
+    return (r0 * r0 + r1 * r1) / 2;
+}
+
+// A material's scattering_function_collection: at most a Lambert lobe (added first) and a
+// GGX microfacet reflection lobe; matte (Lambert), metal (GGX, conductor Fresnel,
+// reflectance 1), plastic (both; GGX with fresnel_effect_dielectric(1, eta) and
+// reflectance `specular`, plastic_material.cpp:30-56).
 struct bsdf {
-    lobe_kind kind = LOBE_NONE;
-    spectrum R{0};          // Lambert reflectance
-    ggx dist{0.1f, 0.1f};   // GGX
-    spectrum eta{1}, k{0};  // conductor Fresnel
-    bool fresnel_one = false;  // fresnel_effect_nop (fresnel_effect.cpp:90-92), used by closed-form tests
+    uint32_t kind = LOBE_NONE;  // lobe bits
+    spectrum R{0};              // Lambert reflectance
+    spectrum Rs{1};             // GGX reflectance
+    ggx dist{0.1f, 0.1f};
+    spectrum eta{1}, k{0};      // conductor Fresnel
+    real ior = 0;               // > 0: dielectric Fresnel (1, ior)
+    bool fresnel_one = false;   // fresnel_effect_nop (fresnel_effect.cpp:90-92), used by closed-form tests
 
-    uint32_t count() const { return kind == LOBE_NONE ? 0u : 1u; }
+    uint32_t count() const { return ((kind & LOBE_LAMBERT) ? 1u : 0u) + ((kind & LOBE_GGX) ? 1u : 0u); }
 
-    // lambertian_reflection.cpp:10-13 / microfacet_reflection.cpp:13-40
-    spectrum lobe_eval(const vec3& wo, const vec3& wi) const {
-        if (kind == LOBE_LAMBERT) return R * k_one_over_pi;
+    // lambertian_reflection.cpp:10-13
+    spectrum lambert_eval() const { return R * k_one_over_pi; }
+    // microfacet_reflection.cpp:13-40
+    spectrum ggx_eval(const vec3& wo, const vec3& wi) const {
         const real cos_o = std::fabs(cos_theta(wo));
         const real cos_i = std::fabs(cos_theta(wi));
         vec3 wh = wi + wo;
         if (cos_o == 0 || cos_i == 0) return spectrum(0);
         if (wh.x == 0 && wh.y == 0 && wh.z == 0) return spectrum(0);
         wh = face_forward(normalize(wh), vec3(0, 0, 1));
-        const spectrum F = fresnel_one ? spectrum(1) : fresnel_conductor(std::fabs(dot(wi, wh)), eta, k);
+        const spectrum F = fresnel_one ? spectrum(1)
+                           : ior > 0   ? spectrum(fresnel_dielectric(dot(wi, wh), 1, ior))
+                                       : fresnel_conductor(std::fabs(dot(wi, wh)), eta, k);
         const real G = dist.G(wo, wi);
         const real D = dist.D(wh);
-        return spectrum(1) * (G * D) * F / (4 * cos_i * cos_o);
+        return Rs * (G * D) * F / (4 * cos_i * cos_o);
     }
     // reflection_function.cpp:27-30 / microfacet_reflection.cpp:66-73
-    real lobe_pdf(const vec3& wo, const vec3& wi) const {
-        if (kind == LOBE_LAMBERT) return same_hemisphere(wo, wi) ? std::fabs(cos_theta(wi)) * k_one_over_pi : 0;
+    real lambert_pdf(const vec3& wo, const vec3& wi) const {
+        return same_hemisphere(wo, wi) ? std::fabs(cos_theta(wi)) * k_one_over_pi : 0;
+    }
+    real ggx_pdf(const vec3& wo, const vec3& wi) const {
         if (!same_hemisphere(wo, wi)) return 0;
         const vec3 wh = normalize(wo + wi);
         return dist.pdf(wo, wh) / (4 * dot(wo, wh));
     }
-    // returns false for the empty scattering_sample {}
-    // reflection_function.cpp:12-25 / microfacet_reflection.cpp:42-64
-    bool lobe_sample(const vec3& wo, const vec2& u, vec3& wi, spectrum& f, real& pdf) const {
-        if (kind == LOBE_LAMBERT) {
-            wi = cosine_sample_hemisphere(u);
-            if (wo.z < 0) wi.z = -wi.z;
-            f = lobe_eval(wo, wi);
-            pdf = lobe_pdf(wo, wi);
-            return true;
-        }
+    // microfacet_reflection.cpp:42-64; false for the empty scattering_sample {}
+    bool ggx_sample(const vec3& wo, const vec2& u, vec3& wi, spectrum& f, real& pdf) const {
         if (wo.z == 0) return false;
         const vec3 wh = dist.sample(wo, u);
         if (dot(wo, wh) < 0) return false;
         wi = reflect(wo, wh);
         if (!same_hemisphere(wo, wi)) return false;
-        f = lobe_eval(wo, wi);
+        f = ggx_eval(wo, wi);
         pdf = dist.pdf(wo, wh) / (4 * dot(wo, wh));
         return true;
     }
 
-    // scattering_function_collection.cpp:29-46 (include matches the lobe in every call made)
+    // scattering_function_collection.cpp:29-46 (every lobe reflects)
     spectrum evaluate(const vec3& wo, const vec3& wi) const {
         if (wo.z == 0) return spectrum(0);
         spectrum f(0);
-        // reflection lobes contribute only when wo and wi share the hemisphere
-        if (count() && same_hemisphere(wo, wi)) f = f + lobe_eval(wo, wi);
+        if (count() && same_hemisphere(wo, wi)) {
+            if (kind & LOBE_LAMBERT) f = f + lambert_eval();
+            if (kind & LOBE_GGX) f = f + ggx_eval(wo, wi);
+        }
         return f;
     }
     // scattering_function_collection.cpp:131-148
     real pdf(const vec3& wo, const vec3& wi) const {
         if (count() == 0 || wo.z == 0) return 0;
         real p = 0;
-        p = p + lobe_pdf(wo, wi);
-        return p / 1;
+        if (kind & LOBE_LAMBERT) p = p + lambert_pdf(wo, wi);
+        if (kind & LOBE_GGX) p = p + ggx_pdf(wo, wi);
+        return p / count();
     }
-    // scattering_function_collection.cpp:48-103; wi returned in world space
-    bool sample(const surface_interaction& si, const vec2& u, vec3& wi_world, spectrum& f, real& pdf) const {
+    // scattering_function_collection.cpp:48-103; wi in shading space
+    bool sample_local(const vec3& wo, const vec2& u, vec3& wi, spectrum& f, real& pdf) const {
         if (count() == 0) return false;
-        const size_t n = 1;
+        const size_t n = count();
         const size_t which = std::min(static_cast<size_t>(std::floor(u.x * n)), n - 1);
         const vec2 remapped(std::min(u.x * n - which, static_cast<real>(1) - std::numeric_limits<real>::epsilon()), u.y);
-        const vec3 wo = world_to_local(si.shading, si.wo);
         if (wo.z == 0) return false;
-        vec3 wi;
-        if (!lobe_sample(wo, remapped, wi, f, pdf)) return false;
+        if ((kind & LOBE_LAMBERT) && which == 0) {  // reflection_function.cpp:12-25
+            wi = cosine_sample_hemisphere(remapped);
+            if (wo.z < 0) wi.z = -wi.z;
+            pdf = lambert_pdf(wo, wi);
+        } else {
+            spectrum unused;
+            if (!ggx_sample(wo, remapped, wi, unused, pdf)) return false;
+        }
         if (pdf == 0) return false;
-        // non-specular lobe: re-evaluate value and pdf over the matched lobes
+        // non-specular lobes: value and pdf over every lobe
         const bool reflection = same_hemisphere(wo, wi);
         f = spectrum(0);
         pdf = 0;
-        if (reflection) f = f + lobe_eval(wo, wi);
-        pdf = pdf + lobe_pdf(wo, wi);
+        if (kind & LOBE_LAMBERT) {
+            if (reflection) f = f + lambert_eval();
+            pdf = pdf + lambert_pdf(wo, wi);
+        }
+        if (kind & LOBE_GGX) {
+            if (reflection) f = f + ggx_eval(wo, wi);
+            pdf = pdf + ggx_pdf(wo, wi);
+        }
         pdf = pdf / n;
+        return true;
+    }
+    bool sample(const surface_interaction& si, const vec2& u, vec3& wi_world, spectrum& f, real& pdf) const {
+        if (count() == 0) return false;
+        const vec3 wo = world_to_local(si.shading, si.wo);
+        vec3 wi;
+        if (!sample_local(wo, u, wi, f, pdf)) return false;
         wi_world = local_to_world(si.shading, wi);
         return true;
     }
 };
+
+// the lobes of an rbw_material (matte_material.cpp:13-29, metal_material.cpp:17-37,
+// plastic_material.cpp:30-56)
+inline bsdf material_bsdf(const rbw_material& m) {
+    bsdf b;
+    if (m.type == RBW_MATERIAL_MATTE) {
+        const spectrum d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        if (is_black(d)) return b;
+        b.kind = LOBE_LAMBERT;
+        b.R = d;
+    } else if (m.type == RBW_MATERIAL_METAL) {
+        b.kind = LOBE_GGX;
+        b.dist = ggx{m.alpha_u, m.alpha_v};
+        b.eta = spectrum(m.eta[0], m.eta[1], m.eta[2]);
+        b.k = spectrum(m.k[0], m.k[1], m.k[2]);
+    } else {
+        const spectrum d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        const spectrum sp(m.specular[0], m.specular[1], m.specular[2]);
+        if (!is_black(d)) {
+            b.kind |= LOBE_LAMBERT;
+            b.R = d;
+        }
+        if (!is_black(sp)) {
+            b.kind |= LOBE_GGX;
+            b.Rs = sp;
+            b.dist = ggx{m.alpha_u, m.alpha_v};
+            b.ior = m.ior;
+        }
+    }
+    return b;
+}
 
 // ---------------------------------------------------------------- scene
 
@@ -887,24 +958,9 @@ struct render_ctx {
 };
 
 inline bsdf build_bsdf(const orc_scene& s, const surface_interaction& si) {
-    bsdf b;
     uint32_t mid = s.ents[si.entity].e.material;
     if (si.prim < s.num_tris && !s.tri_material.empty() && s.tri_material[si.prim] != NONE) mid = s.tri_material[si.prim];
-    const rbw_material& m = s.mats[mid];
-    if (m.type == RBW_MATERIAL_MATTE) {
-        // matte_material.cpp:13-29
-        const spectrum d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-        if (is_black(d)) return b;
-        b.kind = LOBE_LAMBERT;
-        b.R = d;
-    } else {
-        // metal_material.cpp:17-37
-        b.kind = LOBE_GGX;
-        b.dist = ggx{m.alpha_u, m.alpha_v};
-        b.eta = spectrum(m.eta[0], m.eta[1], m.eta[2]);
-        b.k = spectrum(m.k[0], m.k[1], m.k[2]);
-    }
-    return b;
+    return material_bsdf(s.mats[mid]);
 }
 
 // integrator.cpp:423-426
@@ -1555,7 +1611,7 @@ void orc_ggx_sample_f(float ax, float ay, const float* eta, const float* k, cons
     vec3 w;
     spectrum ff(0);
     real p = 0;
-    if (!b.lobe_sample(v3(wo), vec2(u0, u1), w, ff, p)) { w = vec3(0); ff = spectrum(0); p = 0; }
+    if (!b.ggx_sample(v3(wo), vec2(u0, u1), w, ff, p)) { w = vec3(0); ff = spectrum(0); p = 0; }
     wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
     f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
     *pdf = p;
@@ -1563,13 +1619,37 @@ void orc_ggx_sample_f(float ax, float ay, const float* eta, const float* k, cons
 
 float orc_ggx_pdf(float ax, float ay, const float* wo, const float* wi) {
     const bsdf b = make_ggx(ax, ay, nullptr, nullptr);
-    return b.lobe_pdf(v3(wo), v3(wi));
+    return b.ggx_pdf(v3(wo), v3(wi));
 }
 
 void orc_ggx_eval(float ax, float ay, const float* eta, const float* k, const float* wo, const float* wi, float* f) {
     const bsdf b = make_ggx(ax, ay, eta, k);
-    const spectrum r = b.lobe_eval(v3(wo), v3(wi));
+    const spectrum r = b.ggx_eval(v3(wo), v3(wi));
     f[0] = r.x; f[1] = r.y; f[2] = r.z;
+}
+
+float orc_fresnel_dielectric(float cos_theta_i, float eta_i, float eta_o) {
+    return fresnel_dielectric(cos_theta_i, eta_i, eta_o);
+}
+
+float orc_material_pdf(const rbw_material* m, const float* wo, const float* wi) {
+    return material_bsdf(*m).pdf(v3(wo), v3(wi));
+}
+
+void orc_material_eval(const rbw_material* m, const float* wo, const float* wi, float* f) {
+    const spectrum r = material_bsdf(*m).evaluate(v3(wo), v3(wi));
+    f[0] = r.x; f[1] = r.y; f[2] = r.z;
+}
+
+int orc_material_sample(const rbw_material* m, const float* wo, float u0, float u1, float* wi, float* f, float* pdf) {
+    vec3 w(0);
+    spectrum ff(0);
+    real p = 0;
+    const bool ok = material_bsdf(*m).sample_local(v3(wo), vec2(u0, u1), w, ff, p);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
+    *pdf = ok ? p : 0;
+    return ok ? 1 : 0;
 }
 
 void orc_fresnel_conductor(float cos_theta_i, const float* eta, const float* k, float* out3) {

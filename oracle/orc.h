@@ -89,6 +89,12 @@ float orc_ggx_pdf(float ax, float ay, const float* wo, const float* wi);
 void orc_ggx_eval(float ax, float ay, const float* eta, const float* k, const float* wo,
                   const float* wi, float* f);
 void orc_fresnel_conductor(float cos_theta_i, const float* eta, const float* k, float* out3);
+/* fresnel_reflect_dielectric (fresnel_effect.cpp:5-33) */
+float orc_fresnel_dielectric(float cos_theta_i, float eta_i, float eta_o);
+/* a material's lobe collection in shading space (scattering_function_collection.cpp) */
+float orc_material_pdf(const rbw_material* m, const float* wo, const float* wi);
+void orc_material_eval(const rbw_material* m, const float* wo, const float* wi, float* f);
+int orc_material_sample(const rbw_material* m, const float* wo, float u0, float u1, float* wi, float* f, float* pdf);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ RBW_ENTITY_EMITTER = 1
 RBW_ENTITY_REVERSE_ORIENTATION = 2
 RBW_ENTITY_HAS_NORMALS = 4
 RBW_ENTITY_HAS_UVS = 8
-RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL = 1, 2
+RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL, RBW_MATERIAL_PLASTIC = 1, 2, 3
 RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY, RBW_SAMPLER_STRATIFIED = 0, 1, 2
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
@@ -44,7 +44,7 @@ u64p = C.POINTER(C.c_uint64)
 class rbw_material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("diffuse", C.c_float * 3), ("sigma", C.c_float),
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("alpha_u", C.c_float),
-                ("alpha_v", C.c_float)]
+                ("alpha_v", C.c_float), ("specular", C.c_float * 3), ("ior", C.c_float)]
 
 
 class rbw_entity(C.Structure):

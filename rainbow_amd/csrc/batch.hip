@@ -271,7 +271,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths
     stripe_add64(B.counters, 0, rays);
 }
 
-__global__ __launch_bounds__(k_batch_max_block) void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
+__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;

@@ -30,6 +30,8 @@ struct dmaterial {
     float eta[3];
     float k[3];
     float alpha_u, alpha_v;
+    float specular[3];
+    float ior;
 };
 
 struct dscene {

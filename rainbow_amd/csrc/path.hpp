@@ -29,6 +29,8 @@ inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * b
 // The shadow/MIS kernels are latency bound on dependent node fetches: six waves per
 // SIMD (at most 80 VGPRs, a few spilled) beat the five their natural 95 VGPRs allow.
 #define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
+// The shading kernel keeps four waves per SIMD (at most 128 VGPRs).
+#define RBW_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 
 // ------------------------------------------------------------------ traversal
 

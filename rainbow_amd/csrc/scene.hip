@@ -92,8 +92,10 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
     require(d->num_materials > 0 && d->materials, RBW_ERR_INVALID_ARGUMENT, "scene has no materials");
     for (uint32_t i = 0; i < d->num_materials; i++) {
         const rbw_material& m = d->materials[i];
-        require(m.type == RBW_MATERIAL_MATTE || m.type == RBW_MATERIAL_METAL, RBW_ERR_INVALID_ARGUMENT,
-                "unknown material type");
+        require(m.type == RBW_MATERIAL_MATTE || m.type == RBW_MATERIAL_METAL || m.type == RBW_MATERIAL_PLASTIC,
+                RBW_ERR_INVALID_ARGUMENT, "unknown material type");
+        require(m.type != RBW_MATERIAL_PLASTIC || (m.alpha_u > 0.0f && m.alpha_v > 0.0f && m.ior > 0.0f),
+                RBW_ERR_INVALID_ARGUMENT, "plastic alpha and ior must be positive");
         require(m.type != RBW_MATERIAL_MATTE || m.sigma == 0.0f, RBW_ERR_UNSUPPORTED,
                 "matte sigma != 0 (Oren-Nayar) is not supported");
         require(m.type != RBW_MATERIAL_METAL || (m.alpha_u > 0.0f && m.alpha_v > 0.0f), RBW_ERR_INVALID_ARGUMENT,
@@ -220,9 +222,11 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
             o.diffuse[c] = m.diffuse[c];
             o.eta[c] = m.eta[c];
             o.k[c] = m.k[c];
+            o.specular[c] = m.specular[c];
         }
         o.alpha_u = m.alpha_u;
         o.alpha_v = m.alpha_v;
+        o.ior = m.ior;
     }
     h.nt = nt;
     h.ns = ns;

@@ -310,81 +310,116 @@ struct ggx {
 enum : uint32_t { LOBE_NONE = 0, LOBE_LAMBERT = 1, LOBE_GGX = 2 };
 
 // A one-lobe scattering_function_collection (scatterings/scattering_function_collection.cpp)
-struct bsdf {
-    uint32_t kind;
-    f3 R;
-    ggx dist;
-    f3 eta, k;
+// scatterings/fresnel_effect.cpp:5-33
+RBW_HD float fresnel_dielectric(float cos_i, float eta_i, float eta_o) {
+    cos_i = gclamp(cos_i, -1.0f, 1.0f);
+    if (cos_i < 0.0f) {  // wi on the eta_o side: the roles swap
+        const float t = eta_i;
+        eta_i = eta_o;
+        eta_o = t;
+        cos_i = fabsf(cos_i);
+    }
+    const float sin_i = sqrtf(gmax(0.0f, 1.0f - cos_i * cos_i));
+    const float sin_o = sin_i * eta_i / eta_o;
+    const float cos_o = sqrtf(gmax(0.0f, 1.0f - sin_o * sin_o));
+    if (sin_o >= 1.0f) return 1.0f;
+    const float r0 = (eta_o * cos_i - eta_i * cos_o) / (eta_o * cos_i + eta_i * cos_o);
+    const float r1 = (eta_i * cos_i - eta_o * cos_o) / (eta_i * cos_i + eta_o * cos_o);
+    return (r0 * r0 + r1 * r1) / 2.0f;
+}
 
-    RBW_HD f3 lobe_eval(f3 wo, f3 wi) const {
-        if (kind == LOBE_LAMBERT) return R * k_one_over_pi;  // lambertian_reflection.cpp:10-13
-        // microfacet_reflection.cpp:13-40
+// A material's scattering_function_collection (scattering_function_collection.cpp):
+// at most a Lambert lobe (index 0) and a GGX reflection lobe (index 1), every lobe a
+// reflection lobe.  Matte: Lambert; metal: GGX with conductor Fresnel, reflectance 1;
+// plastic: both, GGX with dielectric Fresnel (1, ior) and reflectance `Rs`.
+struct bsdf {
+    uint32_t kind;        // LOBE_NONE, or the lobes present (LOBE_LAMBERT | LOBE_GGX bits)
+    f3 R;                 // Lambert reflectance
+    f3 Rs;                // GGX reflectance
+    ggx dist;
+    f3 eta, k;            // conductor Fresnel
+    float ior;            // > 0: dielectric Fresnel instead
+    RBW_HD uint32_t count() const { return ((kind & LOBE_LAMBERT) ? 1u : 0u) + ((kind & LOBE_GGX) ? 1u : 0u); }
+
+    // lambertian_reflection.cpp:10-13
+    RBW_HD f3 lambert_eval() const { return R * k_one_over_pi; }
+    // microfacet_reflection.cpp:13-40
+    RBW_HD f3 ggx_eval(f3 wo, f3 wi) const {
         const float cos_o = fabsf(wo.z);
         const float cos_i = fabsf(wi.z);
         f3 wh = wi + wo;
         if (cos_o == 0.0f || cos_i == 0.0f) return splat(0.0f);
         if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return splat(0.0f);
         wh = face_forward(normalize(wh), mk(0.0f, 0.0f, 1.0f));
-        const f3 F = fresnel_conductor(fabsf(dot(wi, wh)), eta, k);
+        const f3 F = ior > 0.0f ? splat(fresnel_dielectric(dot(wi, wh), 1.0f, ior))
+                                : fresnel_conductor(fabsf(dot(wi, wh)), eta, k);
         const float G = dist.G(wo, wi);
         const float D = dist.D(wh);
-        return splat(1.0f) * (G * D) * F / (4.0f * cos_i * cos_o);
+        return Rs * (G * D) * F / (4.0f * cos_i * cos_o);
     }
-    RBW_HD float lobe_pdf(f3 wo, f3 wi) const {
-        if (kind == LOBE_LAMBERT) return same_hemisphere(wo, wi) ? fabsf(wi.z) * k_one_over_pi : 0.0f;
+    // reflection_function.cpp:27-30 / microfacet_reflection.cpp:66-73
+    RBW_HD float lambert_pdf(f3 wo, f3 wi) const { return same_hemisphere(wo, wi) ? fabsf(wi.z) * k_one_over_pi : 0.0f; }
+    RBW_HD float ggx_pdf(f3 wo, f3 wi) const {
         if (!same_hemisphere(wo, wi)) return 0.0f;
         const f3 wh = normalize(wo + wi);
         return dist.pdf(wo, wh) / (4.0f * dot(wo, wh));
     }
-    RBW_HD bool lobe_sample(f3 wo, f2 u, f3& wi, f3& f, float& pdf) const {
-        if (kind == LOBE_LAMBERT) {  // reflection_function.cpp:12-25
-            wi = cosine_sample_hemisphere(u);
-            if (wo.z < 0.0f) wi.z = -wi.z;
-            f = lobe_eval(wo, wi);
-            pdf = lobe_pdf(wo, wi);
-            return true;
-        }
-        // microfacet_reflection.cpp:42-64
-        if (wo.z == 0.0f) return false;
-        const f3 wh = dist.sample(wo, u);
-        if (dot(wo, wh) < 0.0f) return false;
-        wi = reflect(wo, wh);
-        if (!same_hemisphere(wo, wi)) return false;
-        f = lobe_eval(wo, wi);
-        pdf = dist.pdf(wo, wh) / (4.0f * dot(wo, wh));
-        return true;
-    }
-    // scattering_function_collection.cpp:29-46
+    // scattering_function_collection.cpp:29-46 (every lobe reflects)
     RBW_HD f3 evaluate(f3 wo, f3 wi) const {
         if (wo.z == 0.0f) return splat(0.0f);
         f3 f = splat(0.0f);
-        if (kind != LOBE_NONE && same_hemisphere(wo, wi)) f = f + lobe_eval(wo, wi);
+        if (kind != LOBE_NONE && same_hemisphere(wo, wi)) {
+            if (kind & LOBE_LAMBERT) f = f + lambert_eval();
+            if (kind & LOBE_GGX) f = f + ggx_eval(wo, wi);
+        }
         return f;
     }
     // scattering_function_collection.cpp:131-148
     RBW_HD float pdf(f3 wo, f3 wi) const {
         if (kind == LOBE_NONE || wo.z == 0.0f) return 0.0f;
         float p = 0.0f;
-        p = p + lobe_pdf(wo, wi);
-        return p / 1.0f;
+        if (kind & LOBE_LAMBERT) p = p + lambert_pdf(wo, wi);
+        if (kind & LOBE_GGX) p = p + ggx_pdf(wo, wi);
+        return p / (float)count();
     }
-    // scattering_function_collection.cpp:48-103 (one lobe: which = 0, u.x unchanged)
+    // scattering_function_collection.cpp:48-103: lobe `which` = floor(u.x * n), u.x remapped
     RBW_HD bool sample(const surf& s, f2 u, f3& wi_world, f3& f, float& pdf) const {
         if (kind == LOBE_NONE) return false;
-        // std::min(u.x * n - which, 1 - eps) with n = 1, which = 0
-        const float a = u.x * 1.0f - 0.0f;
+        const uint32_t n = count();
+        uint32_t which = (uint32_t)floorf(u.x * (float)n);
+        if (which > n - 1u) which = n - 1u;
+        const float a = u.x * (float)n - (float)which;
         const f2 rem{(k_one_minus_epsilon < a) ? k_one_minus_epsilon : a, u.y};
         const f3 wo = world_to_local(s.sh, s.wo);
         if (wo.z == 0.0f) return false;
+        const bool pick_lambert = (kind & LOBE_LAMBERT) && which == 0u;
         f3 wi;
-        if (!lobe_sample(wo, rem, wi, f, pdf)) return false;
+        if (pick_lambert) {  // reflection_function.cpp:12-25
+            wi = cosine_sample_hemisphere(rem);
+            if (wo.z < 0.0f) wi.z = -wi.z;
+            pdf = lambert_pdf(wo, wi);
+        } else {  // microfacet_reflection.cpp:42-64
+            if (wo.z == 0.0f) return false;
+            const f3 wh = dist.sample(wo, rem);
+            if (dot(wo, wh) < 0.0f) return false;
+            wi = reflect(wo, wh);
+            if (!same_hemisphere(wo, wi)) return false;
+            pdf = dist.pdf(wo, wh) / (4.0f * dot(wo, wh));
+        }
         if (pdf == 0.0f) return false;
+        // value and pdf over all lobes
         const bool refl = same_hemisphere(wo, wi);
         f = splat(0.0f);
         pdf = 0.0f;
-        if (refl) f = f + lobe_eval(wo, wi);
-        pdf = pdf + lobe_pdf(wo, wi);
-        pdf = pdf / 1.0f;
+        if (kind & LOBE_LAMBERT) {
+            if (refl) f = f + lambert_eval();
+            pdf = pdf + lambert_pdf(wo, wi);
+        }
+        if (kind & LOBE_GGX) {
+            if (refl) f = f + ggx_eval(wo, wi);
+            pdf = pdf + ggx_pdf(wo, wi);
+        }
+        pdf = pdf / (float)n;
         wi_world = local_to_world(s.sh, wi);
         return true;
     }
@@ -394,9 +429,11 @@ RBW_HD bsdf make_bsdf(const dscene& S, uint32_t entity_material, uint32_t prim) 
     bsdf b;
     b.kind = LOBE_NONE;
     b.R = splat(0.0f);
+    b.Rs = splat(1.0f);
     b.dist = ggx{0.1f, 0.1f};
     b.eta = splat(1.0f);
     b.k = splat(0.0f);
+    b.ior = 0.0f;
     uint32_t mid = entity_material;
     if (S.tri_material && prim < S.num_tris && S.tri_material[prim] != RBW_NONE) mid = S.tri_material[prim];
     const dmaterial m = S.mats[mid];
@@ -405,11 +442,24 @@ RBW_HD bsdf make_bsdf(const dscene& S, uint32_t entity_material, uint32_t prim) 
         if (is_black(d)) return b;
         b.kind = LOBE_LAMBERT;
         b.R = d;
-    } else {  // metal_material.cpp:17-37
+    } else if (m.type == RBW_MATERIAL_METAL) {  // metal_material.cpp:17-37
         b.kind = LOBE_GGX;
         b.dist = ggx{m.alpha_u, m.alpha_v};
         b.eta = mk(m.eta[0], m.eta[1], m.eta[2]);
         b.k = mk(m.k[0], m.k[1], m.k[2]);
+    } else {  // plastic_material.cpp:30-56
+        const f3 d = mk(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+        const f3 sp = mk(m.specular[0], m.specular[1], m.specular[2]);
+        if (!is_black(d)) {
+            b.kind |= LOBE_LAMBERT;
+            b.R = d;
+        }
+        if (!is_black(sp)) {
+            b.kind |= LOBE_GGX;
+            b.Rs = sp;
+            b.dist = ggx{m.alpha_u, m.alpha_v};
+            b.ior = m.ior;
+        }
     }
     return b;
 }

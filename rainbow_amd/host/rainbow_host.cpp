@@ -248,6 +248,16 @@ flat_scene flatten(const scene& s) {
                     const real ru = mt->roughness_u->value(), rv = mt->roughness_v->value();
                     rm.alpha_u = mt->map_roughness_to_alpha ? materials::roughness_to_alpha(ru) : ru;
                     rm.alpha_v = mt->map_roughness_to_alpha ? materials::roughness_to_alpha(rv) : rv;
+                } else if (auto pm = std::dynamic_pointer_cast<materials::plastic_material>(m)) {
+                    // plastic_material.cpp:30-56: Lambert + GGX (isotropic) with dielectric Fresnel
+                    rm.type = RBW_MATERIAL_PLASTIC;
+                    for (int c = 0; c < 3; c++) {
+                        rm.diffuse[c] = pm->diffuse->value()[c];
+                        rm.specular[c] = pm->specular->value()[c];
+                    }
+                    const real r = pm->roughness->value();
+                    rm.alpha_u = rm.alpha_v = pm->map_roughness_to_alpha ? materials::roughness_to_alpha(r) : r;
+                    rm.ior = pm->eta->value();
                 } else {
                     throw std::runtime_error("unsupported material");
                 }

@@ -187,6 +187,23 @@ public:
     bool map_roughness_to_alpha;
 };
 
+// materials/plastic_material.cpp:9-28 (eta 1.49 unless given)
+class plastic_material final : public material {
+public:
+    plastic_material(const std::shared_ptr<texture2d<spectrum>>& specular, const std::shared_ptr<texture2d<spectrum>>& diffuse,
+                     const std::shared_ptr<texture2d<real>>& roughness, bool map_roughness_to_alpha = true)
+        : plastic_material(specular, diffuse, roughness, std::make_shared<texture2d<real>>(1.49f),
+                           map_roughness_to_alpha) {}
+    plastic_material(const std::shared_ptr<texture2d<spectrum>>& specular, const std::shared_ptr<texture2d<spectrum>>& diffuse,
+                     const std::shared_ptr<texture2d<real>>& roughness, const std::shared_ptr<texture2d<real>>& eta,
+                     bool map_roughness_to_alpha = true)
+        : specular(specular), diffuse(diffuse), roughness(roughness), eta(eta),
+          map_roughness_to_alpha(map_roughness_to_alpha) {}
+    std::shared_ptr<texture2d<spectrum>> specular, diffuse;
+    std::shared_ptr<texture2d<real>> roughness, eta;
+    bool map_roughness_to_alpha;
+};
+
 // trowbridge_reitz_distribution.cpp:128-136
 real roughness_to_alpha(real roughness);
 }  // namespace materials
@@ -410,6 +427,7 @@ private:
 namespace scenes {
 // A scene description file (JSON): materials, entities, camera and render settings.
 //   {"materials": {"white": {"matte": {"diffuse": [0.73, 0.73, 0.73], "sigma": 0}},
+//                  "red": {"plastic": {"diffuse": [...], "specular": [...], "roughness": 0.1, "eta": 1.49}},
 //                  "gold": {"metal": {"eta": [...], "k": [...], "roughness": 0.1, "remap_roughness": false}}},
 //    "entities": [{"shape": {"box": [w, h, d]} | {"quad": [w, h]} | {"sphere": r} |
 //                           {"mesh": {"positions": [...], "indices": [...], "normals": [...]}},

@@ -176,7 +176,8 @@ std::shared_ptr<textures::constant_texture2d<T>> constant(const T& v) {
 }
 
 std::shared_ptr<materials::material> make_material(const json& m, const std::string& where) {
-    if (m.kind != json::object_v || m.o.size() != 1) bad(where, "a material is {\"matte\": {...}} or {\"metal\": {...}}");
+    if (m.kind != json::object_v || m.o.size() != 1)
+        bad(where, "a material is {\"matte\": {...}}, {\"metal\": {...}} or {\"plastic\": {...}}");
     const std::string& type = m.o[0].first;
     const json& a = m.o[0].second;
     if (type == "matte")
@@ -198,6 +199,13 @@ std::shared_ptr<materials::material> make_material(const json& m, const std::str
                                                            constant(rgb(member(a, "k", where), where + ".k")),
                                                            constant(ru), constant(rv),
                                                            remap ? (remap->kind == json::bool_v && remap->b) : true);
+    }
+    if (type == "plastic") {
+        const json* remap = a.find("remap_roughness");
+        return std::make_shared<materials::plastic_material>(
+            constant(rgb(member(a, "specular", where), where + ".specular")),
+            constant(rgb(member(a, "diffuse", where), where + ".diffuse")), constant(real(number_or(a, "roughness", 0.1, where))),
+            constant(real(number_or(a, "eta", 1.49, where))), remap ? (remap->kind == json::bool_v && remap->b) : true);
     }
     bad(where, "unknown material type \"" + type + "\"");
 }

@@ -28,7 +28,7 @@ from . import abi
 # numpy layout of rbw_material (13 packed 4-byte fields)
 MATERIAL_DTYPE = np.dtype([("type", np.uint32), ("diffuse", np.float32, 3), ("sigma", np.float32),
                            ("eta", np.float32, 3), ("k", np.float32, 3), ("alpha_u", np.float32),
-                           ("alpha_v", np.float32)])
+                           ("alpha_v", np.float32), ("specular", np.float32, 3), ("ior", np.float32)])
 assert MATERIAL_DTYPE.itemsize == C.sizeof(abi.rbw_material)
 
 # gold conductor (pbrt-v3 Au, RGB)
@@ -64,6 +64,14 @@ class Scene:
         m = np.zeros(1, MATERIAL_DTYPE)
         m["type"] = abi.RBW_MATERIAL_METAL
         m["eta"], m["k"], m["alpha_u"], m["alpha_v"] = eta, k, alpha, alpha
+        self.materials.append(m)
+        return self.num_materials() - 1
+
+    def plastic(self, diffuse, specular, alpha, ior=1.49):
+        """plastic_material (plastic_material.cpp:30-56): Lambert + GGX with dielectric Fresnel"""
+        m = np.zeros(1, MATERIAL_DTYPE)
+        m["type"] = abi.RBW_MATERIAL_PLASTIC
+        m["diffuse"], m["specular"], m["alpha_u"], m["alpha_v"], m["ior"] = diffuse, specular, alpha, alpha, ior
         self.materials.append(m)
         return self.num_materials() - 1
 

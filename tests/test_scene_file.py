@@ -4,6 +4,7 @@ and entity transforms.  The Cornell box written as a file must flatten to the sa
 descriptors as the scene built in code; boxes and quads are checked by their geometry
 (corners, outward winding, normals) after the transforms."""
 import json
+import math
 import subprocess
 
 import numpy as np
@@ -155,3 +156,25 @@ def test_gpu_box_with_uvs_matches_oracle(tracer, tmp_path):
     assert g.mean() > 0
     bad = (np.abs(g - o) > 1e-4 * np.maximum(np.abs(g), np.abs(o)) + 1e-6).any(-1)
     assert bad.mean() < 2e-3, bad.sum()
+
+
+def test_plastic_material_in_a_file(tmp_path):
+    scene = {
+        "materials": {"p": {"plastic": {"diffuse": [0.5, 0.1, 0.1], "specular": [0.4, 0.4, 0.4], "roughness": 0.3}},
+                      "q": {"plastic": {"diffuse": [0.1, 0.1, 0.5], "specular": 1, "roughness": 0.2, "eta": 1.7,
+                                        "remap_roughness": False}}},
+        "entities": [{"shape": {"sphere": 0.5}, "material": "p"},
+                     {"shape": {"sphere": 0.2}, "material": "q", "transform": [{"translate": [1, 0, 0]}]}],
+        "camera": {"resolution": [8, 8], "eye": [0, 0, 3], "target": [0, 0, 0]},
+    }
+    r, out = _load(tmp_path, scene)
+    assert r.returncode == 0, r.stderr
+    M = _read_flat(out)["M"]
+    assert M["type"].tolist() == [abi.RBW_MATERIAL_PLASTIC] * 2
+    assert np.allclose(M["specular"][0], 0.4) and np.allclose(M["diffuse"][1], [0.1, 0.1, 0.5])
+    assert M["ior"][0] == np.float32(1.49) and M["ior"][1] == np.float32(1.7)
+    assert M["alpha_u"][1] == np.float32(0.2) and M["alpha_v"][1] == np.float32(0.2)
+    # roughness 0.3 through roughness_to_alpha (trowbridge_reitz_distribution.cpp:128-136)
+    x = math.log(0.3)
+    want = 1.62142 + 0.819955 * x + 0.1734 * x * x + 0.0171201 * x ** 3 + 0.000640711 * x ** 4
+    assert M["alpha_u"][0] == pytest.approx(want, rel=1e-5) and M["alpha_u"][0] == M["alpha_v"][0]
