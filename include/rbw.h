@@ -134,6 +134,16 @@ enum {
 enum { RBW_TRI_MOLLER_TRUMBORE = 0, RBW_TRI_WATERTIGHT = 1 };
 
 enum {
+    /* path_integrator (path_integrator.cpp:15-47), the default */
+    RBW_INTEGRATOR_PATH = 0,
+    /* direct_integrator (direct_integrator.cpp:20-140): emission plus emitter_samples
+     * light samples and bsdf_samples BSDF samples at the first surface with lobes (the
+     * ray passes through surfaces without lobes), MIS by the power heuristic over the
+     * sample fractions; max_depth and rr_threshold are not used */
+    RBW_INTEGRATOR_DIRECT = 1
+};
+
+enum {
     RBW_SHADOW_CLOSEST_VISIBLE = 0,  /* scene.cpp:97-112 semantics */
     RBW_SHADOW_ANY_HIT = 1           /* occlusion test with early exit */
 };
@@ -188,6 +198,8 @@ typedef struct rbw_render_params {
     /* RBW_SAMPLER_STRATIFIED: strata_x * strata_y must equal spp; stratified 1D and 2D
      * dimensions (at most 65535 each) */
     uint32_t strata_x, strata_y, dims_1d, dims_2d;
+    uint32_t integrator;             /* RBW_INTEGRATOR_* */
+    uint32_t emitter_samples, bsdf_samples;  /* RBW_INTEGRATOR_DIRECT (0 = 1) */
 } rbw_render_params;
 /* Schedules: the plain estimator with the per-sample or stratified sampler (no replay, no debug
  * hits, no visit counts, no guiding) runs on persistent path slots regenerated in

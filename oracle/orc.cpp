@@ -1175,6 +1175,75 @@ inline spectrum trace_path(const render_ctx& ctx, sample_stream& smp, const ray&
     return ps.value;
 }
 
+// direct_integrator::trace (direct_integrator.cpp:20-128), recursive through surfaces
+// without scattering functions as the reference recurses
+inline spectrum trace_direct(const render_ctx& ctx, sample_stream& smp, const ray& r, counters& cnt, int guard = 0) {
+    const orc_scene& s = *ctx.s;
+    if (s.emitters.empty()) return spectrum(0);
+    spectrum L(0);
+    cnt.extension++;
+    hit_candidate c;
+    ray rr = r;
+    if (!scene_intersect(s, rr, false, c)) return L;  // no environment emitters
+    const surface_interaction si = candidate_interaction(s, c);
+    const entity_data& ent = s.ents[si.entity];
+    if (ent.emitter()) L = L + emitter_evaluate(s, si.entity, si, -r.direction);
+    bsdf fn;
+    if (ent.e.material != NONE) fn = build_bsdf(s, si);
+    if (fn.count() == 0) {
+        if (guard > 1024) return L;
+        return L + trace_direct(ctx, smp, spawn_ray(si, r.direction), cnt, guard + 1);
+    }
+    const uint32_t ne = ctx.p->emitter_samples ? ctx.p->emitter_samples : 1u;
+    const uint32_t nb = ctx.p->bsdf_samples ? ctx.p->bsdf_samples : 1u;
+    const size_t sum = size_t(ne) + nb;
+    const real we = static_cast<real>(1) / ne, wb = static_cast<real>(1) / nb;
+    const real fe = static_cast<real>(ne) / sum, fb = static_cast<real>(nb) / sum;
+    const vec3 wo = world_to_local(si.shading, si.wo);
+    for (uint32_t i = 0; i < ne; i++) {
+        const size_t n = s.emitters.size();
+        const size_t which = std::min(static_cast<size_t>(std::floor(smp.next() * n)), n - 1);
+        const real pick_pdf = static_cast<real>(1) / n;
+        const uint32_t eid = s.emitters[which];
+        emitter_sample es = sample_emitter(s, eid, si, smp.next2());
+        es.pdf = es.pdf * pick_pdf;
+        if (is_black(es.intensity) || !(es.pdf > 0)) continue;
+        const vec3 wi = world_to_local(si.shading, es.wi);
+        spectrum f = fn.evaluate(wo, wi);
+        const real fp = fn.pdf(wo, wi);
+        f = f * std::fabs(dot(es.wi, si.shading.z));
+        if (is_black(f) || fp <= 0) continue;
+        ray sr = spawn_ray_to(si, es.it.point);
+        cnt.shadow++;
+        hit_candidate sc;
+        const bool hit = scene_intersect(s, sr, true, sc);
+        if (!hit || sc.entity == eid) {
+            const real w = power_heuristic(es.pdf * fe, fp * fb) * we;
+            L = L + f * es.intensity * w / es.pdf;
+        }
+    }
+    for (uint32_t i = 0; i < nb; i++) {
+        vec3 wi;
+        spectrum f;
+        real pdf = 0;
+        if (!fn.sample(si, smp.next2(), wi, f, pdf)) continue;
+        f = f * std::fabs(dot(wi, si.shading.z));
+        if (is_black(f) || !(pdf > 0)) continue;
+        ray er = spawn_ray(si, wi);
+        cnt.mis++;
+        hit_candidate ec;
+        if (!scene_intersect(s, er, true, ec) || !s.ents[ec.entity].emitter()) continue;
+        const real pick_pdf = static_cast<real>(1) / s.emitters.size();
+        const surface_interaction ei = candidate_interaction(s, ec);
+        const spectrum Le = emitter_evaluate(s, ec.entity, ei, -wi);
+        const real lp = emitter_pdf(s, ec.entity, si, wi, cnt) * pick_pdf;
+        if (is_black(Le) || lp <= 0) continue;
+        const real w = power_heuristic(pdf * fb, lp * fe) * wb;
+        L = L + f * Le * w / pdf;
+    }
+    return L;
+}
+
 // perspective_camera::sample(position, sample) (perspective_camera.cpp:78-99)
 inline ray camera_ray(const render_ctx& ctx, const vec2& pos, const vec2& lens_u) {
     const vec3 target = ctx.raster_to_camera.mul(vec3(pos.x, pos.y, 0), 1);
@@ -1453,7 +1522,8 @@ int orc_render(orc_scene* s, const rbw_camera_desc* cam, const rbw_render_params
                         uint32_t* dbg = p->debug_hits ? p->debug_hits + sid * 3 * p->max_depth : nullptr;
                         if (dbg)
                             for (uint32_t k = 0; k < 3 * p->max_depth; k++) dbg[k] = RBW_NONE;
-                        const spectrum L = trace_path(ctx, smp, r, cnt, dbg);
+                        const spectrum L = p->integrator == RBW_INTEGRATOR_DIRECT ? trace_direct(ctx, smp, r, cnt)
+                                                                                  : trace_path(ctx, smp, r, cnt, dbg);
                         // film_tile::add_sample (film.cpp:57-92)
                         const vec2 dp(pos.x - 0.5f, pos.y - 0.5f);
                         int32_t sx0 = static_cast<int32_t>(std::ceil(dp.x - 0.5f));

@@ -32,6 +32,7 @@ RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL, RBW_MATERIAL_PLASTIC = 1, 2, 3
 RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY, RBW_SAMPLER_STRATIFIED = 0, 1, 2
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
+RBW_INTEGRATOR_PATH, RBW_INTEGRATOR_DIRECT = 0, 1
 RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "shade_guided", "guide_train"]
 RBW_GUIDE_PARAMS = 34160
 RBW_KERNEL_COUNT = 8
@@ -80,7 +81,8 @@ class rbw_render_params(C.Structure):
                 ("replay_offsets", u64p), ("replay_values", f32p), ("replay_count", C.c_uint64),
                 ("debug_hits", u32p), ("count_visits", C.c_uint32),
                 ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("strata_x", C.c_uint32), ("strata_y", C.c_uint32), ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32)]
+                ("strata_x", C.c_uint32), ("strata_y", C.c_uint32), ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32),
+                ("integrator", C.c_uint32), ("emitter_samples", C.c_uint32), ("bsdf_samples", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):

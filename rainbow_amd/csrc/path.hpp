@@ -487,6 +487,7 @@ struct dparams {
     const float* replay_values;
     uint32_t* debug;  // device debug hit buffer or null
     uint32_t strata_x, strata_y, dims_1d, dims_2d;  // RBW_SAMPLER_STRATIFIED
+    uint32_t emitter_samples, bsdf_samples;          // RBW_INTEGRATOR_DIRECT
     dguide g;
 };
 

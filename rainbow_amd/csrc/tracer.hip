@@ -463,6 +463,118 @@ __global__ void k_sampler_draws(dparams prm, uint32_t pixel, uint32_t n1, uint32
     }
 }
 
+// direct_integrator::trace (direct_integrator.cpp:20-128) at the path's hit: emission;
+// a surface without lobes lets the ray through (the path stays in the queue, its depth
+// unchanged); otherwise emitter_samples light samples and bsdf_samples BSDF samples,
+// each with its shadow or emitter-finding ray traced here, weighted by the power
+// heuristic over the sample fractions, and the path ends.
+template <int W>
+__global__ RBW_CONNECT_OCCUPANCY void k_direct(dscene S, dpaths P, dparams prm, const uint32_t* qin,
+                                               const uint32_t* qin_count, uint32_t* qout, uint32_t* qout_count,
+                                               unsigned long long* counters) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = *qin_count;
+    bool alive = false;
+    uint32_t slot = 0, nshadow = 0, nmis = 0, nlight = 0;
+    trace_counters cnt{0, 0};
+    if (q < count) {
+        slot = qin[q];
+        const uint32_t prim = P.U(U_HPRIM, slot);
+        if (prim != RBW_MISS && S.num_emitters > 0) {
+            const f3 o_w = P.F3(F_OX, slot), d_w = P.F3(F_DX, slot);
+            uint32_t eid;
+            const surf si = hit_surface(S, prim, o_w, d_w, eid);
+            const dentity E = S.ents[eid];
+            // L + trace(next): the vertices passed through so far are a prefix added to this one's sum
+            const f3 prefix = P.F3(F_VX, slot);
+            f3 L = f3{0.0f, 0.0f, 0.0f};
+            if (E.flags & RBW_ENTITY_EMITTER) L = L + light_evaluate(E, si.normal, -d_w);
+            bsdf fn;
+            fn.kind = LOBE_NONE;
+            if (E.material != RBW_NONE) fn = make_bsdf(S, E.material, prim);
+            if (fn.kind == LOBE_NONE) {
+                // no scattering functions: straight on (direct_integrator.cpp:52-56)
+                f3 no, nd;
+                spawn_ray(si.point, si.normal, d_w, no, nd);
+                P.setF3(F_OX, slot, no);
+                P.setF3(F_DX, slot, nd);
+                alive = true;
+            } else {
+                sampler_state smp = load_sampler(P, prm, slot);
+                const uint32_t ne = prm.emitter_samples, nb = prm.bsdf_samples;
+                const float fe = (float)ne / (float)(ne + nb), fb = (float)nb / (float)(ne + nb);
+                const float we = 1.0f / (float)ne, wb = 1.0f / (float)nb;
+                const uint32_t nem = S.num_emitters;
+                const f3 wo = world_to_local(si.sh, si.wo);
+                for (uint32_t i = 0; i < ne; i++) {  // :58-98
+                    const float u1 = smp.next();
+                    uint32_t which = (uint32_t)floorf(u1 * (float)nem);
+                    if (which > nem - 1) which = nem - 1;
+                    const float pick = 1.0f / (float)nem;
+                    const uint32_t lid = S.emitters[which];
+                    const dentity Lt = S.ents[lid];
+                    light_sample ls = sample_light(S, Lt, si.point, si.normal, smp.next2());
+                    ls.pdf = ls.pdf * pick;
+                    if (is_black(ls.L) || !(ls.pdf > 0.0f)) continue;
+                    const f3 wi = world_to_local(si.sh, ls.wi);
+                    f3 f = fn.evaluate(wo, wi);
+                    const float fp = fn.pdf(wo, wi);
+                    f = f * fabsf(dot(ls.wi, si.sh.z));
+                    if (is_black(f) || fp <= 0.0f) continue;
+                    f3 so, sd;
+                    float slen;
+                    spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
+                    nshadow++;
+                    hitrec h;
+                    if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
+                        h = traverse<W, true, true, false, true>(S, so, sd, slen, prm.tri_test, lid, cnt);
+                    else
+                        h = traverse<W, true, false, false, true>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
+                    bool unoccluded = h.prim == RBW_MISS;
+                    if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT)
+                        unoccluded = (h.prim < S.num_tris ? S.tri_entity[h.prim] : S.sphere_entity[h.prim - S.num_tris]) == lid;
+                    if (unoccluded) {
+                        const float w = power_heuristic(ls.pdf * fe, fp * fb) * we;
+                        L = L + f * ls.L * w / ls.pdf;
+                    }
+                }
+                for (uint32_t i = 0; i < nb; i++) {  // :100-128
+                    f3 mwi, mf;
+                    float mpdf = 0.0f;
+                    if (!fn.sample(si, smp.next2(), mwi, mf, mpdf)) continue;
+                    mf = mf * fabsf(dot(mwi, si.sh.z));
+                    if (is_black(mf) || !(mpdf > 0.0f)) continue;
+                    // find_emitter (integrator.cpp:49-82)
+                    f3 eo, ed;
+                    spawn_ray(si.point, si.normal, mwi, eo, ed);
+                    nmis++;
+                    const hitrec h = traverse<W, true, false, false, true>(S, eo, ed, __builtin_inff(), prm.tri_test,
+                                                                           RBW_NONE, cnt);
+                    if (h.prim == RBW_MISS) continue;
+                    uint32_t he;
+                    const surf hs = hit_surface(S, h.prim, eo, ed, he);
+                    const dentity Eh = S.ents[he];
+                    if (!(Eh.flags & RBW_ENTITY_EMITTER)) continue;
+                    nlight++;
+                    const float pick = 1.0f / (float)nem;
+                    const f3 Le = light_evaluate(Eh, hs.normal, -mwi);
+                    const float lp = light_pdf(S, Eh, si.point, si.normal, mwi) * pick;
+                    if (is_black(Le) || lp <= 0.0f) continue;
+                    const float w = power_heuristic(mpdf * fb, lp * fe) * wb;
+                    L = L + mf * Le * w / mpdf;
+                }
+                store_sampler(P, slot, smp);
+            }
+            P.setF3(F_VX, slot, prefix + L);
+        }
+    }
+    const uint32_t a = block_append(qout_count, alive);
+    if (alive) qout[a] = slot;
+    stripe_add64(counters, 2, nshadow);
+    stripe_add64(counters, 3, nmis);
+    stripe_add64(counters, 4, nlight);
+}
+
 // scene::intersect / intersect_with_shadow_ray for caller-given rays
 template <int W>
 __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* d, const float* tmax, uint32_t visible,
@@ -580,8 +692,14 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         require(pr.sampler != RBW_SAMPLER_REPLAY, RBW_ERR_INVALID_ARGUMENT,
                 "guiding draws a different number of samples per vertex than the replayed reference stream");
     }
+    require(pr.integrator == RBW_INTEGRATOR_PATH || pr.integrator == RBW_INTEGRATOR_DIRECT, RBW_ERR_INVALID_ARGUMENT,
+            "unknown integrator");
+    const bool direct = pr.integrator == RBW_INTEGRATOR_DIRECT;
+    if (direct)
+        require(!guided && !pr.debug_hits && !pr.count_visits, RBW_ERR_INVALID_ARGUMENT,
+                "the direct integrator takes no guiding, debug hits or visit counts");
     // the plain estimator runs on persistent, in-place regenerated path slots
-    if (pr.sampler != RBW_SAMPLER_REPLAY && !pr.debug_hits && !pr.count_visits && !guided) {
+    if (pr.sampler != RBW_SAMPLER_REPLAY && !pr.debug_hits && !pr.count_visits && !guided && !direct) {
         render_scheduled(ctx, cam, pr, out_rgb, stats);
         return;
     }
@@ -629,6 +747,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     dp.strata_y = pr.strata_y;
     dp.dims_1d = pr.dims_1d;
     dp.dims_2d = pr.dims_2d;
+    dp.emitter_samples = pr.emitter_samples ? pr.emitter_samples : 1u;
+    dp.bsdf_samples = pr.bsdf_samples ? pr.bsdf_samples : 1u;
     dp.tri_test = pr.triangle_test;
     dp.shadow_mode = pr.shadow_mode;
     dp.replay_offsets = nullptr;
@@ -759,6 +879,20 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                                    P, dp, qin, counts + cin, counters);
             }
             if (timed) ctx->timer.end(s);
+            if (direct) {
+                // the direct integrator's vertex, its shadow and emitter-finding rays in one kernel
+                if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
+                hipLaunchKernelGGL(S.width == 8 ? k_direct<8> : k_direct<2>, dim3(blocks(active, BS)), dim3(BS),
+                                   lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout, counters);
+                if (timed) ctx->timer.end(s);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipMemcpyAsync(ctx->host_count, counts + cout, 4, hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                active = *ctx->host_count;
+                std::swap(qin, qout);
+                std::swap(cin, cout);
+                continue;
+            }
             if (timed) ctx->timer.begin(RBW_KERNEL_SHADE, s);
             if (guided) {
                 HIP_CHECK(hipMemsetAsync(counts + 3, 0, 4, s));

@@ -412,6 +412,9 @@ void wavefront_path_integrator::render(const std::shared_ptr<camera>& cam, const
     }
     p.triangle_test = triangle_test;
     p.shadow_mode = shadow_mode;
+    p.integrator = integrator_kind;
+    p.emitter_samples = emitter_samples;
+    p.bsdf_samples = bsdf_samples;
     std::vector<float> rgb(size_t(cd.width) * cd.height * 3);
     check(rbw_render(mCtx, &cd, &p, rgb.data(), &last_stats));
     // film::set_pixel (cameras/film.cpp:202-205) for every pixel in bounds

@@ -411,6 +411,9 @@ public:
     uint32_t seed = 0;
     uint32_t triangle_test = RBW_TRI_MOLLER_TRUMBORE;
     uint32_t shadow_mode = RBW_SHADOW_CLOSEST_VISIBLE;
+    // RBW_INTEGRATOR_DIRECT makes this wavefront_direct_integrator's engine
+    uint32_t integrator_kind = RBW_INTEGRATOR_PATH;
+    uint32_t emitter_samples = 1, bsdf_samples = 1;
     rbw_stats last_stats{};
 private:
     std::shared_ptr<samplers::sampler2d> mSampler2D;
@@ -421,6 +424,26 @@ private:
     rbw_ctx* mCtx = nullptr;
     const scene* mUploaded = nullptr;
     uint64_t mUploadedGeneration = 0;
+};
+
+// integrators/direct_integrator.hpp: direct lighting with emitter_samples light samples and
+// bsdf_samples BSDF samples per camera sample, on the same wavefront engine
+class wavefront_direct_integrator final : public integrator {
+public:
+    wavefront_direct_integrator(const std::shared_ptr<samplers::sampler2d>& sampler2d,
+                                const std::shared_ptr<samplers::sampler1d>& sampler1d, size_t emitter_samples = 1,
+                                size_t bsdf_samples = 1, int device = 0)
+        : mEngine(sampler2d, sampler1d, 1, 1, device) {
+        mEngine.integrator_kind = RBW_INTEGRATOR_DIRECT;
+        mEngine.emitter_samples = uint32_t(emitter_samples);
+        mEngine.bsdf_samples = uint32_t(bsdf_samples);
+    }
+    void render(const std::shared_ptr<camera>& camera, const std::shared_ptr<scene>& scene) override {
+        mEngine.render(camera, scene);
+    }
+    wavefront_path_integrator& engine() noexcept { return mEngine; }  // seed, options, last_stats
+private:
+    wavefront_path_integrator mEngine;
 };
 }  // namespace integrators
 
@@ -437,7 +460,8 @@ namespace scenes {
 //                  "reverse_orientation": false}, ...],
 //    "camera": {"resolution": [w, h], "fov": 40, "eye": [...], "target": [...], "up": [0, 1, 0],
 //               "lens_radius": 0, "focus": 1e6, "crop": [x0, y0, x1, y1], "scale": 1},
-//    "render": {"spp": 16, "max_depth": 5, "threshold": 1, "seed": 0,
+//    "render": {"spp": 16, "max_depth": 5, "threshold": 1, "seed": 0, "integrator": "path" | "direct",
+//               "emitter_samples": 1, "bsdf_samples": 1,
 //               "stratified": {"strata": [4, 4], "dims_1d": 8, "dims_2d": 12}, "output": "name"}}
 // Transforms compose left to right; the screen window is [-1, 1]^2 scaled by the aspect.
 struct scene_file {
@@ -448,9 +472,11 @@ struct scene_file {
     uint32_t seed = 0;
     bool stratified = false;
     size_t strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
+    bool direct = false;  // "integrator": "direct"
+    size_t emitter_samples = 1, bsdf_samples = 1;
     std::string output;  // film::write name ("" = none)
-    // the integrator the render settings describe
-    std::shared_ptr<integrators::wavefront_path_integrator> integrator() const;
+    // the integrator the render settings describe (path, or direct with its sample counts)
+    std::shared_ptr<integrators::integrator> integrator() const;
 };
 scene_file load_scene(const std::string& json_text);
 scene_file load_scene_file(const std::string& path);

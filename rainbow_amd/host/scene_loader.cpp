@@ -347,6 +347,14 @@ scene_file load_scene(const std::string& text) {
             out.dims_2d = size_t(number_or(*st, "dims_2d", 0, "render.stratified"));
             out.spp = out.strata_x * out.strata_y;
         }
+        if (const json* it = r->find("integrator")) {
+            if (it->kind != json::string_v || (it->s != "path" && it->s != "direct"))
+                bad("render.integrator", "expected \"path\" or \"direct\"");
+            out.direct = it->s == "direct";
+        }
+        out.emitter_samples = size_t(number_or(*r, "emitter_samples", 1, "render"));
+        out.bsdf_samples = size_t(number_or(*r, "bsdf_samples", 1, "render"));
+        if (out.emitter_samples == 0 || out.bsdf_samples == 0) bad("render", "sample counts must be positive");
         if (const json* o = r->find("output")) {
             if (o->kind != json::string_v) bad("render.output", "expected a file name");
             out.output = o->s;
@@ -364,12 +372,17 @@ scene_file load_scene_file(const std::string& path) {
     return load_scene(ss.str());
 }
 
-std::shared_ptr<integrators::wavefront_path_integrator> scene_file::integrator() const {
+std::shared_ptr<integrators::integrator> scene_file::integrator() const {
     std::shared_ptr<samplers::sampler2d> s2 = std::make_shared<samplers::random_sampler2d>(spp);
     std::shared_ptr<samplers::sampler1d> s1 = std::make_shared<samplers::random_sampler1d>(spp);
     if (stratified) {
         s2 = std::make_shared<samplers::stratified_sampler2d>(strata_x, strata_y, dims_2d);
         s1 = std::make_shared<samplers::stratified_sampler1d>(strata_x, strata_y, dims_1d);
+    }
+    if (direct) {
+        auto integ = std::make_shared<integrators::wavefront_direct_integrator>(s2, s1, emitter_samples, bsdf_samples);
+        integ->engine().seed = seed;
+        return integ;
     }
     auto integ = std::make_shared<integrators::wavefront_path_integrator>(s2, s1, max_depth, threshold);
     integ->seed = seed;

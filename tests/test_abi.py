@@ -37,12 +37,31 @@ def test_python_signatures_cover_the_header():
     assert set(declared_functions()) <= set(abi.RBW_SIGNATURES)
 
 
-def test_struct_sizes_match_header_layout():
-    # plain 4-byte fields, no padding surprises between C and ctypes
-    assert C.sizeof(abi.rbw_material) == 52
-    assert C.sizeof(abi.rbw_entity) == 32
-    assert C.sizeof(abi.rbw_hit) == 20
-    assert C.sizeof(abi.rbw_camera_desc) == 16 * 4 * 2 + 4 * 2 + 4 * 4 + 4 * 2
+STRUCTS = ["rbw_material", "rbw_entity", "rbw_scene_desc", "rbw_camera_desc", "rbw_guiding_opts",
+           "rbw_render_params", "rbw_stats", "rbw_device_opts", "rbw_hit", "rbw_job", "rbw_batch_opts",
+           "rbw_job_stats", "rbw_batch_stats"]
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof and the offset of every field as the C compiler lays out include/rbw.h,
+    against the ctypes mirror"""
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rbw.h"', "int main(void) {"]
+    for name in STRUCTS:
+        lines.append(f'    printf("{name} %zu\\n", sizeof({name}));')
+        for fname, _ in getattr(abi, name)._fields_:
+            lines.append(f'    printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for name in STRUCTS:
+        t = getattr(abi, name)
+        assert int(got[name]) == C.sizeof(t), name
+        for fname, _ in t._fields_:
+            assert int(got[f"{name}.{fname}"]) == getattr(t, fname).offset, (name, fname)
 
 
 def _has_gpu():

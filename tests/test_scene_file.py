@@ -178,3 +178,22 @@ def test_plastic_material_in_a_file(tmp_path):
     x = math.log(0.3)
     want = 1.62142 + 0.819955 * x + 0.1734 * x * x + 0.0171201 * x ** 3 + 0.000640711 * x ** 4
     assert M["alpha_u"][0] == pytest.approx(want, rel=1e-5) and M["alpha_u"][0] == M["alpha_v"][0]
+
+
+@pytest.mark.gpu
+def test_gpu_render_file_direct_integrator(tracer, tmp_path):
+    """"integrator": "direct" selects wavefront_direct_integrator with the file's sample counts."""
+    from rainbow_amd.tracer import render_params
+    scene = json.load(open(CORNELL))
+    scene["render"].update({"integrator": "direct", "emitter_samples": 2, "bsdf_samples": 3})
+    src = tmp_path / "scene.json"
+    src.write_text(json.dumps(scene))
+    out = tmp_path / "film.bin"
+    subprocess.run([_binary(), "render_file", str(src), str(out)], check=True)
+    img = np.fromfile(out, np.float32).reshape(256, 256, 3)
+    sc, cam = scenes.cornell_box()
+    tracer.upload(sc)
+    ref = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3, integrator=abi.RBW_INTEGRATOR_DIRECT,
+                                           emitter_samples=2, bsdf_samples=3))["image"]
+    bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
+    assert bad.mean() < 1e-3, bad.sum()
