@@ -26,9 +26,10 @@ constexpr int RBW_LDS_STACK = 16;
 extern __shared__ uint32_t rbw_lds_stack[];
 inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * block * sizeof(uint32_t); }
 
-// The shadow/MIS kernels are latency bound on dependent node fetches: six waves per
-// SIMD (at most 80 VGPRs, a few spilled) beat the five their natural 95 VGPRs allow.
-#define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
+// The shadow/MIS kernels are latency bound on dependent node fetches: with the binary
+// walk six waves per SIMD (at most 80 VGPRs, a few spilled) beat the five its natural
+// 95 VGPRs allow; the wide walk's larger state is left unconstrained (W: the width).
+#define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 6 : 1)))
 // The shading kernel keeps four waves per SIMD (at most 128 VGPRs).
 #define RBW_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 
@@ -148,51 +149,80 @@ __device__ __forceinline__ hitrec traverse2(const dscene& S, f3 o_w, f3 d_w, flo
     const f3 o = lr.o;
     const f3 inv = lr.inv;
     const f3 oi = f3{o.x * inv.x, o.y * inv.y, o.z * inv.z};
+    // while-while walk with postponed leaves (Aila and Laine, "Understanding the efficiency of
+    // ray traversal on GPUs", 2009): a wave walks inner nodes until every lane holds a leaf
+    // to test (a lane that holds one keeps walking ahead), then tests leaves together, so
+    // node and leaf work are not interleaved lane by lane
+    bool done = false, has_leaf = false;
+    int32_t leaf = 0;
+    auto advance = [&]() {
+        if (sp == 0) done = true;
+        else cur = (int32_t)pop();
+    };
     for (;;) {
-        if (cur >= 0) {
-            const float4* n = S.nodes + 4 * (size_t)cur;
-            const float4 a = n[0], b = n[1], c = n[2];
-            const int4 ch = *reinterpret_cast<const int4*>(n + 3);
-            if (COUNT) cnt.nodes++;
-            const float tmax = st.thr_local * 1.0000002f;
-            // slab distances (lo - o) / d as fma(lo, 1/d, -o/d); the bounds below are widened
-            // by a few ulps so that rounding never culls a box that holds an accepted hit
-            const float lx0 = __builtin_fmaf(a.x, inv.x, -oi.x), lx1 = __builtin_fmaf(a.y, inv.x, -oi.x);
-            const float ly0 = __builtin_fmaf(a.z, inv.y, -oi.y), ly1 = __builtin_fmaf(a.w, inv.y, -oi.y);
-            const float lz0 = __builtin_fmaf(c.x, inv.z, -oi.z), lz1 = __builtin_fmaf(c.y, inv.z, -oi.z);
-            const float rx0 = __builtin_fmaf(b.x, inv.x, -oi.x), rx1 = __builtin_fmaf(b.y, inv.x, -oi.x);
-            const float ry0 = __builtin_fmaf(b.z, inv.y, -oi.y), ry1 = __builtin_fmaf(b.w, inv.y, -oi.y);
-            const float rz0 = __builtin_fmaf(c.z, inv.z, -oi.z), rz1 = __builtin_fmaf(c.w, inv.z, -oi.z);
-            const float lmin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), 0.0f));
-            const float lmax = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax));
-            const float rmin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), 0.0f));
-            const float rmax = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax));
-            const bool hl = lmin <= lmax * 1.0000004f;
-            const bool hr = rmin <= rmax * 1.0000004f;
-            if (hl && hr) {
-                const bool lfirst = lmin <= rmin;
-                cur = lfirst ? ch.x : ch.y;
-                push((uint32_t)(lfirst ? ch.y : ch.x));
-                continue;
-            } else if (hl) {
-                cur = ch.x;
-                continue;
-            } else if (hr) {
-                cur = ch.y;
-                continue;
+        for (;;) {
+            if (!done && !has_leaf && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
             }
-        } else if (cur != k_empty_child) {
-            const uint32_t first = (uint32_t)cur & 0x07FFFFFFu;
-            const uint32_t count = (((uint32_t)cur >> 27) & 15u) + 1u;
+            const bool walk = !done && cur >= 0;
+            if (__ballot(walk) == 0) break;
+            if (walk) {
+                const float4* n = S.nodes + 4 * (size_t)cur;
+                const float4 a = n[0], b = n[1], c = n[2];
+                const int4 ch = *reinterpret_cast<const int4*>(n + 3);
+                if (COUNT) cnt.nodes++;
+                const float tmax = st.thr_local * 1.0000002f;
+                const float lx0 = __builtin_fmaf(a.x, inv.x, -oi.x), lx1 = __builtin_fmaf(a.y, inv.x, -oi.x);
+                const float ly0 = __builtin_fmaf(a.z, inv.y, -oi.y), ly1 = __builtin_fmaf(a.w, inv.y, -oi.y);
+                const float lz0 = __builtin_fmaf(c.x, inv.z, -oi.z), lz1 = __builtin_fmaf(c.y, inv.z, -oi.z);
+                const float rx0 = __builtin_fmaf(b.x, inv.x, -oi.x), rx1 = __builtin_fmaf(b.y, inv.x, -oi.x);
+                const float ry0 = __builtin_fmaf(b.z, inv.y, -oi.y), ry1 = __builtin_fmaf(b.w, inv.y, -oi.y);
+                const float rz0 = __builtin_fmaf(c.z, inv.z, -oi.z), rz1 = __builtin_fmaf(c.w, inv.z, -oi.z);
+                const float lmin = fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fmaxf(fminf(lz0, lz1), 0.0f));
+                const float lmax = fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fminf(fmaxf(lz0, lz1), tmax));
+                const float rmin = fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fmaxf(fminf(rz0, rz1), 0.0f));
+                const float rmax = fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fminf(fmaxf(rz0, rz1), tmax));
+                const bool hl = lmin <= lmax * 1.0000004f;
+                const bool hr = rmin <= rmax * 1.0000004f;
+                if (hl && hr) {
+                    const bool lfirst = lmin <= rmin;
+                    cur = lfirst ? ch.x : ch.y;
+                    push((uint32_t)(lfirst ? ch.y : ch.x));
+                } else if (hl) {
+                    cur = ch.x;
+                } else if (hr) {
+                    cur = ch.y;
+                } else {
+                    advance();
+                }
+            }
+            // every lane holds a leaf or is done: test them
+            if (__ballot(!(has_leaf || done)) == 0) break;
+        }
+        while (has_leaf) {
+            has_leaf = false;
+            const uint32_t first = (uint32_t)leaf & 0x07FFFFFFu;
+            const uint32_t count = (((uint32_t)leaf >> 27) & 15u) + 1u;
             if (test_leaf<VISIBLE_ONLY, ANY_HIT, COUNT>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt,
                                                         tri_test, target_entity, cnt, st))
                 return st.best;
+            if (!done && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
+            }
         }
-        if (sp == 0) break;
-        cur = (int32_t)pop();
+        if (done) break;
     }
     return st.best;
 }
+
 
 // closest hit (or closest visible hit) through the eight-wide BVH.  The walk keeps
 // a group (a node's inner children still to visit, as a bit mask in visiting

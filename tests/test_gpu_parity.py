@@ -164,3 +164,23 @@ def test_render_icosphere_window(tracer, icosphere):
     frac, idf = _compare_render(g, c, "icosphere window")
     assert idf < 1e-5
     assert frac < 1e-3
+
+
+def test_render_icosphere_full_film_tiles(tracer, icosphere):
+    """The benchmarked render itself (1024x1024, 64 spp, depth 8, persistent path slots)
+    against the oracle on every 16th 16x16 tile of the film (256 tiles, 4.2M samples)."""
+    sc, cam, os_ = icosphere
+    orc = _oracle()
+    g = tracer.render(cam, render_params(spp=64, max_depth=8, seed=0))["image"]
+    stride = 16
+    c = os_.render(cam, orc.render_params(spp=64, max_depth=8, seed=0), sampler=orc.ORC_SAMPLER_PER_SAMPLE,
+                   tile_stride=stride)["image"]
+    tiles_x = (cam.width + 15) // 16
+    mask = np.zeros((cam.height, cam.width), bool)
+    for t in range(0, tiles_x * ((cam.height + 15) // 16), stride):
+        tx, ty = t % tiles_x, t // tiles_x
+        mask[ty * 16:(ty + 1) * 16, tx * 16:(tx + 1) * 16] = True
+    frac = pixel_mismatch(g[mask], c[mask]).mean()
+    print("full film: tile pixels", int(mask.sum()), "mismatch", frac)
+    assert mask.sum() > 60000
+    assert frac < 1e-3
