@@ -47,7 +47,9 @@ enum {
     RBW_ENTITY_EMITTER = 1u,             /* surface_light component (emitters/surface_light.hpp:7) */
     RBW_ENTITY_REVERSE_ORIENTATION = 2u, /* shape::reverse_orientation() (shapes/shape.hpp:81) */
     RBW_ENTITY_HAS_NORMALS = 4u,         /* mesh::has_normal() (shapes/mesh.cpp:215) */
-    RBW_ENTITY_HAS_UVS = 8u              /* mesh::has_uv() (shapes/mesh.cpp:220) */
+    RBW_ENTITY_HAS_UVS = 8u,             /* mesh::has_uv() (shapes/mesh.cpp:220) */
+    RBW_ENTITY_HAS_MEDIA = 16u           /* media component (media/medium.hpp:60-75): the entity's pair
+                                            in rbw_scene_desc::entity_media */
 };
 
 /* materials: matte (materials/matte_material.cpp:13-29, Lambert only: sigma must be 0)
@@ -67,6 +69,26 @@ typedef struct rbw_material {
     float specular[3];
     float ior;
 } rbw_material;
+
+/* Participating media (media/medium.hpp:37-75) for the volume path integrator:
+ *   homogeneous (media/homogeneous_medium.cpp:8-58): sigma_a, sigma_s, g;
+ *   grid (media/heterogeneous_medium.cpp:5-120): sigma_t an image_texture3d over the unit
+ *     cube of the medium's local space, sampled trilinearly as
+ *     textures/detail/image_texture.hpp:119-162 does (x fastest, then y, then z), with
+ *     constant albedo and g (constant_texture3d); world_to_local maps world points and
+ *     vectors into that space and is not re-normalised (heterogeneous_medium.cpp:28-33). */
+enum { RBW_MEDIUM_HOMOGENEOUS = 0, RBW_MEDIUM_GRID = 1 };
+
+typedef struct rbw_medium {
+    uint32_t type;             /* RBW_MEDIUM_* */
+    float sigma_a[3];          /* homogeneous: absorption per unit length */
+    float sigma_s[3];          /* homogeneous: scattering per unit length */
+    float g;                   /* Henyey-Greenstein asymmetry, |g| < 1 (shared/phases/henyey_greenstein.cpp) */
+    float albedo[3];           /* grid: sigma_s / sigma_t */
+    uint32_t grid_offset;      /* grid: index of its first density in rbw_scene_desc::medium_grid */
+    uint32_t grid_size[3];     /* grid: nx, ny, nz */
+    float world_to_local[12];  /* grid: 3x4 row-major affine map (rows of [R | t]) */
+} rbw_medium;
 
 /* One rainbow entity (scenes/entity.hpp:45-70): shape + material + emitter.
  * Mesh entities hold world-space triangles [first, first+count) and carry the
@@ -98,6 +120,14 @@ typedef struct rbw_scene_desc {
     const rbw_entity* entities;      /* in scene::add_entity order (scenes/scene.cpp:15-28) */
     uint32_t num_materials;
     const rbw_material* materials;
+    /* media component of each entity (media::media, media/medium.hpp:60-75): two medium
+     * indices per entity, outside then inside, RBW_NONE for vacuum; NULL: no entity has
+     * media.  Only the volume path integrator reads them. */
+    uint32_t num_media;
+    const rbw_medium* media;
+    const uint32_t* entity_media;
+    uint32_t medium_grid_count;
+    const float* medium_grid;        /* the grids' densities (sigma_t per unit length) */
 } rbw_scene_desc;
 
 /* ----------------------------------------------------------------- camera */
@@ -140,7 +170,13 @@ enum {
      * light samples and bsdf_samples BSDF samples at the first surface with lobes (the
      * ray passes through surfaces without lobes), MIS by the power heuristic over the
      * sample fractions; max_depth and rr_threshold are not used */
-    RBW_INTEGRATOR_DIRECT = 1
+    RBW_INTEGRATOR_DIRECT = 1,
+    /* volume_path_integrator (volume_path_integrator.cpp:13-79): the path integrator
+     * through the scene's media (rbw_scene_desc::media): distance sampling in the
+     * current medium, Henyey-Greenstein vertices with next-event estimation, beam
+     * transmittance on light and MIS rays through invisible medium boundaries
+     * (scene.cpp:114-148).  Camera rays start in vacuum. */
+    RBW_INTEGRATOR_VOLUME_PATH = 2
 };
 
 enum {
@@ -200,6 +236,12 @@ typedef struct rbw_render_params {
     uint32_t strata_x, strata_y, dims_1d, dims_2d;
     uint32_t integrator;             /* RBW_INTEGRATOR_* */
     uint32_t emitter_samples, bsdf_samples;  /* RBW_INTEGRATOR_DIRECT (0 = 1) */
+    /* RBW_INTEGRATOR_VOLUME_PATH: 0 = as the reference, where an emitter that a phase-function
+     * sample reaches from a medium vertex contributes nothing (entity::pdf<emitter>,
+     * detail/entity.hpp:66-72, normalises the medium interaction's zero normal, so the light
+     * pdf is not a number and fails the test at integrator.cpp:262); 1 = it contributes with
+     * the power-heuristic weight, which makes the estimator unbiased */
+    uint32_t medium_mis;
 } rbw_render_params;
 /* Schedules: the plain estimator with the per-sample or stratified sampler (no replay, no debug
  * hits, no visit counts, no guiding) runs on persistent path slots regenerated in

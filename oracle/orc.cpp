@@ -583,6 +583,12 @@ struct orc_scene {
     std::vector<orc::entity_data> ents;
     std::vector<rbw_material> mats;
     std::vector<uint32_t> emitters;
+    // media (media/medium.hpp:37-75): table, (outside, inside) per entity, grid densities,
+    // and 1 / the largest density of each grid medium (heterogeneous_medium.cpp:12-22)
+    std::vector<rbw_medium> media;
+    std::vector<uint32_t> ent_media;
+    std::vector<float> grid;
+    std::vector<float> inv_max_density;
     orc::bvh<orc::entity_ref> top;
     uint32_t num_tris = 0, num_spheres = 0;
 };
@@ -1025,9 +1031,11 @@ inline spectrum emitter_evaluate(const orc_scene& s, uint32_t eid, const interac
 }
 
 // entity::pdf<emitter> -> surface_light::pdf -> shape::pdf (shape.cpp:46-60): a cast against the light's own shape
-inline real emitter_pdf(const orc_scene& s, uint32_t eid, const interaction& ref_world, const vec3& wi_world, counters& cnt) {
+// (raw: the reference interaction is used untransformed, keeping a medium vertex's zero normal)
+inline real emitter_pdf(const orc_scene& s, uint32_t eid, const interaction& ref_world, const vec3& wi_world, counters& cnt,
+                        bool raw = false) {
     const entity_data& ent = s.ents[eid];
-    const interaction ref = transform_interaction(ent.to_local, ref_world);
+    const interaction ref = raw ? ref_world : transform_interaction(ent.to_local, ref_world);
     const vec3 wi = transform_vector(ent.to_local, wi_world);
     ray r = spawn_ray(ref, wi);
     cnt.light++;
@@ -1244,6 +1252,379 @@ inline spectrum trace_direct(const render_ctx& ctx, sample_stream& smp, const ra
     return L;
 }
 
+// ---------------------------------------------------------------- media
+
+constexpr real k_one_over_four_pi = static_cast<real>(0.0795774715459476678844418816862571810);
+
+// henyey_greenstein_phase_function (utilities.cpp:46-51)
+inline real hg_phase(real cos_theta, real g) {
+    const real den = 1 + g * g + 2 * g * cos_theta;
+    return k_one_over_four_pi * (1 - g * g) / (den * std::sqrt(den));
+}
+
+// henyey_greenstein::sample (henyey_greenstein.cpp:17-48); returns wi, pdf = the phase value
+inline vec3 hg_sample(const vec3& wo, real g, const vec2& u, real& pdf) {
+    real cos_t;
+    if (std::fabs(g) >= 1e-3) {
+        const real sq = (1 - g * g) / (1 - g + 2 * g * u.x);
+        cos_t = (1 + g * g - sq * sq) / (2 * g);
+    } else {
+        cos_t = 1 - 2 * u.x;
+    }
+    const real sin_t = std::sqrt(fmax_g(static_cast<real>(0), 1 - cos_t * cos_t));
+    const real phi = k_two_pi * u.y;
+    const vec3 wl(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+    coordinate_system sys = coordinate_system_from_normal(wo);
+    sys.z = -wo;
+    pdf = hg_phase(-cos_t, g);
+    return local_to_world(sys, wl);
+}
+
+// image_texture_t<3, real>::sample (textures/detail/image_texture.hpp:119-162)
+inline real grid_sample(const orc_scene& s, const rbw_medium& m, const vec3& p) {
+    const int n[3] = {static_cast<int>(m.grid_size[0]), static_cast<int>(m.grid_size[1]), static_cast<int>(m.grid_size[2])};
+    const real pc[3] = {p.x, p.y, p.z};
+    real uv[3];
+    int a[3], b[3];
+    real f[3];
+    for (int k = 0; k < 3; k++) {
+        const real md = pc[k] - 1 * std::floor(pc[k] / 1);  // glm::mod
+        uv[k] = static_cast<real>(md * static_cast<real>(n[k]) - 0.5);
+        a[k] = std::min(std::max(static_cast<int>(std::floor(uv[k])), 0), n[k] - 1);
+        b[k] = std::min(std::max(a[k] + 1, 0), n[k] - 1);
+        f[k] = std::min(std::max(uv[k] - static_cast<real>(a[k]), static_cast<real>(0)), static_cast<real>(1));
+    }
+    auto at = [&](int x, int y, int z) {
+        return s.grid[m.grid_offset + static_cast<size_t>(z) * n[0] * n[1] + static_cast<size_t>(y) * n[0] + x];
+    };
+    auto lerp = [](real v0, real v1, real x) { return v0 * (1 - x) + v1 * x; };
+    const real c00 = lerp(at(a[0], a[1], a[2]), at(b[0], a[1], a[2]), f[0]);
+    const real c01 = lerp(at(a[0], a[1], b[2]), at(b[0], a[1], b[2]), f[0]);
+    const real c10 = lerp(at(a[0], b[1], a[2]), at(b[0], b[1], a[2]), f[0]);
+    const real c11 = lerp(at(a[0], b[1], b[2]), at(b[0], b[1], b[2]), f[0]);
+    return lerp(lerp(c00, c10, f[1]), lerp(c01, c11, f[1]), f[2]);
+}
+
+// the grid's local ray: world_to_local applied without re-normalising (heterogeneous_medium.cpp:28-33)
+inline void grid_local_ray(const rbw_medium& m, const ray& r, vec3& lo, vec3& ld) {
+    const float* w = m.world_to_local;
+    lo = vec3(w[0] * r.origin.x + w[1] * r.origin.y + w[2] * r.origin.z + w[3],
+              w[4] * r.origin.x + w[5] * r.origin.y + w[6] * r.origin.z + w[7],
+              w[8] * r.origin.x + w[9] * r.origin.y + w[10] * r.origin.z + w[11]);
+    ld = vec3(w[0] * r.direction.x + w[1] * r.direction.y + w[2] * r.direction.z,
+              w[4] * r.direction.x + w[5] * r.direction.y + w[6] * r.direction.z,
+              w[8] * r.direction.x + w[9] * r.direction.y + w[10] * r.direction.z);
+}
+
+// heterogeneous_medium::intersect (heterogeneous_medium.cpp:122-140) with the unit box
+inline bool unit_box(const vec3& o, const vec3& d, real len, real& t0, real& t1) {
+    t0 = 0;
+    t1 = len;
+    for (int k = 0; k < 3; k++) {
+        const real inv = 1 / d[k];
+        real near_t = (0 - o[k]) * inv, far_t = (1 - o[k]) * inv;
+        if (near_t > far_t) std::swap(near_t, far_t);
+        t0 = fmax_g(t0, near_t);
+        t1 = fmin_g(t1, far_t);
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+struct medium_sample_result {
+    bool scattered = false;
+    vec3 point{0};
+    spectrum value{1};
+    real g = 0;
+};
+
+// medium_info::sample (medium.cpp:45-53) -> homogeneous_medium::sample (homogeneous_medium.cpp:20-58)
+// or heterogeneous_medium::sample (heterogeneous_medium.cpp:70-120).  A grid collision's world
+// point is the world ray at the collision parameter.
+inline medium_sample_result medium_sample(const orc_scene& s, uint32_t mid, sample_stream& smp, const ray& r) {
+    medium_sample_result out;
+    const rbw_medium& m = s.media[mid];
+    out.g = m.g;
+    if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
+        const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+        const size_t channel = std::min(static_cast<size_t>(smp.next() * 3), static_cast<size_t>(2));  // sample_function.cpp:97-102
+        const real t = fmin_g(-std::log(1 - smp.next()) / st[static_cast<int>(channel)], r.length);
+        const bool sampled = t < r.length;
+        const real tc = fmin_g(t, std::numeric_limits<real>::max());
+        const spectrum value(std::exp(-st.x * tc), std::exp(-st.y * tc), std::exp(-st.z * tc));
+        const spectrum density = sampled ? st * value : value;
+        real pdf = 0;
+        pdf = pdf + density.x;
+        pdf = pdf + density.y;
+        pdf = pdf + density.z;
+        pdf = pdf / 3;
+        if (pdf == 0) pdf = 1;
+        if (!sampled) {
+            out.value = value / pdf;
+            return out;
+        }
+        out.scattered = true;
+        out.point = r.origin + r.direction * t;
+        out.value = value * spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]) / pdf;
+        return out;
+    }
+    vec3 lo, ld;
+    grid_local_ray(m, r, lo, ld);
+    real t0, t1;
+    if (!unit_box(lo, ld, r.length, t0, t1)) return out;
+    const real inv = s.inv_max_density[mid];
+    real t = t0;
+    while (true) {
+        t = t - std::log(1 - smp.next()) * inv;
+        if (t >= t1) break;
+        const real sigma_t = grid_sample(s, m, lo + ld * t);
+        if (sigma_t * inv > smp.next()) {
+            out.scattered = true;
+            out.point = r.origin + r.direction * t;
+            out.value = spectrum(m.albedo[0], m.albedo[1], m.albedo[2]);
+            return out;
+        }
+    }
+    return out;
+}
+
+// medium_info::evaluate (medium.cpp:35-43): vacuum 1; homogeneous_medium::evaluate
+// (homogeneous_medium.cpp:12-18); heterogeneous_medium::evaluate, ratio tracking (:24-68)
+inline spectrum medium_evaluate(const orc_scene& s, uint32_t mid, sample_stream& smp, const ray& r) {
+    if (mid == NONE) return spectrum(1);
+    const rbw_medium& m = s.media[mid];
+    if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
+        const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+        const real tc = fmin_g(r.length, std::numeric_limits<real>::max());
+        return spectrum(std::exp(-st.x * tc), std::exp(-st.y * tc), std::exp(-st.z * tc));
+    }
+    vec3 lo, ld;
+    grid_local_ray(m, r, lo, ld);
+    real t0, t1;
+    if (!unit_box(lo, ld, r.length, t0, t1)) return spectrum(1);
+    const real inv = s.inv_max_density[mid];
+    real value = 1, t = t0;
+    while (true) {
+        t = t - std::log(1 - smp.next()) * inv;
+        if (t >= t1) break;
+        const real sigma_t = grid_sample(s, m, lo + ld * t);
+        value = value * (1 - fmax_g(static_cast<real>(0), sigma_t * inv));
+    }
+    return spectrum(value);
+}
+
+// medium_info(entity, normal, wi) (medium.cpp:24-33)
+inline uint32_t medium_after(const orc_scene& s, uint32_t entity, const vec3& normal, const vec3& wi) {
+    return dot(normal, wi) > 0 ? s.ent_media[2 * entity] : s.ent_media[2 * entity + 1];
+}
+inline bool has_media(const orc_scene& s, uint32_t entity) { return (s.ents[entity].e.flags & RBW_ENTITY_HAS_MEDIA) != 0; }
+
+// scene::evaluate_media_beam (scene.cpp:114-148)
+inline spectrum evaluate_media_beam(const orc_scene& s, sample_stream& smp, uint32_t medium, const interaction& from,
+                                    const vec3& to, counters& cnt) {
+    spectrum L(1);
+    ray beam = spawn_ray_to(from, to);
+    hit_candidate c;
+    cnt.shadow++;
+    bool hit = scene_intersect(s, beam, false, c);
+    while (hit) {
+        L = L * medium_evaluate(s, medium, smp, beam);
+        if (s.ents[c.entity].visible()) return spectrum(0);
+        const surface_interaction si = candidate_interaction(s, c);
+        beam = spawn_ray_to(si, to);
+        if (has_media(s, c.entity)) medium = medium_after(s, c.entity, si.normal, beam.direction);
+        c = hit_candidate();
+        cnt.shadow++;
+        hit = scene_intersect(s, beam, false, c);
+    }
+    L = L * medium_evaluate(s, medium, smp, beam);
+    return L;
+}
+
+// uniform_sample_one_emitter, surface overload with media = true (integrator.cpp:101-201)
+inline spectrum sample_one_emitter_media(const render_ctx& ctx, sample_stream& smp, const surface_interaction& si,
+                                         const bsdf& fn, uint32_t medium, counters& cnt) {
+    const orc_scene& s = *ctx.s;
+    spectrum L(0);
+    {
+        const size_t ne = s.emitters.size();
+        const size_t which = std::min(static_cast<size_t>(std::floor(smp.next() * ne)), ne - 1);
+        const real pick_pdf = static_cast<real>(1) / ne;
+        const uint32_t eid = s.emitters[which];
+        emitter_sample es = sample_emitter(s, eid, si, smp.next2());
+        es.pdf = es.pdf * pick_pdf;
+        if (!is_black(es.intensity) && es.pdf > 0) {
+            const vec3 wi = world_to_local(si.shading, es.wi);
+            const vec3 wo = world_to_local(si.shading, si.wo);
+            spectrum f = fn.evaluate(wo, wi);
+            const real fp = fn.pdf(wo, wi);
+            f = f * std::fabs(dot(es.wi, si.shading.z));
+            if (!is_black(f) && fp > 0) {
+                ray sr = spawn_ray_to(si, es.it.point);
+                cnt.shadow++;
+                hit_candidate c;
+                const bool hit = scene_intersect(s, sr, true, c);
+                if (!hit || c.entity == eid) {
+                    f = f * evaluate_media_beam(s, smp, medium, si, es.it.point, cnt);
+                    const real w = power_heuristic(es.pdf, fp);
+                    L = L + f * es.intensity * w / es.pdf;
+                }
+            }
+        }
+    }
+    {
+        vec3 wi;
+        spectrum f;
+        real pdf = 0;
+        const vec2 u = smp.next2();
+        if (fn.sample(si, u, wi, f, pdf)) {
+            f = f * std::fabs(dot(wi, si.shading.z));
+            if (!is_black(f) && pdf > 0) {
+                ray er = spawn_ray(si, wi);
+                cnt.mis++;
+                hit_candidate c;
+                const bool hit = scene_intersect(s, er, true, c);
+                if (hit && s.ents[c.entity].emitter()) {
+                    const real pick_pdf = static_cast<real>(1) / s.emitters.size();
+                    const surface_interaction ei = candidate_interaction(s, c);
+                    const spectrum Le = emitter_evaluate(s, c.entity, ei, -wi);
+                    const real lp = emitter_pdf(s, c.entity, si, wi, cnt) * pick_pdf;
+                    if (!is_black(Le) && lp > 0) {
+                        f = f * evaluate_media_beam(s, smp, medium, si, ei.point, cnt);
+                        const real w = power_heuristic(pdf, lp);
+                        L = L + f * Le * w / pdf;
+                    }
+                }
+            }
+        }
+    }
+    return L;
+}
+
+// uniform_sample_one_emitter, medium overload (integrator.cpp:203-291).  The emitter pdf
+// of a phase sample is entity::pdf<emitter> of the medium interaction, whose zero normal
+// transform_normal normalises to NaN (detail/entity.hpp:66-72): the test `emitter_pdf > 0`
+// fails and the phase-sampled emitter contributes nothing.  medium_mis = 1 evaluates the
+// pdf from the untransformed interaction instead.
+inline spectrum sample_one_emitter_medium(const render_ctx& ctx, sample_stream& smp, const interaction& mi, real g,
+                                          uint32_t medium, counters& cnt) {
+    const orc_scene& s = *ctx.s;
+    spectrum L(0);
+    {
+        const size_t ne = s.emitters.size();
+        const size_t which = std::min(static_cast<size_t>(std::floor(smp.next() * ne)), ne - 1);
+        const real pick_pdf = static_cast<real>(1) / ne;
+        const uint32_t eid = s.emitters[which];
+        emitter_sample es = sample_emitter(s, eid, mi, smp.next2());
+        es.pdf = es.pdf * pick_pdf;
+        if (!is_black(es.intensity) && es.pdf > 0) {
+            const real phase = hg_phase(dot(mi.wo, es.wi), g);
+            const spectrum f(phase);
+            const real fp = phase;
+            if (!is_black(f) && fp > 0) {
+                ray sr = spawn_ray_to(mi, es.it.point);
+                cnt.shadow++;
+                hit_candidate c;
+                const bool hit = scene_intersect(s, sr, true, c);
+                if (!hit || c.entity == eid) {
+                    const spectrum beam = evaluate_media_beam(s, smp, medium, mi, es.it.point, cnt);
+                    const real w = power_heuristic(es.pdf, fp);
+                    L = L + f * es.intensity * beam * w / es.pdf;
+                }
+            }
+        }
+    }
+    {
+        real fp = 0;
+        const vec3 wi = hg_sample(mi.wo, g, smp.next2(), fp);
+        const spectrum f(fp);
+        if (!is_black(f) && fp > 0) {
+            ray er = spawn_ray(mi, wi);
+            cnt.mis++;
+            hit_candidate c;
+            const bool hit = scene_intersect(s, er, true, c);
+            if (hit && s.ents[c.entity].emitter()) {
+                const real pick_pdf = static_cast<real>(1) / s.emitters.size();
+                const surface_interaction ei = candidate_interaction(s, c);
+                const spectrum Le = emitter_evaluate(s, c.entity, ei, -wi);
+                const real lp = emitter_pdf(s, c.entity, mi, wi, cnt, ctx.p->medium_mis != 0) * pick_pdf;
+                if (!is_black(Le) && lp > 0) {
+                    const spectrum beam = evaluate_media_beam(s, smp, medium, mi, ei.point, cnt);
+                    const real w = power_heuristic(fp, lp);
+                    L = L + f * Le * beam * w / fp;
+                }
+            }
+        }
+    }
+    return L;
+}
+
+// volume_path_integrator::trace (volume_path_integrator.cpp:13-79); camera rays start in vacuum
+inline spectrum trace_volume(const render_ctx& ctx, sample_stream& smp, const ray& first, counters& cnt) {
+    const orc_scene& s = *ctx.s;
+    path_state ps;
+    ps.r = first;
+    uint32_t medium = NONE;
+    const int max_depth = static_cast<int>(ctx.p->max_depth);
+    for (int bounces = 0; bounces < max_depth; bounces++) {
+        cnt.extension++;
+        hit_candidate c;
+        const bool hit = scene_intersect(s, ps.r, false, c);  // the ray's length becomes the hit distance
+        bool surface = true;
+        if (medium != NONE) {
+            const medium_sample_result ms = medium_sample(s, medium, smp, ps.r);
+            ps.beta = ps.beta * ms.value;
+            if (is_black(ps.beta)) break;
+            if (ms.scattered) {
+                // sample_medium_interaction (integrator.cpp:406-421)
+                interaction mi;
+                mi.point = ms.point;
+                mi.normal = vec3(0);
+                mi.wo = -ps.r.direction;
+                ps.value = ps.value + ps.beta * sample_one_emitter_medium(ctx, smp, mi, ms.g, medium, cnt);
+                real pp;
+                const vec3 wi = hg_sample(mi.wo, ms.g, smp.next2(), pp);
+                ps.specular = false;
+                ps.r = spawn_ray(mi, wi);
+                surface = false;
+            }
+        }
+        if (surface) {
+            // sample_surface_interaction (integrator.cpp:329-404), media = true
+            if (!hit) break;  // no environment lights
+            const surface_interaction si = candidate_interaction(s, c);
+            const entity_data& ent = s.ents[si.entity];
+            if ((bounces == 0 || ps.specular) && ent.emitter())
+                ps.value = ps.value + ps.beta * emitter_evaluate(s, si.entity, si, -ps.r.direction);
+            if (ent.e.material == NONE) {
+                ps.r = spawn_ray(si, ps.r.direction);
+                bounces--;
+            } else {
+                const bsdf fn = build_bsdf(s, si);
+                if (fn.count() != 0)
+                    ps.value = ps.value + ps.beta * sample_one_emitter_media(ctx, smp, si, fn, medium, cnt);
+                vec3 wi;
+                spectrum f;
+                real pdf = 0;
+                const vec2 u = smp.next2();
+                const bool ok = fn.sample(si, u, wi, f, pdf);
+                if (!ok || is_black(f) || pdf == 0) break;
+                ps.beta = ps.beta * (f * std::fabs(dot(wi, si.shading.z)) / pdf);
+                ps.specular = false;
+                ps.r = spawn_ray(si, wi);
+            }
+            if (has_media(s, si.entity)) medium = medium_after(s, si.entity, si.normal, ps.r.direction);
+        }
+        const real mc = smax_component(ps.beta * ps.eta);
+        if (mc < ctx.p->rr_threshold && bounces > 3) {
+            const real q = fmax_g(static_cast<real>(0.05), 1 - mc);
+            if (smp.next() < q) break;
+            ps.beta = ps.beta / (1 - q);
+        }
+    }
+    return ps.value;
+}
+
 // perspective_camera::sample(position, sample) (perspective_camera.cpp:78-99)
 inline ray camera_ray(const render_ctx& ctx, const vec2& pos, const vec2& lens_u) {
     const vec3 target = ctx.raster_to_camera.mul(vec3(pos.x, pos.y, 0), 1);
@@ -1298,6 +1679,18 @@ int orc_scene_create(const rbw_scene_desc* d, orc_scene** out) {
         s->sphere_entity.assign(d->sphere_entity, d->sphere_entity + d->num_spheres);
     }
     s->mats.assign(d->materials, d->materials + d->num_materials);
+    if (d->entity_media) {
+        s->media.assign(d->media, d->media + d->num_media);
+        s->ent_media.assign(d->entity_media, d->entity_media + 2 * static_cast<size_t>(d->num_entities));
+        s->grid.assign(d->medium_grid, d->medium_grid + d->medium_grid_count);
+        for (const rbw_medium& m : s->media) {
+            float mx = 0;
+            if (m.type == RBW_MEDIUM_GRID)
+                for (size_t k = 0; k < size_t(m.grid_size[0]) * m.grid_size[1] * m.grid_size[2]; k++)
+                    mx = std::max(mx, s->grid[m.grid_offset + k]);
+            s->inv_max_density.push_back(mx > 0 ? 1 / mx : 0);
+        }
+    }
     s->ents.resize(d->num_entities);
     // reference_count of every shape is 1 (each entity owns its shape)
     constexpr size_t sub_shape_threshold = 1 << 17;  // scene.cpp:43
@@ -1523,7 +1916,8 @@ int orc_render(orc_scene* s, const rbw_camera_desc* cam, const rbw_render_params
                         if (dbg)
                             for (uint32_t k = 0; k < 3 * p->max_depth; k++) dbg[k] = RBW_NONE;
                         const spectrum L = p->integrator == RBW_INTEGRATOR_DIRECT ? trace_direct(ctx, smp, r, cnt)
-                                                                                  : trace_path(ctx, smp, r, cnt, dbg);
+                                           : p->integrator == RBW_INTEGRATOR_VOLUME_PATH ? trace_volume(ctx, smp, r, cnt)
+                                                                                         : trace_path(ctx, smp, r, cnt, dbg);
                         // film_tile::add_sample (film.cpp:57-92)
                         const vec2 dp(pos.x - 0.5f, pos.y - 0.5f);
                         int32_t sx0 = static_cast<int32_t>(std::ceil(dp.x - 0.5f));
@@ -1720,6 +2114,26 @@ int orc_material_sample(const rbw_material* m, const float* wo, float u0, float 
     f[0] = ff.x; f[1] = ff.y; f[2] = ff.z;
     *pdf = ok ? p : 0;
     return ok ? 1 : 0;
+}
+
+float orc_hg_phase(float cos_theta, float g) { return hg_phase(cos_theta, g); }
+
+float orc_hg_sample(const float* wo, float g, float u0, float u1, float* wi) {
+    real pdf = 0;
+    const vec3 w = hg_sample(v3(wo), g, vec2(u0, u1), pdf);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    return pdf;
+}
+
+// the emitter pdf of direction wi from a medium vertex at p (zero normal), as the reference
+// computes it (raw = 0) or from the untransformed interaction (raw = 1)
+float orc_medium_emitter_pdf(orc_scene* s, uint32_t entity, const float* p, const float* wi, int raw) {
+    interaction it;
+    it.point = v3(p);
+    it.normal = vec3(0);
+    it.wo = vec3(0, 0, 1);
+    counters cnt;
+    return emitter_pdf(*s, entity, it, v3(wi), cnt, raw != 0);
 }
 
 void orc_fresnel_conductor(float cos_theta_i, const float* eta, const float* k, float* out3) {

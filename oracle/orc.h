@@ -95,6 +95,12 @@ float orc_fresnel_dielectric(float cos_theta_i, float eta_i, float eta_o);
 float orc_material_pdf(const rbw_material* m, const float* wo, const float* wi);
 void orc_material_eval(const rbw_material* m, const float* wo, const float* wi, float* f);
 int orc_material_sample(const rbw_material* m, const float* wo, float u0, float u1, float* wi, float* f, float* pdf);
+/* Henyey-Greenstein (utilities.cpp:46-51, henyey_greenstein.cpp:17-48): phase value; sample -> pdf */
+float orc_hg_phase(float cos_theta, float g);
+float orc_hg_sample(const float* wo, float g, float u0, float u1, float* wi);
+/* emitter pdf from a medium vertex (zero normal): raw = 0 as the reference (entity::pdf<emitter>
+ * transforms the interaction, normalising the zero normal), raw = 1 untransformed */
+float orc_medium_emitter_pdf(orc_scene* s, uint32_t entity, const float* p, const float* wi, int raw);
 
 #ifdef __cplusplus
 }

@@ -28,11 +28,13 @@ RBW_ENTITY_EMITTER = 1
 RBW_ENTITY_REVERSE_ORIENTATION = 2
 RBW_ENTITY_HAS_NORMALS = 4
 RBW_ENTITY_HAS_UVS = 8
+RBW_ENTITY_HAS_MEDIA = 16
 RBW_MATERIAL_MATTE, RBW_MATERIAL_METAL, RBW_MATERIAL_PLASTIC = 1, 2, 3
 RBW_SAMPLER_PER_SAMPLE_PCG, RBW_SAMPLER_REPLAY, RBW_SAMPLER_STRATIFIED = 0, 1, 2
 RBW_TRI_MOLLER_TRUMBORE, RBW_TRI_WATERTIGHT = 0, 1
 RBW_SHADOW_CLOSEST_VISIBLE, RBW_SHADOW_ANY_HIT = 0, 1
-RBW_INTEGRATOR_PATH, RBW_INTEGRATOR_DIRECT = 0, 1
+RBW_INTEGRATOR_PATH, RBW_INTEGRATOR_DIRECT, RBW_INTEGRATOR_VOLUME_PATH = 0, 1, 2
+RBW_MEDIUM_HOMOGENEOUS, RBW_MEDIUM_GRID = 0, 1
 RBW_KERNEL_NAMES = ["raygen", "extend", "shade", "connect", "film", "guide_infer", "shade_guided", "guide_train"]
 RBW_GUIDE_PARAMS = 34160
 RBW_KERNEL_COUNT = 8
@@ -53,13 +55,21 @@ class rbw_entity(C.Structure):
                 ("material", C.c_uint32), ("flags", C.c_uint32), ("radiance", C.c_float * 3)]
 
 
+class rbw_medium(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("sigma_a", C.c_float * 3), ("sigma_s", C.c_float * 3), ("g", C.c_float),
+                ("albedo", C.c_float * 3), ("grid_offset", C.c_uint32), ("grid_size", C.c_uint32 * 3),
+                ("world_to_local", C.c_float * 12)]
+
+
 class rbw_scene_desc(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("positions", f32p), ("normals", f32p), ("uvs", f32p),
                 ("num_triangles", C.c_uint32), ("indices", u32p), ("triangle_entity", u32p),
                 ("triangle_material", u32p), ("num_spheres", C.c_uint32), ("spheres", f32p),
                 ("sphere_entity", u32p), ("num_entities", C.c_uint32),
                 ("entities", C.POINTER(rbw_entity)), ("num_materials", C.c_uint32),
-                ("materials", C.POINTER(rbw_material))]
+                ("materials", C.POINTER(rbw_material)), ("num_media", C.c_uint32),
+                ("media", C.POINTER(rbw_medium)), ("entity_media", u32p), ("medium_grid_count", C.c_uint32),
+                ("medium_grid", f32p)]
 
 
 class rbw_camera_desc(C.Structure):
@@ -82,7 +92,8 @@ class rbw_render_params(C.Structure):
                 ("debug_hits", u32p), ("count_visits", C.c_uint32),
                 ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32),
                 ("strata_x", C.c_uint32), ("strata_y", C.c_uint32), ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32),
-                ("integrator", C.c_uint32), ("emitter_samples", C.c_uint32), ("bsdf_samples", C.c_uint32)]
+                ("integrator", C.c_uint32), ("emitter_samples", C.c_uint32), ("bsdf_samples", C.c_uint32),
+                ("medium_mis", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):

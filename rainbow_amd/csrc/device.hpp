@@ -34,6 +34,18 @@ struct dmaterial {
     float ior;
 };
 
+// A participating medium (media/homogeneous_medium.cpp, media/heterogeneous_medium.cpp)
+struct dmedium {
+    uint32_t type;               // RBW_MEDIUM_*
+    float sigma_t[3];            // homogeneous: sigma_a + sigma_s
+    float sigma_s[3];
+    float g;
+    float albedo[3];             // grid
+    float inv_max_density;       // grid: 1 / the largest density
+    uint32_t grid_offset, nx, ny, nz;
+    float w2l[12];               // grid: world -> unit cube, 3x4 row-major
+};
+
 struct dscene {
     const float4* nodes;     // bvh2 nodes, 4 float4 each
     const float4* prim_a;    // leaf slots: (p0 | centre, id word)
@@ -61,6 +73,11 @@ struct dscene {
     const float4* prim8_c;
     uint32_t root8;
     uint32_t width;          // 2 or 8: the tree traversals use
+    // media (volume path integrator only)
+    const dmedium* media;
+    const uint32_t* ent_media;  // 2 per entity: outside, inside (RBW_NONE = vacuum); null = no media
+    const float* grid;          // grid media densities
+    uint32_t num_media;
 };
 
 struct hitrec {

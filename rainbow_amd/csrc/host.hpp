@@ -117,6 +117,9 @@ struct host_scene {
     std::vector<float4> spheres;
     std::vector<dentity> ents;
     std::vector<dmaterial> mats;
+    std::vector<dmedium> media;     // empty: no media
+    std::vector<uint32_t> ent_media;  // 2 per entity when media is not empty
+    std::vector<float> grid;
     uint32_t nt = 0, ns = 0, nv = 0;
     bool has_nrm = false, has_uv = false, has_tri_material = false;
     uint32_t depth = 0, depth8 = 0;
@@ -137,8 +140,8 @@ constexpr uint32_t k_default_width = 2;  // BVH width of a render that does not 
 
 struct dev_scene {
     dbuf nodes, prim_a, prim_b, prim_c, nodes8, prim8_a, prim8_b, prim8_c, pos, nrm, uv, idx, tri_entity,
-        tri_material, spheres, sphere_entity, ents, mats, emitters;
-    uint32_t nt = 0, ns = 0, num_emitters = 0;
+        tri_material, spheres, sphere_entity, ents, mats, emitters, media, ent_media, grid;
+    uint32_t nt = 0, ns = 0, num_emitters = 0, num_media = 0;
     bool has_uv = false, has_tri_material = false, valid = false;
     void upload(const host_scene& h, hipStream_t s);
     dscene view(uint32_t width = k_default_width) const;

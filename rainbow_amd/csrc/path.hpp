@@ -450,7 +450,9 @@ enum fidx : uint32_t {
     F_COUNT
 };
 // U_DIMS: the stratified sampler's draws made by the sample (1D | 2D << 16)
-enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_DIMS, U_COUNT };
+// U_MEDIUM: the medium the path's ray travels in (volume path integrator; RBW_NONE = vacuum)
+enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_DIMS, U_MEDIUM,
+                       U_COUNT };
 
 constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
 
@@ -518,6 +520,7 @@ struct dparams {
     uint32_t* debug;  // device debug hit buffer or null
     uint32_t strata_x, strata_y, dims_1d, dims_2d;  // RBW_SAMPLER_STRATIFIED
     uint32_t emitter_samples, bsdf_samples;          // RBW_INTEGRATOR_DIRECT
+    uint32_t medium_mis;                             // RBW_INTEGRATOR_VOLUME_PATH: phase-sampled emitter hits count
     dguide g;
 };
 

@@ -33,6 +33,66 @@ inline uint32_t word_of(const float4& v) {
     return w;
 }
 
+// The media table (media/medium.hpp:37-75): homogeneous media keep sigma_t = sigma_a + sigma_s
+// (homogeneous_medium.cpp:7-10); a grid keeps 1 / its largest density
+// (heterogeneous_medium.cpp:12-22).  Entity media are (outside, inside) pairs.
+void build_media(const rbw_scene_desc* d, host_scene& h) {
+    h.media.clear();
+    h.ent_media.clear();
+    h.grid.clear();
+    if (!d->entity_media) {
+        require(d->num_media == 0 || d->media, RBW_ERR_INVALID_ARGUMENT, "media table missing");
+        for (uint32_t e = 0; e < d->num_entities; e++)
+            require(!(d->entities[e].flags & RBW_ENTITY_HAS_MEDIA), RBW_ERR_INVALID_ARGUMENT,
+                    "an entity has media but the scene has no entity_media");
+        return;
+    }
+    require(d->num_media > 0 && d->media, RBW_ERR_INVALID_ARGUMENT, "entity media given without a media table");
+    require(d->medium_grid_count == 0 || d->medium_grid, RBW_ERR_INVALID_ARGUMENT, "medium grid missing");
+    h.grid.assign(d->medium_grid, d->medium_grid + d->medium_grid_count);
+    for (float v : h.grid)
+        require(std::isfinite(v) && v >= 0.0f, RBW_ERR_INVALID_ARGUMENT, "grid densities must be finite and >= 0");
+    h.media.resize(d->num_media);
+    for (uint32_t i = 0; i < d->num_media; i++) {
+        const rbw_medium& m = d->media[i];
+        dmedium& o = h.media[i];
+        std::memset(&o, 0, sizeof(o));
+        o.type = m.type;
+        o.g = m.g;
+        require(std::fabs(m.g) < 1.0f, RBW_ERR_INVALID_ARGUMENT, "Henyey-Greenstein g must lie in (-1, 1)");
+        if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
+            for (int c = 0; c < 3; c++) {
+                require(m.sigma_a[c] >= 0.0f && m.sigma_s[c] >= 0.0f && std::isfinite(m.sigma_a[c] + m.sigma_s[c]),
+                        RBW_ERR_INVALID_ARGUMENT, "medium coefficients must be finite and >= 0");
+                o.sigma_t[c] = m.sigma_a[c] + m.sigma_s[c];
+                o.sigma_s[c] = m.sigma_s[c];
+            }
+        } else if (m.type == RBW_MEDIUM_GRID) {
+            const uint64_t n = uint64_t(m.grid_size[0]) * m.grid_size[1] * m.grid_size[2];
+            require(n > 0 && uint64_t(m.grid_offset) + n <= d->medium_grid_count, RBW_ERR_INVALID_ARGUMENT,
+                    "medium grid out of range");
+            float mx = 0.0f;
+            for (uint64_t k = 0; k < n; k++) mx = std::max(mx, d->medium_grid[m.grid_offset + k]);
+            require(mx > 0.0f, RBW_ERR_INVALID_ARGUMENT, "medium grid has no density");
+            o.inv_max_density = 1.0f / mx;
+            o.grid_offset = m.grid_offset;
+            o.nx = m.grid_size[0];
+            o.ny = m.grid_size[1];
+            o.nz = m.grid_size[2];
+            for (int c = 0; c < 3; c++) o.albedo[c] = m.albedo[c];
+            for (int k = 0; k < 12; k++) o.w2l[k] = m.world_to_local[k];
+        } else {
+            require(false, RBW_ERR_INVALID_ARGUMENT, "unknown medium type");
+        }
+    }
+    h.ent_media.assign(d->entity_media, d->entity_media + size_t(d->num_entities) * 2);
+    for (uint32_t v : h.ent_media)
+        require(v == RBW_NONE || v < d->num_media, RBW_ERR_INVALID_ARGUMENT, "entity medium index out of range");
+    // only entities with the media component switch the path's medium
+    for (uint32_t e = 0; e < d->num_entities; e++)
+        if (!(d->entities[e].flags & RBW_ENTITY_HAS_MEDIA)) h.ent_media[2 * e] = h.ent_media[2 * e + 1] = RBW_NONE;
+}
+
 // leaf slots in BVH order: (p0 | sphere centre, id word), (p1 | radius), p2
 void fill_slots(const rbw_scene_desc* d, const host_scene& h, const std::vector<uint32_t>& order,
                 std::vector<float4>& pa, std::vector<float4>& pb, std::vector<float4>& pc) {
@@ -228,6 +288,7 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
         o.alpha_v = m.alpha_v;
         o.ior = m.ior;
     }
+    build_media(d, h);
     h.nt = nt;
     h.ns = ns;
     h.nv = nv;
@@ -419,6 +480,12 @@ void dev_scene::upload(const host_scene& h, hipStream_t s) {
     ents.upload(h.ents.data(), h.ents.size(), s);
     mats.upload(h.mats.data(), h.mats.size(), s);
     emitters.upload(h.emitters.data(), h.emitters.size(), s);
+    num_media = uint32_t(h.media.size());
+    if (num_media) {
+        media.upload(h.media.data(), h.media.size(), s);
+        ent_media.upload(h.ent_media.data(), h.ent_media.size(), s);
+        if (!h.grid.empty()) grid.upload(h.grid.data(), h.grid.size(), s);
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     nt = h.nt;
     ns = h.ns;
@@ -453,6 +520,10 @@ dscene dev_scene::view(uint32_t width) const {
     S.prim8_c = prim8_c.as<float4>();
     S.root8 = 0;
     S.width = width;
+    S.num_media = num_media;
+    S.media = num_media ? media.as<dmedium>() : nullptr;
+    S.ent_media = num_media ? ent_media.as<uint32_t>() : nullptr;
+    S.grid = num_media && grid.p ? grid.as<float>() : nullptr;
     return S;
 }
 

@@ -525,9 +525,11 @@ RBW_HD f3 light_evaluate(const dentity& E, f3 normal_w, f3 w_world) {
 
 // entity::pdf<emitter> -> shape::pdf (shape.cpp:46-60): a ray cast against the
 // light's own triangles, closest hit by linear scan (mesh.cpp:45-58)
-RBW_HD float light_pdf(const dscene& S, const dentity& E, f3 ref_point_w, f3 ref_normal_w, f3 wi_w) {
+// (raw_normal: the reference normal is used as given -- the zero normal of a medium vertex)
+RBW_HD float light_pdf(const dscene& S, const dentity& E, f3 ref_point_w, f3 ref_normal_w, f3 wi_w,
+                       bool raw_normal = false) {
     const f3 ref_point = ident(ref_point_w);
-    const f3 ref_normal = normalize(ident(ref_normal_w));
+    const f3 ref_normal = raw_normal ? ref_normal_w : normalize(ident(ref_normal_w));
     const f3 wi = ident(wi_w);
     f3 o, d;
     spawn_ray(ref_point, ref_normal, wi, o, d);

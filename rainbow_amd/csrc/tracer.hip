@@ -29,6 +29,7 @@
 #include "bvh.hpp"
 #include "device.hpp"
 #include "guiding.hpp"
+#include "volume.hpp"
 #include "image.hpp"
 #include "host.hpp"
 #include "path.hpp"
@@ -39,14 +40,16 @@ using namespace rbw;
 // ------------------------------------------------------------------ kernels
 
 // sampler_integrator.cpp:98-127 + perspective_camera.cpp:78-99: one camera sample per slot
+// (with_medium: the volume path integrator's camera rays start in vacuum, volume_path_integrator.cpp:21-24)
 __global__ void k_raygen(dscene S, dpaths P, dparams prm, dcamera cam, uint32_t n, uint32_t s0, uint32_t* queue,
-                         uint32_t* queue_count) {
+                         uint32_t* queue_count, uint32_t with_medium) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t bw = (uint32_t)(cam.bx1 - cam.bx0), bh = (uint32_t)(cam.by1 - cam.by0);
     const uint32_t npix = bw * bh;
     const uint32_t li = i % npix, sl = i / npix;
     raygen_path(cam, prm, P, i, cam.bx0 + (int32_t)(li % bw), cam.by0 + (int32_t)(li / bw), s0 + sl);
+    if (with_medium) P.U(U_MEDIUM, i) = RBW_NONE;
     queue[i] = i;
     if (i == 0) *queue_count = n;
 }
@@ -575,6 +578,111 @@ __global__ RBW_CONNECT_OCCUPANCY void k_direct(dscene S, dpaths P, dparams prm, 
     stripe_add64(counters, 4, nlight);
 }
 
+// volume_path_integrator::trace (volume_path_integrator.cpp:29-72), one bounce of the
+// path in `slot` after its extension hit: distance sampling in the current medium
+// (a medium vertex: next-event estimation with the phase function, then a phase
+// sample), else the surface vertex of sample_surface_interaction (integrator.cpp:329-404,
+// media = true) and the medium switch at entities with media (medium.cpp:24-33);
+// Russian roulette as path_integrator.cpp:35-43.
+template <int W>
+__global__ __launch_bounds__(256) void k_volume(dscene S, dpaths P, dparams prm, const uint32_t* qin,
+                                               const uint32_t* qin_count, uint32_t* qout, uint32_t* qout_count,
+                                               unsigned long long* counters) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = *qin_count;
+    bool alive = false;
+    uint32_t slot = 0, nshadow = 0, nmis = 0, nlight = 0;
+    trace_counters cnt{0, 0};
+    if (q < count) {
+        slot = qin[q];
+        const uint32_t prim = P.U(U_HPRIM, slot);
+        int bounce = (int)P.U(U_BOUNCE, slot);
+        uint32_t medium = P.U(U_MEDIUM, slot);
+        f3 value = P.F3(F_VX, slot), beta = P.F3(F_BX, slot);
+        const f3 o_w = P.F3(F_OX, slot), d_w = P.F3(F_DX, slot);
+        sampler_state smp = load_sampler(P, prm, slot);
+        bool cont = true, at_surface = true;
+        f3 no = o_w, nd = d_w;
+        if (medium != RBW_NONE) {
+            const float len = prim == RBW_MISS ? __builtin_inff() : P.F(F_HT, slot);
+            const medium_event ev = medium_sample(S, S.media[medium], smp, o_w, d_w, len);
+            beta = beta * ev.value;
+            if (is_black(beta)) {
+                cont = false;
+                at_surface = false;
+            } else if (ev.scattered) {
+                // sample_medium_interaction (integrator.cpp:406-421)
+                at_surface = false;
+                surf mi;
+                mi.point = o_w + d_w * ev.t;
+                mi.normal = splat(0.0f);
+                mi.wo = -d_w;
+                const bsdf none{};
+                value = value + beta * volume_nee<W, true>(S, prm, smp, mi, none, ev.g, medium, cnt, nshadow, nmis,
+                                                           nlight);
+                float pp;
+                const f3 wi = hg_sample(mi.wo, ev.g, smp.next2(), pp);
+                spawn_ray(mi.point, mi.normal, wi, no, nd);
+            }
+        }
+        if (at_surface) {
+            if (prim == RBW_MISS) {
+                cont = false;  // no environment emitters
+            } else {
+                uint32_t eid;
+                const surf si = hit_surface(S, prim, o_w, d_w, eid);
+                const dentity E = S.ents[eid];
+                // emission at the first vertex (integrator.cpp:350-351; no specular lobes exist)
+                if (bounce == 0 && (E.flags & RBW_ENTITY_EMITTER)) value = value + beta * light_evaluate(E, si.normal, -d_w);
+                if (E.material == RBW_NONE) {
+                    // invisible entity: straight through, the bounce does not count (integrator.cpp:355-363)
+                    spawn_ray(si.point, si.normal, d_w, no, nd);
+                    bounce -= 1;
+                } else {
+                    const bsdf fn = make_bsdf(S, E.material, prim);
+                    if (fn.kind != LOBE_NONE)
+                        value = value + beta * volume_nee<W, false>(S, prm, smp, si, fn, 0.0f, medium, cnt, nshadow,
+                                                                    nmis, nlight);
+                    f3 wi, f;
+                    float pdf = 0.0f;
+                    const bool ok = fn.sample(si, smp.next2(), wi, f, pdf);
+                    if (!ok || is_black(f) || pdf == 0.0f) {
+                        cont = false;
+                    } else {
+                        beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
+                        spawn_ray(si.point, si.normal, wi, no, nd);
+                    }
+                }
+                if (cont && (E.flags & RBW_ENTITY_HAS_MEDIA)) medium = medium_after(S, eid, si.normal, nd);
+            }
+        }
+        if (cont) {
+            const float mc = spectrum_max(beta * 1.0f);
+            if (mc < prm.rr_threshold && bounce > 3) {
+                const float qq = gmax(0.05f, 1.0f - mc);
+                if (smp.next() < qq) cont = false;
+                else beta = beta / (1.0f - qq);
+            }
+        }
+        bounce += 1;
+        alive = cont && (uint32_t)bounce < prm.max_depth;
+        P.setF3(F_VX, slot, value);
+        if (alive) {
+            P.setF3(F_BX, slot, beta);
+            P.setF3(F_OX, slot, no);
+            P.setF3(F_DX, slot, nd);
+            P.U(U_BOUNCE, slot) = (uint32_t)bounce;
+            P.U(U_MEDIUM, slot) = medium;
+            store_sampler(P, slot, smp);
+        }
+    }
+    const uint32_t a = block_append(qout_count, alive);
+    if (alive) qout[a] = slot;
+    stripe_add64(counters, 2, nshadow);
+    stripe_add64(counters, 3, nmis);
+    stripe_add64(counters, 4, nlight);
+}
+
 // scene::intersect / intersect_with_shadow_ray for caller-given rays
 template <int W>
 __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* d, const float* tmax, uint32_t visible,
@@ -692,14 +800,20 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         require(pr.sampler != RBW_SAMPLER_REPLAY, RBW_ERR_INVALID_ARGUMENT,
                 "guiding draws a different number of samples per vertex than the replayed reference stream");
     }
-    require(pr.integrator == RBW_INTEGRATOR_PATH || pr.integrator == RBW_INTEGRATOR_DIRECT, RBW_ERR_INVALID_ARGUMENT,
-            "unknown integrator");
+    require(pr.integrator == RBW_INTEGRATOR_PATH || pr.integrator == RBW_INTEGRATOR_DIRECT ||
+                pr.integrator == RBW_INTEGRATOR_VOLUME_PATH,
+            RBW_ERR_INVALID_ARGUMENT, "unknown integrator");
     const bool direct = pr.integrator == RBW_INTEGRATOR_DIRECT;
+    const bool volume = pr.integrator == RBW_INTEGRATOR_VOLUME_PATH;
     if (direct)
         require(!guided && !pr.debug_hits && !pr.count_visits, RBW_ERR_INVALID_ARGUMENT,
                 "the direct integrator takes no guiding, debug hits or visit counts");
+    if (volume)
+        require(!guided && !pr.debug_hits && !pr.count_visits && pr.sampler != RBW_SAMPLER_REPLAY,
+                RBW_ERR_INVALID_ARGUMENT,
+                "the volume path integrator takes no guiding, replay, debug hits or visit counts");
     // the plain estimator runs on persistent, in-place regenerated path slots
-    if (pr.sampler != RBW_SAMPLER_REPLAY && !pr.debug_hits && !pr.count_visits && !guided && !direct) {
+    if (pr.sampler != RBW_SAMPLER_REPLAY && !pr.debug_hits && !pr.count_visits && !guided && !direct && !volume) {
         render_scheduled(ctx, cam, pr, out_rgb, stats);
         return;
     }
@@ -749,6 +863,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
     dp.dims_2d = pr.dims_2d;
     dp.emitter_samples = pr.emitter_samples ? pr.emitter_samples : 1u;
     dp.bsdf_samples = pr.bsdf_samples ? pr.bsdf_samples : 1u;
+    dp.medium_mis = pr.medium_mis ? 1u : 0u;
     dp.tri_test = pr.triangle_test;
     dp.shadow_mode = pr.shadow_mode;
     dp.replay_offsets = nullptr;
@@ -855,7 +970,8 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         // the pass's training samples are its closed continuation records
         if (guided && go->train) HIP_CHECK(hipMemsetAsync(counts + 4, 0, 4, s));
         if (timed) ctx->timer.begin(RBW_KERNEL_RAYGEN, s);
-        hipLaunchKernelGGL(k_raygen, dim3(blocks(n, BS)), dim3(BS), 0, s, S, P, dp, dc, n, s0, qa, counts + 0);
+        hipLaunchKernelGGL(k_raygen, dim3(blocks(n, BS)), dim3(BS), 0, s, S, P, dp, dc, n, s0, qa, counts + 0,
+                           volume ? 1u : 0u);
         if (timed) ctx->timer.end(s);
         HIP_CHECK(hipGetLastError());
         uint32_t active = n;
@@ -879,11 +995,18 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                                    P, dp, qin, counts + cin, counters);
             }
             if (timed) ctx->timer.end(s);
-            if (direct) {
-                // the direct integrator's vertex, its shadow and emitter-finding rays in one kernel
+            if (direct || volume) {
+                // the direct integrator's vertex, or the volume path integrator's bounce, with
+                // their shadow, emitter-finding and beam rays in one kernel
                 if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
-                hipLaunchKernelGGL(S.width == 8 ? k_direct<8> : k_direct<2>, dim3(blocks(active, BS)), dim3(BS),
-                                   lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout, counters);
+                if (direct)
+                    hipLaunchKernelGGL(S.width == 8 ? k_direct<8> : k_direct<2>, dim3(blocks(active, BS)), dim3(BS),
+                                       lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout,
+                                       counters);
+                else
+                    hipLaunchKernelGGL(S.width == 8 ? k_volume<8> : k_volume<2>, dim3(blocks(active, BS)), dim3(BS),
+                                       lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout,
+                                       counters);
                 if (timed) ctx->timer.end(s);
                 HIP_CHECK(hipGetLastError());
                 HIP_CHECK(hipMemcpyAsync(ctx->host_count, counts + cout, 4, hipMemcpyDeviceToHost, s));

@@ -47,10 +47,44 @@ class Scene:
     sphere_entity: list = field(default_factory=list)
     entities: list = field(default_factory=list)    # dicts
     materials: list = field(default_factory=list)   # dicts
+    media: list = field(default_factory=list)      # abi.rbw_medium
+    grid: list = field(default_factory=list)       # float32 blocks of grid densities
     nv: int = 0
     nt: int = 0
     has_normals: bool = False
     _keep: list = field(default_factory=list)
+
+    # -- media (media/medium.hpp) --------------------------------------------
+    def homogeneous_medium(self, sigma_a, sigma_s, g=0.0):
+        """homogeneous_medium (media/homogeneous_medium.cpp:7-10)"""
+        m = abi.rbw_medium(type=abi.RBW_MEDIUM_HOMOGENEOUS, g=g)
+        m.sigma_a[:] = [float(v) for v in np.broadcast_to(sigma_a, 3)]
+        m.sigma_s[:] = [float(v) for v in np.broadcast_to(sigma_s, 3)]
+        self.media.append(m)
+        return len(self.media) - 1
+
+    def grid_medium(self, density, albedo, g=0.0, box_min=(0, 0, 0), box_max=(1, 1, 1)):
+        """heterogeneous_medium (media/heterogeneous_medium.cpp:5-22): density[z, y, x] (sigma_t)
+        over the axis-aligned box [box_min, box_max], trilinear; constant albedo and g"""
+        density = np.ascontiguousarray(density, np.float32)
+        nz, ny, nx = density.shape
+        off = int(sum(len(b) for b in self.grid))
+        self.grid.append(density.reshape(-1))
+        lo, hi = np.asarray(box_min, np.float64), np.asarray(box_max, np.float64)
+        sc = 1.0 / (hi - lo)
+        w2l = [sc[0], 0, 0, -lo[0] * sc[0], 0, sc[1], 0, -lo[1] * sc[1], 0, 0, sc[2], -lo[2] * sc[2]]
+        m = abi.rbw_medium(type=abi.RBW_MEDIUM_GRID, g=g, grid_offset=off)
+        m.albedo[:] = [float(v) for v in np.broadcast_to(albedo, 3)]
+        m.grid_size[:] = [nx, ny, nz]
+        m.world_to_local[:] = [float(v) for v in w2l]
+        self.media.append(m)
+        return len(self.media) - 1
+
+    def set_media(self, entity, outside=None, inside=None):
+        """the entity's media component (media/medium.hpp:60-75): None = vacuum"""
+        e = self.entities[entity]
+        e["flags"] |= abi.RBW_ENTITY_HAS_MEDIA
+        e["media"] = (abi.RBW_NONE if outside is None else outside, abi.RBW_NONE if inside is None else inside)
 
     # -- materials ---------------------------------------------------------
     def matte(self, rgb):
@@ -152,6 +186,15 @@ class Scene:
             num_entities=len(self.entities), entities=ents, num_materials=self.num_materials(),
             materials=mats.ctypes.data_as(C.POINTER(abi.rbw_material)))
         self._keep = [P, N, I, TE, TM, S, SE, ents, mats]
+        if self.media:
+            med = (abi.rbw_medium * len(self.media))(*self.media)
+            em = np.array([e.get("media", (abi.RBW_NONE, abi.RBW_NONE)) for e in self.entities], np.uint32).reshape(-1)
+            G = np.concatenate(self.grid).astype(np.float32) if self.grid else np.zeros(1, np.float32)
+            d.num_media, d.media = len(self.media), med
+            d.entity_media = abi.ptr(em, abi.u32p)
+            d.medium_grid_count = int(sum(len(b) for b in self.grid))
+            d.medium_grid = abi.ptr(G, abi.f32p)
+            self._keep += [med, em, G]
         return d
 
 
@@ -277,6 +320,55 @@ def cornell_box(width=256, height=256):
     """Cornell-style box, 10 wall triangles, 2 spheres, quad light; returns (scene, camera)."""
     sc = Scene()
     add_cornell_box(sc)
+    cam = perspective_camera(width, height, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
+    return sc, cam
+
+
+def box_mesh(lo, hi):
+    """an axis-aligned box of 12 triangles with outward face normals: (P, I, N)"""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = [((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)),   # +z
+             ((x1, y0, z0), (x0, y0, z0), (x0, y1, z0), (x1, y1, z0)),   # -z
+             ((x1, y0, z1), (x1, y0, z0), (x1, y1, z0), (x1, y1, z1)),   # +x
+             ((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0)),   # -x
+             ((x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (x0, y1, z0)),   # +y
+             ((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1))]   # -y
+    Ps, Is, Ns = [], [], []
+    for k, f in enumerate(faces):
+        P, I, N = _quad(*f)
+        Ps.append(P)
+        Is.append(I + 4 * k)
+        Ns.append(N)
+    return np.concatenate(Ps), np.concatenate(Is), np.concatenate(Ns)
+
+
+def smoke_grid(n=16, seed=3):
+    """a soft blob of density on an n^3 grid (density[z, y, x], sigma_t up to ~6)"""
+    rng = np.random.default_rng(seed)
+    c = (np.arange(n) + 0.5) / n
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    r2 = (x - 0.5) ** 2 + (y - 0.45) ** 2 + (z - 0.5) ** 2
+    d = 6.0 * np.exp(-r2 / 0.06) * (0.75 + 0.5 * rng.random((n, n, n)))
+    return d.astype(np.float32)
+
+
+def foggy_cornell_box(width=256, height=256, grid=True):
+    """The Cornell box with participating media for the volume path integrator: an
+    invisible sphere holding a homogeneous forward-scattering medium and, when `grid`,
+    an invisible box holding a density grid; camera outside both (it starts in vacuum)."""
+    sc = Scene()
+    add_cornell_box(sc, spheres=False)
+    fog = sc.homogeneous_medium(sigma_a=0.2, sigma_s=(1.6, 1.2, 0.9), g=0.5)
+    e = sc.add_sphere((-0.4, -0.5, -0.25), 0.45, abi.RBW_NONE)
+    sc.set_media(e, inside=fog)
+    if grid:
+        # the box stops short of the floor: coplanar faces would make the closest hit a tie
+        smoke = sc.grid_medium(smoke_grid(), albedo=(0.9, 0.85, 0.8), g=-0.2, box_min=(0.05, -0.99, -0.6),
+                               box_max=(0.9, 0.1, 0.4))
+        P, I, N = box_mesh((0.05, -0.99, -0.6), (0.9, 0.1, 0.4))
+        e = sc.add_mesh(P, I, None, normals=N)
+        sc.set_media(e, inside=smoke)
     cam = perspective_camera(width, height, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
     return sc, cam
 

@@ -37,7 +37,7 @@ def test_python_signatures_cover_the_header():
     assert set(declared_functions()) <= set(abi.RBW_SIGNATURES)
 
 
-STRUCTS = ["rbw_material", "rbw_entity", "rbw_scene_desc", "rbw_camera_desc", "rbw_guiding_opts",
+STRUCTS = ["rbw_material", "rbw_entity", "rbw_medium", "rbw_scene_desc", "rbw_camera_desc", "rbw_guiding_opts",
            "rbw_render_params", "rbw_stats", "rbw_device_opts", "rbw_hit", "rbw_job", "rbw_batch_opts",
            "rbw_job_stats", "rbw_batch_stats"]
 

@@ -1,0 +1,297 @@
+"""The volume path integrator (volume_path_integrator.cpp:13-79) and its media
+(media/homogeneous_medium.cpp, media/heterogeneous_medium.cpp, scene.cpp:114-148).
+
+The oracle's restatement is pinned by closed forms: the Henyey-Greenstein phase
+function's normalisation, mean cosine and sampling density; Beer-Lambert transmittance
+of camera rays through homogeneous and gridded slabs (delta tracking passes with
+probability exp(-tau)); the irradiance of a plane under an infinite light through an
+absorbing slab (2 rho Le E3(tau), which exercises the light and BSDF samples' beam
+transmittance); and the furnace -- a unit-albedo medium inside an enclosure that emits
+Le everywhere is lit to Le everywhere.  The furnace holds only with medium_mis = 1: the
+reference drops the phase-sampled half of next-event estimation at medium vertices
+(its emitter pdf from a zero normal is not a number), which the default reproduces and a
+test shows.  The GPU is compared with the oracle on the same per-sample pcg32 streams
+and checked against the same closed forms."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy.special import expn
+
+from oracle import orc_py
+from rainbow_amd import abi, scenes
+
+VOL = abi.RBW_INTEGRATOR_VOLUME_PATH
+
+
+def _olib():
+    L = orc_py.lib()
+    L.orc_hg_phase.argtypes = [C.c_float, C.c_float]
+    L.orc_hg_phase.restype = C.c_float
+    L.orc_hg_sample.argtypes = [abi.f32p, C.c_float, C.c_float, C.c_float, abi.f32p]
+    L.orc_hg_sample.restype = C.c_float
+    L.orc_medium_emitter_pdf.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, C.c_int]
+    L.orc_medium_emitter_pdf.restype = C.c_float
+    return L
+
+
+def _vparams(spp, depth=8, seed=0, **kw):
+    return orc_py.render_params(spp=spp, max_depth=depth, seed=seed, integrator=VOL, **kw)
+
+
+# ------------------------------------------------------------------ scenes
+
+def slab_scene(medium, res=24, Le=(2.0, 1.0, 0.5)):
+    """an emitter quad at z = -2 facing +z seen through the invisible box z in [-1, 0]
+    holding `medium(scene)`; camera at z = 3 with a 2 degree field of view, so every
+    camera ray crosses the slab along |d_z| > 0.9998"""
+    sc = scenes.Scene()
+    black = sc.matte((0.0, 0.0, 0.0))
+    P, I, N = scenes._quad((-4, -4, -2), (4, -4, -2), (4, 4, -2), (-4, 4, -2))
+    sc.add_mesh(P, I, black, normals=N, radiance=Le)
+    P, I, N = scenes.box_mesh((-3, -3, -1), (3, 3, 0))
+    e = sc.add_mesh(P, I, None, normals=N)
+    sc.set_media(e, inside=medium(sc))
+    cam = scenes.perspective_camera(res, res, 2.0, eye=(0, 0, 3), target=(0, 0, 0))
+    return sc, cam, np.array(Le)
+
+
+def lit_plane_scene(sigma_a, rho=0.5, Le=1.0, res=16):
+    """a matte plane z = 0 under a 400 x 400 light at z = 1 (facing down) with an absorbing
+    slab z in [0.2, 0.7] between them; camera at z = 0.9 looking down through the slab"""
+    sc = scenes.Scene()
+    plane = sc.matte((rho, rho, rho))
+    black = sc.matte((0.0, 0.0, 0.0))
+    P, I, N = scenes._quad((-3, -3, 0), (3, -3, 0), (3, 3, 0), (-3, 3, 0))
+    sc.add_mesh(P, I, plane, normals=N)
+    P, I, N = scenes._quad((-200, -200, 1), (-200, 200, 1), (200, 200, 1), (200, -200, 1))
+    sc.add_mesh(P, I, black, normals=N, radiance=(Le, Le, Le))
+    P, I, N = scenes.box_mesh((-100, -100, 0.2), (100, 100, 0.7))
+    e = sc.add_mesh(P, I, None, normals=N)
+    sc.set_media(e, inside=sc.homogeneous_medium(sigma_a, 0.0))
+    cam = scenes.perspective_camera(res, res, 4.0, eye=(0, 0, 0.9), target=(0, 0, 0), up=(0, 1, 0))
+    tau = sigma_a * 0.5
+    # camera ray: exp(-tau); plane: rho / pi * integral of Le cos exp(-tau / cos) over the hemisphere
+    expect = math.exp(-tau) * 2.0 * rho * Le * expn(3, tau)
+    return sc, cam, expect
+
+
+def furnace_scene(grid=False, res=16, g=0.6):
+    """a closed cube [-1, 1]^3 whose inner faces emit 1 and reflect nothing, holding an
+    invisible sphere (radius 0.6) of unit-albedo medium; camera inside the cube, outside
+    the sphere"""
+    sc = scenes.Scene()
+    black = sc.matte((0.0, 0.0, 0.0))
+    P, I, N = scenes.box_mesh((-1, -1, -1), (1, 1, 1))
+    I = I[:, ::-1].copy()  # inward winding and normals
+    sc.add_mesh(P, I, black, normals=-N, radiance=(1.0, 1.0, 1.0))
+    if grid:
+        d = 2.5 * (0.5 + np.random.default_rng(5).random((6, 6, 6)))
+        m = sc.grid_medium(d, albedo=1.0, g=g, box_min=(-0.5, -0.5, -0.5), box_max=(0.5, 0.5, 0.5))
+        P, I, N = scenes.box_mesh((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+        e = sc.add_mesh(P, I, None, normals=N)
+    else:
+        m = sc.homogeneous_medium(0.0, 2.0, g=g)
+        e = sc.add_sphere((0.0, 0.0, 0.0), 0.6, abi.RBW_NONE)
+    sc.set_media(e, inside=m)
+    cam = scenes.perspective_camera(res, res, 50.0, eye=(0.1, 0.05, 0.95), target=(0, 0, 0))
+    return sc, cam
+
+
+def _homog(sa):
+    return lambda sc: sc.homogeneous_medium(sa, 0.0)
+
+
+def _grid_const(c):
+    return lambda sc: sc.grid_medium(np.full((3, 3, 3), c, np.float32), albedo=0.0, box_min=(-3, -3, -1),
+                                     box_max=(3, 3, 0))
+
+
+RAMP = np.array([0.2, 1.4, 0.5, 2.0], np.float32)  # density of the four z layers (texel centres)
+
+
+def _grid_ramp(sc):
+    d = np.broadcast_to(RAMP[:, None, None], (4, 2, 2)).astype(np.float32)
+    return sc.grid_medium(d, albedo=0.0, box_min=(-3, -3, -1), box_max=(3, 3, 0))
+
+
+def _ramp_depth():
+    """optical depth through the unit cube along z of the trilinear interpolant: constant
+    in the outer half texels, linear between texel centres"""
+    n = len(RAMP)
+    h = 0.5 / n
+    tau = RAMP[0] * h + RAMP[-1] * h
+    for k in range(n - 1):
+        tau += 0.5 * (RAMP[k] + RAMP[k + 1]) / n
+    return float(tau)
+
+
+def _check_mean(img, expect, tol_sigma=5.0, rel_floor=2e-3):
+    """per channel: image mean against the closed form within tol_sigma standard errors"""
+    px = img.reshape(-1, 3).astype(np.float64)
+    m = px.mean(0)
+    se = px.std(0) / math.sqrt(len(px))
+    for c in range(3):
+        assert abs(m[c] - expect[c]) < tol_sigma * se[c] + rel_floor * expect[c], (c, m, expect, se)
+
+
+# ------------------------------------------------------------------ phase function
+
+@pytest.mark.parametrize("g", [-0.7, 0.0, 0.3, 0.9])
+def test_hg_normalisation_and_mean_cosine(g):
+    L = _olib()
+    mu_s = np.linspace(-1, 1, 100001)
+    p = np.array([L.orc_hg_phase(float(x), g) for x in mu_s])
+    integral = 2 * math.pi * np.trapezoid(p, mu_s)
+    assert abs(integral - 1.0) < 2e-3, integral
+    # dot(wo, wi) = -cos(scattering angle): the mean cosine of the scattering angle is g
+    mean_cos = 2 * math.pi * np.trapezoid(p * (-mu_s), mu_s)
+    assert abs(mean_cos - g) < 3e-3, (mean_cos, g)
+
+
+@pytest.mark.parametrize("g", [-0.5, 0.0005, 0.7])
+def test_hg_sample_density_and_pdf(g):
+    """sampled directions: pdf returned == phase(dot(wo, wi)); histogram of the scattering
+    cosine against the closed-form HG density"""
+    L = _olib()
+    rng = np.random.default_rng(1)
+    wo = np.array([0.3, -0.5, 0.81], np.float32)
+    wo /= np.linalg.norm(wo)
+    n = 40000
+    u = rng.random((n, 2)).astype(np.float32)
+    cs = np.empty(n)
+    wi = np.zeros(3, np.float32)
+    for i in range(n):
+        pdf = L.orc_hg_sample(abi.ptr(wo, abi.f32p), g, float(u[i, 0]), float(u[i, 1]), abi.ptr(wi, abi.f32p))
+        d = float(np.dot(wo, wi))
+        if i < 200:
+            assert abs(np.linalg.norm(wi) - 1) < 1e-5
+            assert abs(pdf - L.orc_hg_phase(d, g)) < 1e-4 * pdf
+        cs[i] = -d
+    edges = np.linspace(-1, 1, 21)
+    hist = np.histogram(cs, edges)[0] / n
+
+    def cdf(x):  # P(cos <= x) of the HG density 2 pi p(-x) in the scattering cosine
+        if abs(g) < 1e-3:
+            return (x + 1) / 2
+        return (1 - g * g) / (2 * g) * (1 / math.sqrt(1 + g * g - 2 * g * x) - 1 / (1 + g))
+    want = np.array([cdf(b) - cdf(a) for a, b in zip(edges[:-1], edges[1:])])
+    assert np.all(np.abs(hist - want) < 5 * np.sqrt(want / n) + 1e-3), (hist, want)
+
+
+# ------------------------------------------------------------------ oracle closed forms
+
+@pytest.mark.parametrize("name,medium,tau", [
+    ("homogeneous", _homog((0.3, 0.8, 1.6)), (0.3, 0.8, 1.6)),
+    ("grid constant", _grid_const(0.9), (0.9, 0.9, 0.9)),
+    ("grid ramp", _grid_ramp, None),
+])
+def test_oracle_beer_lambert_through_a_slab(name, medium, tau):
+    sc, cam, Le = slab_scene(medium)
+    tau = np.array(tau if tau is not None else (_ramp_depth(),) * 3)
+    img = orc_py.OracleScene(sc).render(cam, _vparams(256, seed=3), sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    _check_mean(img, Le * np.exp(-tau))
+
+
+@pytest.mark.parametrize("sigma_a", [0.0, 0.7, 2.5])
+def test_oracle_plane_lit_through_an_absorbing_slab(sigma_a):
+    sc, cam, expect = lit_plane_scene(sigma_a)
+    img = orc_py.OracleScene(sc).render(cam, _vparams(256, depth=3, seed=4), sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    _check_mean(img, np.full(3, expect), rel_floor=5e-3)
+
+
+@pytest.mark.parametrize("grid", [False, True])
+def test_oracle_furnace_with_phase_mis(grid):
+    sc, cam = furnace_scene(grid)
+    img = orc_py.OracleScene(sc).render(cam, _vparams(128, depth=64, seed=5, medium_mis=1),
+                                        sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    _check_mean(img, np.ones(3), rel_floor=3e-3)
+
+
+def test_oracle_reference_drops_phase_sampled_emitters():
+    """medium_mis = 0 (the reference): the emitter pdf from a medium vertex is not > 0, so
+    the furnace comes out darker than Le; medium_mis = 1 gives the solid-angle pdf of the
+    light quad"""
+    L = _olib()
+    sc, cam, _ = lit_plane_scene(0.5)
+    o = orc_py.OracleScene(sc)
+    p = np.array([0.1, -0.2, 0.5], np.float32)
+    wi = np.array([0.0, 0.0, 1.0], np.float32)
+    ref = L.orc_medium_emitter_pdf(o.h, 1, abi.ptr(p, abi.f32p), abi.ptr(wi, abi.f32p), 0)
+    raw = L.orc_medium_emitter_pdf(o.h, 1, abi.ptr(p, abi.f32p), abi.ptr(wi, abi.f32p), 1)
+    assert not ref > 0
+    assert abs(raw - 0.5 ** 2 / (400.0 * 400.0)) < 1e-4 * raw   # distance^2 / (cos * area)
+    fs, fc = furnace_scene()
+    img = orc_py.OracleScene(fs).render(fc, _vparams(64, depth=64, seed=5), sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    assert img.mean() < 0.97
+
+
+def test_oracle_vacuum_media_change_nothing():
+    """a scene whose media are all vacuum renders as with the path integrator, draw for draw"""
+    sc, cam = scenes.cornell_box(24, 24)
+    P, I, N = scenes.box_mesh((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    e = sc.add_mesh(P, I, None, normals=N)
+    sc.homogeneous_medium(1.0, 1.0)
+    sc.set_media(e)          # media component with vacuum on both sides
+    o = orc_py.OracleScene(sc)
+    v = o.render(cam, _vparams(8, depth=5, seed=6), sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    p = o.render(cam, orc_py.render_params(spp=8, max_depth=5, seed=6), sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)["image"]
+    assert np.array_equal(v, p)
+
+
+# ------------------------------------------------------------------ GPU
+
+def _gpu(tracer, sc, cam, spp, depth=8, seed=0, **kw):
+    from rainbow_amd.tracer import render_params
+    tracer.upload(sc)
+    return tracer.render(cam, render_params(spp=spp, max_depth=depth, seed=seed, integrator=VOL, **kw))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", [False, True])
+@pytest.mark.parametrize("medium_mis", [0, 1])
+def test_gpu_volume_matches_oracle(tracer, grid, medium_mis):
+    sc, cam = scenes.foggy_cornell_box(32, 32, grid=grid)
+    r = _gpu(tracer, sc, cam, 16, seed=7, medium_mis=medium_mis)
+    o = orc_py.OracleScene(sc).render(cam, _vparams(16, seed=7, medium_mis=medium_mis),
+                                      sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)
+    g, c = r["image"], o["image"]
+    bad = (np.abs(g - c) > 1e-4 * np.maximum(np.abs(g), np.abs(c)) + 1e-6).any(-1)
+    # logf/expf may differ from the host's by an ulp, which can flip a distance-sampling
+    # decision and send a path elsewhere: a few pixels may differ, the rest are exact
+    assert bad.mean() < 0.02, bad.sum()
+    assert abs(float(g.mean()) - float(c.mean())) < 0.02 * float(c.mean())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,medium,tau", [
+    ("homogeneous", _homog((0.3, 0.8, 1.6)), (0.3, 0.8, 1.6)),
+    ("grid ramp", _grid_ramp, None),
+])
+def test_gpu_beer_lambert_through_a_slab(tracer, name, medium, tau):
+    sc, cam, Le = slab_scene(medium, res=64)
+    tau = np.array(tau if tau is not None else (_ramp_depth(),) * 3)
+    _check_mean(_gpu(tracer, sc, cam, 256, seed=3)["image"], Le * np.exp(-tau))
+
+
+@pytest.mark.gpu
+def test_gpu_plane_lit_through_an_absorbing_slab(tracer):
+    sc, cam, expect = lit_plane_scene(0.7, res=32)
+    _check_mean(_gpu(tracer, sc, cam, 512, depth=3, seed=4)["image"], np.full(3, expect), rel_floor=5e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", [False, True])
+def test_gpu_furnace_with_phase_mis(tracer, grid):
+    sc, cam = furnace_scene(grid, res=32)
+    _check_mean(_gpu(tracer, sc, cam, 256, depth=64, seed=5, medium_mis=1)["image"], np.ones(3), rel_floor=3e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_volume_rejects_guiding(tracer):
+    from rainbow_amd.tracer import RbwError, guiding_opts
+    sc, cam = scenes.foggy_cornell_box(8, 8)
+    go = guiding_opts()
+    with pytest.raises(RbwError):
+        _gpu(tracer, sc, cam, 4, guiding=C.pointer(go))
