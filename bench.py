@@ -26,12 +26,12 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s + time to relMSE 1e-3: guided vs plain GPU PT, 1 MI355X"
 
 
-def cpu_baseline(sc, cam, spp, depth, target_s=15.0):
+def cpu_baseline(sc, cam, spp, depth, target_s=15.0, **pkw):
     """The oracle on a bounded subset of tiles (every `stride`-th 16x16 tile), all spp."""
     from oracle import orc_py  # the CPU restatement, timed as the baseline
     threads = min(16, os.cpu_count() or 1)
     os_ = orc_py.OracleScene(sc)
-    p = orc_py.render_params(spp=spp, max_depth=depth)
+    p = orc_py.render_params(spp=spp, max_depth=depth, **pkw)
     ntiles = ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
     stride = max(1, ntiles // 8)
     r = os_.render(cam, p, sampler=orc_py.ORC_SAMPLER_TILE, threads=threads, tile_stride=stride)
@@ -121,6 +121,41 @@ def guided_leg(full, threshold=1e-3):
     }
 
 
+def volume_leg(cpu=True, steps=2):
+    """The volume path integrator on the foggy Cornell box (a homogeneous fog sphere and a
+    16^3 smoke grid behind invisible boundaries), 512x512, 64 spp, depth 8, beside the
+    oracle's rate on a bounded sample of the same workload."""
+    from rainbow_amd import abi, scenes
+    from rainbow_amd.tracer import Tracer, render_params
+    sc, cam = scenes.foggy_cornell_box(512, 512, grid=True)
+    spp, depth = 64, 8
+    tr = Tracer(0)
+    tr.upload(sc)
+    params = render_params(spp=spp, max_depth=depth, seed=0, integrator=abi.RBW_INTEGRATOR_VOLUME_PATH)
+    tr.render(cam, params)
+    rays, samples = 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st_ = abi.rbw_stats()
+        rc = tr.lib.rbw_render(tr.h, C.byref(cam), C.byref(params), None, C.byref(st_))
+        if rc != 0:
+            raise RuntimeError(tr.lib.rbw_last_error().decode())
+        d = st_.as_dict()
+        samples += d["samples"]
+        rays += d["extension_rays"] + d["shadow_rays"] + d["mis_rays"]
+    wall = time.perf_counter() - t0
+    out = {"workload": "Cornell box with a homogeneous fog sphere (g 0.5) and a 16^3 smoke grid (g -0.2), "
+                       "512x512, 64 spp, max depth 8, reference MIS weighting",
+           "value": round(samples / wall / 1e6, 3), "unit": "Msamples/s",
+           "mrays_per_s": round(rays / wall / 1e6, 2), "ms_per_render": round(wall / steps * 1e3, 2)}
+    print(f"# volume leg: {out['value']} Msamples/s", file=sys.stderr, flush=True)
+    if cpu:
+        cb = cpu_baseline(sc, cam, spp, depth, target_s=5.0, integrator=abi.RBW_INTEGRATOR_VOLUME_PATH)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(out["value"] / cb["value"], 1)
+    return out
+
+
 def batch_leg(n_jobs=64, autotune=True):
     """The batched-jobs workload: n_jobs seeded jobs of uneven size (grid-sphere meshes of
     1e4-2e6 triangles log-uniform in the Cornell box, 128^2-1024^2, 4-256 spp, depth 3-12)
@@ -202,6 +237,8 @@ def main():
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
     ap.add_argument("--batch", action=argparse.BooleanOptionalAction, default=True,
                     help="batched-jobs leg (64 jobs of uneven size in one submission)")
+    ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
+                    help="volume path integrator leg (foggy Cornell box)")
     ap.add_argument("--guided", default="reduced", choices=["reduced", "full", "off"],
                     help="guided-vs-plain time-to-relMSE leg: 256^2 with a 16384-spp judge (reduced), "
                          "512^2 with a 65536-spp judge (full), or skipped")
@@ -296,6 +333,8 @@ def main():
         out["batch"] = batch_leg()
     if args.guided != "off":
         out["guided"] = guided_leg(args.guided == "full")
+    if args.volume:
+        out["volume"] = volume_leg(cpu=not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(sc, cam, spp, depth)
         out["cpu_baseline"] = cb

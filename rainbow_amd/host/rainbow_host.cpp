@@ -219,12 +219,64 @@ rbw_scene_desc flat_scene::desc() const {
     d.entities = entities.data();
     d.num_materials = uint32_t(materials.size());
     d.materials = materials.data();
+    if (!media.empty()) {
+        d.num_media = uint32_t(media.size());
+        d.media = media.data();
+        d.entity_media = entity_media.data();
+        d.medium_grid_count = uint32_t(medium_grid.size());
+        d.medium_grid = medium_grid.empty() ? nullptr : medium_grid.data();
+    }
     return d;
 }
+
+namespace {
+// one rbw_medium per distinct medium object: homogeneous (homogeneous_medium.cpp:7-10) or a
+// grid (heterogeneous_medium.cpp:5-22) with world_to_local = the inverse of its transform
+uint32_t flatten_medium(flat_scene& f, std::map<const media::medium*, uint32_t>& index,
+                        const std::shared_ptr<media::medium>& m) {
+    if (!m) return RBW_NONE;
+    auto it = index.find(m.get());
+    if (it != index.end()) return it->second;
+    rbw_medium out;
+    std::memset(&out, 0, sizeof(out));
+    if (auto h = std::dynamic_pointer_cast<media::homogeneous_medium>(m)) {
+        out.type = RBW_MEDIUM_HOMOGENEOUS;
+        for (int c = 0; c < 3; c++) {
+            out.sigma_a[c] = h->sigma_a[c];
+            out.sigma_s[c] = h->sigma_s[c];
+        }
+        out.g = h->g;
+    } else if (auto g = std::dynamic_pointer_cast<media::heterogeneous_medium>(m)) {
+        require(g->sigma_t && g->albedo && g->g, "heterogeneous_medium needs sigma_t, albedo and g");
+        const auto size = g->sigma_t->size();
+        const auto& vals = g->sigma_t->values();
+        require(vals.size() >= size[0] * size[1] * size[2] && size[0] * size[1] * size[2] > 0,
+                "sigma_t grid is smaller than its size");
+        out.type = RBW_MEDIUM_GRID;
+        out.grid_offset = uint32_t(f.medium_grid.size());
+        for (int k = 0; k < 3; k++) out.grid_size[k] = uint32_t(size[k]);
+        f.medium_grid.insert(f.medium_grid.end(), vals.begin(), vals.begin() + size[0] * size[1] * size[2]);
+        for (int c = 0; c < 3; c++) out.albedo[c] = g->albedo->value()[c];
+        out.g = g->g->value();
+        float Minv[16];
+        to_float(g->local_to_world.inverse_matrix(), Minv);  // column-major
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) out.world_to_local[4 * r + c] = Minv[c * 4 + r];
+    } else {
+        throw std::runtime_error("unsupported medium");
+    }
+    const uint32_t id = uint32_t(f.media.size());
+    f.media.push_back(out);
+    index.emplace(m.get(), id);
+    return id;
+}
+}  // namespace
 
 flat_scene flatten(const scene& s) {
     flat_scene f;
     std::map<const materials::material*, uint32_t> mat_index;
+    std::map<const media::medium*, uint32_t> med_index;
+    std::vector<uint32_t> ent_media;
     for (const auto& ep : s.entities()) {
         const scenes::entity& e = *ep;
         rbw_entity out;
@@ -341,9 +393,27 @@ flat_scene flatten(const scene& s) {
         } else {
             throw std::runtime_error("unsupported shape");
         }
+        uint32_t mo = RBW_NONE, mi = RBW_NONE;
+        if (auto md = e.media()) {
+            out.flags |= RBW_ENTITY_HAS_MEDIA;
+            mo = flatten_medium(f, med_index, md->outside());
+            mi = flatten_medium(f, med_index, md->inside());
+        }
+        ent_media.insert(ent_media.end(), {mo, mi});
         f.entities.push_back(out);
     }
     f.triangle_material.assign(f.triangle_entity.size(), RBW_NONE);
+    bool any_media = false;
+    for (const auto& en : f.entities) any_media = any_media || (en.flags & RBW_ENTITY_HAS_MEDIA);
+    if (any_media) {
+        f.entity_media = std::move(ent_media);
+        if (f.media.empty()) {
+            // media components that are vacuum on both sides still need a (unused) table
+            rbw_medium none;
+            std::memset(&none, 0, sizeof(none));
+            f.media.push_back(none);
+        }
+    }
     return f;
 }
 
@@ -414,6 +484,7 @@ void wavefront_path_integrator::render(const std::shared_ptr<camera>& cam, const
     p.shadow_mode = shadow_mode;
     p.integrator = integrator_kind;
     p.emitter_samples = emitter_samples;
+    p.medium_mis = medium_mis;
     p.bsdf_samples = bsdf_samples;
     std::vector<float> rgb(size_t(cd.width) * cd.height * 3);
     check(rbw_render(mCtx, &cd, &p, rgb.data(), &last_stats));

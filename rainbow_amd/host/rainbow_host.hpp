@@ -125,6 +125,19 @@ private:
 };
 template <typename T>
 using texture2d = constant_texture2d<T>;
+template <typename T>
+using constant_texture3d = constant_texture2d<T>;
+// textures/image_texture.hpp:58 (image_texture_t<3, T>): size.x * size.y * size.z values, x fastest
+template <typename T>
+class image_texture3d final : public interfaces::noncopyable {
+public:
+    image_texture3d(std::array<size_t, 3> size, std::vector<T> values) : mSize(size), mValues(std::move(values)) {}
+    std::array<size_t, 3> size() const noexcept { return mSize; }
+    const std::vector<T>& values() const noexcept { return mValues; }
+private:
+    std::array<size_t, 3> mSize;
+    std::vector<T> mValues;
+};
 }  // namespace textures
 
 namespace shapes {
@@ -219,7 +232,45 @@ public:
 }  // namespace emitters
 
 namespace media {
-class media : public interfaces::noncopyable {};
+using shared::spectrums::spectrum;
+// media/medium.hpp:51-58
+class medium : public interfaces::noncopyable {
+public:
+    virtual ~medium() = default;
+};
+// media/homogeneous_medium.hpp
+class homogeneous_medium final : public medium {
+public:
+    homogeneous_medium(const spectrum& sigma_a, const spectrum& sigma_s, real g)
+        : sigma_a(sigma_a), sigma_s(sigma_s), g(g) {}
+    spectrum sigma_a, sigma_s;
+    real g;
+};
+// media/heterogeneous_medium.hpp: sigma_t an image texture over the unit cube of the
+// medium's local space (transform: local to world); albedo and g constant textures
+class heterogeneous_medium final : public medium {
+public:
+    heterogeneous_medium(const std::shared_ptr<textures::constant_texture3d<spectrum>>& albedo,
+                         const std::shared_ptr<textures::image_texture3d<real>>& sigma_t,
+                         const std::shared_ptr<textures::constant_texture3d<real>>& g, const shared::transform& transform)
+        : albedo(albedo), sigma_t(sigma_t), g(g), local_to_world(transform) {}
+    std::shared_ptr<textures::constant_texture3d<spectrum>> albedo;
+    std::shared_ptr<textures::image_texture3d<real>> sigma_t;
+    std::shared_ptr<textures::constant_texture3d<real>> g;
+    shared::transform local_to_world;
+};
+// media/medium.hpp:60-75: the media component of an entity (null = vacuum)
+class media final : public interfaces::noncopyable {
+public:
+    media(const std::shared_ptr<medium>& outside, const std::shared_ptr<medium>& inside)
+        : mOutside(outside), mInside(inside) {}
+    explicit media(const std::shared_ptr<medium>& two_side) : media(two_side, two_side) {}
+    std::shared_ptr<medium> outside() const noexcept { return mOutside; }
+    std::shared_ptr<medium> inside() const noexcept { return mInside; }
+    bool different_sides() const noexcept { return mOutside != mInside; }
+private:
+    std::shared_ptr<medium> mOutside, mInside;
+};
 }  // namespace media
 
 namespace scenes {
@@ -389,6 +440,9 @@ struct flat_scene {
     std::vector<uint32_t> sphere_entity;
     std::vector<rbw_entity> entities;
     std::vector<rbw_material> materials;
+    std::vector<rbw_medium> media;        // empty: no entity has media
+    std::vector<uint32_t> entity_media;   // (outside, inside) per entity when media is not empty
+    std::vector<float> medium_grid;
     bool has_normals = false, has_uvs = false;
     rbw_scene_desc desc() const;
 };
@@ -414,6 +468,7 @@ public:
     // RBW_INTEGRATOR_DIRECT makes this wavefront_direct_integrator's engine
     uint32_t integrator_kind = RBW_INTEGRATOR_PATH;
     uint32_t emitter_samples = 1, bsdf_samples = 1;
+    uint32_t medium_mis = 0;  // volume path: 1 = phase-sampled emitter hits count (see rbw_render_params)
     rbw_stats last_stats{};
 private:
     std::shared_ptr<samplers::sampler2d> mSampler2D;
@@ -445,6 +500,24 @@ public:
 private:
     wavefront_path_integrator mEngine;
 };
+
+// integrators/volume_path_integrator.hpp: the path integrator through the scene's media
+// (entities' media components), same constructor arguments as path_integrator
+class wavefront_volume_path_integrator final : public integrator {
+public:
+    wavefront_volume_path_integrator(const std::shared_ptr<samplers::sampler2d>& sampler2d,
+                                     const std::shared_ptr<samplers::sampler1d>& sampler1d, size_t max_depth = 5,
+                                     real threshold = 1, int device = 0)
+        : mEngine(sampler2d, sampler1d, max_depth, threshold, device) {
+        mEngine.integrator_kind = RBW_INTEGRATOR_VOLUME_PATH;
+    }
+    void render(const std::shared_ptr<camera>& camera, const std::shared_ptr<scene>& scene) override {
+        mEngine.render(camera, scene);
+    }
+    wavefront_path_integrator& engine() noexcept { return mEngine; }  // seed, medium_mis, last_stats
+private:
+    wavefront_path_integrator mEngine;
+};
 }  // namespace integrators
 
 namespace scenes {
@@ -452,15 +525,20 @@ namespace scenes {
 //   {"materials": {"white": {"matte": {"diffuse": [0.73, 0.73, 0.73], "sigma": 0}},
 //                  "red": {"plastic": {"diffuse": [...], "specular": [...], "roughness": 0.1, "eta": 1.49}},
 //                  "gold": {"metal": {"eta": [...], "k": [...], "roughness": 0.1, "remap_roughness": false}}},
+//    "media": {"fog": {"homogeneous": {"sigma_a": [...], "sigma_s": [...], "g": 0.5}},
+//              "smoke": {"grid": {"size": [nx, ny, nz], "density": [...], "albedo": [...], "g": 0,
+//                                 "min": [x, y, z], "max": [x, y, z]}}},
 //    "entities": [{"shape": {"box": [w, h, d]} | {"quad": [w, h]} | {"sphere": r} |
 //                           {"mesh": {"positions": [...], "indices": [...], "normals": [...]}},
 //                  "material": "white" | {...}, "emitter": {"radiance": [r, g, b]},
+//                  "media": "fog" | {"outside": null, "inside": "fog"},  (no material: invisible boundary)
 //                  "transform": [{"translate": [x, y, z]}, {"scale": s | [x, y, z]},
 //                                {"rotate": {"axis": [x, y, z], "degrees": a}}],
 //                  "reverse_orientation": false}, ...],
 //    "camera": {"resolution": [w, h], "fov": 40, "eye": [...], "target": [...], "up": [0, 1, 0],
 //               "lens_radius": 0, "focus": 1e6, "crop": [x0, y0, x1, y1], "scale": 1},
-//    "render": {"spp": 16, "max_depth": 5, "threshold": 1, "seed": 0, "integrator": "path" | "direct",
+//    "render": {"spp": 16, "max_depth": 5, "threshold": 1, "seed": 0,
+//               "integrator": "path" | "direct" | "volume_path", "medium_mis": false,
 //               "emitter_samples": 1, "bsdf_samples": 1,
 //               "stratified": {"strata": [4, 4], "dims_1d": 8, "dims_2d": 12}, "output": "name"}}
 // Transforms compose left to right; the screen window is [-1, 1]^2 scaled by the aspect.
@@ -473,6 +551,8 @@ struct scene_file {
     bool stratified = false;
     size_t strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
     bool direct = false;  // "integrator": "direct"
+    bool volume = false;  // "integrator": "volume_path"
+    bool medium_mis = false;
     size_t emitter_samples = 1, bsdf_samples = 1;
     std::string output;  // film::write name ("" = none)
     // the integrator the render settings describe (path, or direct with its sample counts)

@@ -268,6 +268,39 @@ std::shared_ptr<shapes::shape> make_shape(const json& s, bool reverse, const std
     bad(where, "unknown shape \"" + type + "\"");
 }
 
+// a named medium: {"homogeneous": {"sigma_a": rgb, "sigma_s": rgb, "g": g}} or
+// {"grid": {"size": [nx, ny, nz], "density": [...], "albedo": rgb, "g": g, "min": xyz, "max": xyz}}
+// (the grid spans the box [min, max]; density x fastest)
+std::shared_ptr<media::medium> make_medium(const json& m, const std::string& where) {
+    if (m.kind != json::object_v || m.o.size() != 1)
+        bad(where, "a medium is {\"homogeneous\": {...}} or {\"grid\": {...}}");
+    const std::string& type = m.o.begin()->first;
+    const json& a = m.o.begin()->second;
+    const real g = real(number_or(a, "g", 0.0, where));
+    if (!(std::fabs(g) < 1)) bad(where + ".g", "must lie in (-1, 1)");
+    if (type == "homogeneous")
+        return std::make_shared<media::homogeneous_medium>(rgb(member(a, "sigma_a", where), where + ".sigma_a"),
+                                                           rgb(member(a, "sigma_s", where), where + ".sigma_s"), g);
+    if (type == "grid") {
+        const auto n = numbers(member(a, "size", where), 3, where + ".size");
+        if (n[0] < 1 || n[1] < 1 || n[2] < 1) bad(where + ".size", "must be positive");
+        const std::array<size_t, 3> size{size_t(n[0]), size_t(n[1]), size_t(n[2])};
+        const json& d = member(a, "density", where);
+        const auto dens = numbers(d, size[0] * size[1] * size[2], where + ".density");
+        std::vector<real> vals(dens.begin(), dens.end());
+        const vector3 lo = vec3(member(a, "min", where), where + ".min");
+        const vector3 hi = vec3(member(a, "max", where), where + ".max");
+        if (!(hi.x > lo.x && hi.y > lo.y && hi.z > lo.z)) bad(where, "max must exceed min");
+        const shared::transform t = shared::translate(lo) * shared::scale({hi.x - lo.x, hi.y - lo.y, hi.z - lo.z});
+        return std::make_shared<media::heterogeneous_medium>(
+            std::make_shared<textures::constant_texture3d<spectrum>>(rgb(member(a, "albedo", where), where + ".albedo")),
+            std::make_shared<textures::image_texture3d<real>>(size, std::move(vals)),
+            std::make_shared<textures::constant_texture3d<real>>(g), t);
+    }
+    bad(where, "unknown medium type \"" + type + "\"");
+    return nullptr;
+}
+
 }  // namespace
 
 namespace scenes {
@@ -282,6 +315,18 @@ scene_file load_scene(const std::string& text) {
         if (m->kind != json::object_v) bad("materials", "expected an object of named materials");
         for (const auto& kv : m->o) mats[kv.first] = make_material(kv.second, "materials." + kv.first);
     }
+    std::map<std::string, std::shared_ptr<media::medium>> meds;
+    if (const json* m = root.find("media")) {
+        if (m->kind != json::object_v) bad("media", "expected an object of named media");
+        for (const auto& kv : m->o) meds[kv.first] = make_medium(kv.second, "media." + kv.first);
+    }
+    auto medium_ref = [&](const json* v, const std::string& w) -> std::shared_ptr<media::medium> {
+        if (!v || v->kind == json::null_v) return nullptr;
+        if (v->kind != json::string_v) bad(w, "expected the name of a medium or null");
+        auto it = meds.find(v->s);
+        if (it == meds.end()) bad(w, "unknown medium \"" + v->s + "\"");
+        return it->second;
+    };
     const json& ents = member(root, "entities", "root");
     if (ents.kind != json::array_v) bad("entities", "expected an array");
     for (size_t i = 0; i < ents.a.size(); i++) {
@@ -303,10 +348,23 @@ scene_file load_scene(const std::string& text) {
         std::shared_ptr<emitters::emitter> em;
         if (const json* el = e.find("emitter"))
             em = std::make_shared<emitters::surface_light>(rgb(member(*el, "radiance", w + ".emitter"), w + ".emitter"));
-        if (!mat && !em) bad(w, "an entity needs a material or an emitter");
-        if (!mat)  // an emitter without a material reflects nothing
+        // "media": "name" (both sides) or {"outside": name | null, "inside": name | null}
+        std::shared_ptr<media::media> md;
+        if (const json* mm = e.find("media")) {
+            if (mm->kind == json::string_v || mm->kind == json::null_v) {
+                md = std::make_shared<media::media>(medium_ref(mm, w + ".media"));
+            } else if (mm->kind == json::object_v) {
+                md = std::make_shared<media::media>(medium_ref(mm->find("outside"), w + ".media.outside"),
+                                                    medium_ref(mm->find("inside"), w + ".media.inside"));
+            } else {
+                bad(w + ".media", "expected a medium name or {\"outside\", \"inside\"}");
+            }
+        }
+        if (!mat && !em && !md) bad(w, "an entity needs a material, an emitter or media");
+        if (!mat && em)  // an emitter without a material reflects nothing
             mat = std::make_shared<materials::matte_material>(constant(spectrum(0)), constant(real(0)));
-        out.scene->add_entity(std::make_shared<entity>(mat, em, shape, nullptr, make_transform(e.find("transform"), w + ".transform")));
+        // (a media boundary without material is invisible: rays pass through it)
+        out.scene->add_entity(std::make_shared<entity>(mat, em, shape, md, make_transform(e.find("transform"), w + ".transform")));
     }
     out.scene->build_accelerator();
 
@@ -348,9 +406,14 @@ scene_file load_scene(const std::string& text) {
             out.spp = out.strata_x * out.strata_y;
         }
         if (const json* it = r->find("integrator")) {
-            if (it->kind != json::string_v || (it->s != "path" && it->s != "direct"))
-                bad("render.integrator", "expected \"path\" or \"direct\"");
+            if (it->kind != json::string_v || (it->s != "path" && it->s != "direct" && it->s != "volume_path"))
+                bad("render.integrator", "expected \"path\", \"direct\" or \"volume_path\"");
             out.direct = it->s == "direct";
+            out.volume = it->s == "volume_path";
+        }
+        if (const json* mm = r->find("medium_mis")) {
+            if (mm->kind != json::bool_v) bad("render.medium_mis", "expected true or false");
+            out.medium_mis = mm->b;
         }
         out.emitter_samples = size_t(number_or(*r, "emitter_samples", 1, "render"));
         out.bsdf_samples = size_t(number_or(*r, "bsdf_samples", 1, "render"));
@@ -382,6 +445,12 @@ std::shared_ptr<integrators::integrator> scene_file::integrator() const {
     if (direct) {
         auto integ = std::make_shared<integrators::wavefront_direct_integrator>(s2, s1, emitter_samples, bsdf_samples);
         integ->engine().seed = seed;
+        return integ;
+    }
+    if (volume) {
+        auto integ = std::make_shared<integrators::wavefront_volume_path_integrator>(s2, s1, max_depth, threshold);
+        integ->engine().seed = seed;
+        integ->engine().medium_mis = medium_mis ? 1u : 0u;
         return integ;
     }
     auto integ = std::make_shared<integrators::wavefront_path_integrator>(s2, s1, max_depth, threshold);

@@ -111,7 +111,18 @@ int main(int argc, char** argv) {
         std::fwrite(fs.entities.data(), sizeof(rbw_entity), fs.entities.size(), out);
         std::fwrite(fs.materials.data(), sizeof(rbw_material), fs.materials.size(), out);
         std::fwrite(&cd, sizeof(cd), 1, out);
+        const uint32_t has_uvs = fs.has_uvs ? 1u : 0u;
+        std::fwrite(&has_uvs, 4, 1, out);
         if (fs.has_uvs) std::fwrite(fs.uvs.data(), 4, fs.uvs.size(), out);  // (u, v) per vertex, when any
+        // media: count, table, (outside, inside) per entity, grid densities
+        const uint32_t nmed = uint32_t(fs.media.size()), ngrid = uint32_t(fs.medium_grid.size());
+        std::fwrite(&nmed, 4, 1, out);
+        if (nmed) {
+            std::fwrite(fs.media.data(), sizeof(rbw_medium), nmed, out);
+            std::fwrite(fs.entity_media.data(), 4, fs.entity_media.size(), out);
+        }
+        std::fwrite(&ngrid, 4, 1, out);
+        if (ngrid) std::fwrite(fs.medium_grid.data(), 4, ngrid, out);
     } else {
         const size_t spp = argc > 3 ? std::stoul(argv[3]) : 16;
         const size_t depth = argc > 4 ? std::stoul(argv[4]) : 5;

@@ -34,7 +34,17 @@ def _read_flat(path):
     off += nm * MATERIAL_DTYPE.itemsize
     out["CAM"] = abi.rbw_camera_desc.from_buffer_copy(b[off: off + C.sizeof(abi.rbw_camera_desc)])
     off += C.sizeof(abi.rbw_camera_desc)
-    out["UV"] = take(np.float32, 2 * nv) if off < len(b) else None
+    has_uvs = int(take(np.uint32, 1)[0])
+    out["UV"] = take(np.float32, 2 * nv) if has_uvs else None
+    nmed = int(take(np.uint32, 1)[0])
+    out["MED"] = None
+    if nmed:
+        out["MED"] = (abi.rbw_medium * nmed).from_buffer_copy(b[off: off + nmed * C.sizeof(abi.rbw_medium)])
+        off += nmed * C.sizeof(abi.rbw_medium)
+        out["EM"] = take(np.uint32, 2 * ne)
+    ngrid = int(take(np.uint32, 1)[0])
+    out["G"] = take(np.float32, ngrid) if ngrid else np.zeros(0, np.float32)
+    assert off == len(b)
     return out
 
 
@@ -61,6 +71,12 @@ def flat_desc(f):
     d.entities = C.cast(ents, C.POINTER(abi.rbw_entity))
     d.num_materials = len(mats)
     d.materials = C.cast(mats, C.POINTER(abi.rbw_material))
+    if f.get("MED") is not None:
+        d.num_media = len(f["MED"])
+        d.media = C.cast(f["MED"], C.POINTER(abi.rbw_medium))
+        d.entity_media = abi.ptr(keep["EM"], abi.u32p)
+        d.medium_grid_count = len(keep["G"])
+        d.medium_grid = abi.ptr(keep["G"], abi.f32p) if len(keep["G"]) else abi.ptr(None, abi.f32p)
     return d
 
 

@@ -197,3 +197,78 @@ def test_gpu_render_file_direct_integrator(tracer, tmp_path):
                                            emitter_samples=2, bsdf_samples=3))["image"]
     bad = (np.abs(img - ref) > 1e-4 * np.maximum(np.abs(img), np.abs(ref)) + 1e-7).any(-1)
     assert bad.mean() < 1e-3, bad.sum()
+
+
+FOG = {
+    "materials": {"white": {"matte": {"diffuse": [0.73, 0.73, 0.73]}}},
+    "media": {"fog": {"homogeneous": {"sigma_a": [0.2, 0.2, 0.2], "sigma_s": [1.6, 1.2, 0.9], "g": 0.5}},
+              "smoke": {"grid": {"size": [3, 2, 2], "density": [float(v) for v in range(12)],
+                                 "albedo": [0.9, 0.85, 0.8], "g": -0.2, "min": [0.0, -1.0, -0.5],
+                                 "max": [0.6, 0.0, 0.5]}}},
+    "entities": [
+        {"shape": {"box": [2, 2, 2]}, "material": "white", "reverse_orientation": True},
+        {"shape": {"quad": [0.5, 0.5]}, "emitter": {"radiance": [17, 12, 4]},
+         "transform": [{"translate": [0, 0.99, 0]}, {"rotate": {"axis": [1, 0, 0], "degrees": 90}}]},
+        {"shape": {"sphere": 0.4}, "media": {"outside": None, "inside": "fog"},
+         "transform": [{"translate": [-0.4, -0.5, -0.2]}]},
+        {"shape": {"box": [0.6, 1.0, 1.0]}, "media": {"inside": "smoke"},
+         "transform": [{"translate": [0.3, -0.5, 0.0]}]},
+    ],
+    "camera": {"resolution": [24, 24], "fov": 60, "eye": [0, 0, 0.95], "target": [0, -0.2, 0]},
+    "render": {"spp": 8, "max_depth": 6, "seed": 9, "integrator": "volume_path", "medium_mis": True},
+}
+
+
+def test_media_flatten_to_the_abi(tmp_path):
+    """media (media/medium.hpp): a homogeneous and a grid medium, invisible boundaries with
+    (outside, inside) pairs and the entity flag; the grid's world_to_local maps its box to
+    the unit cube"""
+    r, out = _load(tmp_path, FOG)
+    assert r.returncode == 0, r.stderr
+    f = _read_flat(out)
+    E = (abi.rbw_entity * 4).from_buffer_copy(f["E"])
+    assert [e.flags & abi.RBW_ENTITY_HAS_MEDIA for e in E] == [0, 0, abi.RBW_ENTITY_HAS_MEDIA,
+                                                               abi.RBW_ENTITY_HAS_MEDIA]
+    assert E[2].material == abi.RBW_NONE and E[3].material == abi.RBW_NONE
+    med = f["MED"]
+    assert len(med) == 2
+    fog, smoke = med[0], med[1]
+    assert fog.type == abi.RBW_MEDIUM_HOMOGENEOUS and list(fog.sigma_s) == pytest.approx([1.6, 1.2, 0.9])
+    assert fog.g == pytest.approx(0.5)
+    assert smoke.type == abi.RBW_MEDIUM_GRID and list(smoke.grid_size) == [3, 2, 2]
+    assert list(f["G"]) == list(range(12)) and smoke.grid_offset == 0
+    assert list(f["EM"]) == [abi.RBW_NONE] * 4 + [abi.RBW_NONE, 0, abi.RBW_NONE, 1]
+    W = np.array(smoke.world_to_local).reshape(3, 4)
+    for p, want in [((0.0, -1.0, -0.5), (0, 0, 0)), ((0.6, 0.0, 0.5), (1, 1, 1)), ((0.3, -0.5, 0.0), (0.5, 0.5, 0.5))]:
+        assert W[:, :3] @ np.array(p) + W[:, 3] == pytest.approx(want, abs=1e-6)
+
+
+def test_media_file_errors(tmp_path):
+    bad = json.loads(json.dumps(FOG))
+    bad["entities"][2]["media"]["inside"] = "nope"
+    r, _ = _load(tmp_path, bad)
+    assert r.returncode != 0 and "unknown medium" in r.stderr
+    bad = json.loads(json.dumps(FOG))
+    bad["media"]["fog"]["homogeneous"]["g"] = 1.0
+    r, _ = _load(tmp_path, bad)
+    assert r.returncode != 0 and "(-1, 1)" in r.stderr
+
+
+@pytest.mark.gpu
+def test_volume_file_renders_like_the_abi(tmp_path, tracer):
+    """render_file through wavefront_volume_path_integrator equals rbw_render of the same
+    flattened scene with the file's settings"""
+    from rainbow_amd.tracer import render_params
+    src = tmp_path / "fog.json"
+    src.write_text(json.dumps(FOG))
+    flat, img = tmp_path / "flat.bin", tmp_path / "img.bin"
+    subprocess.run([_binary(), "load", str(src), str(flat)], check=True)
+    subprocess.run([_binary(), "render_file", str(src), str(img)], check=True)
+    got = np.fromfile(img, np.float32).reshape(24, 24, 3)
+    f = _read_flat(flat)
+    d = flat_desc(f)
+    tracer.upload_desc(d)
+    ref = tracer.render(f["CAM"], render_params(spp=8, max_depth=6, seed=9, integrator=abi.RBW_INTEGRATOR_VOLUME_PATH,
+                                                medium_mis=1))["image"]
+    assert np.array_equal(got, ref)
+    assert ref.mean() > 0
