@@ -325,7 +325,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "triangles": ntris, "width": cam.width, "height": cam.height,
                    "spp": spp, "max_depth": depth, "sampler": "per-sample pcg32", "triangle_test": "moller-trumbore",
-                   "shadow_rays": "closest-visible"},
+                   "shadow_rays": "closest-visible results (any-hit walk: the light is planar)",
+                   "mis_rays": "closest-visible results (emitter triangles first, then an occlusion walk)"},
         "mrays_per_s": round(rays / wall / 1e6, 2),
         "roofline": roof,
     }

@@ -52,7 +52,8 @@ struct djob {
     uint32_t max_depth, seed, npix, bw;
     float rr_threshold;
     uint32_t root8;
-    uint32_t pad[2];
+    uint32_t light_flags;  // k_lights_* of the job's scene
+    uint32_t pad;
 };
 
 struct dctl {
@@ -101,6 +102,7 @@ __device__ __forceinline__ void job_view(const dscene& S, const dparams& base, c
     Sj.root8 = J.root8;
     Sj.emitters = S.emitters + J.em_first;
     Sj.num_emitters = J.em_count;
+    Sj.light_flags = J.light_flags;
     pj = base;
     pj.spp = J.spp;
     pj.max_depth = J.max_depth;
@@ -326,7 +328,7 @@ struct rbw_batch_state {
     // merged scene of the last submission
     std::vector<uint32_t> scene_ids;
     dev_scene scene;
-    std::vector<uint32_t> roots, roots8, em_first, em_count;
+    std::vector<uint32_t> roots, roots8, em_first, em_count, light_flags;
     uint32_t merged_tris = 0;
     // path pool
     dbuf pf, pu, prng;
@@ -426,6 +428,8 @@ void prepare_scene(rbw_ctx* ctx, const rbw_job* jobs, size_t n, std::vector<uint
     for (uint32_t id : ids) parts.push_back(ctx->pool[id].get());
     host_scene merged;
     merge_scenes(parts, merged, b.roots, b.roots8, b.em_first, b.em_count);
+    b.light_flags.clear();
+    for (const host_scene* h : parts) b.light_flags.push_back(h->light_flags);
     b.scene.upload(merged, ctx->stream);
     b.scene_ids = ids;
     b.merged_tris = merged.nt;
@@ -647,6 +651,7 @@ std::vector<djob> make_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const s
         d.root8 = b.roots8[p];
         d.em_first = b.em_first[p];
         d.em_count = b.em_count[p];
+        d.light_flags = b.light_flags[p];
         d.bw = uint32_t(J.camera.bound_max[0] - J.camera.bound_min[0]);
         d.npix = d.bw * uint32_t(J.camera.bound_max[1] - J.camera.bound_min[1]);
         d.spp = spp_override ? spp_override : J.spp;
@@ -802,6 +807,7 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
     J.root8 = 0;
     J.em_first = 0;
     J.em_count = ctx->main.num_emitters;
+    J.light_flags = ctx->main.light_flags;
     J.bw = uint32_t(cam->bound_max[0] - cam->bound_min[0]);
     J.npix = J.bw * uint32_t(cam->bound_max[1] - cam->bound_min[1]);
     J.spp = pr.spp;

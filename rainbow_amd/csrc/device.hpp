@@ -73,12 +73,19 @@ struct dscene {
     const float4* prim8_c;
     uint32_t root8;
     uint32_t width;          // 2 or 8: the tree traversals use
+    uint32_t light_flags;    // k_lights_* : what the connect stage may shortcut (set per scene)
     // media (volume path integrator only)
     const dmedium* media;
     const uint32_t* ent_media;  // 2 per entity: outside, inside (RBW_NONE = vacuum); null = no media
     const float* grid;          // grid media densities
     uint32_t num_media;
 };
+
+// light_flags: the emitters' triangles are few enough to test an MIS ray against them
+// first; every emitter mesh is planar (a shadow ray to a point on it cannot reach it
+// earlier, so "closest visible hit is the light or none" is "no other visible hit")
+constexpr uint32_t k_lights_few = 1u, k_lights_planar = 2u;
+constexpr uint32_t k_light_first_max_tris = 64;
 
 struct hitrec {
     uint32_t prim;  // global primitive id or RBW_MISS

@@ -123,6 +123,7 @@ struct host_scene {
     uint32_t nt = 0, ns = 0, nv = 0;
     bool has_nrm = false, has_uv = false, has_tri_material = false;
     uint32_t depth = 0, depth8 = 0;
+    uint32_t light_flags = 0;  // k_lights_*
     f3 smin{0, 0, 0}, smax{0, 0, 0};
     size_t bytes() const;
 };
@@ -141,7 +142,7 @@ constexpr uint32_t k_default_width = 2;  // BVH width of a render that does not 
 struct dev_scene {
     dbuf nodes, prim_a, prim_b, prim_c, nodes8, prim8_a, prim8_b, prim8_c, pos, nrm, uv, idx, tri_entity,
         tri_material, spheres, sphere_entity, ents, mats, emitters, media, ent_media, grid;
-    uint32_t nt = 0, ns = 0, num_emitters = 0, num_media = 0;
+    uint32_t nt = 0, ns = 0, num_emitters = 0, num_media = 0, light_flags = 0;
     bool has_uv = false, has_tri_material = false, valid = false;
     void upload(const host_scene& h, hipStream_t s);
     dscene view(uint32_t width = k_default_width) const;

@@ -106,6 +106,41 @@ __device__ __forceinline__ bool test_leaf(const dscene& S, const float4* __restr
     return false;
 }
 
+// the nearest hit among the visible emitters' triangles (mesh lights, world space), with the
+// leaf test's acceptance rule; used only while few emitter triangles exist (k_lights_few)
+__device__ __forceinline__ hitrec nearest_emitter_hit(const dscene& S, f3 o_w, f3 d_w, uint32_t tri_test) {
+    const local_ray lr = make_local_ray(o_w, d_w);
+    wt_setup wt;
+    if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+    hitrec best;
+    best.prim = RBW_MISS;
+    best.t = 0.0f;
+    best.b1 = best.b2 = 0.0f;
+    float len_w = __builtin_inff();
+    for (uint32_t k = 0; k < S.num_emitters; k++) {
+        const dentity E = S.ents[S.emitters[k]];
+        if (E.material == RBW_NONE) continue;
+        for (uint32_t i = 0; i < E.count; i++) {
+            const uint32_t tri = E.first + i;
+            const f3 p0 = load3(S.pos, S.idx[3 * tri]), p1 = load3(S.pos, S.idx[3 * tri + 1]),
+                     p2 = load3(S.pos, S.idx[3 * tri + 2]);
+            float t, b1, b2;
+            const float thr = len_w * lr.L0;
+            const bool hit = tri_test == RBW_TRI_WATERTIGHT
+                                 ? wt_test(p0, p1, p2, lr.o, lr.d, thr, wt.kz, wt.Sx, wt.Sy, wt.Sz, t, b1, b2)
+                                 : mt_test(p0, p1, p2, lr.o, lr.d, thr, t, b1, b2);
+            if (hit) {
+                best.prim = tri;
+                best.t = t;
+                best.b1 = b1;
+                best.b2 = b2;
+                len_w = t * lr.L1;
+            }
+        }
+    }
+    return best;
+}
+
 // closest hit (or closest visible hit) through the binary BVH: near child first,
 // the far child on a private stack
 template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS>
@@ -927,14 +962,17 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const uint32_t lid = P.U(U_SENT, slot);
         const f3 so = P.F3(F_SOX, slot), sd = P.F3(F_SDX, slot);
         const float slen = P.F(F_SLEN, slot);
+        // a planar light cannot be hit before the sample point on it, so the closest visible
+        // hit is the light or none exactly when no other visible surface is in the way
+        const bool any = prm.shadow_mode == RBW_SHADOW_ANY_HIT || (!dbg && (S.light_flags & k_lights_planar));
         hitrec h;
-        if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
+        if (any)
             h = traverse<W, true, true, COUNT, LDS>(S, so, sd, slen, prm.tri_test, lid, cnt);
         else
             h = traverse<W, true, false, COUNT, LDS>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
         if (dbg) dbg[1] = h.prim;
         bool unoccluded = h.prim == RBW_MISS;
-        if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT) {
+        if (!unoccluded && !any) {
             const uint32_t he = h.prim < S.num_tris ? S.tri_entity[h.prim] : S.sphere_entity[h.prim - S.num_tris];
             unoccluded = he == lid;
         }
@@ -947,7 +985,22 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
         f3 eo, ed;
         spawn_ray(mp, mn, mw, eo, ed);
-        const hitrec h = traverse<W, true, false, COUNT, LDS>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+        hitrec h;
+        if (!dbg && (S.light_flags & k_lights_few)) {
+            // the closest visible hit is an emitter exactly when the nearest visible emitter
+            // triangle along the ray has no other visible surface before it: test the few
+            // emitter triangles, then walk the tree only for rays that reach one, as an
+            // occlusion query up to it
+            h = nearest_emitter_hit(S, eo, ed, prm.tri_test);
+            if (h.prim != RBW_MISS) {
+                const uint32_t le = S.tri_entity[h.prim];
+                const local_ray lr = make_local_ray(eo, ed);
+                const hitrec o = traverse<W, true, true, COUNT, LDS>(S, eo, ed, h.t * lr.L1, prm.tri_test, le, cnt);
+                if (o.prim != RBW_MISS) h.prim = RBW_MISS;
+            }
+        } else {
+            h = traverse<W, true, false, COUNT, LDS>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+        }
         if (dbg) dbg[2] = h.prim;
         if (h.prim != RBW_MISS) {
             uint32_t he;

@@ -33,6 +33,41 @@ inline uint32_t word_of(const float4& v) {
     return w;
 }
 
+// What the connect stage may shortcut for this scene's emitters (k_lights_*): few visible
+// emitter triangles (an MIS ray is tested against them before any walk), and planar
+// emitter meshes (every vertex within 1e-6 of the first triangle's plane, relative to the
+// mesh's extent).
+uint32_t light_flags_of(const rbw_scene_desc* d, const host_scene& h) {
+    uint64_t tris = 0;
+    bool planar = true;
+    for (uint32_t e : h.emitters) {
+        const rbw_entity& in = d->entities[e];
+        if (in.material != RBW_NONE) tris += in.count;
+        const uint32_t* I = d->indices + size_t(in.first) * 3;
+        auto P = [&](uint32_t v) { return &d->positions[size_t(v) * 3]; };
+        const float* a = P(I[0]);
+        const float* b = P(I[1]);
+        const float* c = P(I[2]);
+        const double u[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
+        const double v[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
+        double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (!(nl > 0)) {
+            planar = false;
+            continue;
+        }
+        double ext = 0, dev = 0;
+        for (uint32_t k = 0; k < 3 * in.count; k++) {
+            const float* p = P(I[k]);
+            const double w[3] = {double(p[0]) - a[0], double(p[1]) - a[1], double(p[2]) - a[2]};
+            ext = std::max(ext, std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));
+            dev = std::max(dev, std::fabs(w[0] * n[0] + w[1] * n[1] + w[2] * n[2]) / nl);
+        }
+        if (dev > 1e-6 * ext) planar = false;
+    }
+    return (tris <= k_light_first_max_tris ? k_lights_few : 0u) | (planar ? k_lights_planar : 0u);
+}
+
 // The media table (media/medium.hpp:37-75): homogeneous media keep sigma_t = sigma_a + sigma_s
 // (homogeneous_medium.cpp:7-10); a grid keeps 1 / its largest density
 // (heterogeneous_medium.cpp:12-22).  Entity media are (outside, inside) pairs.
@@ -220,6 +255,7 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
     }
     // integrator.cpp:89,121 dereference the sampled emitter: a scene needs one
     require(!h.emitters.empty(), RBW_ERR_INVALID_ARGUMENT, "scene has no emitter");
+    h.light_flags = light_flags_of(d, h);
     if (d->triangle_material)
         for (uint32_t t = 0; t < nt; t++)
             require(d->triangle_material[t] == RBW_NONE || d->triangle_material[t] < d->num_materials,
@@ -311,6 +347,8 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
 void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, std::vector<uint32_t>& roots,
                   std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count) {
     u = host_scene();
+    u.light_flags = k_lights_few | k_lights_planar;
+    for (const host_scene* p : parts) u.light_flags &= p->light_flags;
     uint64_t T = 0, Sp = 0, V = 0, N = 0, L = 0, E = 0, M = 0, N8 = 0, L8 = 0;
     bool any_nrm = false, any_uv = false, any_tm = false;
     for (const host_scene* p : parts) {
@@ -490,6 +528,7 @@ void dev_scene::upload(const host_scene& h, hipStream_t s) {
     nt = h.nt;
     ns = h.ns;
     num_emitters = uint32_t(h.emitters.size());
+    light_flags = h.light_flags;
     valid = true;
 }
 
@@ -520,6 +559,7 @@ dscene dev_scene::view(uint32_t width) const {
     S.prim8_c = prim8_c.as<float4>();
     S.root8 = 0;
     S.width = width;
+    S.light_flags = light_flags;
     S.num_media = num_media;
     S.media = num_media ? media.as<dmedium>() : nullptr;
     S.ent_media = num_media ? ent_media.as<uint32_t>() : nullptr;
