@@ -210,6 +210,8 @@ typedef struct rbw_guiding_opts {
     float scene_min[3], scene_max[3];/* position normalisation; all zero = scene bounds */
     uint32_t light_records;          /* also train on unoccluded light samples (their MIS-weighted
                                         radiance), so the guide learns all incident light */
+    uint32_t guide_depth;            /* vertices 0 .. guide_depth-1 of a path are guided; later ones
+                                        use the plain estimator (light + BSDF samples); 0 = all */
 } rbw_guiding_opts;
 
 typedef struct rbw_render_params {

@@ -541,6 +541,7 @@ struct dguide {
     uint32_t* t_count;
     uint32_t t_cap;
     uint32_t light_records;      // also train on unoccluded light samples
+    uint32_t depth;              // vertices below this bounce are guided (>= max_depth: all)
 };
 
 struct dparams {
@@ -834,8 +835,10 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
         bool cont = true;
         // emission at the first vertex (integrator.cpp:350-351; no specular lobes exist)
         if (bounce == 0 && (E.flags & RBW_ENTITY_EMITTER)) value = value + beta * light_evaluate(E, si.normal, -d_w);
-        if (GUIDED && bounce > 0 && (E.flags & RBW_ENTITY_EMITTER)) {
-            // the guided continuation is the second technique of next-event estimation
+        // a vertex is guided below prm.g.depth; deeper ones shade as the plain estimator
+        const bool guide_here = GUIDED && (uint32_t)bounce < prm.g.depth;
+        if (GUIDED && bounce > 0 && (uint32_t)bounce <= prm.g.depth && (E.flags & RBW_ENTITY_EMITTER)) {
+            // the previous vertex's guided continuation is the second technique of its next-event estimation
             const f3 Le = light_evaluate(E, si.normal, -d_w);
             if (!is_black(Le)) {
                 const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
@@ -853,7 +856,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
         } else {
             const bsdf fn = make_bsdf(S, E.material, prim);
             uint32_t cflags = 0;
-            if (!GUIDED && fn.kind != LOBE_NONE) {
+            if (!guide_here && fn.kind != LOBE_NONE) {
                 // uniform_sample_one_emitter (integrator.cpp:101-161): light sample
                 const uint32_t ne = S.num_emitters;
                 const float u1 = smp.next();
@@ -904,12 +907,12 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
                     need_connect = true;
                 }
             }
-            if (GUIDED && fn.kind != LOBE_NONE) {
+            if (guide_here && fn.kind != LOBE_NONE) {
                 to_guide = true;
                 gsi = si;
                 guide_mat = (S.tri_material && prim < S.num_tris && S.tri_material[prim] != RBW_NONE)
                                 ? S.tri_material[prim] : E.material;
-            } else if (GUIDED) {
+            } else if (guide_here) {
                 cont = false;  // an empty lobe list ends the path (integrator.cpp:380-382)
             } else {
                 // continuation sample (integrator.cpp:380-397)

@@ -171,14 +171,16 @@ def test_gpu_training_learns_a_direction(tracer):
 
 
 @pytest.mark.gpu
-def test_gpu_guided_render_unbiased(tracer):
-    """the guided estimator's expectation equals the plain one (Cornell box, mean radiance)"""
+@pytest.mark.parametrize("guide_depth", [0, 1, 2])
+def test_gpu_guided_render_unbiased(tracer, guide_depth):
+    """the guided estimator's expectation equals the plain one (Cornell box, mean radiance),
+    also when only the first vertices are guided and later ones shade as the plain estimator"""
     from rainbow_amd.tracer import guiding_opts, render_params
     sc, cam = scenes.cornell_box()
     tracer.upload(sc)
     tracer.guide_reset(1)
     plain = tracer.render(cam, render_params(spp=64, max_depth=5, seed=21))["image"]
-    go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 16)
+    go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 16, guide_depth=guide_depth)
     guided = tracer.render(cam, render_params(spp=64, max_depth=5, seed=22, guiding=C_ref(go)))
     gi = guided["image"]
     assert np.isfinite(gi).all()
@@ -195,6 +197,12 @@ def test_gpu_guided_furnace(tracer):
     sc, cam = closed_box(0.5, 1.0)
     tracer.upload(sc)
     tracer.guide_reset(2)
+    for gd in (0, 2):
+        go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 14, guide_depth=gd)
+        r = tracer.render(cam, render_params(spp=256, max_depth=FURNACE_DEPTH, seed=4, guiding=C_ref(go)))
+        mean = float(r["image"].mean())
+        expect = furnace_expected(0.5, 1.0, FURNACE_DEPTH)
+        assert abs(mean - expect) < 0.01 * expect, (gd, mean, expect)
     go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 14)
     r = tracer.render(cam, render_params(spp=256, max_depth=FURNACE_DEPTH, seed=4, guiding=C_ref(go)))
     mean = float(r["image"].mean())
