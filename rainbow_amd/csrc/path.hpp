@@ -404,6 +404,26 @@ __device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, floa
     return traverse2<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
 }
 
+// find_emitter's walk (integrator.cpp:49-82): the closest visible hit of a BSDF-sampled
+// ray, or, while the scene has few emitter triangles (k_lights_few), the nearest emitter
+// triangle hit when no other visible surface lies before it (RBW_MISS otherwise) -- the
+// same answer for the caller, which only uses hits on emitters
+template <int W, bool COUNT, bool LDS>
+__device__ __forceinline__ hitrec find_emitter_walk(const dscene& S, f3 o, f3 d, uint32_t tri_test,
+                                                    trace_counters& cnt) {
+    if (S.light_flags & k_lights_few) {
+        hitrec h = nearest_emitter_hit(S, o, d, tri_test);
+        if (h.prim != RBW_MISS) {
+            const uint32_t le = S.tri_entity[h.prim];
+            const hitrec occ = traverse<W, true, true, COUNT, LDS>(S, o, d, h.t * make_local_ray(o, d).L1, tri_test,
+                                                                   le, cnt);
+            if (occ.prim != RBW_MISS) h.prim = RBW_MISS;
+        }
+        return h;
+    }
+    return traverse<W, true, false, COUNT, LDS>(S, o, d, __builtin_inff(), tri_test, RBW_NONE, cnt);
+}
+
 // ------------------------------------------------------------------ wave helpers
 
 // slot in an append queue for every lane with `pred` set; call with the whole wave converged

@@ -472,7 +472,7 @@ __global__ void k_sampler_draws(dparams prm, uint32_t pixel, uint32_t n1, uint32
 // each with its shadow or emitter-finding ray traced here, weighted by the power
 // heuristic over the sample fractions, and the path ends.
 template <int W>
-__global__ RBW_CONNECT_OCCUPANCY void k_direct(dscene S, dpaths P, dparams prm, const uint32_t* qin,
+__global__ __launch_bounds__(256) void k_direct(dscene S, dpaths P, dparams prm, const uint32_t* qin,
                                                const uint32_t* qin_count, uint32_t* qout, uint32_t* qout_count,
                                                unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,12 +529,13 @@ __global__ RBW_CONNECT_OCCUPANCY void k_direct(dscene S, dpaths P, dparams prm, 
                     spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
                     nshadow++;
                     hitrec h;
-                    if (prm.shadow_mode == RBW_SHADOW_ANY_HIT)
+                    const bool any = prm.shadow_mode == RBW_SHADOW_ANY_HIT || (S.light_flags & k_lights_planar);
+                    if (any)
                         h = traverse<W, true, true, false, true>(S, so, sd, slen, prm.tri_test, lid, cnt);
                     else
                         h = traverse<W, true, false, false, true>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
                     bool unoccluded = h.prim == RBW_MISS;
-                    if (!unoccluded && prm.shadow_mode != RBW_SHADOW_ANY_HIT)
+                    if (!unoccluded && !any)
                         unoccluded = (h.prim < S.num_tris ? S.tri_entity[h.prim] : S.sphere_entity[h.prim - S.num_tris]) == lid;
                     if (unoccluded) {
                         const float w = power_heuristic(ls.pdf * fe, fp * fb) * we;
@@ -551,8 +552,7 @@ __global__ RBW_CONNECT_OCCUPANCY void k_direct(dscene S, dpaths P, dparams prm, 
                     f3 eo, ed;
                     spawn_ray(si.point, si.normal, mwi, eo, ed);
                     nmis++;
-                    const hitrec h = traverse<W, true, false, false, true>(S, eo, ed, __builtin_inff(), prm.tri_test,
-                                                                           RBW_NONE, cnt);
+                    const hitrec h = find_emitter_walk<W, false, true>(S, eo, ed, prm.tri_test, cnt);
                     if (h.prim == RBW_MISS) continue;
                     uint32_t he;
                     const surf hs = hit_surface(S, h.prim, eo, ed, he);

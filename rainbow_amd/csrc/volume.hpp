@@ -77,8 +77,11 @@ __device__ __forceinline__ f3 volume_nee(const dscene& S, const dparams& prm, sa
                 float slen;
                 spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
                 nshadow++;
-                const hitrec h = traverse<W, true, false, false, true>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
-                if (h.prim == RBW_MISS || prim_entity(S, h.prim) == lid) {
+                // (a planar light: the any-hit walk gives the closest-visible answer, as connect_path)
+                const bool any = (S.light_flags & k_lights_planar) != 0;
+                const hitrec h = any ? traverse<W, true, true, false, true>(S, so, sd, slen, prm.tri_test, lid, cnt)
+                                     : traverse<W, true, false, false, true>(S, so, sd, slen, prm.tri_test, RBW_NONE, cnt);
+                if (h.prim == RBW_MISS || (!any && prim_entity(S, h.prim) == lid)) {
                     const f3 beam = media_beam<W>(S, prm, smp, medium, si.point, si.normal, ls.point, cnt, nshadow);
                     const float w = power_heuristic(ls.pdf, fp);
                     if (MEDIUM) {
@@ -103,24 +106,23 @@ __device__ __forceinline__ f3 volume_nee(const dscene& S, const dparams& prm, sa
             ok = fn.sample(si, smp.next2(), mwi, mf, mpdf);
             if (ok) mf = mf * fabsf(dot(mwi, si.sh.z));
         }
-        if (ok && !is_black(mf) && mpdf > 0.0f) {
+        // at a medium vertex with the reference's weighting an emitter hit never contributes
+        // (its light pdf is not a number, see rbw_render_params.medium_mis): no walk needed
+        const bool walk = !MEDIUM || prm.medium_mis;
+        if (ok && !is_black(mf) && mpdf > 0.0f) nmis++;
+        if (walk && ok && !is_black(mf) && mpdf > 0.0f) {
             // find_emitter (integrator.cpp:49-82)
             f3 eo, ed;
             spawn_ray(si.point, si.normal, mwi, eo, ed);
-            nmis++;
-            const hitrec h =
-                traverse<W, true, false, false, true>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+            const hitrec h = find_emitter_walk<W, false, true>(S, eo, ed, prm.tri_test, cnt);
             if (h.prim != RBW_MISS) {
                 uint32_t he;
                 const surf hs = hit_surface(S, h.prim, eo, ed, he);
                 const dentity Eh = S.ents[he];
                 if (Eh.flags & RBW_ENTITY_EMITTER) {
                     const f3 Le = light_evaluate(Eh, hs.normal, -mwi);
-                    float lp = 0.0f;
-                    if (!MEDIUM || prm.medium_mis) {
-                        nlight++;
-                        lp = light_pdf(S, Eh, si.point, si.normal, mwi, MEDIUM) * pick;
-                    }
+                    nlight++;
+                    const float lp = light_pdf(S, Eh, si.point, si.normal, mwi, MEDIUM) * pick;
                     if (!is_black(Le) && lp > 0.0f) {
                         const f3 beam = media_beam<W>(S, prm, smp, medium, si.point, si.normal, hs.point, cnt, nshadow);
                         const float w = power_heuristic(mpdf, lp);
