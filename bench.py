@@ -239,9 +239,9 @@ def main():
                     help="batched-jobs leg (64 jobs of uneven size in one submission)")
     ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
                     help="volume path integrator leg (foggy Cornell box)")
-    ap.add_argument("--guided", default="reduced", choices=["reduced", "full", "off"],
-                    help="guided-vs-plain time-to-relMSE leg: 256^2 with a 16384-spp judge (reduced), "
-                         "512^2 with a 65536-spp judge (full), or skipped")
+    ap.add_argument("--guided", default="full", choices=["reduced", "full", "off"],
+                    help="guided-vs-plain time-to-relMSE leg: 512^2 with a 65536-spp judge (full, the "
+                         "configured workload, ~90 s), 256^2 with a 16384-spp judge (reduced), or skipped")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
