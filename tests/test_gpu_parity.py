@@ -184,3 +184,51 @@ def test_render_icosphere_full_film_tiles(tracer, icosphere):
     print("full film: tile pixels", int(mask.sum()), "mismatch", frac)
     assert mask.sum() > 60000
     assert frac < 1e-3
+
+
+def _lights_scene(kind):
+    """the Cornell box with lights that exercise the connect stage's shortcuts:
+    'two': the planar ceiling quad plus a small emissive box (non-planar: shadow rays take
+    the closest-visible walk, MIS rays still test the emitter triangles first);
+    'many': a ceiling light mesh of 2 x 40 triangles (more than the 64 emitter triangles the
+    MIS shortcut allows: the full find_emitter walk)."""
+    sc = scenes.Scene()
+    light = scenes.add_cornell_box(sc)
+    black = sc.matte((0.0, 0.0, 0.0))
+    if kind == "two":
+        P, I, N = scenes.box_mesh((0.3, -0.2, -0.3), (0.5, 0.0, -0.1))
+        sc.add_mesh(P, I, black, normals=N, radiance=(2.0, 4.0, 6.0))
+    else:
+        xs = np.linspace(-0.6, 0.6, 41)
+        P = np.array([[x, 0.998, z] for x in xs for z in (-0.3, 0.3)], np.float32)
+        I = []
+        for k in range(40):
+            a, b, c, d = 2 * k, 2 * k + 1, 2 * k + 3, 2 * k + 2
+            I += [(a, c, b), (a, d, c)]
+        I = np.array(I, np.uint32)
+        N = np.tile(np.array([0, -1, 0], np.float32), (len(P), 1))
+        sc.add_mesh(P, I, black, normals=N, radiance=(3.0, 3.0, 3.0))
+    cam = scenes.perspective_camera(64, 64, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
+    return sc, cam
+
+
+@pytest.mark.parametrize("kind", ["two", "many"])
+def test_render_light_shortcuts_match_the_oracle(tracer, kind):
+    """Renders without debug hits take the connect shortcuts the scene's lights allow (MIS
+    rays against the emitter triangles first, any-hit shadow rays for planar lights); their
+    radiance matches the oracle's closest-visible semantics, and equals the debug-hit
+    render (full walks) of the same seed."""
+    orc = _oracle()
+    sc, cam = _lights_scene(kind)
+    tracer.upload(sc)
+    c = orc.OracleScene(sc).render(cam, orc.render_params(spp=16, max_depth=5, seed=3),
+                                   sampler=orc.ORC_SAMPLER_PER_SAMPLE)
+    g = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3))
+    bad = pixel_mismatch(g["image"], c["image"])
+    print(kind, "pixels off", bad.sum(), "/", bad.size, g["stats"])
+    assert bad.mean() < 1e-3
+    # (a path that takes another branch at a tie changes the counts by a few rays)
+    for k in ("mis_rays", "shadow_rays"):
+        assert abs(g["stats"][k] - c["stats"][k]) <= 1e-4 * c["stats"][k] + 2, k
+    full = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3), debug=True)["image"]
+    assert pixel_mismatch(g["image"], full).mean() < 1e-3
