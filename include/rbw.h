@@ -379,10 +379,13 @@ typedef struct rbw_job {
 
 typedef struct rbw_batch_opts {
     uint32_t block_size;             /* threads per block of the wavefront kernels: 64, 128, 256 or 512; 0 = 256 */
-    uint32_t pool_size;              /* path slots shared by all jobs; 0 = default 2^21 */
+    uint32_t pool_size;              /* path slots of the shared launch; 0 = default 2^21.  Jobs with at
+                                        least this many samples run alone, one after another, with at
+                                        least a single render's 2^23 slots; the rest share the launch */
     uint32_t bvh_width;              /* 0 (default: 2), 2 or 8; the autotuner tries both */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
-    uint32_t autotune;               /* 1: time candidate block/pool sizes on a 1-spp probe first */
+    uint32_t autotune;               /* 1: time candidate BVH widths and block sizes on a 1-spp probe
+                                        first (the pool size stays as given) */
     uint32_t triangle_test;          /* RBW_TRI_* */
     uint32_t shadow_mode;            /* RBW_SHADOW_* */
 } rbw_batch_opts;

@@ -358,9 +358,11 @@ void batch_free(rbw_ctx* ctx) {
 
 namespace {
 
-// the single render's schedule (measured best on the 1.3M-triangle icosphere at 1024^2)
-constexpr uint32_t k_render_block = 128;
-constexpr uint32_t k_render_pool = 1u << 22;
+// the single render's schedule (measured best on the 1.3M-triangle icosphere at 1024^2, 64 spp:
+// block 64 / 128 / 256 with 2^23 slots: 348 / 332 / 309 Msamples/s; block 64 with 2^22 / 2^24 /
+// 2^25 slots: 332 / 337 / 312)
+constexpr uint32_t k_render_block = 64;
+constexpr uint32_t k_render_pool = 1u << 23;
 
 struct batch_config {
     uint32_t block = 256, pool = 1u << 21, width = k_default_width;
@@ -676,7 +678,7 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
     prepare_scene(ctx, jobs, n, part);
     batch_config cfg = config_of(inout);
     const auto t0 = std::chrono::steady_clock::now();
-    // probe: every job at 1 spp; coordinate search over block size, then pool size
+    // probe: every job at 1 spp; coordinate search over BVH width, then block size
     auto probe = [&](uint32_t block, uint32_t pool, uint32_t width) {
         batch_config c = cfg;
         c.block = block;
@@ -687,8 +689,8 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
         run_batch(ctx, ctx->batch->scene.view(c.width), hj, c, rr, false);
         return rr.ms;
     };
-    // coordinate search: BVH width, then block size, then pool size
-    uint32_t best_block = 256, best_pool = 1u << 21, best_width = k_default_width;
+    // coordinate search: BVH width, then block size
+    uint32_t best_block = 256, best_pool = cfg.pool, best_width = k_default_width;
     double best = 1e300;
     for (uint32_t width : {2u, 8u}) {
         const double t = probe(best_block, best_pool, width);
@@ -704,13 +706,10 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
             best_block = block;
         }
     }
-    for (uint32_t pool : {1u << 20, 1u << 22}) {
-        const double t = probe(best_block, pool, best_width);
-        if (t < best) {
-            best = t;
-            best_pool = pool;
-        }
-    }
+    // (the pool is not searched: at 1 spp every job is smaller than the pool and shares one
+    // launch, while in the real run the jobs that fill the pool run alone; the shared
+    // launch's pool stays 2^21, measured best for the 64-job workload: 384 Msamples/s
+    // against 376 at 2^22 and 353 at 2^23)
     inout->block_size = best_block;
     inout->pool_size = best_pool;
     inout->bvh_width = best_width;
@@ -734,8 +733,58 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     prepare_scene(ctx, jobs, n, part);
     rbw_batch_state& b = *ctx->batch;
     std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 0);
+    // Jobs with at least as many samples as the pool has slots fill it on their own: they run
+    // alone, one after another in submission order, and the other jobs then share one
+    // persistent launch.  (Measured on the 64-job workload: letting large jobs share the
+    // pool only mixes their scenes in the caches -- the shared launch ran 14 % slower than
+    // the same jobs rendered one by one.)  Images do not depend on the grouping: a sample's
+    // stream and film sums depend only on its job, pixel and index.
+    std::vector<std::vector<size_t>> groups;
+    {
+        std::vector<size_t> small;
+        for (size_t i = 0; i < n; i++) {
+            if (n > 1 && hj[i].count >= cfg.pool) groups.push_back({i});
+            else small.push_back(i);
+        }
+        if (!small.empty()) groups.push_back(small);
+    }
     run_result rr;
-    run_batch(ctx, b.scene.view(cfg.width), hj, cfg, rr, job_stats != nullptr);
+    std::vector<double> job_active_ms(n, 0.0);
+    rr.first_iter.assign(n, 0);
+    rr.last_iter.assign(n, 0);
+    const dscene Sv = b.scene.view(cfg.width);
+    for (const auto& g : groups) {
+        std::vector<djob> sub;
+        unsigned long long base = 0;
+        for (size_t i : g) {
+            djob d = hj[i];
+            d.base = base;
+            base += d.count;
+            sub.push_back(d);
+        }
+        // a job alone gets at least the single render's slot count (capped by its samples)
+        batch_config gcfg = cfg;
+        if (g.size() == 1 && groups.size() > 1) {
+            const unsigned long long want = (sub[0].count + 1023) & ~1023ull;
+            gcfg.pool = std::max<uint32_t>(cfg.pool, (uint32_t)std::min<unsigned long long>(k_render_pool, want));
+        }
+        run_result r;
+        run_batch(ctx, Sv, sub, gcfg, r, job_stats != nullptr);
+        if (job_stats) {
+            for (size_t k = 0; k < g.size(); k++) {
+                const size_t i = g[k];
+                rr.first_iter[i] = r.first_iter[k] + rr.iterations;
+                rr.last_iter[i] = r.last_iter[k] + rr.iterations;
+                // launch l (1-based) runs iterations 2l-1 and 2l
+                const uint32_t l0 = (r.first_iter[k] + 1) / 2, l1 = (r.last_iter[k] + 1) / 2;
+                if (l0 >= 1 && l1 >= l0 && l1 < r.launch_ms.size()) job_active_ms[i] = r.launch_ms[l1] - r.launch_ms[l0 - 1];
+            }
+        }
+        rr.ms += r.ms;
+        rr.iterations += r.iterations;
+        rr.launches += r.launches;
+        for (int k = 0; k < 4; k++) rr.counters[k] += r.counters[k];
+    }
     // films -> caller
     hipStream_t s = ctx->stream;
     size_t max_px = 0;
@@ -759,12 +808,8 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
             js.samples = hj[i].count;
             js.first_iteration = rr.first_iter[i];
             js.last_iteration = rr.last_iter[i];
-            // launch l (1-based) runs iterations 2l-1 and 2l
-            const uint32_t l0 = (rr.first_iter[i] + 1) / 2, l1 = (rr.last_iter[i] + 1) / 2;
-            if (l0 >= 1 && l1 >= l0 && l1 < rr.launch_ms.size()) {
-                js.active_ms = rr.launch_ms[l1] - rr.launch_ms[l0 - 1];
-                js.msamples_per_s = js.active_ms > 0 ? double(js.samples) / js.active_ms * 1e-3 : 0.0;
-            }
+            js.active_ms = job_active_ms[i];
+            js.msamples_per_s = js.active_ms > 0 ? double(js.samples) / js.active_ms * 1e-3 : 0.0;
         }
     }
     if (stats) {
