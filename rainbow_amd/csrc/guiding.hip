@@ -92,7 +92,15 @@ __global__ __launch_bounds__(FWD_WAVES * 64) void k_forward(const _Float16* __re
                                                             const float* __restrict__ w32,
                                                             const float* __restrict__ inputs, uint32_t n,
                                                             uint32_t n_pad, float* __restrict__ out,
-                                                            _Float16* __restrict__ acts, sample_map map) {
+                                                            _Float16* __restrict__ acts, sample_map map,
+                                                            const uint32_t* __restrict__ n_dev = nullptr) {
+    // (n_dev: the row count lives in device memory, written by an earlier kernel of the stream)
+    if (n_dev) {
+        n = *n_dev;
+        n_pad = (n + 15) & ~15u;
+        map.n_src = n;
+        if (n == 0) return;
+    }
     extern __shared__ __align__(16) _Float16 lds[];
     _Float16* wl = lds;                                  // LDS_W_HALFS
     _Float16* tiles = lds + LDS_W_HALFS;                 // FWD_WAVES x 16 x TILE_PAD
@@ -592,6 +600,16 @@ void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t 
     hipLaunchKernelGGL(k_forward<false>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
                        M.w32, inputs, n, n_pad,
                        out, nullptr, id);
+    GCHECK(hipGetLastError());
+}
+
+void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s) {
+    if (n_max == 0) return;
+    sample_map id;
+    id.n_src = n_max;
+    allow_forward_lds();
+    hipLaunchKernelGGL(k_forward<false>, dim3(forward_blocks((n_max + 15) & ~15u)), dim3(FWD_WAVES * 64),
+                       FWD_LDS_BYTES, s, M.w16, M.w32, inputs, n_max, (n_max + 15) & ~15u, out, nullptr, id, n_dev);
     GCHECK(hipGetLastError());
 }
 

@@ -73,6 +73,8 @@ sample_map make_sample_map(uint32_t n_src, uint64_t seed);
 
 // inputs: n x 9 floats in [0,1] (device); out: n x N_OUT_PAD raw outputs (device)
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s);
+// the same with the row count in device memory (at most n_max rows; no host synchronisation)
+void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s);
 // one training step on n samples: inputs n x 9, dirs n x 3, weights n (radiance / pdf), all device.
 // Leaves the gradient in M.grad (scaled to the mean loss) and, when apply, takes an Adam step.
 void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,

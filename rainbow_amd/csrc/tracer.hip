@@ -1028,19 +1028,16 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             }
             if (timed) ctx->timer.end(s);
             if (guided) {
-                HIP_CHECK(hipMemcpyAsync(ctx->host_count + 1, counts + 3, 4, hipMemcpyDeviceToHost, s));
-                HIP_CHECK(hipStreamSynchronize(s));
-                const uint32_t ng = ctx->host_count[1];
-                if (ng > 0) {
-                    if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_INFER, s);
-                    guide::forward(ctx->gm, dp.g.query, ng, dp.g.mix, s);
-                    if (timed) ctx->timer.end(s);
-                    if (timed) ctx->timer.begin(RBW_KERNEL_SHADE_GUIDED, s);
-                    hipLaunchKernelGGL(k_shade_guided, dim3(blocks(ng, BS)), dim3(BS), 0, s, S, P, dp, qout,
-                                       counts + cout, cqp, counts + 2);
-                    if (timed) ctx->timer.end(s);
-                    HIP_CHECK(hipGetLastError());
-                }
+                // the guided rows (at most `active`) are counted on the device: the MLP and the
+                // guided vertex kernel read the count there, no host round trip
+                if (timed) ctx->timer.begin(RBW_KERNEL_GUIDE_INFER, s);
+                guide::forward_dev(ctx->gm, dp.g.query, counts + 3, active, dp.g.mix, s);
+                if (timed) ctx->timer.end(s);
+                if (timed) ctx->timer.begin(RBW_KERNEL_SHADE_GUIDED, s);
+                hipLaunchKernelGGL(k_shade_guided, dim3(blocks(active, BS)), dim3(BS), 0, s, S, P, dp, qout,
+                                   counts + cout, cqp, counts + 2);
+                if (timed) ctx->timer.end(s);
+                HIP_CHECK(hipGetLastError());
             }
             if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
             {
