@@ -156,6 +156,57 @@ def volume_leg(cpu=True, steps=2):
     return out
 
 
+def room_variants_leg(n=16, res=256, judge_spp=16384, threshold=1e-3):
+    """The guided-plus-batched workload's scenes: 16 narrow-aperture room variants (gap
+    0.1-0.5, 1e5-1.3e6 triangles, GGX alpha 0.02-0.6), each with its own guiding network
+    trained online from scratch; time to relMSE <= threshold, guided vs plain, per variant.
+    The jobs run one after another (one network per job; the batched multi-network kernels
+    and the shared launch are not built)."""
+    import numpy as np
+    from rainbow_amd import scenes
+    from rainbow_amd.tracer import Tracer, guiding_opts, render_params
+    rooms = scenes.room_variants(n=n, res=res)
+    tr = Tracer(0)
+    depth, chunk, max_spp, train_calls = 8, 8, 8192, 32
+    per = []
+    for k, room in enumerate(rooms):
+        sc, cam = room["scene"], room["camera"]
+        tr.upload(sc)
+        ref = tr.render(cam, render_params(spp=judge_spp, max_depth=depth, seed=1 << 20), stats=False)["image"]
+
+        def run(guided):
+            acc = np.zeros_like(ref)
+            elapsed, curve, calls = 0.0, [], 0
+            if guided:
+                tr.guide_reset(7 + k)
+            while calls * chunk < max_spp:
+                kw = {}
+                if guided:
+                    kw["guiding"] = C.pointer(guiding_opts(train=calls < train_calls, steps_per_pass=4, batch=1 << 18))
+                t = time.perf_counter()
+                img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=(2 if guided else 1) * 100000 + calls,
+                                                   **kw), stats=False)["image"]
+                elapsed += time.perf_counter() - t
+                acc += img
+                calls += 1
+                e = scenes.rel_mse(acc / calls, ref)
+                curve.append((elapsed * 1e3, e))
+                if e <= threshold * 0.8:
+                    break
+            return curve
+        tp, tg = _time_to(run(False), threshold), _time_to(run(True), threshold)
+        per.append({"gap": round(room["gap"], 3), "triangles": room["triangles"],
+                    "plain_ms": None if tp is None else round(tp, 1), "guided_ms": None if tg is None else round(tg, 1),
+                    "speedup": None if (tp is None or tg is None) else round(tp / tg, 3)})
+        print(f"# room variant {k}: gap {room['gap']:.2f}, {room['triangles']} tris: {per[-1]}", file=sys.stderr,
+              flush=True)
+    sp = [p["speedup"] for p in per if p["speedup"]]
+    return {"workload": f"{n} narrow-aperture room variants (gap 0.1-0.5, 1e5-1.3e6 tris, GGX alpha 0.02-0.6), "
+                        f"{res}x{res}, depth 8, judge {judge_spp} spp, one network per job trained online",
+            "per_variant": per, "median_speedup": None if not sp else sorted(sp)[len(sp) // 2],
+            "min_speedup": None if not sp else min(sp), "max_speedup": None if not sp else max(sp)}
+
+
 def batch_leg(n_jobs=64, autotune=True):
     """The batched-jobs workload: n_jobs seeded jobs of uneven size (grid-sphere meshes of
     1e4-2e6 triangles log-uniform in the Cornell box, 128^2-1024^2, 4-256 spp, depth 3-12)
@@ -237,6 +288,8 @@ def main():
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
     ap.add_argument("--batch", action=argparse.BooleanOptionalAction, default=True,
                     help="batched-jobs leg (64 jobs of uneven size in one submission)")
+    ap.add_argument("--rooms", action="store_true",
+                    help="guided vs plain on 16 narrow-aperture room variants, one network each (~4 min)")
     ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
                     help="volume path integrator leg (foggy Cornell box)")
     ap.add_argument("--guided", default="full", choices=["reduced", "full", "off"],
@@ -336,6 +389,8 @@ def main():
         out["guided"] = guided_leg(args.guided == "full")
     if args.volume:
         out["volume"] = volume_leg(cpu=not args.no_cpu)
+    if args.rooms:
+        out["room_variants"] = room_variants_leg()
     if not args.no_cpu:
         cb = cpu_baseline(sc, cam, spp, depth)
         out["cpu_baseline"] = cb

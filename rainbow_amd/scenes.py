@@ -474,12 +474,12 @@ def fbm_displaced_icosphere(level=8, seed=1, center=(0.05, -0.35, -0.3), radius=
     return P32, F, N.astype(np.float32)
 
 
-def per_triangle_materials(sc: Scene, ntri, seed):
-    """50% Lambert albedo U[0.2,0.9]^3, else GGX conductor alpha U[0.05,0.5]; one material per triangle."""
+def per_triangle_materials(sc: Scene, ntri, seed, alpha_range=(0.05, 0.5)):
+    """50% Lambert albedo U[0.2,0.9]^3, else GGX conductor alpha U[alpha_range]; one material per triangle."""
     rng = np.random.default_rng(seed)
     coin = rng.random(ntri) < 0.5
     albedo = rng.uniform(0.2, 0.9, (ntri, 3))
-    alpha = rng.uniform(0.05, 0.5, ntri)
+    alpha = rng.uniform(alpha_range[0], alpha_range[1], ntri)
     m = np.zeros(ntri, MATERIAL_DTYPE)
     m["type"] = np.where(coin, abi.RBW_MATERIAL_MATTE, abi.RBW_MATERIAL_METAL)
     m["diffuse"] = np.where(coin[:, None], albedo, 0.0)
@@ -610,6 +610,24 @@ def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0,
         sc.add_mesh(P, F, tm[0], normals=Nn, tri_material=tm)
     cam = perspective_camera(width, height, 65.0, eye=(0.6, 0.3, 0.95), target=(-0.3, -0.3, -0.6))
     return sc, cam
+
+
+def room_variants(n=16, seed=4, res=256):
+    """Variants of the narrow-aperture room: gap U[0.1, 0.5], a seeded fBm-displaced grid
+    sphere of 1e5-1.3e6 triangles (log-uniform) with per-triangle Lambert / GGX materials
+    (alpha U[0.02, 0.6]), the light aimed 45 degrees up through the gap; res x res.
+    Returns a list of dicts {scene, camera, gap, triangles, seed}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        gap = float(rng.uniform(0.1, 0.5))
+        t = int(round(math.exp(rng.uniform(math.log(1e5), math.log(1.3e6)))))
+        sc, cam = narrow_aperture_room(level=8, width=res, height=res, gap=gap, light_tilt=45.0, mesh=False)
+        P, F, N = fbm_displaced_grid_sphere(t, seed=2000 + i, center=(-0.2, -0.45, -0.3), radius=0.45)
+        tm = per_triangle_materials(sc, len(F), 2000 + i, alpha_range=(0.02, 0.6))
+        sc.add_mesh(P, F, tm[0], normals=N, tri_material=tm)
+        out.append({"scene": sc, "camera": cam, "gap": gap, "triangles": len(F), "seed": 2000 + i})
+    return out
 
 
 def rel_mse(img, ref):
