@@ -156,6 +156,46 @@ def volume_leg(cpu=True, steps=2):
     return out
 
 
+def integrator_variants_leg(sc, cam, spp, depth, cpu=True, steps=2):
+    """The headline workload rendered by the other estimators the build carries: the direct
+    integrator (one light and one BSDF sample, beside the oracle's rate) and the path
+    integrator with the stratified sampler (8 x 8 strata, 8 1D and 12 2D dimensions)."""
+    from rainbow_amd import abi
+    from rainbow_amd.tracer import render_params
+
+    def rate(tr, params):
+        tr.render(cam, params, stats=False)
+        samples = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st_ = abi.rbw_stats()
+            rc = tr.lib.rbw_render(tr.h, C.byref(cam), C.byref(params), None, C.byref(st_))
+            if rc != 0:
+                raise RuntimeError(tr.lib.rbw_last_error().decode())
+            samples += st_.as_dict()["samples"]
+        return samples / (time.perf_counter() - t0) / 1e6
+
+    from rainbow_amd.tracer import Tracer
+    tr = Tracer(0)
+    tr.upload(sc)
+    direct = render_params(spp=spp, max_depth=depth, seed=0, integrator=abi.RBW_INTEGRATOR_DIRECT,
+                           emitter_samples=1, bsdf_samples=1)
+    strat = render_params(spp=spp, max_depth=depth, seed=0, sampler=abi.RBW_SAMPLER_STRATIFIED, strata_x=8,
+                          strata_y=spp // 8, dims_1d=8, dims_2d=12)
+    out = {"direct": {"value": round(rate(tr, direct), 2), "unit": "Msamples/s",
+                      "workload": "the headline scene, direct integrator, 1 light + 1 BSDF sample"},
+           "stratified": {"value": round(rate(tr, strat), 2), "unit": "Msamples/s",
+                          "workload": f"the headline workload, stratified sampler 8x{spp // 8}, 8 1D + 12 2D dims"}}
+    print(f"# direct {out['direct']['value']} Msamples/s, stratified {out['stratified']['value']} Msamples/s",
+          file=sys.stderr, flush=True)
+    if cpu:
+        cb = cpu_baseline(sc, cam, spp, depth, target_s=5.0, integrator=abi.RBW_INTEGRATOR_DIRECT,
+                          emitter_samples=1, bsdf_samples=1)
+        out["direct"]["cpu_baseline"] = cb
+        out["direct"]["gpu_over_cpu"] = round(out["direct"]["value"] / cb["value"], 1)
+    return out
+
+
 def room_variants_leg(n=16, res=256, judge_spp=16384, threshold=1e-3):
     """The guided-plus-batched workload's scenes: 16 narrow-aperture room variants (gap
     0.1-0.5, 1e5-1.3e6 triangles, GGX alpha 0.02-0.6), each with its own guiding network
@@ -288,6 +328,8 @@ def main():
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
     ap.add_argument("--batch", action=argparse.BooleanOptionalAction, default=True,
                     help="batched-jobs leg (64 jobs of uneven size in one submission)")
+    ap.add_argument("--variants", action=argparse.BooleanOptionalAction, default=True,
+                    help="direct integrator and stratified sampler on the headline workload")
     ap.add_argument("--rooms", action="store_true",
                     help="guided vs plain on 16 narrow-aperture room variants, one network each (~4 min)")
     ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
@@ -389,6 +431,8 @@ def main():
         out["guided"] = guided_leg(args.guided == "full")
     if args.volume:
         out["volume"] = volume_leg(cpu=not args.no_cpu)
+    if args.variants:
+        out["integrator_variants"] = integrator_variants_leg(sc, cam, spp, depth, cpu=not args.no_cpu)
     if args.rooms:
         out["room_variants"] = room_variants_leg()
     if not args.no_cpu:
