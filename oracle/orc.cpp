@@ -1273,7 +1273,7 @@ inline vec3 hg_sample(const vec3& wo, real g, const vec2& u, real& pdf) {
     }
     const real sin_t = std::sqrt(fmax_g(static_cast<real>(0), 1 - cos_t * cos_t));
     const real phi = k_two_pi * u.y;
-    const vec3 wl(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+    const vec3 wl(sin_t * rcos(phi), sin_t * rsin(phi), cos_t);
     coordinate_system sys = coordinate_system_from_normal(wo);
     sys.z = -wo;
     pdf = hg_phase(-cos_t, g);
@@ -1348,10 +1348,10 @@ inline medium_sample_result medium_sample(const orc_scene& s, uint32_t mid, samp
     if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
         const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
         const size_t channel = std::min(static_cast<size_t>(smp.next() * 3), static_cast<size_t>(2));  // sample_function.cpp:97-102
-        const real t = fmin_g(-std::log(1 - smp.next()) / st[static_cast<int>(channel)], r.length);
+        const real t = fmin_g(-rlog(1 - smp.next()) / st[static_cast<int>(channel)], r.length);
         const bool sampled = t < r.length;
         const real tc = fmin_g(t, std::numeric_limits<real>::max());
-        const spectrum value(std::exp(-st.x * tc), std::exp(-st.y * tc), std::exp(-st.z * tc));
+        const spectrum value(rexp(-st.x * tc), rexp(-st.y * tc), rexp(-st.z * tc));
         const spectrum density = sampled ? st * value : value;
         real pdf = 0;
         pdf = pdf + density.x;
@@ -1375,7 +1375,7 @@ inline medium_sample_result medium_sample(const orc_scene& s, uint32_t mid, samp
     const real inv = s.inv_max_density[mid];
     real t = t0;
     while (true) {
-        t = t - std::log(1 - smp.next()) * inv;
+        t = t - rlog(1 - smp.next()) * inv;
         if (t >= t1) break;
         const real sigma_t = grid_sample(s, m, lo + ld * t);
         if (sigma_t * inv > smp.next()) {
@@ -1396,7 +1396,7 @@ inline spectrum medium_evaluate(const orc_scene& s, uint32_t mid, sample_stream&
     if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
         const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
         const real tc = fmin_g(r.length, std::numeric_limits<real>::max());
-        return spectrum(std::exp(-st.x * tc), std::exp(-st.y * tc), std::exp(-st.z * tc));
+        return spectrum(rexp(-st.x * tc), rexp(-st.y * tc), rexp(-st.z * tc));
     }
     vec3 lo, ld;
     grid_local_ray(m, r, lo, ld);
@@ -1405,7 +1405,7 @@ inline spectrum medium_evaluate(const orc_scene& s, uint32_t mid, sample_stream&
     const real inv = s.inv_max_density[mid];
     real value = 1, t = t0;
     while (true) {
-        t = t - std::log(1 - smp.next()) * inv;
+        t = t - rlog(1 - smp.next()) * inv;
         if (t >= t1) break;
         const real sigma_t = grid_sample(s, m, lo + ld * t);
         value = value * (1 - fmax_g(static_cast<real>(0), sigma_t * inv));

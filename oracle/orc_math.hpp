@@ -39,6 +39,8 @@ inline real rsin(real x) { return static_cast<real>(std::sin(static_cast<double>
 inline real rcos(real x) { return static_cast<real>(std::cos(static_cast<double>(x))); }
 inline real racos(real x) { return static_cast<real>(std::acos(static_cast<double>(x))); }
 inline real ratan2(real y, real x) { return static_cast<real>(std::atan2(static_cast<double>(y), static_cast<double>(x))); }
+inline real rlog(real x) { return static_cast<real>(std::log(static_cast<double>(x))); }
+inline real rexp(real x) { return static_cast<real>(std::exp(static_cast<double>(x))); }
 
 // glm::min/max on scalars: (y < x) ? y : x  and  (x < y) ? y : x
 inline real fmin_g(real x, real y) { return (y < x) ? y : x; }

@@ -86,6 +86,8 @@ RBW_HD float rsin(float x) { return (float)sin((double)x); }
 RBW_HD float rcos(float x) { return (float)cos((double)x); }
 RBW_HD float racos(float x) { return (float)acos((double)x); }
 RBW_HD float ratan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+RBW_HD float rlog(float x) { return (float)log((double)x); }
+RBW_HD float rexp(float x) { return (float)exp((double)x); }
 
 // std::nextafter for float (IEEE 754 nextUp/nextDown on the bit pattern)
 RBW_HD float next_after(float from, float to) {

@@ -3,7 +3,9 @@
 // and of the density grid (media/heterogeneous_medium.cpp:24-120: delta tracking
 // to sample, ratio tracking to evaluate, trilinear lookup as
 // textures/detail/image_texture.hpp:119-162), and the Henyey-Greenstein phase
-// function (shared/phases/henyey_greenstein.cpp:11-48, utilities.cpp:46-51).
+// function (shared/phases/henyey_greenstein.cpp:11-48, utilities.cpp:46-51).  log, exp,
+// sin and cos are the correctly rounded float values (via the double routines), as the
+// oracle computes them.
 #pragma once
 
 #include "device.hpp"
@@ -34,7 +36,7 @@ RBW_HD f3 hg_sample(f3 wo, float g, f2 u, float& pdf) {
     frame fr = frame_from_normal(wo);
     fr.z = -wo;
     pdf = hg_phase(-ct, g);
-    return local_to_world(fr, mk(st * cosf(phi), st * sinf(phi), ct));
+    return local_to_world(fr, mk(st * rcos(phi), st * rsin(phi), ct));
 }
 
 // the medium's local ray of a grid: direction not re-normalised (heterogeneous_medium.cpp:28-33)
@@ -127,10 +129,10 @@ __device__ __forceinline__ medium_event medium_sample(const dscene& S, const dme
         const float st[3] = {M.sigma_t[0], M.sigma_t[1], M.sigma_t[2]};
         const float u1 = smp.next();
         const float sc = ch == 0 ? st[0] : (ch == 1 ? st[1] : st[2]);
-        const float t = gmin(-logf(1.0f - u1) / sc, len);
+        const float t = gmin(-rlog(1.0f - u1) / sc, len);
         const bool sampled = t < len;
         const float tc = gmin(t, k_flt_max);
-        const f3 value = mk(expf(-st[0] * tc), expf(-st[1] * tc), expf(-st[2] * tc));
+        const f3 value = mk(rexp(-st[0] * tc), rexp(-st[1] * tc), rexp(-st[2] * tc));
         const f3 dens = sampled ? mk(st[0], st[1], st[2]) * value : value;
         float pdf = 0.0f;
         pdf = pdf + dens.x;
@@ -148,7 +150,7 @@ __device__ __forceinline__ medium_event medium_sample(const dscene& S, const dme
     if (!unit_box_interval(lo, ld, len, t0, t1)) return ev;
     float t = t0;
     while (true) {
-        t = t - logf(1.0f - smp.next()) * M.inv_max_density;
+        t = t - rlog(1.0f - smp.next()) * M.inv_max_density;
         if (t >= t1) break;
         const float sigma_t = grid_density(S, M, lo + ld * t);
         if (sigma_t * M.inv_max_density > smp.next()) {
@@ -167,14 +169,14 @@ template <class SMP>
 __device__ __forceinline__ f3 medium_transmittance(const dscene& S, const dmedium& M, SMP& smp, f3 o, f3 d, float len) {
     if (M.type == RBW_MEDIUM_HOMOGENEOUS) {
         const float tc = gmin(len, k_flt_max);
-        return mk(expf(-M.sigma_t[0] * tc), expf(-M.sigma_t[1] * tc), expf(-M.sigma_t[2] * tc));
+        return mk(rexp(-M.sigma_t[0] * tc), rexp(-M.sigma_t[1] * tc), rexp(-M.sigma_t[2] * tc));
     }
     const f3 lo = w2l_point(M, o), ld = w2l_vector(M, d);
     float t0, t1;
     if (!unit_box_interval(lo, ld, len, t0, t1)) return splat(1.0f);
     float value = 1.0f, t = t0;
     while (true) {
-        t = t - logf(1.0f - smp.next()) * M.inv_max_density;
+        t = t - rlog(1.0f - smp.next()) * M.inv_max_density;
         if (t >= t1) break;
         const float sigma_t = grid_density(S, M, lo + ld * t);
         value = value * (1.0f - gmax(0.0f, sigma_t * M.inv_max_density));

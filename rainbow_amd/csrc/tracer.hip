@@ -604,7 +604,8 @@ __global__ __launch_bounds__(256) void k_volume(dscene S, dpaths P, dparams prm,
         bool cont = true, at_surface = true;
         f3 no = o_w, nd = d_w;
         if (medium != RBW_NONE) {
-            const float len = prim == RBW_MISS ? __builtin_inff() : P.F(F_HT, slot);
+            // the ray's world length after the hit: the local hit parameter mapped back (entity.cpp:384)
+            const float len = prim == RBW_MISS ? __builtin_inff() : P.F(F_HT, slot) * make_local_ray(o_w, d_w).L1;
             const medium_event ev = medium_sample(S, S.media[medium], smp, o_w, d_w, len);
             beta = beta * ev.value;
             if (is_black(beta)) {

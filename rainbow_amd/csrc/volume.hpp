@@ -28,7 +28,8 @@ __device__ __forceinline__ f3 media_beam(const dscene& S, const dparams& prm, sa
         nrays++;
         const hitrec h = traverse<W, false, false, false, true>(S, o, d, len, prm.tri_test, RBW_NONE, cnt);
         if (h.prim == RBW_MISS) break;
-        if (medium != RBW_NONE) L = L * medium_transmittance(S, S.media[medium], smp, o, d, h.t);
+        // (the segment's world length: the local hit parameter mapped back, entity.cpp:384)
+        if (medium != RBW_NONE) L = L * medium_transmittance(S, S.media[medium], smp, o, d, h.t * make_local_ray(o, d).L1);
         uint32_t eid;
         const surf hs = hit_surface(S, h.prim, o, d, eid);
         const dentity E = S.ents[eid];

@@ -258,10 +258,11 @@ def test_gpu_volume_matches_oracle(tracer, grid, medium_mis):
                                       sampler=orc_py.ORC_SAMPLER_PER_SAMPLE)
     g, c = r["image"], o["image"]
     bad = (np.abs(g - c) > 1e-4 * np.maximum(np.abs(g), np.abs(c)) + 1e-6).any(-1)
-    # logf/expf may differ from the host's by an ulp, which can flip a distance-sampling
-    # decision and send a path elsewhere: a few pixels may differ, the rest are exact
-    assert bad.mean() < 0.02, bad.sum()
-    assert abs(float(g.mean()) - float(c.mean())) < 0.02 * float(c.mean())
+    print(f"grid {grid} medium_mis {medium_mis}: pixels off {bad.sum()} / {bad.size}")
+    # both sides use correctly rounded log/exp/sin/cos and the reference's world ray lengths,
+    # so the paths agree draw for draw; the bar is the path integrator's (ties aside)
+    assert bad.mean() < 1e-3, bad.sum()
+    assert abs(float(g.mean()) - float(c.mean())) < 1e-4 * float(c.mean())
 
 
 @pytest.mark.gpu
