@@ -1348,10 +1348,10 @@ inline medium_sample_result medium_sample(const orc_scene& s, uint32_t mid, samp
     if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
         const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
         const size_t channel = std::min(static_cast<size_t>(smp.next() * 3), static_cast<size_t>(2));  // sample_function.cpp:97-102
-        const real t = fmin_g(-rlog(1 - smp.next()) / st[static_cast<int>(channel)], r.length);
+        const real t = fmin_g(-flog(1 - smp.next()) / st[static_cast<int>(channel)], r.length);
         const bool sampled = t < r.length;
         const real tc = fmin_g(t, std::numeric_limits<real>::max());
-        const spectrum value(rexp(-st.x * tc), rexp(-st.y * tc), rexp(-st.z * tc));
+        const spectrum value(fexp(-st.x * tc), fexp(-st.y * tc), fexp(-st.z * tc));
         const spectrum density = sampled ? st * value : value;
         real pdf = 0;
         pdf = pdf + density.x;
@@ -1375,7 +1375,7 @@ inline medium_sample_result medium_sample(const orc_scene& s, uint32_t mid, samp
     const real inv = s.inv_max_density[mid];
     real t = t0;
     while (true) {
-        t = t - rlog(1 - smp.next()) * inv;
+        t = t - flog(1 - smp.next()) * inv;
         if (t >= t1) break;
         const real sigma_t = grid_sample(s, m, lo + ld * t);
         if (sigma_t * inv > smp.next()) {
@@ -1396,7 +1396,7 @@ inline spectrum medium_evaluate(const orc_scene& s, uint32_t mid, sample_stream&
     if (m.type == RBW_MEDIUM_HOMOGENEOUS) {
         const spectrum st = spectrum(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]) + spectrum(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
         const real tc = fmin_g(r.length, std::numeric_limits<real>::max());
-        return spectrum(rexp(-st.x * tc), rexp(-st.y * tc), rexp(-st.z * tc));
+        return spectrum(fexp(-st.x * tc), fexp(-st.y * tc), fexp(-st.z * tc));
     }
     vec3 lo, ld;
     grid_local_ray(m, r, lo, ld);
@@ -1405,7 +1405,7 @@ inline spectrum medium_evaluate(const orc_scene& s, uint32_t mid, sample_stream&
     const real inv = s.inv_max_density[mid];
     real value = 1, t = t0;
     while (true) {
-        t = t - rlog(1 - smp.next()) * inv;
+        t = t - flog(1 - smp.next()) * inv;
         if (t >= t1) break;
         const real sigma_t = grid_sample(s, m, lo + ld * t);
         value = value * (1 - fmax_g(static_cast<real>(0), sigma_t * inv));
@@ -2117,6 +2117,8 @@ int orc_material_sample(const rbw_material* m, const float* wo, float u0, float 
 }
 
 float orc_hg_phase(float cos_theta, float g) { return hg_phase(cos_theta, g); }
+float orc_flog(float x) { return flog(x); }
+float orc_fexp(float x) { return fexp(x); }
 
 float orc_hg_sample(const float* wo, float g, float u0, float u1, float* wi) {
     real pdf = 0;

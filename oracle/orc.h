@@ -97,6 +97,9 @@ void orc_material_eval(const rbw_material* m, const float* wo, const float* wi, 
 int orc_material_sample(const rbw_material* m, const float* wo, float u0, float u1, float* wi, float* f, float* pdf);
 /* Henyey-Greenstein (utilities.cpp:46-51, henyey_greenstein.cpp:17-48): phase value; sample -> pdf */
 float orc_hg_phase(float cos_theta, float g);
+/* the float log and exp the media code uses on both sides (orc_math.hpp flog / fexp) */
+float orc_flog(float x);
+float orc_fexp(float x);
 float orc_hg_sample(const float* wo, float g, float u0, float u1, float* wi);
 /* emitter pdf from a medium vertex (zero normal): raw = 0 as the reference (entity::pdf<emitter>
  * transforms the interaction, normalising the zero normal), raw = 1 untransformed */

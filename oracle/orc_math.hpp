@@ -39,8 +39,60 @@ inline real rsin(real x) { return static_cast<real>(std::sin(static_cast<double>
 inline real rcos(real x) { return static_cast<real>(std::cos(static_cast<double>(x))); }
 inline real racos(real x) { return static_cast<real>(std::acos(static_cast<double>(x))); }
 inline real ratan2(real y, real x) { return static_cast<real>(std::atan2(static_cast<double>(y), static_cast<double>(x))); }
-inline real rlog(real x) { return static_cast<real>(std::log(static_cast<double>(x))); }
-inline real rexp(real x) { return static_cast<real>(std::exp(static_cast<double>(x))); }
+// Float log and exp with one fixed sequence of IEEE operations (no contraction), so that
+// the oracle, which carries the same definition, agrees with the kernels bit for bit at
+// real cost.  Published methods: log by reduction to m in [sqrt(1/2), sqrt(2)) and the
+// series log(m) = 2 atanh(s), s = (m - 1) / (m + 1), through s^9; exp by Cody-Waite
+// reduction x = k ln2 + r and the Taylor polynomial of degree 7, scaled by 2^k.  Both
+// are within 2 ulp of the correctly rounded value on the ranges used (tested).
+inline real flog(real x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    int e = 0;
+    if (b < 0x00800000u) {  // subnormal: scale into the normal range
+        const real y = x * 8388608.0f;
+        __builtin_memcpy(&b, &y, 4);
+        e = -23;
+    }
+    e += (int)(b >> 23) - 127;
+    b = (b & 0x007FFFFFu) | 0x3F800000u;
+    real m;
+    __builtin_memcpy(&m, &b, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const real s = (m - 1.0f) / (m + 1.0f);
+    const real s2 = s * s;
+    const real p = s2 * (0.666666667f + s2 * (0.4f + s2 * (0.285714286f + s2 * 0.222222222f)));
+    const real lm = (s + s) + s * p;
+    const real fe = static_cast<real>(e);
+    return fe * 0.693145752f + (lm + fe * 1.42860677e-6f);
+}
+
+inline real fexp(real x) {
+    if (x != x) return x;
+    if (x > 88.7228394f) return __builtin_huge_valf();
+    if (x < -103.972084f) return 0.0f;
+    const real kf = __builtin_rintf(x * 1.44269504f);
+    const real r = (x - kf * 0.693145752f) - kf * 1.42860677e-6f;
+    const real p =
+        1.0f + r * (1.0f + r * (0.5f + r * (0.166666667f + r * (0.0416666667f + r * (0.00833333333f +
+                                                                                    r * (0.00138888889f +
+                                                                                         r * 0.000198412698f))))));
+    int k = (int)kf;
+    real scale = 1.0f;
+    if (k < -125) {  // subnormal result: scale in two steps
+        scale = 5.42101086e-20f;  // 2^-64
+        k += 64;
+    }
+    const uint32_t eb = (uint32_t)(k + 127) << 23;
+    real two_k;
+    __builtin_memcpy(&two_k, &eb, 4);
+    return p * two_k * scale;
+}
 
 // glm::min/max on scalars: (y < x) ? y : x  and  (x < y) ? y : x
 inline real fmin_g(real x, real y) { return (y < x) ? y : x; }

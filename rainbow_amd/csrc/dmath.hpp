@@ -86,8 +86,60 @@ RBW_HD float rsin(float x) { return (float)sin((double)x); }
 RBW_HD float rcos(float x) { return (float)cos((double)x); }
 RBW_HD float racos(float x) { return (float)acos((double)x); }
 RBW_HD float ratan2(float y, float x) { return (float)atan2((double)y, (double)x); }
-RBW_HD float rlog(float x) { return (float)log((double)x); }
-RBW_HD float rexp(float x) { return (float)exp((double)x); }
+// Float log and exp with one fixed sequence of IEEE operations (no contraction), so that
+// the oracle, which carries the same definition, agrees with the kernels bit for bit at
+// float cost.  Published methods: log by reduction to m in [sqrt(1/2), sqrt(2)) and the
+// series log(m) = 2 atanh(s), s = (m - 1) / (m + 1), through s^9; exp by Cody-Waite
+// reduction x = k ln2 + r and the Taylor polynomial of degree 7, scaled by 2^k.  Both
+// are within 2 ulp of the correctly rounded value on the ranges used (tested).
+RBW_HD float flog(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    int e = 0;
+    if (b < 0x00800000u) {  // subnormal: scale into the normal range
+        const float y = x * 8388608.0f;
+        __builtin_memcpy(&b, &y, 4);
+        e = -23;
+    }
+    e += (int)(b >> 23) - 127;
+    b = (b & 0x007FFFFFu) | 0x3F800000u;
+    float m;
+    __builtin_memcpy(&m, &b, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    const float p = s2 * (0.666666667f + s2 * (0.4f + s2 * (0.285714286f + s2 * 0.222222222f)));
+    const float lm = (s + s) + s * p;
+    const float fe = (float)e;
+    return fe * 0.693145752f + (lm + fe * 1.42860677e-6f);
+}
+
+RBW_HD float fexp(float x) {
+    if (x != x) return x;
+    if (x > 88.7228394f) return __builtin_huge_valf();
+    if (x < -103.972084f) return 0.0f;
+    const float kf = __builtin_rintf(x * 1.44269504f);
+    const float r = (x - kf * 0.693145752f) - kf * 1.42860677e-6f;
+    const float p =
+        1.0f + r * (1.0f + r * (0.5f + r * (0.166666667f + r * (0.0416666667f + r * (0.00833333333f +
+                                                                                    r * (0.00138888889f +
+                                                                                         r * 0.000198412698f))))));
+    int k = (int)kf;
+    float scale = 1.0f;
+    if (k < -125) {  // subnormal result: scale in two steps
+        scale = 5.42101086e-20f;  // 2^-64
+        k += 64;
+    }
+    const uint32_t eb = (uint32_t)(k + 127) << 23;
+    float two_k;
+    __builtin_memcpy(&two_k, &eb, 4);
+    return p * two_k * scale;
+}
 
 // std::nextafter for float (IEEE 754 nextUp/nextDown on the bit pattern)
 RBW_HD float next_after(float from, float to) {

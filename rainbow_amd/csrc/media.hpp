@@ -3,9 +3,9 @@
 // and of the density grid (media/heterogeneous_medium.cpp:24-120: delta tracking
 // to sample, ratio tracking to evaluate, trilinear lookup as
 // textures/detail/image_texture.hpp:119-162), and the Henyey-Greenstein phase
-// function (shared/phases/henyey_greenstein.cpp:11-48, utilities.cpp:46-51).  log, exp,
-// sin and cos are the correctly rounded float values (via the double routines), as the
-// oracle computes them.
+// function (shared/phases/henyey_greenstein.cpp:11-48, utilities.cpp:46-51).  log and exp
+// are dmath.hpp's flog / fexp (one fixed float sequence, shared with the oracle), sin and
+// cos the correctly rounded values via the double routines.
 #pragma once
 
 #include "device.hpp"
@@ -129,10 +129,10 @@ __device__ __forceinline__ medium_event medium_sample(const dscene& S, const dme
         const float st[3] = {M.sigma_t[0], M.sigma_t[1], M.sigma_t[2]};
         const float u1 = smp.next();
         const float sc = ch == 0 ? st[0] : (ch == 1 ? st[1] : st[2]);
-        const float t = gmin(-rlog(1.0f - u1) / sc, len);
+        const float t = gmin(-flog(1.0f - u1) / sc, len);
         const bool sampled = t < len;
         const float tc = gmin(t, k_flt_max);
-        const f3 value = mk(rexp(-st[0] * tc), rexp(-st[1] * tc), rexp(-st[2] * tc));
+        const f3 value = mk(fexp(-st[0] * tc), fexp(-st[1] * tc), fexp(-st[2] * tc));
         const f3 dens = sampled ? mk(st[0], st[1], st[2]) * value : value;
         float pdf = 0.0f;
         pdf = pdf + dens.x;
@@ -150,7 +150,7 @@ __device__ __forceinline__ medium_event medium_sample(const dscene& S, const dme
     if (!unit_box_interval(lo, ld, len, t0, t1)) return ev;
     float t = t0;
     while (true) {
-        t = t - rlog(1.0f - smp.next()) * M.inv_max_density;
+        t = t - flog(1.0f - smp.next()) * M.inv_max_density;
         if (t >= t1) break;
         const float sigma_t = grid_density(S, M, lo + ld * t);
         if (sigma_t * M.inv_max_density > smp.next()) {
@@ -169,14 +169,14 @@ template <class SMP>
 __device__ __forceinline__ f3 medium_transmittance(const dscene& S, const dmedium& M, SMP& smp, f3 o, f3 d, float len) {
     if (M.type == RBW_MEDIUM_HOMOGENEOUS) {
         const float tc = gmin(len, k_flt_max);
-        return mk(rexp(-M.sigma_t[0] * tc), rexp(-M.sigma_t[1] * tc), rexp(-M.sigma_t[2] * tc));
+        return mk(fexp(-M.sigma_t[0] * tc), fexp(-M.sigma_t[1] * tc), fexp(-M.sigma_t[2] * tc));
     }
     const f3 lo = w2l_point(M, o), ld = w2l_vector(M, d);
     float t0, t1;
     if (!unit_box_interval(lo, ld, len, t0, t1)) return splat(1.0f);
     float value = 1.0f, t = t0;
     while (true) {
-        t = t - rlog(1.0f - smp.next()) * M.inv_max_density;
+        t = t - flog(1.0f - smp.next()) * M.inv_max_density;
         if (t >= t1) break;
         const float sigma_t = grid_density(S, M, lo + ld * t);
         value = value * (1.0f - gmax(0.0f, sigma_t * M.inv_max_density));

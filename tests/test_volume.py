@@ -136,6 +136,43 @@ def _check_mean(img, expect, tol_sigma=5.0, rel_floor=2e-3):
         assert abs(m[c] - expect[c]) < tol_sigma * se[c] + rel_floor * expect[c], (c, m, expect, se)
 
 
+# ------------------------------------------------------------------ float log / exp
+
+def _ulps(a, b):
+    """distance in float32 ulps between float32 a and the float64 value b"""
+    a = np.asarray(a, np.float32)
+    ref = np.asarray(b, np.float64).astype(np.float32)
+    ai = a.view(np.int32).astype(np.int64)
+    ri = ref.view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7FFFFFFF), ai)
+    ri = np.where(ri < 0, -(ri & 0x7FFFFFFF), ri)
+    return np.abs(ai - ri)
+
+
+def test_flog_fexp_within_two_ulp():
+    """the float log / exp both sides of the media code use (one fixed operation sequence):
+    within 2 ulp of the correctly rounded value, exact at 1 and 0, special values"""
+    L = orc_py.lib()
+    L.orc_flog.argtypes = [C.c_float]
+    L.orc_flog.restype = C.c_float
+    L.orc_fexp.argtypes = [C.c_float]
+    L.orc_fexp.restype = C.c_float
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.random(20000), 1.0 - rng.random(20000) * 1e-3, np.exp(rng.uniform(-80, 80, 20000)),
+                         [1e-45, 1e-40, 1.17549435e-38, 0.5, 1.0, 2.0, 3.4e38]]).astype(np.float32)
+    got = np.array([L.orc_flog(float(x)) for x in xs], np.float32)
+    assert _ulps(got, np.log(xs.astype(np.float64))).max() <= 2
+    assert L.orc_flog(1.0) == 0.0 and L.orc_flog(0.0) == -np.inf and np.isnan(L.orc_flog(-1.0))
+    ys = np.concatenate([rng.uniform(-103, 88, 40000), rng.uniform(-1, 1, 20000), [-0.0, 0.0, -87.3, -100.0]])
+    ys = ys.astype(np.float32)
+    got = np.array([L.orc_fexp(float(y)) for y in ys], np.float32)
+    want = np.exp(ys.astype(np.float64))
+    normal = want >= 1.17549435e-38
+    assert _ulps(got[normal], want[normal]).max() <= 2
+    assert np.all(np.abs(got[~normal] - want[~normal]) <= 2 * 1.4e-45)   # subnormal results: absolute
+    assert L.orc_fexp(0.0) == 1.0 and L.orc_fexp(-200.0) == 0.0 and L.orc_fexp(100.0) == np.inf
+
+
 # ------------------------------------------------------------------ phase function
 
 @pytest.mark.parametrize("g", [-0.7, 0.0, 0.3, 0.9])
