@@ -6,7 +6,16 @@
 // reference files line by line (file:line cited at each function) with the
 // reference's floating-point operation order.  Scenes arrive as the flat
 // rbw_scene_desc of include/rbw.h: mesh entities carry the identity transform
-// (world-space triangles), sphere entities a translation.
+// (world-space triangles), sphere entities a translation.  Also restated: the
+// direct integrator (direct_integrator.cpp) and the volume path integrator with
+// homogeneous and grid media (volume_path_integrator.cpp, media/*.cpp).
+//
+// How each part is pinned lives with its tests (tests/test_oracle.py,
+// test_direct.py, test_volume.py, ...): golden pcg32 vectors from the reference's
+// own random_generator.cpp, closed forms (furnaces, polygon irradiance, Beer-Lambert,
+// slab irradiance 2 rho Le E3(tau), HG normalisation and sampling), brute-force
+// intersection.  Parity unpinned: the last bits of MSVC's float transcendentals
+// (here correctly rounded sin/cos/acos/atan2 and the fixed flog/fexp sequence).
 #include "orc.h"
 #include "orc_math.hpp"
 
