@@ -156,6 +156,47 @@ def volume_leg(cpu=True, steps=2):
     return out
 
 
+def mlp_leg(rows=1 << 18, calls=20):
+    """The guiding network's kernels at the training batch of 2^18 records: one training
+    step (forward with stored activations, mixture gradient, data and weight gradients,
+    Adam) and batched inference, each timed with HIP events on the library's stream
+    (transfers excluded), against the dense FP16 MFMA peak.  Records are synthetic:
+    inputs uniform in [0,1]^9, directions uniform on the sphere, weights exponential."""
+    import numpy as np
+
+    from rainbow_amd import roofline
+    from rainbow_amd.tracer import Tracer
+    rng = np.random.default_rng(7)
+    x = rng.random((rows, 9), dtype=np.float32)
+    d = rng.normal(size=(rows, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = rng.exponential(size=rows).astype(np.float32)
+    tr = Tracer(0)
+    tr.guide_reset(1)
+    for _ in range(3):
+        tr.guide_train(x, d, c, want_loss=False)
+        tr.guide_forward(x)
+    train_ms, fwd_ms = [], []
+    for _ in range(calls):
+        tr.guide_train(x, d, c, want_loss=False)
+        train_ms.append(tr.guide_last_ms())
+    for _ in range(calls):
+        tr.guide_forward(x)
+        fwd_ms.append(tr.guide_last_ms())
+    tr.close()
+    out = {"workload": f"{rows} synthetic records, 9 inputs -> 216 sin/cos features -> 5 x 64 ReLU -> 8 vMF lobes",
+           "peak": roofline.MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "bound": "mfma"}
+    for name, ms, flop in (("train_step", train_ms, roofline.MLP_FLOP_TRAIN),
+                           ("inference", fwd_ms, roofline.MLP_FLOP_FORWARD)):
+        m = float(np.median(ms))
+        tf = rows * flop / (m * 1e-3) / 1e12
+        out[name] = {"ms": round(m, 4), "flop_per_record": flop, "achieved": round(tf, 1),
+                     "frac": round(tf / roofline.MFMA_FP16_PEAK_TFLOPS, 4)}
+    print(f"# mlp leg: train step {out['train_step']['ms']} ms, inference {out['inference']['ms']} ms",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def integrator_variants_leg(sc, cam, spp, depth, cpu=True, steps=2):
     """The headline workload rendered by the other estimators the build carries: the direct
     integrator (one light and one BSDF sample, beside the oracle's rate) and the path
@@ -334,6 +375,8 @@ def main():
                     help="guided vs plain on 16 narrow-aperture room variants, one network each (~4 min)")
     ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
                     help="volume path integrator leg (foggy Cornell box)")
+    ap.add_argument("--mlp", action=argparse.BooleanOptionalAction, default=True,
+                    help="guiding-network kernels (training step, inference) at 2^18 records vs the MFMA peak")
     ap.add_argument("--guided", default="full", choices=["reduced", "full", "off"],
                     help="guided-vs-plain time-to-relMSE leg: 512^2 with a 65536-spp judge (full, the "
                          "configured workload, ~90 s), 256^2 with a 16384-spp judge (reduced), or skipped")
@@ -431,6 +474,8 @@ def main():
         out["guided"] = guided_leg(args.guided == "full")
     if args.volume:
         out["volume"] = volume_leg(cpu=not args.no_cpu)
+    if args.mlp:
+        out["guiding_mlp"] = mlp_leg()
     if args.variants:
         out["integrator_variants"] = integrator_variants_leg(sc, cam, spp, depth, cpu=not args.no_cpu)
     if args.rooms:

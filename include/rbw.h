@@ -350,6 +350,11 @@ rbw_status rbw_guide_forward(rbw_ctx* ctx, uint32_t n, const float* inputs, floa
  * an Adam step with learning_rate.  loss (nullable) receives the loss. */
 rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const float* dirs, const float* c,
                            float learning_rate, uint32_t apply, float* loss, float* grad);
+/* HIP-event time, on the context stream, of the network kernels of the last
+ * rbw_guide_forward or rbw_guide_train call: host transfers of the inputs and
+ * outputs excluded, the read-back of grad / loss included when requested;
+ * 0 before the first such call */
+rbw_status rbw_guide_last_ms(rbw_ctx* ctx, double* ms);
 
 /* ---------------------------------------------------------------- batches
  * Many render jobs of uneven size in one wavefront (no counterpart in the

@@ -180,6 +180,7 @@ RBW_SIGNATURES = {
     "rbw_guide_set_params": (C.c_int32, [C.c_void_p, f32p]),
     "rbw_guide_forward": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p]),
     "rbw_guide_train": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_uint32, f32p, f32p]),
+    "rbw_guide_last_ms": (C.c_int32, [C.c_void_p, C.POINTER(C.c_double)]),
     "rbw_add_scene": (C.c_int32, [C.c_void_p, C.POINTER(rbw_scene_desc), C.POINTER(C.c_uint32)]),
     "rbw_remove_scene": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "rbw_submit_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),

@@ -312,7 +312,10 @@ __global__ __launch_bounds__(256) void k_mixture_grad(const float* __restrict__ 
     }
 }
 
-// dZ_{l-1} = (dZ_l W_l) masked by the ReLU of layer l's input
+// dZ_{l-1} = (dZ_l W_l) masked by the ReLU of layer l's input.  The product is formed
+// transposed, dZ_{l-1}^T = W_l^T dZ_l^T (A = W_l^T, B = dZ_l^T), so that a lane's four
+// accumulators are four consecutive features of one row: 8-byte loads of the stored
+// activations and 8-byte stores of the result.
 __global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restrict__ w16, int l,
                                                        const _Float16* __restrict__ dz_in, int in_stride,
                                                        const _Float16* __restrict__ acts, uint32_t n_pad,
@@ -340,20 +343,23 @@ __global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restric
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             if (s < K / 16) {
-                const h4 a = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
+                // B fragment of dZ_l^T: k = 16 s + 4 lg + i (a feature), column li (a row)
+                const h4 b = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = mfma16(a, wb[s][t], acc[t]);
+                for (int t = 0; t < 4; t++) acc[t] = mfma16(wb[s][t], b, acc[t]);
             }
         }
+        // acc[t][j] = dZ_{l-1}[row r0 + li][feature 16 t + 4 lg + j]
+        const uint32_t r = r0 + li;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < 4; t++) {
+            const h4 a = load_h4(acts + (size_t)r * ACT_STRIDE + act_offset(l) + 16 * t + 4 * lg);
+            h4 o;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t r = r0 + 4 * lg + j;
-                const _Float16 a = acts[(size_t)r * ACT_STRIDE + act_offset(l) + 16 * t + li];
-                const float v = (float)a > 0.0f ? fminf(fmaxf(acc[t][j], -30000.0f), 30000.0f) : 0.0f;
-                dz_out[(size_t)r * HIDDEN + 16 * t + li] = (_Float16)v;
-            }
+            for (int j = 0; j < 4; j++)
+                o[j] = (_Float16)((float)a[j] > 0.0f ? fminf(fmaxf(acc[t][j], -30000.0f), 30000.0f) : 0.0f);
+            *reinterpret_cast<h4*>(dz_out + (size_t)r * HIDDEN + 16 * t + 4 * lg) = o;
+        }
     }
 }
 

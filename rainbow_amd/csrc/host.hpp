@@ -100,6 +100,12 @@ struct kernel_timer {
         recs.back().b = get();
         HIP_CHECK(hipEventRecord(recs.back().b, s));
     }
+    // milliseconds of the last begin/end pair (its end event must have completed)
+    double elapsed_last() const {
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, recs.back().a, recs.back().b));
+        return ms;
+    }
     ~kernel_timer() {
         for (auto e : pool) (void)hipEventDestroy(e);
     }
@@ -177,6 +183,7 @@ struct rbw_ctx {
     rbw::host::dbuf g_surf, g_mat, g_q, g_query, g_mix, g_rec, g_flag, g_tin, g_tdir, g_tc, g_io_in, g_io_out,
         g_io_dir, g_io_c;
     uint32_t g_cap = 0, g_rec_cap = 0, g_t_cap = 0;
+    double guide_last_ms = 0.0;  // network kernels of the last rbw_guide_forward / rbw_guide_train
     // batched jobs: scenes added with rbw_add_scene, and the scheduler
     std::vector<std::unique_ptr<rbw::host::host_scene>> pool;
     rbw_batch_state* batch = nullptr;  // owned; released by batch_free

@@ -1340,10 +1340,14 @@ rbw_status rbw_guide_forward(rbw_ctx* ctx, uint32_t n, const float* inputs, floa
         ctx->g_io_in.upload(inputs, size_t(n) * 9, s);
         const uint32_t n_pad = (n + 15) & ~15u;
         ctx->g_io_out.ensure(size_t(n_pad) * guide::N_OUT_PAD * sizeof(float));
+        ctx->timer.reset();
+        ctx->timer.begin(RBW_KERNEL_GUIDE_INFER, s);
         guide::forward(ctx->gm, ctx->g_io_in.as<float>(), n, ctx->g_io_out.as<float>(), s);
+        ctx->timer.end(s);
         HIP_CHECK(hipMemcpyAsync(out, ctx->g_io_out.p, size_t(n) * guide::N_OUT_PAD * sizeof(float),
                                  hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
+        ctx->guide_last_ms = ctx->timer.elapsed_last();
     });
 }
 
@@ -1359,6 +1363,8 @@ rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const 
         ctx->g_io_in.upload(inputs, size_t(n) * 9, s);
         ctx->g_io_dir.upload(dirs, size_t(n) * 3, s);
         ctx->g_io_c.upload(c, n, s);
+        ctx->timer.reset();
+        ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
         HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
         guide::adam_opts ao;
         ao.lr = learning_rate;
@@ -1381,8 +1387,17 @@ rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const 
             const float c2 = 1.0f - std::pow(ao.b2, (float)ctx->gm.step);
             guide::adam_apply(ctx->gm, n, ao, c1, c2, s);
         }
+        ctx->timer.end(s);
         HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
         HIP_CHECK(hipStreamSynchronize(s));
+        ctx->guide_last_ms = ctx->timer.elapsed_last();
+    });
+}
+
+rbw_status rbw_guide_last_ms(rbw_ctx* ctx, double* ms) {
+    return guarded([&] {
+        require(ctx != nullptr && ms != nullptr, RBW_ERR_INVALID_ARGUMENT, "null arguments");
+        *ms = ctx->guide_last_ms;
     });
 }
 

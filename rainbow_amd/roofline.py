@@ -46,3 +46,15 @@ def kernel_bytes(stats, width=2):
         "connect": records * B_CONNECT_REC + con_nodes * node + con_prims * S_PRIM
         + stats["light_casts"] * B_LIGHT_CAST,
     }
+
+
+# The guiding network (9 inputs -> 216 sin/cos features -> 5 x 64 ReLU -> 40 head
+# outputs): multiply-adds per record of the live (unpadded) matrices.
+MLP_MACS_FORWARD = 216 * 64 + 4 * 64 * 64 + 64 * 40          # 32,768
+MLP_MACS_DATA_GRAD = 4 * 64 * 64 + 64 * 40                   # layers 1-5 (the encoding needs none)
+MLP_FLOP_FORWARD = 2 * MLP_MACS_FORWARD                      # 65,536 per record
+# one training record: forward with stored activations, data gradients of layers 1-5,
+# weight gradients of all six layers (as many multiply-adds as the forward pass)
+MLP_FLOP_TRAIN = 2 * (2 * MLP_MACS_FORWARD + MLP_MACS_DATA_GRAD)  # 168,960 per record
+
+MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 matrix peak, 2.5 PFLOP/s

@@ -122,16 +122,22 @@ class Tracer:
         _check(self.lib.rbw_guide_forward(self.h, len(x), abi.ptr(x, abi.f32p), abi.ptr(out, abi.f32p)))
         return out
 
-    def guide_train(self, inputs, dirs, c, lr=1e-2, apply=True, want_grad=False):
+    def guide_train(self, inputs, dirs, c, lr=1e-2, apply=True, want_grad=False, want_loss=True):
         x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         w = np.ascontiguousarray(c, np.float32).reshape(-1)
-        loss = np.zeros(1, np.float32)
+        loss = np.zeros(1, np.float32) if want_loss else None
         grad = np.zeros(abi.RBW_GUIDE_PARAMS, np.float32) if want_grad else None
         _check(self.lib.rbw_guide_train(self.h, len(x), abi.ptr(x, abi.f32p), abi.ptr(d, abi.f32p),
                                         abi.ptr(w, abi.f32p), float(lr), int(apply), abi.ptr(loss, abi.f32p),
                                         abi.ptr(grad, abi.f32p)))
-        return float(loss[0]), grad
+        return (float(loss[0]) if want_loss else None), grad
+
+    def guide_last_ms(self):
+        """HIP-event time of the network kernels of the last guide_forward / guide_train call"""
+        ms = C.c_double(0.0)
+        _check(self.lib.rbw_guide_last_ms(self.h, C.byref(ms)))
+        return ms.value
 
 
     # -- batched jobs (no counterpart in the reference) ----------------------

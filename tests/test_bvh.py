@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module")
-def checker(tmp_path_factory):
+def bvh8_check_exe(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("bvh8") / "bvh8_check")
     subprocess.run(["g++", "-std=c++17", "-O2", "-o", exe, os.path.join(ROOT, "tests", "cpp", "bvh8_check.cpp"),
                     os.path.join(ROOT, "rainbow_amd", "csrc", "bvh.cpp")], check=True)
@@ -19,8 +19,8 @@ def checker(tmp_path_factory):
 
 
 @pytest.mark.parametrize("ntris", [1, 2, 5, 9, 300, 20000])
-def test_bvh8_structure_and_closest_hits(checker, ntris):
-    r = subprocess.run([checker, str(ntris)], capture_output=True, text=True)
+def test_bvh8_structure_and_closest_hits(bvh8_check_exe, ntris):
+    r = subprocess.run([bvh8_check_exe, str(ntris)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("ok")

@@ -144,6 +144,29 @@ def test_gpu_adam_step(tracer):
 
 
 @pytest.mark.gpu
+def test_gpu_guide_kernel_time_and_loss_free_step(tracer):
+    """rbw_guide_last_ms reports the network kernels of the last call; a step without the
+    loss read-back updates the parameters as one with it"""
+    Ws, bs = G.init_params(5)
+    p0 = G.to_padded(Ws, bs)
+    x, d, c = _batch(4096, 6)
+    tracer.guide_set_params(p0)
+    tracer.guide_train(x, d, c, lr=1e-3, apply=True)
+    ms_train = tracer.guide_last_ms()
+    p_with = tracer.guide_params()
+    tracer.guide_set_params(p0)
+    loss, _ = tracer.guide_train(x, d, c, lr=1e-3, apply=True, want_loss=False)
+    p_without = tracer.guide_params()
+    assert loss is None
+    # (float atomics in the batch sums: equal up to their summation order)
+    assert np.allclose(p_with, p_without, rtol=0, atol=1e-6)
+    tracer.guide_forward(x)
+    ms_fwd = tracer.guide_last_ms()
+    print("train step", ms_train, "ms, inference", ms_fwd, "ms for", len(x), "records")
+    assert 0.0 < ms_train < 1000.0 and 0.0 < ms_fwd < 1000.0
+
+
+@pytest.mark.gpu
 def test_gpu_training_learns_a_direction(tracer):
     """samples whose radiance comes from one direction (depending on the input) are learned"""
     tracer.guide_reset(5)
