@@ -446,7 +446,11 @@ def main():
                 "traffic": profiled_traffic(f"k_b_{dom}<2>") if args.workload == "icosphere" else None,
                 "algorithmic_bytes_per_launch": int(kb[dom] / max(1, launches)),
                 "avg_launch_ms": kms[dom] / max(1, launches),
-                "kernel_ms": {k: round(v, 3) for k, v in kms.items()}}
+                "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
+                # every stage kernel against the same roofline (the dominant one is above)
+                "kernels": {k: {"achieved": round(kb[k] / (kms[k] * 1e-3) / 1e9, 1),
+                                "frac": round(kb[k] / (kms[k] * 1e-3) / 1e9 / roofline.HBM_PEAK_GBS, 4),
+                                "avg_launch_ms": kms[k] / max(1, last["kernel_launches"][k])} for k in kms}}
 
     out = {
         "metric": METRIC,

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
 }
 
 template <int W>
-__global__ __launch_bounds__(k_batch_max_block) void k_b_extend(dscene S, dpaths P, dparams base, dbatch B) {
+__global__ __launch_bounds__(k_batch_max_block) RBW_EXTEND_OCCUPANCY void k_b_extend(dscene S, dpaths P, dparams base, dbatch B) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
     uint32_t rays = 0;

@@ -32,6 +32,9 @@ inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * b
 #define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 6 : 1)))
 // The shading kernel keeps four waves per SIMD (at most 128 VGPRs).
 #define RBW_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
+// The binary walk of extension rays: seven waves per SIMD (at most 72 VGPRs, 4 spilled)
+// beat the six of its natural 79 VGPRs; eight (64 VGPRs) spill 11 and run slower.
+#define RBW_EXTEND_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 7 : 1)))
 
 // ------------------------------------------------------------------ traversal
 

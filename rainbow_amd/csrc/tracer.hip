@@ -55,7 +55,7 @@ __global__ void k_raygen(dscene S, dpaths P, dparams prm, dcamera cam, uint32_t 
 }
 
 template <int W, bool COUNT>
-__global__ void k_extend(dscene S, dpaths P, dparams prm, const uint32_t* queue, const uint32_t* queue_count,
+__global__ RBW_EXTEND_OCCUPANCY void k_extend(dscene S, dpaths P, dparams prm, const uint32_t* queue, const uint32_t* queue_count,
                          unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = *queue_count;
