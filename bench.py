@@ -62,14 +62,17 @@ def _time_to(curve, threshold):
     return None
 
 
-def guided_leg(full, threshold=1e-3):
+def guided_leg(full, threshold=1e-3, guide_depth=1):
     """Time to relMSE <= threshold, plain vs guided, on the narrow-aperture room.
 
     The room is lit only through a 0.3x0.3 gap in its +x wall by a 0.5x0.5 quad light
     (radiance 50) outside, aimed 45 degrees upward through the gap, so the beam lands on
     the ceiling and the camera's view is lit indirectly.  The judge is a plain render of
     judge_spp samples per pixel; time is the sum of the render calls' wall time (all
-    kernels: tracing, MLP inference, training), relMSE computed between calls.
+    kernels: tracing, MLP inference, training), relMSE computed between calls.  The guided
+    estimator guides the first guide_depth vertices of each path and shades later ones as
+    the plain tracer (0: every vertex); the leg also reports the every-vertex schedule
+    against the same judge.
     """
     import numpy as np
     from rainbow_amd import scenes
@@ -84,7 +87,7 @@ def guided_leg(full, threshold=1e-3):
     judge_s = time.perf_counter() - t0
     print(f"# guided leg: judge {judge_spp} spp in {judge_s:.1f} s", file=sys.stderr, flush=True)
 
-    def run(guided):
+    def run(guided, gd=0):
         acc = np.zeros_like(ref)
         elapsed, curve, n = 0.0, [], 0
         if guided:
@@ -92,7 +95,7 @@ def guided_leg(full, threshold=1e-3):
         while n * chunk < max_spp:
             kw = {}
             if guided:
-                go = guiding_opts(train=n < train_calls, steps_per_pass=4, batch=1 << 18)
+                go = guiding_opts(train=n < train_calls, steps_per_pass=4, batch=1 << 18, guide_depth=gd)
                 kw["guiding"] = C.pointer(go)
             t = time.perf_counter()
             img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=(2 if guided else 1) * 100000 + n,
@@ -107,8 +110,10 @@ def guided_leg(full, threshold=1e-3):
         return curve, n * chunk
 
     pc, pspp = run(False)
-    gc, gspp = run(True)
+    gc, gspp = run(True, guide_depth)
     tp, tg = _time_to(pc, threshold), _time_to(gc, threshold)
+    ac, aspp = run(True, 0) if guide_depth != 0 else (gc, gspp)
+    ta = _time_to(ac, threshold)
     return {
         "workload": f"narrow-aperture room, displaced icosphere level 8, {res}x{res}, depth 8, "
                     f"beam through a 0.3x0.3 gap onto the ceiling",
@@ -118,6 +123,10 @@ def guided_leg(full, threshold=1e-3):
         "speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
         "plain_spp_run": pspp, "guided_spp_run": gspp,
         "guided_training": f"online, first {train_calls} calls of {chunk} spp, 4 Adam steps of 2^18 per pass",
+        "guided_vertices": "all" if guide_depth == 0 else f"the first {guide_depth}",
+        "every_vertex_guided": {"guided_ms_to_threshold": None if ta is None else round(ta, 1),
+                                "speedup": None if (tp is None or ta is None) else round(tp / ta, 3),
+                                "guided_spp_run": aspp},
     }
 
 
@@ -237,10 +246,11 @@ def integrator_variants_leg(sc, cam, spp, depth, cpu=True, steps=2):
     return out
 
 
-def room_variants_leg(n=16, res=256, judge_spp=16384, threshold=1e-3):
+def room_variants_leg(n=16, res=256, judge_spp=16384, threshold=1e-3, guide_depth=1):
     """The guided-plus-batched workload's scenes: 16 narrow-aperture room variants (gap
     0.1-0.5, 1e5-1.3e6 triangles, GGX alpha 0.02-0.6), each with its own guiding network
-    trained online from scratch; time to relMSE <= threshold, guided vs plain, per variant.
+    trained online from scratch (the first guide_depth vertices guided, 0: all); time to
+    relMSE <= threshold, guided vs plain, per variant.
     The jobs run one after another (one network per job; the batched multi-network kernels
     and the shared launch are not built)."""
     import numpy as np
@@ -263,7 +273,8 @@ def room_variants_leg(n=16, res=256, judge_spp=16384, threshold=1e-3):
             while calls * chunk < max_spp:
                 kw = {}
                 if guided:
-                    kw["guiding"] = C.pointer(guiding_opts(train=calls < train_calls, steps_per_pass=4, batch=1 << 18))
+                    kw["guiding"] = C.pointer(guiding_opts(train=calls < train_calls, steps_per_pass=4, batch=1 << 18,
+                                                             guide_depth=guide_depth))
                 t = time.perf_counter()
                 img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=(2 if guided else 1) * 100000 + calls,
                                                    **kw), stats=False)["image"]
@@ -375,6 +386,8 @@ def main():
                     help="guided vs plain on 16 narrow-aperture room variants, one network each (~4 min)")
     ap.add_argument("--volume", action=argparse.BooleanOptionalAction, default=True,
                     help="volume path integrator leg (foggy Cornell box)")
+    ap.add_argument("--guide-depth", type=int, default=1,
+                    help="guided leg: guide the first N vertices of a path, the plain estimator after (0: all)")
     ap.add_argument("--mlp", action=argparse.BooleanOptionalAction, default=True,
                     help="guiding-network kernels (training step, inference) at 2^18 records vs the MFMA peak")
     ap.add_argument("--guided", default="full", choices=["reduced", "full", "off"],
@@ -475,7 +488,7 @@ def main():
     if args.batch:
         out["batch"] = batch_leg()
     if args.guided != "off":
-        out["guided"] = guided_leg(args.guided == "full")
+        out["guided"] = guided_leg(args.guided == "full", guide_depth=args.guide_depth)
     if args.volume:
         out["volume"] = volume_leg(cpu=not args.no_cpu)
     if args.mlp:
@@ -483,7 +496,7 @@ def main():
     if args.variants:
         out["integrator_variants"] = integrator_variants_leg(sc, cam, spp, depth, cpu=not args.no_cpu)
     if args.rooms:
-        out["room_variants"] = room_variants_leg()
+        out["room_variants"] = room_variants_leg(guide_depth=args.guide_depth)
     if not args.no_cpu:
         cb = cpu_baseline(sc, cam, spp, depth)
         out["cpu_baseline"] = cb
