@@ -2032,6 +2032,24 @@ int orc_render(orc_scene* s, const rbw_camera_desc* cam, const rbw_render_params
     return RBW_OK;
 }
 
+// perspective_camera::sample(position, sample) alone: n raster positions (x, y) and lens
+// samples (u, v) -> world rays (origin, normalised direction)
+void orc_camera_rays(const rbw_camera_desc* cam, uint32_t n, const float* raster_xy, const float* lens_uv,
+                     float* out_o, float* out_d) {
+    render_ctx ctx{};
+    ctx.cam = cam;
+    std::memcpy(ctx.raster_to_camera.m, cam->raster_to_camera, sizeof(float) * 16);
+    std::memcpy(ctx.camera_to_world.m, cam->camera_to_world, sizeof(float) * 16);
+    for (uint32_t i = 0; i < n; i++) {
+        const ray r = camera_ray(ctx, vec2(raster_xy[2 * i], raster_xy[2 * i + 1]),
+                                 vec2(lens_uv[2 * i], lens_uv[2 * i + 1]));
+        for (int c = 0; c < 3; c++) {
+            out_o[3 * i + c] = r.origin[c];
+            out_d[3 * i + c] = r.direction[c];
+        }
+    }
+}
+
 void orc_pcg32_reals(uint64_t seed, uint32_t n, float* out) {
     pcg32 g(seed);
     for (uint32_t i = 0; i < n; i++) out[i] = g.uniform_real();

@@ -64,6 +64,11 @@ int orc_trace_rays(orc_scene* scene, uint32_t n, const float* origins, const flo
 int orc_trace_rays_linear(orc_scene* scene, uint32_t n, const float* origins, const float* directions,
                           const float* tmax, uint32_t visible_only, rbw_hit* out);
 
+/* perspective_camera::sample(position, sample) (perspective_camera.cpp:78-99) for n raster
+ * positions raster_xy (2n) and lens samples lens_uv (2n): world origins and directions (3n each) */
+void orc_camera_rays(const rbw_camera_desc* cam, uint32_t n, const float* raster_xy, const float* lens_uv,
+                     float* out_o, float* out_d);
+
 /* pcg32 of the reference: n uniform_real() draws of pcg32(seed) */
 void orc_pcg32_reals(uint64_t seed, uint32_t n, float* out);
 void orc_pcg32_u32(uint64_t seed, uint32_t n, uint32_t* out);

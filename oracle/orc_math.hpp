@@ -87,6 +87,9 @@ inline real fexp(real x) {
     if (k < -125) {  // subnormal result: scale in two steps
         scale = 5.42101086e-20f;  // 2^-64
         k += 64;
+    } else if (k > 127) {  // x in (88.38, 88.72]: 2^128 is not a float, so 2^127 * 2
+        scale = 2.0f;
+        k -= 1;
     }
     const uint32_t eb = (uint32_t)(k + 127) << 23;
     real two_k;

@@ -59,6 +59,8 @@ def lib():
                                  C.POINTER(orc_render_opts), abi.f32p, C.POINTER(orc_stats)]
         for fn in (L.orc_trace_rays, L.orc_trace_rays_linear):
             fn.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, abi.f32p, C.c_uint32, C.c_void_p]
+        L.orc_camera_rays.argtypes = [C.POINTER(abi.rbw_camera_desc), C.c_uint32, abi.f32p, abi.f32p, abi.f32p,
+                                      abi.f32p]
         L.orc_pcg32_reals.argtypes = [C.c_uint64, C.c_uint32, abi.f32p]
         L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint32, abi.u32p]
         L.orc_pcg32_range.argtypes = [C.c_uint64, C.c_uint32, abi.u32p, abi.u32p, abi.u32p]
@@ -164,6 +166,18 @@ def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def camera_rays(cam, raster_xy, lens_uv=None):
+    """perspective_camera::sample for raster positions (n x 2) and lens samples (n x 2):
+    world (origins, directions), n x 3 each"""
+    xy = np.ascontiguousarray(raster_xy, np.float32).reshape(-1, 2)
+    uv = np.zeros_like(xy) if lens_uv is None else np.ascontiguousarray(lens_uv, np.float32).reshape(-1, 2)
+    o = np.zeros((len(xy), 3), np.float32)
+    d = np.zeros((len(xy), 3), np.float32)
+    lib().orc_camera_rays(C.byref(cam), len(xy), abi.ptr(xy, abi.f32p), abi.ptr(uv, abi.f32p), abi.ptr(o, abi.f32p),
+                          abi.ptr(d, abi.f32p))
+    return o, d
 
 
 def pcg32_reals(seed, n):

@@ -1056,16 +1056,36 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
 // for bit whether its samples are accumulated in sample order (single render)
 // or as paths retire (batched jobs).  |value| is clamped to 2^30; a non-finite
 // sample marks its channel, which then resolves to NaN (the reference's film
-// sum turns NaN there, film.cpp:138).
+// sum turns NaN there, film.cpp:138).  A pixel's sum holds |sum| < 2^31
+// radiance units; an add that would leave that range marks the channel
+// (FILM_OVER_POS / FILM_OVER_NEG), which then resolves to +inf / -inf instead of
+// a wrapped value.
 struct dfilm {
     unsigned long long* sum;  // 3 per pixel
     uint32_t* weight;         // samples per pixel (box filter weight 1)
-    uint32_t* nan;            // per pixel: channels that saw a non-finite sample
+    uint32_t* nan;            // per pixel: channel flags (non-finite sample, sum overflow)
 };
+
+constexpr uint32_t FILM_OVER_POS = 8u;    // << channel
+constexpr uint32_t FILM_OVER_NEG = 64u;   // << channel
 
 __device__ __forceinline__ long long film_fix(float v) {
     const double x = fmin(fmax((double)v, -1073741824.0), 1073741824.0) * 4294967296.0;
     return __double2ll_rn(x);
+}
+
+// the overflow flag of adding q to a sum that held `old`
+__device__ __forceinline__ uint32_t film_overflow(long long old, long long q, int ch) {
+    if (q > 0 && old > __LONG_LONG_MAX__ - q) return FILM_OVER_POS << ch;
+    if (q < 0 && old < -__LONG_LONG_MAX__ - 1 - q) return FILM_OVER_NEG << ch;
+    return 0u;
+}
+
+// one channel's fixed-point add with the overflow check (the sum is flagged, not wrapped silently)
+__device__ __forceinline__ void film_add_fixed(const dfilm& film, size_t p, int ch, long long q) {
+    const long long old = (long long)atomicAdd(&film.sum[3 * p + ch], (unsigned long long)q);
+    const uint32_t over = film_overflow(old, q, ch);
+    if (over) atomicOr(&film.nan[p], over);
 }
 
 // film_tile::add_sample footprint (film.cpp:57-92, box filter radius 0.5) clamped to the pixel bounds
@@ -1096,7 +1116,7 @@ __device__ __forceinline__ void film_add_span(const dcamera& cam, const dfilm& f
         for (int32_t xx = x0; xx < x1; xx++) {
             const size_t p = (size_t)yy * cam.width + (size_t)xx;
             for (int ch = 0; ch < 3; ch++)
-                if (q[ch]) atomicAdd(&film.sum[3 * p + ch], (unsigned long long)q[ch]);
+                if (q[ch]) film_add_fixed(film, p, ch, q[ch]);
             atomicAdd(&film.weight[p], 1u);
             if (bad) atomicOr(&film.nan[p], bad);
         }
@@ -1121,7 +1141,7 @@ __device__ __forceinline__ void film_add_sample(const dcamera& cam, const dfilm&
         for (int32_t xx = x0; xx < x1; xx++) {
             const size_t p = (size_t)yy * cam.width + (size_t)xx;
             for (int ch = 0; ch < 3; ch++)
-                if (q[ch]) atomicAdd(&film.sum[3 * p + ch], (unsigned long long)q[ch]);
+                if (q[ch]) film_add_fixed(film, p, ch, q[ch]);
             if (count_own || xx != ox || yy != oy) atomicAdd(&film.weight[p], 1u);
             if (bad) atomicOr(&film.nan[p], bad);
         }
@@ -1137,6 +1157,8 @@ __device__ __forceinline__ void film_resolve_pixel(const dcamera& cam, const dfi
     for (int ch = 0; ch < 3; ch++) {
         float v = 0.0f;
         if (in) v = (float)((double)(long long)film.sum[3 * p + ch] * 2.3283064365386963e-10 / (double)w);
+        if (bad & (FILM_OVER_POS << ch)) v = __builtin_huge_valf();
+        if (bad & (FILM_OVER_NEG << ch)) v = -__builtin_huge_valf();
         if (bad & (1u << ch)) v = __builtin_nanf("");
         rgb[3 * p + ch] = v;
     }

@@ -411,8 +411,13 @@ __global__ void k_film_own(dpaths P, dcamera cam, uint32_t k, dfilm film) {
             const f3 v = P.F3(F_VX, slot);
             const float c[3] = {v.x, v.y, v.z};
             for (int ch = 0; ch < 3; ch++) {
-                if (isfinite(c[ch])) acc[ch] += film_fix(c[ch]);
-                else bad |= 1u << ch;
+                if (isfinite(c[ch])) {
+                    const long long q = film_fix(c[ch]);
+                    bad |= film_overflow(acc[ch], q, ch);
+                    acc[ch] = (long long)((unsigned long long)acc[ch] + (unsigned long long)q);
+                } else {
+                    bad |= 1u << ch;
+                }
             }
             w += 1;
         }

@@ -541,7 +541,8 @@ namespace scenes {
 //               "integrator": "path" | "direct" | "volume_path", "medium_mis": false,
 //               "emitter_samples": 1, "bsdf_samples": 1,
 //               "stratified": {"strata": [4, 4], "dims_1d": 8, "dims_2d": 12}, "output": "name"}}
-// Transforms compose left to right; the screen window is [-1, 1]^2 scaled by the aspect.
+// Transforms compose left to right; the screen window is [-1, 1]^2 (square pixels: the projection
+// carries the aspect), "fov" the vertical field of view in degrees.
 struct scene_file {
     std::shared_ptr<scenes::scene> scene;
     std::shared_ptr<cameras::camera> camera;

@@ -384,8 +384,8 @@ scene_file load_scene(const std::string& text) {
     const vector3 target = vec3(member(c, "target", "camera"), "camera.target");
     const json* up = c.find("up");
     const vector3 upv = up ? vec3(*up, "camera.up") : vector3{0, 1, 0};
-    const real aspect = real(res[0] / res[1]);
-    const bound2 screen = aspect >= 1 ? bound2{{-aspect, -1}, {aspect, 1}} : bound2{{-1, -1 / aspect}, {1, 1 / aspect}};
+    // perspective_right_hand already scales x by height / width: the [-1, 1]^2 window gives square pixels
+    const bound2 screen{{-1, -1}, {1, 1}};
     out.camera = std::make_shared<cameras::perspective_camera>(
         film, cameras::camera_system::right_hand, screen, shared::perspective_right_hand(fov, real(res[0]), real(res[1])),
         shared::look_at_right_hand(eye, target, upv).inverse(), real(number_or(c, "focus", 1e6, "camera")),

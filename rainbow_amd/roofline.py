@@ -11,12 +11,11 @@ the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
   shade path     264 B   queue 4, hit 8, ray 24, beta 12, value 12, rng 8,
                          pixel/sample/bounce 12 (read); 3 vertex positions 36,
                          3 normals 36, 3 indices 12, entity 48, material 48 (gather);
-                         ray 24, beta 12, value 12, rng 8, bounce 4 (write) -- the
-                         connect record writes are counted in connect
-  connect record 132 B   queue 4, flags 4, shadow ray 28 + light id 4, contribution 12,
-                         vertex point/normal 24, BSDF direction/value/pdf 28, beta 12,
-                         value 12 (read), value 12 (write) ... written once by shade
-                         and read once here: both sides counted here (x1)
+                         ray 24, beta 12, value 12, rng 8, bounce 4 (write)
+  connect record 112 B   flags 4, shadow ray 28 + light id 4, contribution 12, vertex
+                         point/normal 24, BSDF direction/value/pdf 28, beta 12: written
+                         by shade and read by connect, so counted in both kernels
+  connect extra   28 B   connect's queue index 4, value 12 (read) and 12 (write)
   light cast      96 B   two light triangles' vertex positions and indices
 """
 S_NODE = 64
@@ -24,7 +23,8 @@ S_NODE8 = 80
 S_PRIM = 48
 B_EXTEND_RAY = 36
 B_SHADE_PATH = 264
-B_CONNECT_REC = 132
+B_CONNECT_REC = 112
+B_CONNECT_EXTRA = 28
 B_LIGHT_CAST = 96
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
@@ -42,8 +42,9 @@ def kernel_bytes(stats, width=2):
     records = max(stats["shadow_rays"], stats["mis_rays"])
     return {
         "extend": ext_rays * B_EXTEND_RAY + ext_nodes * node + ext_prims * S_PRIM,
-        "shade": ext_rays * B_SHADE_PATH,
-        "connect": records * B_CONNECT_REC + con_nodes * node + con_prims * S_PRIM
+        # shade writes the connect record, connect reads it back
+        "shade": ext_rays * B_SHADE_PATH + records * B_CONNECT_REC,
+        "connect": records * (B_CONNECT_REC + B_CONNECT_EXTRA) + con_nodes * node + con_prims * S_PRIM
         + stats["light_casts"] * B_LIGHT_CAST,
     }
 

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -254,10 +255,11 @@ def _inv(a):
 
 def perspective_camera(width, height, fov_deg, eye, target, up=(0, 1, 0), lens_radius=0.0, focus=1e6):
     """perspective_camera (cameras/camera.cpp:21-52, perspective_camera.cpp:1-20) with the
-    screen window [-1,1]^2 scaled by the aspect; matrices composed in float64 and rounded."""
-    aspect = width / height
-    sw_min = (-aspect, -1.0) if aspect >= 1 else (-1.0, -1.0 / aspect)
-    sw_max = (aspect, 1.0) if aspect >= 1 else (1.0, 1.0 / aspect)
+    screen window [-1,1]^2; matrices composed in float64 and rounded.  The projection
+    (glm's perspectiveFovRH, shared/transform.cpp:144) already scales x by height/width,
+    so this window gives square pixels: fov_deg is the vertical field of view and the
+    horizontal half-angle is atan(tan(fov/2) * width/height)."""
+    sw_min, sw_max = (-1.0, -1.0), (1.0, 1.0)
     # the reference's perspective_right_hand takes float fov/near/far (shared/transform.hpp:113)
     fov = float(np.float32(math.radians(fov_deg)))
     cam_to_screen = _perspective_fov_rh(fov, float(np.float32(width)), float(np.float32(height)),
@@ -548,20 +550,31 @@ def batch_workload(n_jobs=64, seed=2, tris=(1e4, 2e6), res=(128, 1024), spp=(4, 
     spp log-uniform in their ranges, max depth uniform in [depth[0], depth[1]].
     Returns a list of dicts {scene, camera, spp, max_depth, seed, triangles}."""
     rng = np.random.default_rng(seed)
-    jobs = []
+    params = []
     for i in range(n_jobs):
         t = int(round(math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1])))))
         w = int(round(math.exp(rng.uniform(math.log(res[0]), math.log(res[1])))))
         s = int(round(math.exp(rng.uniform(math.log(spp[0]), math.log(spp[1])))))
         d = int(rng.integers(depth[0], depth[1] + 1))
+        params.append((i, t, w, s, d))
+
+    def build(prm):
+        i, t, w, s, d = prm
         sc = Scene()
         add_cornell_box(sc, spheres=False)
         P, F, N = fbm_displaced_grid_sphere(t, seed=1000 + i)
         tm = per_triangle_materials(sc, len(F), 1000 + i)
         sc.add_mesh(P, F, tm[0], normals=N, tri_material=tm)
         cam = perspective_camera(w, w, 40.0, eye=(0.0, 0.0, 3.5), target=(0.0, 0.0, 0.0))
-        jobs.append({"scene": sc, "camera": cam, "spp": s, "max_depth": d, "seed": 7 + i, "triangles": len(F) + 10})
-    return jobs
+        return {"scene": sc, "camera": cam, "spp": s, "max_depth": d, "seed": 7 + i, "triangles": len(F) + 10}
+
+    # each job's scene depends only on its own parameters and seed: build them on threads
+    # (numpy's array loops run outside the interpreter lock), largest first
+    from concurrent.futures import ThreadPoolExecutor
+    order = sorted(range(n_jobs), key=lambda k: -params[k][1])
+    with ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+        built = dict(zip(order, ex.map(build, [params[k] for k in order])))
+    return [built[k] for k in range(n_jobs)]
 
 
 def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0, seed=1, mesh=True,
