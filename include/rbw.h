@@ -324,6 +324,19 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
                           uint32_t visible_only, uint32_t triangle_test,
                           uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
 
+/* rbw_trace_rays without any device allocation: the same queries, with the rays and hit
+ * records staged in a fixed device buffer that the library holds per device from its load
+ * until the process ends (12 MiB), instead of in memory allocated per call.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code running
+ * under HIP graph capture, serving loops that forbid allocation).  n must be at most
+ * RBW_STATIC_MAX_RAYS (RBW_ERR_INVALID_ARGUMENT beyond).  The buffer is shared by every
+ * context on a device: calls of one process's threads on one device take turns; calls
+ * that overlap on one device from several processes each use their own copy. */
+#define RBW_STATIC_MAX_RAYS 262144u
+rbw_status rbw_trace_rays_static(rbw_ctx* ctx, uint32_t n, const float* origins, const float* directions,
+                                 const float* tmax, uint32_t visible_only, uint32_t triangle_test,
+                                 uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
+
 /* Diagnostic: the draws the sampler of `params` (per-sample pcg32 or stratified)
  * gives the spp samples of pixel p (y*width + x): per sample, n1 1D draws then n2 2D
  * draws (x, y), out = spp * (n1 + 2*n2) floats.  Replaces sampler::next

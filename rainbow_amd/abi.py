@@ -165,6 +165,8 @@ def ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+RBW_STATIC_MAX_RAYS = 262144  # rbw_trace_rays_static's limit per call
+
 # declared C-ABI entry points: name -> (restype, argtypes)
 RBW_SIGNATURES = {
     "rbw_create": (C.c_int32, [C.POINTER(rbw_device_opts), C.POINTER(C.c_void_p)]),
@@ -175,6 +177,8 @@ RBW_SIGNATURES = {
                                f32p, C.POINTER(rbw_stats)]),
     "rbw_trace_rays": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_void_p]),
+    "rbw_trace_rays_static": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_void_p]),
     "rbw_guide_reset": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "rbw_guide_get_params": (C.c_int32, [C.c_void_p, f32p]),
     "rbw_guide_set_params": (C.c_int32, [C.c_void_p, f32p]),

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -710,6 +711,21 @@ __global__ void k_trace_rays(dscene S, uint32_t n, const float* o, const float* 
     out[i] = r;
 }
 
+// ---------------------------------------------------- allocation-free ray queries
+// rbw_trace_rays_static stages its rays and hit records here instead of in memory the
+// library allocates: origins, directions and tmax of up to RBW_STATIC_MAX_RAYS rays, then
+// their hit records (48 bytes per ray, 12 MiB).  Each device holds its own copy of the
+// array from the module's load until the process ends.  Host code never passes `scratch`
+// itself to a kernel or a copy (there it names a host-side stand-in): it uses the device
+// address hipGetSymbolAddress returns for the current device (static_scratch_of).
+constexpr size_t STATIC_O = 0;
+constexpr size_t STATIC_D = STATIC_O + size_t(RBW_STATIC_MAX_RAYS) * 3 * sizeof(float);
+constexpr size_t STATIC_T = STATIC_D + size_t(RBW_STATIC_MAX_RAYS) * 3 * sizeof(float);
+constexpr size_t STATIC_H = STATIC_T + size_t(RBW_STATIC_MAX_RAYS) * sizeof(float);
+constexpr size_t STATIC_BYTES = STATIC_H + size_t(RBW_STATIC_MAX_RAYS) * sizeof(rbw_hit);
+static_assert(STATIC_D % 256 == 0 && STATIC_T % 256 == 0 && STATIC_H % 256 == 0, "aligned sections");
+static __device__ __align__(256) unsigned char scratch[STATIC_BYTES];
+
 // ================================================================== host side
 
 using namespace rbw::host;
@@ -733,6 +749,35 @@ rbw_status guarded(F&& f) {
         g_last_error = e.what();
         return RBW_ERR_DEVICE;
     }
+}
+
+}  // namespace
+
+namespace {
+
+// device address of `scratch` per device, resolved once per device with that device
+// current; the mutex serialises the calls of this process's threads that share it
+struct static_scratch_slot {
+    std::once_flag once;
+    hipError_t status = hipSuccess;
+    unsigned char* ptr = nullptr;
+    std::mutex use;
+};
+constexpr int MAX_DEVICES = 64;
+static_scratch_slot g_static_scratch[MAX_DEVICES];
+
+// call with `device` current
+static_scratch_slot& static_scratch_of(int device) {
+    require(device >= 0 && device < MAX_DEVICES, RBW_ERR_UNSUPPORTED, "device ordinal above the static table");
+    static_scratch_slot& sl = g_static_scratch[device];
+    std::call_once(sl.once, [&] {
+        void* p = nullptr;
+        sl.status = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        sl.ptr = static_cast<unsigned char*>(p);
+    });
+    if (sl.status != hipSuccess || !sl.ptr)
+        throw rbw_error(RBW_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(sl.status));
+    return sl;
 }
 
 }  // namespace
@@ -1152,6 +1197,7 @@ rbw_status rbw_create(const rbw_device_opts* opts, rbw_ctx** out) {
         HIP_CHECK(hipSetDevice(dev));
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_count), 64, hipHostMallocDefault));
+        (void)static_scratch_of(dev);  // the static ray buffers' device address, once per device
         *out = ctx.release();
     });
 }
@@ -1303,6 +1349,44 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const 
             throw;
         }
         o.release(); d.release(); t.release(); h.release();
+    });
+}
+
+rbw_status rbw_trace_rays_static(rbw_ctx* ctx, uint32_t n, const float* origins, const float* directions,
+                                 const float* tmax, uint32_t visible_only, uint32_t triangle_test, uint32_t bvh_width,
+                                 rbw_hit* out) {
+    return guarded([&] {
+        require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
+        require(ctx->has_scene, RBW_ERR_NO_SCENE, "no scene uploaded");
+        require(n == 0 || (origins && directions && tmax && out), RBW_ERR_INVALID_ARGUMENT, "null ray arrays");
+        require(n <= RBW_STATIC_MAX_RAYS, RBW_ERR_INVALID_ARGUMENT,
+                "rbw_trace_rays_static takes at most RBW_STATIC_MAX_RAYS (262144) rays per call");
+        require(triangle_test == RBW_TRI_MOLLER_TRUMBORE || triangle_test == RBW_TRI_WATERTIGHT,
+                RBW_ERR_INVALID_ARGUMENT, "unknown triangle test");
+        const uint32_t width = resolve_width(bvh_width);
+        if (n == 0) return;
+        HIP_CHECK(hipSetDevice(ctx->device));
+        static_scratch_slot& sl = static_scratch_of(ctx->device);
+        std::lock_guard<std::mutex> lock(sl.use);
+        hipStream_t s = ctx->stream;
+        float* o = reinterpret_cast<float*>(sl.ptr + STATIC_O);
+        float* d = reinterpret_cast<float*>(sl.ptr + STATIC_D);
+        float* t = reinterpret_cast<float*>(sl.ptr + STATIC_T);
+        rbw_hit* h = reinterpret_cast<rbw_hit*>(sl.ptr + STATIC_H);
+        try {
+            HIP_CHECK(hipMemcpyAsync(o, origins, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(d, directions, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(t, tmax, size_t(n) * sizeof(float), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256),
+                               lds_stack_bytes(256), s, ctx->main.view(width), n, o, d, t, visible_only, triangle_test,
+                               h);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(out, h, size_t(n) * sizeof(rbw_hit), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);  // the buffers are shared: nothing of this call stays in flight
+            throw;
+        }
     });
 }
 

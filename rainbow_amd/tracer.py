@@ -85,14 +85,16 @@ class Tracer:
         return res
 
     def trace(self, origins, directions, tmax, visible_only=False, triangle_test=abi.RBW_TRI_MOLLER_TRUMBORE,
-              bvh_width=0):
+              bvh_width=0, static=False):
+        """closest-hit / closest-visible queries; static: through rbw_trace_rays_static (no
+        device allocation, at most abi.RBW_STATIC_MAX_RAYS rays)"""
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tmax, np.float32).reshape(-1)
         out = np.zeros(len(o), abi.HIT_DTYPE)
-        _check(self.lib.rbw_trace_rays(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p),
-                                       abi.ptr(t, abi.f32p), int(visible_only), int(triangle_test), int(bvh_width),
-                                       out.ctypes.data_as(C.c_void_p)))
+        fn = self.lib.rbw_trace_rays_static if static else self.lib.rbw_trace_rays
+        _check(fn(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p), abi.ptr(t, abi.f32p), int(visible_only),
+                  int(triangle_test), int(bvh_width), out.ctypes.data_as(C.c_void_p)))
         return out
 
 
