@@ -26,27 +26,66 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s + time to relMSE 1e-3: guided vs plain GPU PT, 1 MI355X"
 
 
-def cpu_baseline(sc, cam, spp, depth, target_s=15.0, **pkw):
-    """The oracle on a bounded subset of tiles (every `stride`-th 16x16 tile), all spp."""
+_NATIVE_BUILD = None  # the child process building liborc_native.so (main starts it)
+
+
+def _join_native_build():
+    global _NATIVE_BUILD
+    if _NATIVE_BUILD is None:
+        return
+    err = _NATIVE_BUILD.communicate()[1]
+    if _NATIVE_BUILD.returncode != 0:
+        print(f"# native oracle build failed ({_NATIVE_BUILD.returncode}): {err.decode()[-400:]}; timing the "
+              "parity build", file=sys.stderr, flush=True)
+    _NATIVE_BUILD = None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sc, cam, spp, depth, target_s=15.0, one_core_s=None, **pkw):
+    """The oracle on a bounded subset of tiles (every `stride`-th 16x16 tile), all spp, on all
+    the host threads the box gives (at most 16) and on one: the timing build
+    (liborc_native.so, -O3 -march=native for this host) when bench.py built it, else the
+    parity build.  The timed region is the oracle's tile loop and film merge, the
+    reference's (sampler_integrator.cpp:90-143)."""
     from oracle import orc_py  # the CPU restatement, timed as the baseline
+    _join_native_build()
     threads = min(16, os.cpu_count() or 1)
-    os_ = orc_py.OracleScene(sc)
+    native = orc_py.native_lib() is not None
+    os_ = orc_py.OracleScene(sc, native=native)
     p = orc_py.render_params(spp=spp, max_depth=depth, **pkw)
     ntiles = ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
-    stride = max(1, ntiles // 8)
-    r = os_.render(cam, p, sampler=orc_py.ORC_SAMPLER_TILE, threads=threads, tile_stride=stride)
-    rate = r["stats"]["samples"] / max(r["stats"]["seconds"], 1e-9)
-    want_tiles = max(1.0, rate * target_s / (256 * spp))
-    stride = max(1, int(ntiles / want_tiles))
-    r = os_.render(cam, p, sampler=orc_py.ORC_SAMPLER_TILE, threads=threads, tile_stride=stride)
-    st = r["stats"]
+
+    def timed(nthreads, seconds):
+        stride = max(1, ntiles // (8 * nthreads))
+        r = os_.render(cam, p, sampler=orc_py.ORC_SAMPLER_TILE, threads=nthreads, tile_stride=stride)
+        rate = r["stats"]["samples"] / max(r["stats"]["seconds"], 1e-9)
+        want_tiles = max(1.0, rate * seconds / (256 * spp))
+        stride = max(1, int(ntiles / want_tiles))
+        st = os_.render(cam, p, sampler=orc_py.ORC_SAMPLER_TILE, threads=nthreads, tile_stride=stride)["stats"]
+        return st, stride
+
+    st, stride = timed(threads, target_s)
+    st1, stride1 = timed(1, one_core_s if one_core_s else target_s / 3)
     return {
         "value": st["samples"] / st["seconds"] / 1e6,
         "unit": "Msamples/s",
         "cores": st["threads"],
         "kind": "port",
+        "value_1core": st1["samples"] / st1["seconds"] / 1e6,
+        "cpu_model": cpu_model(),
+        "build": "g++ -O3 -march=native (built on this host)" if native else "g++ -O2 -ffp-contract=off (parity build)",
         "sample": f"every {stride}th 16x16 tile of the same workload at full spp/depth: "
-                  f"{st['samples']} samples in {st['seconds']:.2f} s",
+                  f"{st['samples']} samples in {st['seconds']:.2f} s on {st['threads']} threads; 1 core: every "
+                  f"{stride1}th tile, {st1['samples']} samples in {st1['seconds']:.2f} s",
     }
 
 
@@ -398,6 +437,14 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the film reduction across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
+    if not args.no_cpu:
+        # the CPU baseline's -O3 -march=native build for this host, compiled by a child process
+        # while the GPU legs run (started before anything touches the GPU)
+        from oracle import orc_py
+        if os.path.exists(orc_py.NATIVE_LIB):
+            os.remove(orc_py.NATIVE_LIB)
+        global _NATIVE_BUILD
+        _NATIVE_BUILD = orc_py.start_native_build()
 
     from rainbow_amd import abi, roofline, scenes
     from rainbow_amd.tracer import Tracer, render_params
@@ -425,17 +472,21 @@ def main():
     for _ in range(args.warmup):
         tr.render(cam, params)
     step_stats = []
+    import numpy as np
+    film = np.zeros((cam.height, cam.width, 3), np.float32)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        # one render call per step; the film stays on the device (no PCIe copy in the timed region)
+        # one render call per step, delivering the film's pixels to host memory as the
+        # drop-in render call does (the PCIe copy is inside the timed region)
         st_ = abi.rbw_stats()
-        rc = tr.lib.rbw_render(tr.h, C.byref(cam), C.byref(params), None, C.byref(st_))
+        rc = tr.lib.rbw_render(tr.h, C.byref(cam), C.byref(params), abi.ptr(film, abi.f32p), C.byref(st_))
         if rc != 0:
             raise RuntimeError(tr.lib.rbw_last_error().decode())
         step_stats.append(st_.as_dict())
         print(f"# step {i}: {st_.render_ms:.1f} ms, {st_.msamples_per_s:.1f} Msamples/s", file=sys.stderr,
               flush=True)
     t1 = time.perf_counter()
+    assert np.isfinite(film).all() and film.any()
     wall = t1 - t0
     samples = sum(s["samples"] for s in step_stats)
     rays = sum(s["extension_rays"] + s["shadow_rays"] + s["mis_rays"] for s in step_stats)
@@ -483,6 +534,9 @@ def main():
                    "shadow_rays": "closest-visible results (any-hit walk: the light is planar)",
                    "mis_rays": "closest-visible results (emitter triangles first, then an occlusion walk)"},
         "mrays_per_s": round(rays / wall / 1e6, 2),
+        # the render on the device alone (HIP events: first kernel to film resolve), without
+        # the film's copy to host memory that ms_per_step includes
+        "render_ms_device": round(sum(s["render_ms"] for s in step_stats) / len(step_stats), 3),
         "roofline": roof,
     }
     if args.batch:

@@ -42,7 +42,26 @@ def build():
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
 
+NATIVE_LIB = os.path.join(HERE, "liborc_native.so")
+
+
+def start_native_build():
+    """starts `make native` (liborc_native.so, -O3 -march=native for this host) as a child
+    process and returns it; the CPU baseline's timing build"""
+    return subprocess.Popen(["make", "-s", "-C", HERE, "liborc_native.so"], stdout=subprocess.DEVNULL,
+                            stderr=subprocess.PIPE)
+
+
 _lib = None
+_native = None
+
+
+def native_lib():
+    """the timing build (None when it was not built); the same C API as lib()"""
+    global _native
+    if _native is None and os.path.exists(NATIVE_LIB):
+        _native = _bind(C.CDLL(NATIVE_LIB))
+    return _native
 
 
 def lib():
@@ -50,39 +69,43 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.orc_scene_create.argtypes = [C.POINTER(abi.rbw_scene_desc), C.POINTER(C.c_void_p)]
-        L.orc_scene_destroy.argtypes = [C.c_void_p]
-        L.orc_scene_num_bvh_nodes.argtypes = [C.c_void_p]
-        L.orc_scene_num_bvh_nodes.restype = C.c_uint32
-        L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.rbw_camera_desc), C.POINTER(abi.rbw_render_params),
-                                 C.POINTER(orc_render_opts), abi.f32p, C.POINTER(orc_stats)]
-        for fn in (L.orc_trace_rays, L.orc_trace_rays_linear):
-            fn.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, abi.f32p, C.c_uint32, C.c_void_p]
-        L.orc_camera_rays.argtypes = [C.POINTER(abi.rbw_camera_desc), C.c_uint32, abi.f32p, abi.f32p, abi.f32p,
-                                      abi.f32p]
-        L.orc_pcg32_reals.argtypes = [C.c_uint64, C.c_uint32, abi.f32p]
-        L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint32, abi.u32p]
-        L.orc_pcg32_range.argtypes = [C.c_uint64, C.c_uint32, abi.u32p, abi.u32p, abi.u32p]
-        L.orc_stratified_tables.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                            abi.f32p, abi.f32p]
-        L.orc_stratified_tables.restype = C.c_uint64
-        L.orc_concentric_sample_disk.argtypes = [C.c_float, C.c_float, abi.f32p]
-        L.orc_cosine_sample_hemisphere.argtypes = [C.c_float, C.c_float, abi.f32p]
-        L.orc_uniform_sample_triangle.argtypes = [C.c_float, C.c_float, abi.f32p]
-        L.orc_ggx_D.argtypes = [C.c_float, C.c_float, abi.f32p]
-        L.orc_ggx_D.restype = C.c_float
-        L.orc_ggx_lambda.argtypes = [C.c_float, C.c_float, abi.f32p]
-        L.orc_ggx_lambda.restype = C.c_float
-        L.orc_ggx_sample_wh.argtypes = [C.c_float, C.c_float, abi.f32p, C.c_float, C.c_float, abi.f32p]
-        L.orc_ggx_sample_f.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, C.c_float, C.c_float,
-                                       abi.f32p, abi.f32p, abi.f32p]
-        L.orc_ggx_pdf.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p]
-        L.orc_ggx_pdf.restype = C.c_float
-        L.orc_ggx_eval.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, abi.f32p, abi.f32p]
-        L.orc_fresnel_conductor.argtypes = [C.c_float, abi.f32p, abi.f32p, abi.f32p]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+def _bind(L):
+    """argument and result types of the oracle's C API on a loaded library"""
+    L.orc_scene_create.argtypes = [C.POINTER(abi.rbw_scene_desc), C.POINTER(C.c_void_p)]
+    L.orc_scene_destroy.argtypes = [C.c_void_p]
+    L.orc_scene_num_bvh_nodes.argtypes = [C.c_void_p]
+    L.orc_scene_num_bvh_nodes.restype = C.c_uint32
+    L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.rbw_camera_desc), C.POINTER(abi.rbw_render_params),
+                             C.POINTER(orc_render_opts), abi.f32p, C.POINTER(orc_stats)]
+    for fn in (L.orc_trace_rays, L.orc_trace_rays_linear):
+        fn.argtypes = [C.c_void_p, C.c_uint32, abi.f32p, abi.f32p, abi.f32p, C.c_uint32, C.c_void_p]
+    L.orc_camera_rays.argtypes = [C.POINTER(abi.rbw_camera_desc), C.c_uint32, abi.f32p, abi.f32p, abi.f32p,
+                                  abi.f32p]
+    L.orc_pcg32_reals.argtypes = [C.c_uint64, C.c_uint32, abi.f32p]
+    L.orc_pcg32_u32.argtypes = [C.c_uint64, C.c_uint32, abi.u32p]
+    L.orc_pcg32_range.argtypes = [C.c_uint64, C.c_uint32, abi.u32p, abi.u32p, abi.u32p]
+    L.orc_stratified_tables.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        abi.f32p, abi.f32p]
+    L.orc_stratified_tables.restype = C.c_uint64
+    L.orc_concentric_sample_disk.argtypes = [C.c_float, C.c_float, abi.f32p]
+    L.orc_cosine_sample_hemisphere.argtypes = [C.c_float, C.c_float, abi.f32p]
+    L.orc_uniform_sample_triangle.argtypes = [C.c_float, C.c_float, abi.f32p]
+    L.orc_ggx_D.argtypes = [C.c_float, C.c_float, abi.f32p]
+    L.orc_ggx_D.restype = C.c_float
+    L.orc_ggx_lambda.argtypes = [C.c_float, C.c_float, abi.f32p]
+    L.orc_ggx_lambda.restype = C.c_float
+    L.orc_ggx_sample_wh.argtypes = [C.c_float, C.c_float, abi.f32p, C.c_float, C.c_float, abi.f32p]
+    L.orc_ggx_sample_f.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, C.c_float, C.c_float,
+                                   abi.f32p, abi.f32p, abi.f32p]
+    L.orc_ggx_pdf.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p]
+    L.orc_ggx_pdf.restype = C.c_float
+    L.orc_ggx_eval.argtypes = [C.c_float, C.c_float, abi.f32p, abi.f32p, abi.f32p, abi.f32p, abi.f32p]
+    L.orc_fresnel_conductor.argtypes = [C.c_float, abi.f32p, abi.f32p, abi.f32p]
+    return L
 
 
 def f3(v):
@@ -90,18 +113,22 @@ def f3(v):
 
 
 class OracleScene:
-    def __init__(self, scene, desc=None):
+    def __init__(self, scene, desc=None, native=False):
+        """native: use the timing build liborc_native.so (must have been built)"""
         self.scene = scene
         self.desc = scene.desc() if desc is None else desc
+        self.L = native_lib() if native else lib()
+        if self.L is None:
+            raise RuntimeError("liborc_native.so is not built (oracle/Makefile target native)")
         self.h = C.c_void_p()
-        rc = lib().orc_scene_create(C.byref(self.desc), C.byref(self.h))
+        rc = self.L.orc_scene_create(C.byref(self.desc), C.byref(self.h))
         if rc != 0:
             raise RuntimeError(f"orc_scene_create failed: {rc}")
 
     def __del__(self):
-        if getattr(self, "h", None) and lib is not None:
+        if getattr(self, "h", None) and getattr(self, "L", None) is not None:
             try:
-                lib().orc_scene_destroy(self.h)
+                self.L.orc_scene_destroy(self.h)
             except Exception:
                 pass
 
@@ -111,14 +138,14 @@ class OracleScene:
         return cls(None, desc)
 
     def num_bvh_nodes(self):
-        return lib().orc_scene_num_bvh_nodes(self.h)
+        return self.L.orc_scene_num_bvh_nodes(self.h)
 
     def trace(self, origins, directions, tmax, visible_only=False, linear=False):
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(tmax, np.float32).reshape(-1)
         out = np.zeros(len(o), abi.HIT_DTYPE)
-        fn = lib().orc_trace_rays_linear if linear else lib().orc_trace_rays
+        fn = self.L.orc_trace_rays_linear if linear else self.L.orc_trace_rays
         rc = fn(self.h, len(o), abi.ptr(o, abi.f32p), abi.ptr(d, abi.f32p), abi.ptr(t, abi.f32p),
                 int(visible_only), out.ctypes.data_as(C.c_void_p))
         assert rc == 0
@@ -146,7 +173,7 @@ class OracleScene:
             dbg = np.full(W * H * params.spp * params.max_depth * 3, abi.RBW_NONE, np.uint32)
             params.debug_hits = abi.ptr(dbg, abi.u32p)
         st = orc_stats()
-        rc = lib().orc_render(self.h, C.byref(cam), C.byref(params), C.byref(opts), abi.ptr(out, abi.f32p),
+        rc = self.L.orc_render(self.h, C.byref(cam), C.byref(params), C.byref(opts), abi.ptr(out, abi.f32p),
                               C.byref(st))
         if debug:
             params.debug_hits = abi.ptr(None, abi.u32p)

@@ -130,3 +130,35 @@ def test_batch_rejects_bad_jobs(tracer, small_batch):
         _submit(tracer, [bad], ids[:1])
     with pytest.raises(RbwError):
         _submit(tracer, jobs[:1], ids[:1], block_size=100)
+
+
+@pytest.mark.gpu
+def test_batch_64_jobs_match_single_renders(tracer):
+    """The batched-jobs workload at its configured size: 64 seeded jobs (seed 2) of
+    1e4-2e6 triangles (25.6 M in all), 128^2-1024^2, 4-256 spp, depth 3-12 in one
+    submission (the autotuner's choice of schedule).  Every job's image equals its single
+    rbw_render (oracle-pinned by the parity tests) bit for bit, and every job's samples
+    are accounted for."""
+    import time
+    from rainbow_amd.tracer import batch_opts, render_params
+    t0 = time.perf_counter()
+    jobs = scenes.batch_workload()
+    ids = [tracer.add_scene(j["scene"]) for j in jobs]
+    print(f"64 jobs built and pooled in {time.perf_counter() - t0:.1f} s", flush=True)
+    js = [{"scene": ids[i], "camera": j["camera"], "spp": j["spp"], "max_depth": j["max_depth"], "seed": j["seed"]}
+          for i, j in enumerate(jobs)]
+    try:
+        imgs, jstats, bst = tracer.submit_jobs(js, batch_opts(autotune=True))
+        print("batch", bst, flush=True)
+        assert bst["samples"] == sum(j["camera"].width * j["camera"].height * j["spp"] for j in jobs) == 1099077143
+        for i, j in enumerate(jobs):
+            tracer.upload(j["scene"])
+            single = tracer.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"]),
+                                   stats=False)["image"]
+            assert np.isfinite(single).all() and single.mean() > 0, i
+            assert np.array_equal(imgs[i], single), (i, np.abs(imgs[i] - single).max())
+            assert jstats[i]["samples"] == j["camera"].width * j["camera"].height * j["spp"]
+    finally:
+        for sid in ids:
+            tracer.remove_scene(sid)
+    print(f"64 jobs checked in {time.perf_counter() - t0:.1f} s", flush=True)

@@ -258,3 +258,55 @@ def test_gpu_trained_guide_reduces_error(tracer):
 def C_ref(opts):
     import ctypes as C
     return C.pointer(opts)
+
+
+@pytest.fixture(scope="module")
+def room512(tracer):
+    """the narrow-aperture room at its configured size (level-8 mesh, 512x512, depth 8, the
+    beam through the 0.3 x 0.3 gap onto the ceiling) and a plain judge of 8192 spp"""
+    from rainbow_amd.tracer import render_params
+    sc, cam = scenes.narrow_aperture_room(level=8, width=512, height=512, light_tilt=45.0)
+    tracer.upload(sc)
+    judge = tracer.render(cam, render_params(spp=8192, max_depth=8, seed=1 << 20), stats=False)["image"]
+    return sc, cam, judge
+
+
+def _chunked(tracer, cam, chunks, spp, seed0, **kw):
+    """the average of `chunks` independent renders and the standard error of its image mean"""
+    from rainbow_amd.tracer import render_params
+    imgs = [tracer.render(cam, render_params(spp=spp, max_depth=8, seed=seed0 + k, **kw), stats=False)["image"]
+            for k in range(chunks)]
+    means = np.array([float(im.mean()) for im in imgs])
+    return np.mean(imgs, axis=0), float(means.std(ddof=1) / np.sqrt(chunks))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("guide_depth,bar", [(1, 0.97), (0, 0.95)])
+def test_gpu_guided_room_at_the_configured_size(tracer, room512, guide_depth, bar):
+    """The learned-guiding workload at its configured size: the network trained online from
+    scratch with the benchmark's schedule (32 renders of 8 spp, 4 Adam steps of 2^18 records
+    after each), then 8 independent 32-spp renders of the guided (vertex 0, or every vertex)
+    and of the plain estimator.  The guided image mean equals the judge's within 4 standard
+    errors (unbiased), and its relMSE at equal spp is below `bar` times the plain one's."""
+    from rainbow_amd.tracer import guiding_opts
+    sc, cam, judge = room512
+    tracer.upload(sc)
+    tracer.guide_reset(7)
+    for i in range(32):
+        go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=guide_depth)
+        tracer.render(cam, render_params_guided(i, go), stats=False)
+    go = guiding_opts(train=False, guide_depth=guide_depth)
+    guided, se_g = _chunked(tracer, cam, 8, 32, 5000, guiding=C_ref(go))
+    plain, se_p = _chunked(tracer, cam, 8, 32, 9000)
+    mj, mg, mp = float(judge.mean()), float(guided.mean()), float(plain.mean())
+    eg, ep = scenes.rel_mse(guided, judge), scenes.rel_mse(plain, judge)
+    print(f"guide_depth {guide_depth}: means judge {mj:.5f} guided {mg:.5f} (se {se_g:.1e}) plain {mp:.5f} "
+          f"(se {se_p:.1e}); relMSE guided {eg:.3e} plain {ep:.3e} ratio {eg / ep:.3f}")
+    assert abs(mg - mj) < 4 * se_g * np.sqrt(1 + 256 / 8192)
+    assert abs(mp - mj) < 4 * se_p * np.sqrt(1 + 256 / 8192)
+    assert eg < bar * ep
+
+
+def render_params_guided(i, go):
+    from rainbow_amd.tracer import render_params
+    return render_params(spp=8, max_depth=8, seed=300 + i, guiding=C_ref(go))
