@@ -192,6 +192,66 @@ def backward(Ws, bs, acts, pre, dout):
     return dWs, dbs
 
 
+# ---- the kernels' rounding, restated (FP16 operands, wide accumulation)
+
+def f16(a):
+    """round to IEEE half and back"""
+    return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+
+
+def encode_f16(x):
+    """the kernels' encoding: t = 2^k x / 2 in float32 (exact), the fraction of t in float32,
+    sin and cos of 2 pi times it rounded to half"""
+    x32 = np.asarray(x, np.float32)
+    out = np.zeros((x32.shape[0], N_FEAT_PAD))
+    for v in range(N_IN):
+        for k in range(N_FREQ):
+            t = x32[:, v] * np.float32(2.0 ** k * 0.5)
+            f = (t - np.floor(t)).astype(np.float64)
+            out[:, 24 * v + 2 * k] = f16(np.sin(2 * math.pi * f))
+            out[:, 24 * v + 2 * k + 1] = f16(np.cos(2 * math.pi * f))
+    return out
+
+
+def forward_f16(Ws, bs, x):
+    """raw outputs with half weights and half post-ReLU activations (biases and sums wide);
+    returns (outputs, layer inputs)"""
+    A = encode_f16(x)
+    acts = [A]
+    for i, (W, b) in enumerate(zip(Ws, bs)):
+        Wh = f16(W)
+        if i == 0:
+            Wh = np.pad(Wh, ((0, 0), (0, N_FEAT_PAD - Wh.shape[1]))) if Wh.shape[1] < N_FEAT_PAD else Wh
+        Z = A @ Wh.T + b
+        if i < len(Ws) - 1:
+            A = f16(np.maximum(Z, 0.0))
+            acts.append(A)
+        else:
+            A = Z
+    return A, acts
+
+
+def grads_f16(Ws, bs, x, d, c):
+    """loss and mean gradients as the training kernel forms them: the head's per-record
+    gradient at weight c / mean(c) rounded to half (clamped to +-30000), data gradients
+    rounded to half after the ReLU mask, weight gradients summed wide over the records"""
+    n = x.shape[0]
+    o, acts = forward_f16(Ws, bs, x)
+    loss, dout = loss_and_grad_out(o, d, c)
+    clip = lambda a: np.clip(a, -30000.0, 30000.0)
+    dZ = f16(clip(dout * n))
+    dWs, dbs = [None] * len(Ws), [None] * len(Ws)
+    for i in range(len(Ws) - 1, -1, -1):
+        A = acts[i]
+        dW = dZ.T @ A / n
+        dWs[i] = dW[:, :N_FEAT] if i == 0 else dW
+        dbs[i] = dZ.sum(axis=0) / n
+        if i > 0:
+            dA = dZ @ f16(Ws[i])
+            dZ = f16(clip(dA)) * (A > 0)
+    return loss, dWs, dbs
+
+
 def loss_fn(Ws, bs, X, d, c):
     o, _, _ = forward(Ws, bs, X)
     return loss_and_grad_out(o, d, c)[0]

@@ -1,12 +1,23 @@
-// FP16-MFMA kernels of the path-guiding MLP (inference, training forward,
-// backward, Adam).  One wavefront owns 16 rows at a time and runs them through
-// the whole network with v_mfma_f32_16x16x16_f16 (FP32 accumulation); layer
-// outputs go through a per-wave LDS tile to become the next layer's A operand.
+// FP16-MFMA kernels of the path-guiding MLP for CDNA4 (gfx950): batched
+// inference and one fused training step, with v_mfma_f32_16x16x32_f16 and FP32
+// accumulation.
 //
-// MFMA operand layout (wave64, 16x16x16):
-//   A (16x16, M x K): lane l holds A[l % 16][4 * (l / 16) + 0..3]
-//   B (16x16, K x N): lane l holds B[4 * (l / 16) + 0..3][l % 16]
-//   D (16x16):        lane l holds D[4 * (l / 16) + 0..3][l % 16]
+// MFMA operand layout (wave64, 16x16x32, lane l, g = l / 16):
+//   A (16 x 32, M x K): lane l holds A[l % 16][8 g + i], i = 0..7
+//   B (32 x 16, K x N): lane l holds B[8 g + i][l % 16]
+//   D (16 x 16):        lane l holds D[4 g + j][l % 16], j = 0..3
+//
+// Layers run transposed: Z^T = W X^T, the weights are the A operand (M = output
+// features), the records the N columns.  A layer's four 16-feature output tiles
+// then sit in the lanes exactly where the next layer needs its B operand, if the
+// 32 features of a K block are taken in the order
+//   position 8 g + i  <->  feature 32 kb + 4 g + i (i < 4), 32 kb + 16 + 4 g + (i - 4)
+// (tile 2 kb's four values of the lane, then tile 2 kb + 1's): the weights are
+// stored in that order (fragment order, packed by k_adam_pack), so activations
+// never leave the registers between layers.  Layer 0's B operand is the
+// frequency encoding, computed in place: K block kb, lane group g holds input
+// v = (4 kb + g) / 3 at the four frequencies 4 ((4 kb + g) % 3) + q, q = 0..3, as
+// (sin, cos) pairs (9 inputs x 3 quads = 27 of the 28 lane groups; the last is zero).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -21,15 +32,8 @@
 namespace rbw {
 namespace guide {
 
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-
-// stored activations: the inputs of layers 1..5 (layer 0's input, the encoding, is
-// recomputed from the 9 raw inputs where the backward pass needs it)
-constexpr int ACT_STRIDE = 5 * HIDDEN;  // 320 halfs per row
-__host__ __device__ constexpr int act_offset(int l) { return (l - 1) * HIDDEN; }
-constexpr int TILE_PAD = HIDDEN + 8;  // LDS row pitch (halfs)
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 #define GCHECK(expr)                                                                                \
     do {                                                                                            \
@@ -37,446 +41,610 @@ constexpr int TILE_PAD = HIDDEN + 8;  // LDS row pitch (halfs)
         if (_e != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-__device__ __forceinline__ f4 mfma16(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ float pick9(const float* x, int v) {
-    float r = x[0];
+// feature of K-block position (kb, g, i) in the chained layers' order
+__host__ __device__ constexpr int hid_feat(int kb, int g, int i) { return 32 * kb + (i < 4 ? 4 * g + i : 12 + 4 * g + i); }
+// fragment index of the forward product of layer l, M tile m, K block kb
+__host__ __device__ constexpr int fwd_frag(int l, int m, int kb) {
+    return l == 0 ? m * 7 + kb : (l < 5 ? 28 + (l - 1) * 8 + m * 2 + kb : 60 + m * 2 + kb);
+}
+// fragment index of the backward product W_l^T (l = 1..5), M tile m (input features), K block kb
+__host__ __device__ constexpr int bwd_frag(int l, int m, int kb) { return FWD_FRAGS + (l - 1) * 8 + m * 2 + kb; }
+
+// sin and cos of 2 pi t for t in [0, 1): the hardware's v_sin / v_cos take revolutions
+__device__ __forceinline__ float sin_rev(float t) { return __builtin_amdgcn_sinf(t); }
+__device__ __forceinline__ float cos_rev(float t) { return __builtin_amdgcn_cosf(t); }
+
+// Layer 0's B fragment for K block kb of one record (inputs x at stride 1): lane group g
+// holds input v at four consecutive frequencies k0..k0+3, sin(2^k pi x) = sin(2 pi t) with
+// t = 2^(k-1) x, each frequency's t the previous one's doubled (exact in FP32)
+__device__ __forceinline__ h8 encode_frag(float xv, int kb, int g) {
+    h8 b;
+    const int Q = 4 * kb + g;
+    if (Q >= 27) {
 #pragma unroll
-    for (int i = 1; i < N_IN; i++) r = (v == i) ? x[i] : r;
-    return r;
-}
-
-// four consecutive encoded features starting at f0 (a multiple of 4): features of one
-// input scalar v = f0 / 24 at frequencies k and k+1 as (sin, cos, sin, cos)
-__device__ __forceinline__ h4 encode4(const float* x, int f0) {
-    h4 a;
-    if (f0 >= N_FEAT) {
-        a[0] = a[1] = a[2] = a[3] = (_Float16)0.0f;
-        return a;
+        for (int i = 0; i < 8; i++) b[i] = (_Float16)0.0f;
+        return b;
     }
-    const int v = f0 / 24;
-    const int k = (f0 % 24) >> 1;
-    const float xv = pick9(x, v);
-    // sin/cos(2^k pi x) = sin/cos(2 pi t) with t = 2^(k-1) x; v_sin/v_cos take turns
-    float t0 = xv * (float)(1 << k) * 0.5f;
-    float t1 = xv * (float)(1 << (k + 1)) * 0.5f;
-    t0 = t0 - floorf(t0);
-    t1 = t1 - floorf(t1);
-    a[0] = (_Float16)__builtin_amdgcn_sinf(t0);
-    a[1] = (_Float16)__builtin_amdgcn_cosf(t0);
-    a[2] = (_Float16)__builtin_amdgcn_sinf(t1);
-    a[3] = (_Float16)__builtin_amdgcn_cosf(t1);
-    return a;
-}
-
-__device__ __forceinline__ h4 load_h4(const _Float16* p) { return *reinterpret_cast<const h4*>(p); }
-
-// LDS copy of the half weights: rows padded so that the 16 rows one MFMA B
-// fragment reads fall into distinct banks (row pitch of 114 dwords for layer 0,
-// 34 dwords for the others)
-constexpr int W0_PITCH = N_FEAT_PAD + 4;  // halfs
-constexpr int WH_PITCH = HIDDEN + 4;      // halfs
-__host__ __device__ constexpr int lds_w_offset(int l) {
-    return l == 0 ? 0 : HIDDEN * W0_PITCH + (l - 1) * HIDDEN * WH_PITCH;
-}
-constexpr int LDS_W_HALFS = lds_w_offset(N_LAYERS - 1) + N_OUT_PAD * WH_PITCH;  // 34,944 halfs
-constexpr int FWD_WAVES = 4;
-
-// Forward pass.  A block stages every layer's half weights in LDS once, then its
-// four waves run 16-row tiles through the network (one wave per tile), persistent
-// over the rows: the B fragments of all MFMAs come from LDS.  STORE writes every
-// layer's input (training) for all rows up to the padded count; rows >= n use the
-// last valid input row.
-template <bool STORE>
-__global__ __launch_bounds__(FWD_WAVES * 64) void k_forward(const _Float16* __restrict__ w16,
-                                                            const float* __restrict__ w32,
-                                                            const float* __restrict__ inputs, uint32_t n,
-                                                            uint32_t n_pad, float* __restrict__ out,
-                                                            _Float16* __restrict__ acts, sample_map map,
-                                                            const uint32_t* __restrict__ n_dev = nullptr) {
-    // (n_dev: the row count lives in device memory, written by an earlier kernel of the stream)
-    if (n_dev) {
-        n = *n_dev;
-        n_pad = (n + 15) & ~15u;
-        map.n_src = n;
-        if (n == 0) return;
+    const int k0 = 4 * (Q % 3);
+    float t = xv * (float)(1 << k0) * 0.5f;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float f = t - floorf(t);
+        b[2 * q] = (_Float16)sin_rev(f);
+        b[2 * q + 1] = (_Float16)cos_rev(f);
+        t = t * 2.0f;
     }
-    extern __shared__ __align__(16) _Float16 lds[];
-    _Float16* wl = lds;                                  // LDS_W_HALFS
-    _Float16* tiles = lds + LDS_W_HALFS;                 // FWD_WAVES x 16 x TILE_PAD
-    // stage the weights: 4 halfs per thread and step, row by row
-    for (int l = 0; l < N_LAYERS; l++) {
-        const int in = layer_in(l), outs = layer_out(l), pitch = l == 0 ? W0_PITCH : WH_PITCH;
-        const _Float16* src = w16 + layer_w_offset(l);
-        _Float16* dst = wl + lds_w_offset(l);
-        const int quads = outs * in / 4;
-        for (int q = threadIdx.x; q < quads; q += blockDim.x) {
-            const int row = (q * 4) / in, col = (q * 4) % in;
-            *reinterpret_cast<h4*>(dst + row * pitch + col) = load_h4(src + row * in + col);
-        }
+    return b;
+}
+__device__ __forceinline__ int encode_input(int kb, int g) { return (4 * kb + g) / 3; }  // input v (< 9 when live)
+
+__device__ __forceinline__ h8 lds_h8(const _Float16* p) { return *reinterpret_cast<const h8*>(p); }
+__device__ __forceinline__ h8 frag(const _Float16* wl, int f, int lane) { return lds_h8(wl + (f * 64 + lane) * 8); }
+// clamp to +-30000 (inside half's range) in one v_med3
+__device__ __forceinline__ float clamp_h(float v) { return __builtin_amdgcn_fmed3f(v, -30000.0f, 30000.0f); }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------------ inference
+
+constexpr int INF_WAVES = 4;
+constexpr size_t INF_XS = size_t(FWD_FRAGS) * FRAG_HALFS * 2 + N_BIAS * 4;  // per-wave input rows
+constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 64 * N_IN * 4;        // 78,272 bytes
+static_assert(INF_XS % 16 == 0, "aligned input rows");
+
+// R record tiles (16 rows each) per wave share every weight fragment read from LDS.
+// Network blockIdx.y: rows [net * stride, net * stride + n), n from n_dev[net] (or n_host).
+template <int R>
+__global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __restrict__ pack,
+                                                          const float* __restrict__ bias,
+                                                          const float* __restrict__ inputs, uint32_t n_host,
+                                                          const uint32_t* __restrict__ n_dev, uint32_t stride,
+                                                          float* __restrict__ out) {
+    const uint32_t net = blockIdx.y;
+    const uint32_t n = n_dev ? n_dev[net] : n_host;
+    if (n == 0) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    _Float16* wl = reinterpret_cast<_Float16*>(smem);
+    float* bl = reinterpret_cast<float*>(smem + size_t(FWD_FRAGS) * FRAG_HALFS * 2);
+    {
+        const h8* src = reinterpret_cast<const h8*>(pack + (size_t)net * PACK_HALFS);
+        h8* dst = reinterpret_cast<h8*>(wl);
+        for (int q = threadIdx.x; q < FWD_FRAGS * 64; q += blockDim.x) dst[q] = src[q];
+        for (int q = threadIdx.x; q < N_BIAS; q += blockDim.x) bl[q] = bias[(size_t)net * N_BIAS + q];
     }
     __syncthreads();
+    const float* in = inputs + (size_t)net * stride * N_IN;
+    float* o = out + (size_t)net * stride * N_OUT_PAD;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int li = lane & 15, lg = lane >> 4;
-    _Float16* tile = tiles + wave * 16 * TILE_PAD;
-    const uint32_t nwaves = gridDim.x * FWD_WAVES;
-    for (uint32_t chunk = blockIdx.x * FWD_WAVES + wave; chunk * 16 < n_pad; chunk += nwaves) {
-        const uint32_t r0 = chunk * 16;
-        const uint32_t arow = r0 + li;  // the row this lane feeds as A operand
-        const uint32_t src = map(arow < n ? arow : n - 1);
-        float x[N_IN];
+    const int li = lane & 15, g = lane >> 4;
+    const uint32_t groups = (n + 16 * R - 1) / (16 * R);
+    // the wave's group of 16 R rows is staged in LDS (coalesced loads, the next group's in
+    // flight while this one computes)
+    float* xs = reinterpret_cast<float*>(smem + INF_XS) + wave * 64 * N_IN;
+    constexpr int XPL = 16 * R * N_IN / 64;  // floats per lane
+    float xpre[XPL];
+    auto load_group = [&](uint32_t grp) {
 #pragma unroll
-        for (int i = 0; i < N_IN; i++) x[i] = inputs[(size_t)src * N_IN + i];
-        f4 acc[4];
-        // ---- layer 0: encoded inputs (224) -> 64
-#pragma unroll
-        for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-        const _Float16* W0 = wl + lds_w_offset(0);
-        for (int s2 = 0; s2 < N_FEAT_PAD / 16; s2++) {
-            const h4 a = encode4(x, 16 * s2 + 4 * lg);
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-                acc[t] = mfma16(a, load_h4(W0 + (16 * t + li) * W0_PITCH + 16 * s2 + 4 * lg), acc[t]);
+        for (int k = 0; k < XPL; k++) {
+            const size_t q = (size_t)grp * 16 * R * N_IN + lane + 64 * k;
+            xpre[k] = grp < groups && q < (size_t)n * N_IN ? in[q] : 0.0f;
         }
-        // ---- hidden layers 1..4 and the output layer 5
+    };
+    load_group(blockIdx.x * INF_WAVES + wave);
+    for (uint32_t grp = blockIdx.x * INF_WAVES + wave; grp < groups; grp += gridDim.x * INF_WAVES) {
+#pragma unroll
+        for (int k = 0; k < XPL; k++) xs[lane + 64 * k] = xpre[k];
+        wave_sync();
+        load_group(grp + gridDim.x * INF_WAVES);
+        uint32_t row[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            row[t] = grp * 16 * R + 16 * t + li;
+        }
+        f4 acc[R][4];
+        h8 X[R][2];
+#pragma unroll
+        for (int t = 0; t < R; t++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) acc[t][m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        // layer 0: the encoding
+#pragma unroll 1
+        for (int kb = 0; kb < 7; kb++) {
+            const int v = encode_input(kb, g);
+            h8 b[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) b[t] = encode_frag(v < N_IN ? xs[(16 * t + li) * N_IN + v] : 0.0f, kb, g);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const h8 a = frag(wl, fwd_frag(0, m, kb), lane);
+#pragma unroll
+                for (int t = 0; t < R; t++) acc[t][m] = mfma32(a, b[t], acc[t][m]);
+            }
+        }
+        // hidden layers (rolled: unrolled, every layer's fragment loads are hoisted together)
+#pragma unroll 1
         for (int l = 1; l < N_LAYERS; l++) {
-            // bias + ReLU of layer l-1, to the wave's LDS tile (and the activation store)
-            const float* bprev = w32 + layer_b_offset(l - 1);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const float b = bprev[16 * t + li];
+            for (int t = 0; t < R; t++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const float z = acc[t][j] + b;
-                    const _Float16 h = (_Float16)(z > 0.0f ? z : 0.0f);
-                    tile[(4 * lg + j) * TILE_PAD + 16 * t + li] = h;
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float z = acc[t][m][j] + bl[bias_offset(l - 1) + 16 * m + 4 * g + j];
+                        X[t][m >> 1][(m & 1) * 4 + j] = (_Float16)(z > 0.0f ? z : 0.0f);
+                    }
+            const int mt = l == N_LAYERS - 1 ? 3 : 4;
+#pragma unroll
+            for (int t = 0; t < R; t++)
+#pragma unroll
+                for (int m = 0; m < 4; m++) acc[t][m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                if (m >= mt) break;
+#pragma unroll
+                for (int kb = 0; kb < 2; kb++) {
+                    const h8 a = frag(wl, fwd_frag(l, m, kb), lane);
+#pragma unroll
+                    for (int t = 0; t < R; t++) acc[t][m] = mfma32(a, X[t][kb], acc[t][m]);
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (STORE) {
-                // the tile's 16 rows x 64 halfs to the activation store, 16 bytes per lane and step
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int id = lane + 64 * h, row = id >> 3, c8 = id & 7;
-                    *reinterpret_cast<h8*>(acts + (size_t)(r0 + row) * ACT_STRIDE + act_offset(l) + 8 * c8) =
-                        *reinterpret_cast<const h8*>(&tile[row * TILE_PAD + 8 * c8]);
-                }
-            }
-            const int nt = layer_out(l) / 16;
-            const _Float16* Wl = wl + lds_w_offset(l);
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s2 = 0; s2 < HIDDEN / 16; s2++) {
-                const h4 a = load_h4(&tile[li * TILE_PAD + 16 * s2 + 4 * lg]);
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    if (t < nt) acc[t] = mfma16(a, load_h4(Wl + (16 * t + li) * WH_PITCH + 16 * s2 + 4 * lg), acc[t]);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        // ---- raw outputs (48 columns, 40 live) with bias
-        const float* b5 = w32 + layer_b_offset(N_LAYERS - 1);
+        // raw outputs (48 columns, 40 live) with bias
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const float b = b5[16 * t + li];
+        for (int t = 0; t < R; t++) {
+            if (row[t] >= n) continue;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t r = r0 + 4 * lg + j;
-                if (r < n_pad) out[(size_t)r * N_OUT_PAD + 16 * t + li] = acc[t][j] + b;
+            for (int m = 0; m < 3; m++) {
+                const int c0 = 16 * m + 4 * g;
+                float4 v;
+                v.x = acc[t][m][0] + bl[bias_offset(5) + c0];
+                v.y = acc[t][m][1] + bl[bias_offset(5) + c0 + 1];
+                v.z = acc[t][m][2] + bl[bias_offset(5) + c0 + 2];
+                v.w = acc[t][m][3] + bl[bias_offset(5) + c0 + 3];
+                *reinterpret_cast<float4*>(o + (size_t)row[t] * N_OUT_PAD + c0) = v;
             }
         }
     }
 }
 
-constexpr size_t FWD_LDS_BYTES = (size_t(LDS_W_HALFS) + size_t(FWD_WAVES) * 16 * TILE_PAD) * sizeof(_Float16);
+// ------------------------------------------------------------------ training
 
-__device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+// the training head's transcendentals on the hardware units (v_exp_f32 / v_log_f32 / v_rcp_f32,
+// ~1 ulp): the head's gradients are rounded to half afterwards
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float hlog(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ __forceinline__ float hrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hsoftplus(float x) { return x > 20.0f ? x : hlog(1.0f + hexp(x)); }
 
-// d loss / d raw output of the vMF-mixture head for one row; returns log q
-__device__ __forceinline__ float mixture_grad(const float* o, const float* d, float scale, float* g) {
-    float w[N_LOBES], vv[N_LOBES], kap[N_LOBES], mu[N_LOBES][3], nm[N_LOBES], cosv[N_LOBES], sig[N_LOBES];
-    bool kcap[N_LOBES];
-    float amax = -3.4e38f;
-    for (int j = 0; j < N_LOBES; j++) amax = fmaxf(amax, o[5 * j + 4]);
-    float wsum = 0.0f;
-    for (int j = 0; j < N_LOBES; j++) {
-        w[j] = expf(o[5 * j + 4] - amax);
-        wsum += w[j];
+struct train_seg {
+    uint32_t base, n, n_src, a, b;
+    float c1, c2;
+    uint32_t idx_off;  // the network's first entry in the mapped-index array
+    __device__ uint32_t src(uint32_t r) const {
+        return base + (uint32_t)(((uint64_t)a * r + b) % n_src);
     }
-    float q = 0.0f;
-    for (int j = 0; j < N_LOBES; j++) {
-        w[j] /= wsum;
-        const float mx = o[5 * j], my = o[5 * j + 1], mz = o[5 * j + 2];
-        nm[j] = fmaxf(sqrtf(mx * mx + my * my + mz * mz), 1e-12f);
-        mu[j][0] = mx / nm[j];
-        mu[j][1] = my / nm[j];
-        mu[j][2] = mz / nm[j];
-        const float sp = softplus(o[5 * j + 3]) + KAPPA_MIN;
-        kcap[j] = sp >= KAPPA_MAX;
-        kap[j] = fminf(sp, KAPPA_MAX);
-        sig[j] = 1.0f / (1.0f + expf(-o[5 * j + 3]));
-        cosv[j] = mu[j][0] * d[0] + mu[j][1] * d[1] + mu[j][2] * d[2];
-        const float norm = kap[j] / (2.0f * 3.14159265f * (-expm1f(-2.0f * kap[j])));
-        vv[j] = norm * expf(kap[j] * (cosv[j] - 1.0f));
-        q += w[j] * vv[j];
-    }
-    q = fmaxf(q, 1e-30f);
-    const float gq = -scale;  // d loss / d log q (scaled)
-    for (int j = 0; j < N_LOBES; j++) {
-        const float r = w[j] * vv[j] / q;
-        const float e = expf(-2.0f * kap[j]);
-        const float dlv = 1.0f / kap[j] - 2.0f * e / (-expm1f(-2.0f * kap[j])) + (cosv[j] - 1.0f);
-        g[5 * j + 3] = kcap[j] ? 0.0f : gq * r * dlv * sig[j];
-        g[5 * j + 4] = gq * (r - w[j]);
-        float gm[3];
-        for (int c = 0; c < 3; c++) gm[c] = gq * r * kap[j] * d[c];
-        const float p = gm[0] * mu[j][0] + gm[1] * mu[j][1] + gm[2] * mu[j][2];
-        for (int c = 0; c < 3; c++) g[5 * j + c] = (gm[c] - p * mu[j][c]) / nm[j];
-    }
-    return logf(q);
-}
+};
+struct train_segs {
+    train_seg s[MAX_NETS];
+};
 
-// sum of the batch's weights: SUM_PER_THREAD rows per thread, one atomic per block
-constexpr uint32_t SUM_PER_THREAD = 8;
-__global__ __launch_bounds__(256) void k_sum(const float* __restrict__ c, uint32_t n, sample_map map,
-                                             float* __restrict__ total) {
+constexpr uint32_t SUM_BLOCKS = 256;
+
+// partial sums of each network's record weights (block (b, net) writes csum_part[net][b]) and
+// the records' mapped indices, idx[idx_off + r] = src(r), for the training kernel
+__global__ __launch_bounds__(256) void k_csum(const float* __restrict__ c, train_segs segs,
+                                              float* __restrict__ csum_part, uint32_t* __restrict__ idx) {
     __shared__ float part[4];
-    const uint32_t first = blockIdx.x * blockDim.x * SUM_PER_THREAD + threadIdx.x;
+    const train_seg sg = segs.s[blockIdx.y];
     float v = 0.0f;
-#pragma unroll
-    for (uint32_t j = 0; j < SUM_PER_THREAD; j++) {
-        const uint32_t i = first + j * blockDim.x;
-        if (i < n) v += c[map(i)];
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < sg.n; r += SUM_BLOCKS * 256) {
+        const uint32_t src = sg.src(r);
+        idx[sg.idx_off + r] = src;
+        v += c[src];
     }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const float t = part[0] + part[1] + part[2] + part[3];
-        if (t != 0.0f) atomicAdd(total, t);
-    }
+    if (threadIdx.x == 0) csum_part[blockIdx.y * SUM_BLOCKS + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// per row: the clamped, loss-scaled output gradient (FP16, 48 columns) and the loss term
-__global__ __launch_bounds__(256) void k_mixture_grad(const float* __restrict__ out, const float* __restrict__ dirs,
-                                                      const float* __restrict__ c, const float* __restrict__ csum,
-                                                      uint32_t n, uint32_t n_pad, sample_map map,
-                                                      _Float16* __restrict__ dz, float* __restrict__ loss) {
-    __shared__ float part[4];
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    float lterm = 0.0f;
-    if (r < n_pad) {
-        float g[N_OUT_PAD];
+constexpr int TR_WAVES = 4;
+constexpr int CHUNK = 16 * TR_WAVES;  // records per block iteration
+constexpr int XB_PITCH = CHUNK + 8;   // halfs per exchange row
+constexpr size_t L_FRAGS = 0;
+constexpr size_t L_BIAS = L_FRAGS + size_t(FWD_FRAGS + BWD_FRAGS) * FRAG_HALFS * 2;
+constexpr size_t L_XIN = L_BIAS + N_BIAS * 4;
+constexpr size_t L_XB_DZ = L_XIN + CHUNK * N_IN * 4;
+constexpr size_t L_XB_X = L_XB_DZ + size_t(HIDDEN) * XB_PITCH * 2;
+constexpr size_t L_RED = L_XB_X + size_t(HIDDEN) * XB_PITCH * 2;
+constexpr size_t TRAIN_LDS = L_RED + 64;  // 130,816 bytes
+// the head's staging (raw outputs, then their gradients as halfs) aliases the exchange rows,
+// which are free between a chunk's last weight-gradient read and its first exchange write
+constexpr size_t L_HEAD = L_XB_DZ;
+constexpr size_t L_DZ5 = L_HEAD + size_t(TR_WAVES) * 16 * N_OUT * 4;
+static_assert(L_DZ5 + size_t(TR_WAVES) * 16 * N_OUT_PAD * 2 <= L_RED, "head staging inside the exchange rows");
+static_assert(TRAIN_LDS <= 160 * 1024, "a CU's LDS");
+static_assert(L_XIN % 16 == 0 && L_HEAD % 16 == 0 && L_DZ5 % 16 == 0 && L_XB_DZ % 16 == 0 && L_XB_X % 16 == 0 &&
+                  (XB_PITCH * 2) % 16 == 0,
+              "16-byte sections");
+constexpr int XIN_PER_THREAD = (CHUNK * N_IN + TR_WAVES * 64 - 1) / (TR_WAVES * 64);
+// weight-gradient tiles per wave: TR_NW waves share an input tile of layers 1..5 and split its
+// output tiles (TR_MS of layers 1..4, TR_S5 of layer 5); layer 0's input tiles go round-robin
+constexpr int TR_NW = TR_WAVES / 4;
+constexpr int TR_MS = 4 / TR_NW;
+constexpr int TR_S5 = (3 + TR_NW - 1) / TR_NW;
+constexpr int TR_S0 = (14 + TR_WAVES - 1) / TR_WAVES;
+static_assert(TR_WAVES == 4 || TR_WAVES == 8, "four or eight waves");
+
+// The vMF-mixture head's gradient for one record, four lanes per record (lane sub takes
+// lobes 2 sub and 2 sub + 1): o the record's raw outputs, d the sampled direction, wr the
+// record's weight (radiance / pdf over the batch mean, at most 64).  Writes d loss / d o
+// (scaled by wr) for the lane's ten outputs as halfs; returns log q.
+__device__ __forceinline__ float mixture_grad4(const float* o, const float* d, float wr, int sub, _Float16* g_out) {
+    float a[2], e[2], w[2], kap[2], sig[2], cosv[2], nm[2], mu[2][3], vv[2], om[2];
+    bool kcap[2];
 #pragma unroll
-        for (int k = 0; k < N_OUT_PAD; k++) g[k] = 0.0f;
-        if (r < n) {
-            const float total = *csum;
-            const float cmean = total / (float)n;
-            const uint32_t src = map(r);
-            const float wr = cmean > 0.0f ? fminf(c[src] / cmean, 64.0f) : 0.0f;
-            if (wr > 0.0f) {
-                float o[N_OUT_PAD];
-                const float4* orow = reinterpret_cast<const float4*>(out + (size_t)r * N_OUT_PAD);
+    for (int t = 0; t < 2; t++) a[t] = o[5 * (2 * sub + t) + 4];
+    float amax = fmaxf(a[0], a[1]);
+    amax = fmaxf(amax, __shfl_xor(amax, 1));
+    amax = fmaxf(amax, __shfl_xor(amax, 2));
 #pragma unroll
-                for (int k = 0; k < N_OUT_PAD / 4; k++) {
-                    const float4 v = orow[k];
-                    o[4 * k] = v.x;
-                    o[4 * k + 1] = v.y;
-                    o[4 * k + 2] = v.z;
-                    o[4 * k + 3] = v.w;
-                }
-                const float lq = mixture_grad(o, dirs + (size_t)src * 3, wr, g);
-                lterm = -(c[src] / total) * lq;
+    for (int t = 0; t < 2; t++) e[t] = hexp(a[t] - amax);
+    float ws = e[0] + e[1];
+    ws += __shfl_xor(ws, 1);
+    ws += __shfl_xor(ws, 2);
+    float qp = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const float* oj = o + 5 * (2 * sub + t);
+        w[t] = e[t] * hrcp(ws);
+        nm[t] = fmaxf(sqrtf(oj[0] * oj[0] + oj[1] * oj[1] + oj[2] * oj[2]), 1e-12f);
+        const float inm = hrcp(nm[t]);
+        mu[t][0] = oj[0] * inm;
+        mu[t][1] = oj[1] * inm;
+        mu[t][2] = oj[2] * inm;
+        const float sp = hsoftplus(oj[3]) + KAPPA_MIN;
+        kcap[t] = sp >= KAPPA_MAX;
+        kap[t] = fminf(sp, KAPPA_MAX);
+        sig[t] = hrcp(1.0f + hexp(-oj[3]));
+        cosv[t] = mu[t][0] * d[0] + mu[t][1] * d[1] + mu[t][2] * d[2];
+        // 1 - e^(-2 kappa) in FP32 keeps ~5e-6 relative accuracy down to kappa = KAPPA_MIN
+        om[t] = 1.0f - hexp(-2.0f * kap[t]);
+        const float norm = kap[t] * hrcp(2.0f * 3.14159265f * om[t]);
+        vv[t] = norm * hexp(kap[t] * (cosv[t] - 1.0f));
+        qp += w[t] * vv[t];
+    }
+    float q = qp + __shfl_xor(qp, 1);
+    q += __shfl_xor(q, 2);
+    q = fmaxf(q, 1e-30f);
+    const float gq = -wr;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const float r = w[t] * vv[t] * hrcp(q);
+        const float dlv = hrcp(kap[t]) - 2.0f * (1.0f - om[t]) * hrcp(om[t]) + (cosv[t] - 1.0f);
+        float gm[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) gm[c] = gq * r * kap[t] * d[c];
+        const float p = gm[0] * mu[t][0] + gm[1] * mu[t][1] + gm[2] * mu[t][2];
+        _Float16* gj = g_out + 5 * (2 * sub + t);
+#pragma unroll
+        for (int c = 0; c < 3; c++) gj[c] = (_Float16)clamp_h((gm[c] - p * mu[t][c]) * hrcp(nm[t]));
+        gj[3] = (_Float16)clamp_h(kcap[t] ? 0.0f : gq * r * dlv * sig[t]);
+        gj[4] = (_Float16)clamp_h(gq * (r - w[t]));
+    }
+    return hlog(q);
+}
+
+// One training step's gradients.  Block (x, net) takes chunks of 128 records of network net
+// (wave w records 16 w .. 16 w + 15 of a chunk): forward with the inputs of layers 1..5 kept
+// in registers, the head's gradient, the data gradients down the layers, and, per layer, the
+// weight gradient dW_l += dZ_l^T X_l over the chunk's records: the waves exchange dZ_l and
+// X_l through LDS (feature-major, a record per column) and each accumulates its share of
+// dW's 16 x 16 tiles in registers across all its chunks (layer 0: input tiles w and w + 8,
+// all output tiles; the other layers: input tile w % 4 and half of the output tiles), the
+// bias gradients as row sums.  The next chunk's inputs and the head's directions and
+// weights are loaded while the current chunk computes.  Every block writes its full partial
+// gradient (and loss) to part[net][x]; k_adam_pack adds them.
+__global__ __launch_bounds__(TR_WAVES * 64) __attribute__((amdgpu_waves_per_eu(TR_NW, TR_NW))) void k_train(
+    const _Float16* __restrict__ pack, const float* __restrict__ bias, const float* __restrict__ inputs,
+    const float* __restrict__ dirs, const float* __restrict__ cw, train_segs segs,
+    const float* __restrict__ csum_part, const uint32_t* __restrict__ idx, float* __restrict__ part,
+    float* __restrict__ loss_part) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    _Float16* wl = reinterpret_cast<_Float16*>(smem + L_FRAGS);
+    float* bl = reinterpret_cast<float*>(smem + L_BIAS);
+    float* xin = reinterpret_cast<float*>(smem + L_XIN);
+    float* red = reinterpret_cast<float*>(smem + L_RED);
+    _Float16* xb_dz = reinterpret_cast<_Float16*>(smem + L_XB_DZ);
+    _Float16* xb_x = reinterpret_cast<_Float16*>(smem + L_XB_X);
+    const uint32_t net = blockIdx.y;
+    const train_seg sg = segs.s[net];
+    const uint32_t n = sg.n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    __builtin_assume(g >= 0 && g < 4);
+    const int nq = wave & 3;              // input tile of layers 1..5
+    const int mh = TR_MS * (wave >> 2);   // first of the TR_MS output tiles of layers 1..4
+    const int m5 = wave >> 2;             // output tiles m5 + TR_NW s (< 3) of layer 5
+    // the bias gradients (row sums of dZ) of output tile m are added by the wave with m % 4 == wave % 4
+    // among those that read tile m (layers 1..5: the waves with input tile 0 .. TR_NW - 1 wrap)
+    float* hs = reinterpret_cast<float*>(smem + L_HEAD) + wave * 16 * N_OUT;
+    _Float16* dz5s = reinterpret_cast<_Float16*>(smem + L_DZ5) + wave * 16 * N_OUT_PAD;
+    {
+        const h8* src = reinterpret_cast<const h8*>(pack + (size_t)net * PACK_HALFS);
+        h8* dst = reinterpret_cast<h8*>(wl);
+        for (int q = threadIdx.x; q < (FWD_FRAGS + BWD_FRAGS) * 64; q += blockDim.x) dst[q] = src[q];
+        for (int q = threadIdx.x; q < N_BIAS; q += blockDim.x) bl[q] = bias[(size_t)net * N_BIAS + q];
+    }
+    float total = 0.0f;
+    for (uint32_t b = 0; b < SUM_BLOCKS; b++) total += csum_part[net * SUM_BLOCKS + b];
+    const float cmean = n ? total / (float)n : 0.0f;
+    f4 dw0[TR_S0][4], dwh[4][TR_MS], dw5[TR_S5];
+    float db0[4], dbh[4][TR_MS], db5[TR_S5];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        db0[m] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < TR_S0; s++) dw0[s][m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < TR_MS; s++) {
+            dwh[m][s] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            dbh[m][s] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < TR_S5; s++) {
+        dw5[s] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        db5[s] = 0.0f;
+    }
+    float lsum = 0.0f;
+    const uint32_t chunks = (n + CHUNK - 1) / CHUNK;
+    // the first chunk's inputs (each thread XIN_PER_THREAD of the chunk's 128 x 9 floats)
+    float xpre[XIN_PER_THREAD];
+    auto load_inputs = [&](uint32_t ch) {
+#pragma unroll
+        for (int k = 0; k < XIN_PER_THREAD; k++) {
+            const int q = threadIdx.x + k * TR_WAVES * 64;
+            xpre[k] = 0.0f;
+            if (q < CHUNK * N_IN && ch < chunks) {
+                const uint32_t r = ch * CHUNK + (uint32_t)(q / N_IN);
+                xpre[k] = inputs[(size_t)idx[sg.idx_off + (r < n ? r : n - 1)] * N_IN + q % N_IN];
             }
         }
-        h8* drow = reinterpret_cast<h8*>(dz + (size_t)r * N_OUT_PAD);
+    };
+    load_inputs(blockIdx.x);
+    for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const uint32_t r0 = ch * CHUNK;
 #pragma unroll
-        for (int k = 0; k < N_OUT_PAD / 8; k++) {
-            h8 v;
-#pragma unroll
-            for (int e = 0; e < 8; e++) v[e] = (_Float16)fminf(fmaxf(g[8 * k + e], -30000.0f), 30000.0f);
-            drow[k] = v;
+        for (int k = 0; k < XIN_PER_THREAD; k++) {
+            const int q = threadIdx.x + k * TR_WAVES * 64;
+            if (q < CHUNK * N_IN) xin[q] = xpre[k];
         }
-    }
-    for (int off = 32; off > 0; off >>= 1) lterm += __shfl_xor(lterm, off);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = lterm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const float t = part[0] + part[1] + part[2] + part[3];
-        if (t != 0.0f) atomicAdd(loss, t);
-    }
-}
-
-// dZ_{l-1} = (dZ_l W_l) masked by the ReLU of layer l's input.  The product is formed
-// transposed, dZ_{l-1}^T = W_l^T dZ_l^T (A = W_l^T, B = dZ_l^T), so that a lane's four
-// accumulators are four consecutive features of one row: 8-byte loads of the stored
-// activations and 8-byte stores of the result.
-__global__ __launch_bounds__(256) void k_backward_data(const _Float16* __restrict__ w16, int l,
-                                                       const _Float16* __restrict__ dz_in, int in_stride,
-                                                       const _Float16* __restrict__ acts, uint32_t n_pad,
-                                                       _Float16* __restrict__ dz_out) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int li = lane & 15, lg = lane >> 4;
-    const int K = layer_out(l);     // reduction over layer l's outputs
-    const int N = layer_in(l);      // 64
-    const _Float16* W = w16 + layer_w_offset(l);
-    // the B fragments (W, at most 4 x 4 tiles) stay in registers for all of the wave's rows
-    h4 wb[4][4];
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                wb[s][t][i] = s < K / 16 ? W[(16 * s + 4 * lg + i) * N + 16 * t + li] : (_Float16)0.0f;
-    const uint32_t nwaves = gridDim.x * 4;
-    for (uint32_t chunk = blockIdx.x * 4 + wave; chunk * 16 < n_pad; chunk += nwaves) {
-        const uint32_t r0 = chunk * 16;
+        // the head's record data (four lanes per record), then the next chunk's inputs
+        const int rec = lane >> 2, sub = lane & 3;
+        const uint32_t rh = r0 + 16 * wave + rec;
+        const bool valid = rh < n;
+        const uint32_t src = idx[sg.idx_off + (valid ? rh : n - 1)];
+        const float d[3] = {dirs[(size_t)src * 3], dirs[(size_t)src * 3 + 1], dirs[(size_t)src * 3 + 2]};
+        const float c = cw[src];
+        load_inputs(ch + gridDim.x);
+        __syncthreads();
+        // ---- forward: the wave's 16 records
+        const float* xr = xin + (16 * wave + li) * N_IN;
+        h8 X[6][2];  // X[l]: the input of layer l (l = 1..5) as B fragments
         f4 acc[4];
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int m = 0; m < 4; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            if (s < K / 16) {
-                // B fragment of dZ_l^T: k = 16 s + 4 lg + i (a feature), column li (a row)
-                const h4 b = load_h4(dz_in + (size_t)(r0 + li) * in_stride + 16 * s + 4 * lg);
+        for (int kb = 0; kb < 7; kb++) {
+            const int v = encode_input(kb, g);
+            const h8 b = encode_frag(v < N_IN ? xr[v] : 0.0f, kb, g);
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = mfma16(wb[s][t], b, acc[t]);
-            }
+            for (int m = 0; m < 4; m++) acc[m] = mfma32(frag(wl, fwd_frag(0, m, kb), lane), b, acc[m]);
         }
-        // acc[t][j] = dZ_{l-1}[row r0 + li][feature 16 t + 4 lg + j]
-        const uint32_t r = r0 + li;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const h4 a = load_h4(acts + (size_t)r * ACT_STRIDE + act_offset(l) + 16 * t + 4 * lg);
-            h4 o;
+        for (int l = 1; l < N_LAYERS; l++) {
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                o[j] = (_Float16)((float)a[j] > 0.0f ? fminf(fmaxf(acc[t][j], -30000.0f), 30000.0f) : 0.0f);
-            *reinterpret_cast<h4*>(dz_out + (size_t)r * HIDDEN + 16 * t + 4 * lg) = o;
-        }
-    }
-}
-
-// Weight gradient of layer L over a block's rows: grad(W)[n][k] = sum_r dZ[r][n] in[r][k],
-// grad(b)[n] = sum_r dZ[r][n].  The rows are the MFMA reduction, so a lane's operand is
-// four consecutive rows of one column: the block stages 64-row slabs of dZ and of the
-// layer input in LDS transposed (row index fastest; pitch 36 dwords, so the 8-byte
-// operand reads of a half-wave hit distinct banks).  Layer 0's input, the frequency
-// encoding, is recomputed from the 9 raw inputs.  Output tiles (N/16 x (K/16 + 1), the
-// last k tile the bias as a column of ones) are dealt to the 4 waves; each block
-// writes its partial sums (weights then biases, the parameter layout of the layer)
-// to part[blockIdx.x], and k_reduce_partials adds them into grad.
-constexpr int BW_ROWS = 64;
-constexpr int BW_PITCH = BW_ROWS + 8;  // halfs
-constexpr uint32_t BW_MAX_BLOCKS = 512;
-__host__ __device__ constexpr int bw_count(int l) { return layer_out(l) * layer_in(l) + layer_out(l); }
-
-template <int L>
-__global__ __launch_bounds__(256) void k_backward_weight(const _Float16* __restrict__ dz,
-                                                         const _Float16* __restrict__ acts,
-                                                         const float* __restrict__ inputs, uint32_t n,
-                                                         uint32_t n_pad, uint32_t rows_per_block, sample_map map,
-                                                         float* __restrict__ part) {
-    constexpr int K = layer_in(L), N = layer_out(L);
-    constexpr int KT = K / 16 + 1, NT = N / 16, T = NT * KT, TPW = (T + 3) / 4;
-    __shared__ __align__(16) _Float16 actT[K * BW_PITCH];
-    __shared__ __align__(16) _Float16 dzT[N * BW_PITCH];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int li = lane & 15, lg = lane >> 4;
-    f4 acc[TPW];
+            for (int m = 0; m < 4; m++)
 #pragma unroll
-    for (int j = 0; j < TPW; j++) acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const uint32_t rbeg = blockIdx.x * rows_per_block;
-    const uint32_t rend = rbeg + rows_per_block < n_pad ? rbeg + rows_per_block : n_pad;
-    const h8 zero8 = h8{0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t s0 = rbeg; s0 < rend; s0 += BW_ROWS) {
-        // stage the slab: lane = row, waves take turns over 8-column groups
-        const uint32_t r = s0 + (uint32_t)lane;
-        const bool valid = r < rend;  // rows past the end: zero dZ, no contribution
+                for (int j = 0; j < 4; j++) {
+                    const float z = acc[m][j] + bl[bias_offset(l - 1) + 16 * m + 4 * g + j];
+                    X[l][m >> 1][(m & 1) * 4 + j] = (_Float16)(z > 0.0f ? z : 0.0f);
+                }
+            const int mt = l == N_LAYERS - 1 ? 3 : 4;
 #pragma unroll
-        for (int c8 = wave; c8 < N / 8; c8 += 4) {
-            const h8 v = valid ? *reinterpret_cast<const h8*>(dz + (size_t)r * N + 8 * c8) : zero8;
+            for (int m = 0; m < 4; m++) {
+                acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (m < mt) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) dzT[(8 * c8 + e) * BW_PITCH + lane] = v[e];
-        }
-        if (L == 0) {
-            const uint32_t src = map(valid && r < n ? r : n - 1);
-            float x[N_IN];
-#pragma unroll
-            for (int i = 0; i < N_IN; i++) x[i] = inputs[(size_t)src * N_IN + i];
-            for (int q = wave; q < K / 4; q += 4) {
-                const h4 a = encode4(x, 4 * q);
-#pragma unroll
-                for (int e = 0; e < 4; e++) actT[(4 * q + e) * BW_PITCH + lane] = a[e];
-            }
-        } else {
-#pragma unroll
-            for (int c8 = wave; c8 < K / 8; c8 += 4) {
-                const h8 v = valid ? *reinterpret_cast<const h8*>(acts + (size_t)r * ACT_STRIDE + act_offset(L) + 8 * c8)
-                                   : zero8;
-#pragma unroll
-                for (int e = 0; e < 8; e++) actT[(8 * c8 + e) * BW_PITCH + lane] = v[e];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int st = 0; st < BW_ROWS / 16; st++) {
-#pragma unroll
-            for (int j = 0; j < TPW; j++) {
-                const int ti = wave + 4 * j;
-                if (ti < T) {
-                    const int tn = ti / KT, tk = ti % KT;
-                    const h4 a = load_h4(&dzT[(16 * tn + li) * BW_PITCH + 16 * st + 4 * lg]);
-                    h4 b;
-                    if (tk == KT - 1) {
-                        const _Float16 one = (_Float16)(li == 0 ? 1.0f : 0.0f);
-                        b = h4{one, one, one, one};
-                    } else {
-                        b = load_h4(&actT[(16 * tk + li) * BW_PITCH + 16 * st + 4 * lg]);
-                    }
-                    acc[j] = mfma16(a, b, acc[j]);
+                    for (int kb = 0; kb < 2; kb++) acc[m] = mfma32(frag(wl, fwd_frag(l, m, kb), lane), X[l][kb], acc[m]);
                 }
             }
         }
-        __syncthreads();
-    }
-    float* dst = part + (size_t)blockIdx.x * bw_count(L);
+        // ---- head: raw outputs to the wave's staging rows, then four lanes per record
 #pragma unroll
-    for (int j = 0; j < TPW; j++) {
-        const int ti = wave + 4 * j;
-        if (ti >= T) break;
-        const int tn = ti / KT, tk = ti % KT;
+        for (int m = 0; m < 3; m++)
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int nrow = 16 * tn + 4 * lg + e;
-            if (tk == KT - 1) {
-                if (li == 0) dst[N * K + nrow] = acc[j][e];
-            } else {
-                dst[nrow * K + 16 * tk + li] = acc[j][e];
+            for (int j = 0; j < 4; j++) {
+                const int col = 16 * m + 4 * g + j;
+                if (col < N_OUT) hs[li * N_OUT + col] = acc[m][j] + bl[bias_offset(5) + col];
+            }
+        wave_sync();
+        {
+            const float wr = valid && cmean > 0.0f ? fminf(c / cmean, 64.0f) : 0.0f;
+            _Float16* grow = dz5s + rec * N_OUT_PAD;
+            const float lq = mixture_grad4(hs + rec * N_OUT, d, wr, sub, grow);
+            if (sub == 3)
+#pragma unroll
+                for (int k = N_OUT; k < N_OUT_PAD; k++) grow[k] = (_Float16)0.0f;
+            if (sub == 0 && wr > 0.0f) lsum += -(c / total) * lq;
+        }
+        wave_sync();
+        // dZ_5 as B fragments (K = the head's outputs in the chained order, N = records)
+        h8 dz[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int f = hid_feat(kb, g, i);
+                dz[kb][i] = f < N_OUT_PAD ? dz5s[li * N_OUT_PAD + f] : (_Float16)0.0f;
+            }
+        __syncthreads();  // every wave's head staging is read before the exchange rows are written
+        // ---- backward, one layer at a time from the head: exchange dZ_l and X_l, the next
+        // data gradient, then the weight-gradient tiles
+#pragma unroll
+        for (int l = N_LAYERS - 1; l >= 0; l--) {
+            const int mo = l == N_LAYERS - 1 ? 3 : 4;  // output tiles of layer l
+            const int col = 16 * wave + li;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int f = hid_feat(kb, g, i);
+                    if (f < 16 * mo) xb_dz[f * XB_PITCH + col] = dz[kb][i];
+                    if (l >= 1) xb_x[f * XB_PITCH + col] = X[l][kb][i];
+                }
+            h8 nd[2];
+            if (l >= 1) {
+                f4 a[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    a[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int kb = 0; kb < 2; kb++) a[m] = mfma32(frag(wl, bwd_frag(l, m, kb), lane), dz[kb], a[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int k = (m & 1) * 4 + j;
+                        nd[m >> 1][k] = X[l][m >> 1][k] > (_Float16)0.0f ? (_Float16)clamp_h(a[m][j]) : (_Float16)0.0f;
+                    }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const int kofs = 32 * kb + 8 * g;
+                if (l >= 1) {
+                    const h8 B = lds_h8(xb_x + (16 * nq + li) * XB_PITCH + kofs);
+#pragma unroll
+                    for (int s = 0; s < (l == N_LAYERS - 1 ? TR_S5 : TR_MS); s++) {
+                        const int m = l == N_LAYERS - 1 ? m5 + TR_NW * s : mh + s;
+                        if (m < mo) {
+                            const h8 A = lds_h8(xb_dz + (16 * m + li) * XB_PITCH + kofs);
+                            float rs = 0.0f;
+                            if ((m & 3) == nq) {
+#pragma unroll
+                                for (int i = 0; i < 8; i++) rs += (float)A[i];
+                            }
+                            if (l == N_LAYERS - 1) {
+                                dw5[s] = mfma32(A, B, dw5[s]);
+                                db5[s] += rs;
+                            } else {
+                                dwh[l - 1][s] = mfma32(A, B, dwh[l - 1][s]);
+                                dbh[l - 1][s] += rs;
+                            }
+                        }
+                    }
+                } else {
+                    // layer 0: B is the encoding, feature 16 nt + li (the parameter layout's
+                    // order, 24 v + 2 k + (0 sin | 1 cos)) of records 32 kb + 8 g + i
+                    h8 A[4];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        A[m] = lds_h8(xb_dz + (16 * m + li) * XB_PITCH + kofs);
+                        if (m == wave % 4 && wave < 4) {
+                            float rs = 0.0f;
+#pragma unroll
+                            for (int i = 0; i < 8; i++) rs += (float)A[m][i];
+                            db0[m] += rs;
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < TR_S0; s++) {
+                        const int nt = wave + TR_WAVES * s;
+                        if (nt >= 14) break;
+                        const int f = 16 * nt + li;
+                        h8 B;
+                        if (f < N_FEAT) {
+                            const int v = f / 24, k = (f % 24) >> 1;
+                            // cos(2 pi t) = sin(2 pi (t + 1/4)): one transcendental per lane
+                            const float shift = (f & 1) ? 0.25f : 0.0f;
+                            const float scale = (float)(1 << k) * 0.5f;
+#pragma unroll
+                            for (int i = 0; i < 8; i++) {
+                                float t = xin[(kofs + i) * N_IN + v] * scale;
+                                t = t - floorf(t) + shift;
+                                B[i] = (_Float16)sin_rev(t - floorf(t));
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 8; i++) B[i] = (_Float16)0.0f;
+                        }
+#pragma unroll
+                        for (int m = 0; m < 4; m++) dw0[s][m] = mfma32(A[m], B, dw0[s][m]);
+                    }
+                }
+            }
+            __syncthreads();
+            if (l >= 1) {
+                dz[0] = nd[0];
+                dz[1] = nd[1];
             }
         }
     }
+    // ---- this block's partial gradient (every parameter written) and loss
+    float* P = part + ((size_t)net * gridDim.x + blockIdx.x) * N_PARAMS;
+#pragma unroll
+    for (int s = 0; s < TR_S0; s++) {
+        const int nt = wave + TR_WAVES * s;
+        if (nt >= 14) break;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                P[layer_w_offset(0) + (16 * m + 4 * g + j) * N_FEAT_PAD + 16 * nt + li] = dw0[s][m][j];
+    }
+#pragma unroll
+    for (int l = 1; l < N_LAYERS; l++)
+#pragma unroll
+        for (int s = 0; s < (l == N_LAYERS - 1 ? TR_S5 : TR_MS); s++) {
+            const int m = l == N_LAYERS - 1 ? m5 + TR_NW * s : mh + s;
+            if (m >= (l == N_LAYERS - 1 ? 3 : 4)) continue;
+            const f4 t = l == N_LAYERS - 1 ? dw5[s] : dwh[l - 1][s];
+#pragma unroll
+            for (int j = 0; j < 4; j++) P[layer_w_offset(l) + (16 * m + 4 * g + j) * HIDDEN + 16 * nq + li] = t[j];
+            if ((m & 3) == nq) {
+                float v = l == N_LAYERS - 1 ? db5[s] : dbh[l - 1][s];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (g == 0) P[layer_b_offset(l) + 16 * m + li] = v;
+            }
+        }
+    if (wave < 4) {
+        const int m = wave;
+        float v = db0[m];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (g == 0) P[layer_b_offset(0) + 16 * m + li] = v;
+    }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0) red[wave] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < TR_WAVES; w++) t += red[w];
+        loss_part[(size_t)net * gridDim.x + blockIdx.x] = t;
+    }
 }
 
-// grad[layer_w_offset(l) + p] += sum over blocks of part[b][p]; blockIdx.y takes a
-// group of BW_GROUP blocks
-constexpr uint32_t BW_GROUP = 16;
-__global__ void k_reduce_partials(const float* __restrict__ part, uint32_t nblocks, uint32_t count,
-                                  float* __restrict__ grad) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
-    const uint32_t b0 = blockIdx.y * BW_GROUP;
-    const uint32_t b1 = b0 + BW_GROUP < nblocks ? b0 + BW_GROUP : nblocks;
-    float s = 0.0f;
-    for (uint32_t b = b0; b < b1; b++) s += part[(size_t)b * count + p];
-    atomicAdd(&grad[p], s);
-}
+// ------------------------------------------------------------------ Adam and packing
 
 __device__ __forceinline__ bool live_param(uint32_t p) {
     // padded input columns of layer 0 and padded outputs of the last layer stay zero
@@ -487,135 +655,251 @@ __device__ __forceinline__ bool live_param(uint32_t p) {
     return true;
 }
 
-__global__ void k_adam(float* __restrict__ w32, _Float16* __restrict__ w16, float* __restrict__ m,
-                       float* __restrict__ v, float* __restrict__ grad, float inv_n, float lr, float b1, float b2,
-                       float eps, float c1, float c2, uint32_t apply) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (uint32_t)N_PARAMS) return;
-    if (!apply) return;
-    if (!live_param(p)) {
-        grad[p] = 0.0f;
-        return;
-    }
-    const float g = grad[p] * inv_n;
-    const float mm = b1 * m[p] + (1.0f - b1) * g;
-    const float vv = b2 * v[p] + (1.0f - b2) * g * g;
-    m[p] = mm;
-    v[p] = vv;
-    const float w = w32[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-    w32[p] = w;
-    w16[p] = (_Float16)w;
-    grad[p] = 0.0f;
+// position (K block, lane group, element) of a chained layer's input feature f
+__device__ __forceinline__ void chained_pos(int f, int& kb, int& g, int& i) {
+    kb = f >> 5;
+    const int r = f & 31;
+    g = (r & 15) >> 2;
+    i = (r & 3) + (r >= 16 ? 4 : 0);
 }
 
-__global__ void k_to_half(const float* __restrict__ w32, _Float16* __restrict__ w16) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < (uint32_t)N_PARAMS) w16[p] = (_Float16)w32[p];
+// writes parameter p's half (and FP32 bias) where the kernels read it
+__device__ __forceinline__ void pack_param(uint32_t p, float w, _Float16* __restrict__ pk, float* __restrict__ bs) {
+    int l = 0;
+    while (l < N_LAYERS - 1 && p >= (uint32_t)layer_w_offset(l + 1)) l++;
+    const uint32_t wo = layer_w_offset(l), bo = layer_b_offset(l);
+    if (p >= bo) {
+        bs[bias_offset(l) + (p - bo)] = w;
+        return;
+    }
+    const int in = layer_in(l);
+    const int o = (int)((p - wo) / in), i = (int)((p - wo) % in);
+    const _Float16 h = (_Float16)w;
+    int kb, g, e;
+    if (l == 0) {
+        // feature i = 24 v + 2 k + sc sits in quad Q = 3 v + k / 4 (kb = Q / 4, g = Q % 4) at 2 (k % 4) + sc
+        const int v = i / 24, k = (i % 24) >> 1, sc = i & 1;
+        const int Q = 3 * v + (k >> 2);
+        kb = Q >> 2;
+        g = Q & 3;
+        e = 2 * (k & 3) + sc;
+    } else {
+        chained_pos(i, kb, g, e);
+    }
+    pk[(size_t)(fwd_frag(l, o >> 4, kb) * 64 + g * 16 + (o & 15)) * 8 + e] = h;
+    if (l >= 1) {
+        // W_l^T: row = input feature i, K = output feature o in the chained order
+        chained_pos(o, kb, g, e);
+        pk[(size_t)(bwd_frag(l, i >> 4, kb) * 64 + g * 16 + (i & 15)) * 8 + e] = h;
+    }
+}
+
+// mode 0: pack w32 into the kernels' layout; 1: gradient (sum of the blocks' partials over
+// n) and Adam step, then pack; 2: gradient only.  A block takes 8 parameter quads: its 256
+// threads add the partial rows of a quad in 32 interleaved splits with 16-byte loads (enough
+// loads in flight to stream the partials), then one thread per quad adds the splits and
+// updates the four parameters.  Block (0, net) also adds the loss partials.
+constexpr int RED_QUADS = 8;
+constexpr int RED_SPLITS = 32;
+constexpr uint32_t ADAM_BLOCKS = (N_PARAMS / 4 + RED_QUADS - 1) / RED_QUADS;
+static_assert(N_PARAMS % 4 == 0, "parameter quads");
+__global__ __launch_bounds__(RED_QUADS * RED_SPLITS) void k_adam_pack(
+    float* __restrict__ w32, float* __restrict__ m, float* __restrict__ v, _Float16* __restrict__ pack,
+    float* __restrict__ bias, float* __restrict__ grad, float* __restrict__ loss, const float* __restrict__ part,
+    const float* __restrict__ loss_part, uint32_t blocks, train_segs segs, float lr, float b1, float b2, float eps,
+    uint32_t mode) {
+    __shared__ float4 red[RED_SPLITS][RED_QUADS];
+    const uint32_t net = blockIdx.y;
+    const train_seg sg = segs.s[net];
+    if (mode != 0 && sg.n == 0) return;
+    const int qi = threadIdx.x % RED_QUADS, sp = threadIdx.x / RED_QUADS;
+    const uint32_t p0 = (blockIdx.x * RED_QUADS + qi) * 4;
+    if (mode != 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        float s = 0.0f;
+        for (uint32_t b = 0; b < blocks; b++) s += loss_part[(size_t)net * blocks + b];
+        loss[net] = s;
+    }
+    if (mode != 0) {
+        float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (p0 < (uint32_t)N_PARAMS) {
+            const float* pp = part + (size_t)net * blocks * N_PARAMS + p0;
+#pragma unroll 4
+            for (uint32_t b = sp; b < blocks; b += RED_SPLITS) {
+                const float4 t = *reinterpret_cast<const float4*>(pp + (size_t)b * N_PARAMS);
+                gs.x += t.x;
+                gs.y += t.y;
+                gs.z += t.z;
+                gs.w += t.w;
+            }
+        }
+        red[sp][qi] = gs;
+        __syncthreads();
+    }
+    if (sp != 0 || p0 >= (uint32_t)N_PARAMS) return;
+    float gq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (mode != 0) {
+        for (int k = 0; k < RED_SPLITS; k++) {
+            const float4 t = red[k][qi];
+            gq[0] += t.x;
+            gq[1] += t.y;
+            gq[2] += t.z;
+            gq[3] += t.w;
+        }
+    }
+    float* W = w32 + (size_t)net * N_PARAMS;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint32_t p = p0 + e;
+        float w = W[p];
+        const bool live = live_param(p);
+        if (mode != 0) {
+            const float gr = live ? gq[e] / (float)sg.n : 0.0f;
+            grad[(size_t)net * N_PARAMS + p] = gr;
+            if (mode == 2 || !live) continue;
+            const size_t q = (size_t)net * N_PARAMS + p;
+            const float mm = b1 * m[q] + (1.0f - b1) * gr;
+            const float vv = b2 * v[q] + (1.0f - b2) * gr * gr;
+            m[q] = mm;
+            v[q] = vv;
+            w = w - lr * (mm / sg.c1) / (sqrtf(vv / sg.c2) + eps);
+            W[p] = w;
+        }
+        if (live) pack_param(p, w, pack + (size_t)net * PACK_HALFS, bias + (size_t)net * N_BIAS);
+    }
 }
 
 // ------------------------------------------------------------------ host
 
 void model_free(model& M) {
-    for (void* p : {(void*)M.w16, (void*)M.w32, (void*)M.m, (void*)M.v, (void*)M.grad, (void*)M.loss,
-                    (void*)M.acts, (void*)M.dz, (void*)M.out, (void*)M.part})
+    for (void* p : {(void*)M.w32, (void*)M.m, (void*)M.v, (void*)M.pack, (void*)M.bias, (void*)M.grad,
+                    (void*)M.loss, (void*)M.part, (void*)M.loss_part, (void*)M.csum_part, (void*)M.idx})
         if (p) (void)hipFree(p);
     M = model();
 }
 
-void model_set(model& M, const float* host, hipStream_t s) {
-    GCHECK(hipMemcpyAsync(M.w32, host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_to_half, dim3((N_PARAMS + 255) / 256), dim3(256), 0, s, M.w32, M.w16);
-    GCHECK(hipMemsetAsync(M.m, 0, sizeof(float) * N_PARAMS, s));
-    GCHECK(hipMemsetAsync(M.v, 0, sizeof(float) * N_PARAMS, s));
-    GCHECK(hipMemsetAsync(M.grad, 0, sizeof(float) * N_PARAMS, s));
-    M.step = 0;
+static uint32_t train_blocks(uint32_t nets) {
+    const uint32_t b = 256 / (nets ? nets : 1);
+    return b < 4 ? 4 : b;
 }
 
-void model_get(model& M, float* host, hipStream_t s) {
-    GCHECK(hipMemcpyAsync(host, M.w32, sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, s));
+static train_segs no_segs() {
+    train_segs t;
+    std::memset(&t, 0, sizeof(t));
+    return t;
+}
+
+static void pack_net(model& M, uint32_t net, hipStream_t s) {
+    train_segs t = no_segs();
+    hipLaunchKernelGGL(k_adam_pack, dim3(ADAM_BLOCKS, net + 1), dim3(RED_QUADS * RED_SPLITS), 0, s, M.w32, M.m, M.v, M.pack,
+                       M.bias, M.grad, M.loss, M.part, M.loss_part, train_blocks(M.nets), t, 0.0f, 0.0f, 0.0f, 0.0f,
+                       0u);
+    GCHECK(hipGetLastError());
+}
+
+void model_set(model& M, const float* host, hipStream_t s, uint32_t net) {
+    if (net >= M.nets) throw std::runtime_error("no such network");
+    const size_t off = (size_t)net * N_PARAMS;
+    GCHECK(hipMemcpyAsync(M.w32 + off, host, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, s));
+    GCHECK(hipMemsetAsync(M.m + off, 0, sizeof(float) * N_PARAMS, s));
+    GCHECK(hipMemsetAsync(M.v + off, 0, sizeof(float) * N_PARAMS, s));
+    pack_net(M, net, s);  // packs networks 0..net (unchanged ones to the same values)
+    M.step[net] = 0;
+}
+
+void model_get(model& M, float* host, hipStream_t s, uint32_t net) {
+    if (net >= M.nets) throw std::runtime_error("no such network");
+    GCHECK(hipMemcpyAsync(host, M.w32 + (size_t)net * N_PARAMS, sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, s));
     GCHECK(hipStreamSynchronize(s));
 }
 
-void model_init(model& M, uint32_t seed, hipStream_t s) {
-    if (!M.w32) {
-        GCHECK(hipMalloc(&M.w16, sizeof(_Float16) * N_PARAMS));
-        GCHECK(hipMalloc(&M.w32, sizeof(float) * N_PARAMS));
-        GCHECK(hipMalloc(&M.m, sizeof(float) * N_PARAMS));
-        GCHECK(hipMalloc(&M.v, sizeof(float) * N_PARAMS));
-        GCHECK(hipMalloc(&M.grad, sizeof(float) * N_PARAMS));
-        GCHECK(hipMalloc(&M.loss, sizeof(float) * 4));
-    }
-    // He-uniform, zero biases; the last layer at half scale
-    std::vector<float> p(N_PARAMS, 0.0f);
-    pcg32 g;
-    g.seed(seed);
-    for (int l = 0; l < N_LAYERS; l++) {
-        const int fan_in = l == 0 ? N_FEAT : HIDDEN;
-        const int outs = l == N_LAYERS - 1 ? N_OUT : HIDDEN;
-        const float lim = std::sqrt(6.0f / fan_in) * (l == N_LAYERS - 1 ? 0.5f : 1.0f);
-        for (int o = 0; o < outs; o++)
-            for (int i = 0; i < (l == 0 ? N_FEAT : HIDDEN); i++)
-                p[layer_w_offset(l) + o * layer_in(l) + i] = (2.0f * g.uniform() - 1.0f) * lim;
-    }
-    model_set(M, p.data(), s);
-    GCHECK(hipStreamSynchronize(s));
-}
-
-void model_ensure(model& M, uint32_t rows) {
-    const uint32_t pad = (rows + 15) & ~15u;
-    if (pad <= M.cap && M.acts) return;
-    for (void* q : {(void*)M.acts, (void*)M.dz, (void*)M.out})
-        if (q) (void)hipFree(q);
-    GCHECK(hipMalloc(&M.acts, sizeof(_Float16) * (size_t)pad * ACT_STRIDE));
-    GCHECK(hipMalloc(&M.dz, sizeof(_Float16) * (size_t)pad * (2 * HIDDEN + N_OUT_PAD)));
-    GCHECK(hipMalloc(&M.out, sizeof(float) * (size_t)pad * N_OUT_PAD));
-    if (!M.part) GCHECK(hipMalloc(&M.part, sizeof(float) * (size_t)BW_MAX_BLOCKS * bw_count(0)));
-    M.cap = pad;
-}
-
-// the forward kernel's dynamic LDS (78 KiB) is above the default limit of a launch
-static void allow_forward_lds() {
+static void allow_lds() {
     static bool done = false;
     if (done) return;
-    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forward<false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(FWD_LDS_BYTES)));
-    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forward<true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(FWD_LDS_BYTES)));
+    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(INF_LDS)));
+    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(TRAIN_LDS)));
     done = true;
 }
 
-// persistent forward blocks: two fit a CU's 160 KiB of LDS
-static uint32_t forward_blocks(uint32_t n_pad) {
-    const uint32_t tiles = n_pad / 16;
-    const uint32_t want = (tiles + FWD_WAVES - 1) / FWD_WAVES;
-    return want < 512 ? (want ? want : 1) : 512;
+void model_init(model& M, uint32_t seed, hipStream_t s, uint32_t nets) {
+    if (nets == 0 || nets > MAX_NETS) throw std::runtime_error("a model holds 1 to 32 networks");
+    if (M.nets != nets) {
+        model_free(M);
+        M.nets = nets;
+        const size_t np = (size_t)nets * N_PARAMS;
+        GCHECK(hipMalloc(&M.w32, sizeof(float) * np));
+        GCHECK(hipMalloc(&M.m, sizeof(float) * np));
+        GCHECK(hipMalloc(&M.v, sizeof(float) * np));
+        GCHECK(hipMalloc(&M.grad, sizeof(float) * np));
+        GCHECK(hipMalloc(&M.pack, sizeof(_Float16) * (size_t)nets * PACK_HALFS));
+        GCHECK(hipMalloc(&M.bias, sizeof(float) * (size_t)nets * N_BIAS));
+        GCHECK(hipMalloc(&M.loss, sizeof(float) * nets));
+        const size_t nb = (size_t)nets * train_blocks(nets);
+        GCHECK(hipMalloc(&M.part, sizeof(float) * nb * N_PARAMS));
+        GCHECK(hipMalloc(&M.loss_part, sizeof(float) * nb));
+        GCHECK(hipMalloc(&M.csum_part, sizeof(float) * (size_t)nets * SUM_BLOCKS));
+        // the padded positions of the packed weights stay zero
+        GCHECK(hipMemsetAsync(M.pack, 0, sizeof(_Float16) * (size_t)nets * PACK_HALFS, s));
+        GCHECK(hipMemsetAsync(M.grad, 0, sizeof(float) * np, s));
+        GCHECK(hipMemsetAsync(M.loss, 0, sizeof(float) * nets, s));
+    }
+    M.step.assign(nets, 0);
+    allow_lds();
+    // He-uniform, zero biases; the last layer at half scale; network k from seed + k
+    std::vector<float> p(N_PARAMS);
+    for (uint32_t k = 0; k < nets; k++) {
+        std::fill(p.begin(), p.end(), 0.0f);
+        pcg32 gen;
+        gen.seed(seed + k);
+        for (int l = 0; l < N_LAYERS; l++) {
+            const int fan_in = l == 0 ? N_FEAT : HIDDEN;
+            const int outs = l == N_LAYERS - 1 ? N_OUT : HIDDEN;
+            const float lim = std::sqrt(6.0f / fan_in) * (l == N_LAYERS - 1 ? 0.5f : 1.0f);
+            for (int o = 0; o < outs; o++)
+                for (int i = 0; i < (l == 0 ? N_FEAT : HIDDEN); i++)
+                    p[layer_w_offset(l) + o * layer_in(l) + i] = (2.0f * gen.uniform() - 1.0f) * lim;
+        }
+        GCHECK(hipMemcpyAsync(M.w32 + (size_t)k * N_PARAMS, p.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice,
+                              s));
+        GCHECK(hipStreamSynchronize(s));  // p is reused for the next network
+    }
+    GCHECK(hipMemsetAsync(M.m, 0, sizeof(float) * (size_t)nets * N_PARAMS, s));
+    GCHECK(hipMemsetAsync(M.v, 0, sizeof(float) * (size_t)nets * N_PARAMS, s));
+    pack_net(M, nets - 1, s);
+    GCHECK(hipStreamSynchronize(s));
 }
 
-static uint32_t grid_waves(uint32_t n_pad) {
-    const uint32_t chunks = n_pad / 16;
-    const uint32_t blocks = (chunks + 3) / 4;
-    return blocks < 2048 ? (blocks ? blocks : 1) : 2048;
+// persistent inference blocks: two fit a CU's LDS
+static uint32_t infer_blocks(uint32_t rows) {
+    const uint32_t groups = (rows + 63) / 64;
+    const uint32_t want = (groups + INF_WAVES - 1) / INF_WAVES;
+    return want < 512 ? (want ? want : 1) : 512;
 }
 
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s) {
     if (n == 0) return;
-    const uint32_t n_pad = (n + 15) & ~15u;
-    sample_map id;
-    id.n_src = n;
-    allow_forward_lds();
-    hipLaunchKernelGGL(k_forward<false>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
-                       M.w32, inputs, n, n_pad,
-                       out, nullptr, id);
+    allow_lds();
+    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n), 1), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs,
+                       n, nullptr, 0u, out);
     GCHECK(hipGetLastError());
 }
 
 void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s) {
     if (n_max == 0) return;
-    sample_map id;
-    id.n_src = n_max;
-    allow_forward_lds();
-    hipLaunchKernelGGL(k_forward<false>, dim3(forward_blocks((n_max + 15) & ~15u)), dim3(FWD_WAVES * 64),
-                       FWD_LDS_BYTES, s, M.w16, M.w32, inputs, n_max, (n_max + 15) & ~15u, out, nullptr, id, n_dev);
+    allow_lds();
+    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n_max), 1), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
+                       inputs, 0u, n_dev, 0u, out);
+    GCHECK(hipGetLastError());
+}
+
+void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint32_t stride, uint32_t n_max,
+                      float* out, hipStream_t s) {
+    if (n_max == 0) return;
+    allow_lds();
+    const uint32_t per = (infer_blocks(n_max * M.nets) + M.nets - 1) / M.nets;
+    hipLaunchKernelGGL(k_infer<4>, dim3(per ? per : 1, M.nets), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
+                       inputs, 0u, n_dev, stride, out);
     GCHECK(hipGetLastError());
 }
 
@@ -640,82 +924,55 @@ sample_map make_sample_map(uint32_t n_src, uint64_t seed) {
     return m;
 }
 
-void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
-                const adam_opts& opt, bool apply, hipStream_t s, sample_map map) {
-    if (n == 0) return;
-    if (map.n_src == 0) map.n_src = n;
-    if (map.n_src < n) throw std::runtime_error("training batch larger than the sample set");
-    model_ensure(M, n);
-    const uint32_t n_pad = (n + 15) & ~15u;
-    _Float16* dz5 = M.dz + (size_t)n_pad * 2 * HIDDEN;
-    _Float16* dzA = M.dz;
-    _Float16* dzB = M.dz + (size_t)n_pad * HIDDEN;
-    GCHECK(hipMemsetAsync(M.loss, 0, sizeof(float) * 4, s));
-    allow_forward_lds();
-    hipLaunchKernelGGL(k_forward<true>, dim3(forward_blocks(n_pad)), dim3(FWD_WAVES * 64), FWD_LDS_BYTES, s, M.w16,
-                       M.w32, inputs, n, n_pad,
-                       M.out, M.acts, map);
-    hipLaunchKernelGGL(k_sum, dim3((n + 256 * SUM_PER_THREAD - 1) / (256 * SUM_PER_THREAD)), dim3(256), 0, s, weights,
-                       n, map, M.loss + 1);
-    hipLaunchKernelGGL(k_mixture_grad, dim3((n_pad + 255) / 256), dim3(256), 0, s, M.out, dirs, weights,
-                       M.loss + 1, n, n_pad, map, dz5, M.loss);
-    // weight gradients: about BW_MAX_BLOCKS blocks of whole 64-row slabs
-    uint32_t rows_per_block = (n_pad + BW_MAX_BLOCKS - 1) / BW_MAX_BLOCKS;
-    rows_per_block = (rows_per_block + BW_ROWS - 1) / BW_ROWS * BW_ROWS;
-    const uint32_t wblocks = (n_pad + rows_per_block - 1) / rows_per_block;
-    const _Float16* dz = dz5;
-    int dz_stride = N_OUT_PAD;
-    for (int l = N_LAYERS - 1; l >= 0; l--) {
-        switch (l) {
-            case 0:
-                hipLaunchKernelGGL(k_backward_weight<0>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-            case 1:
-                hipLaunchKernelGGL(k_backward_weight<1>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-            case 2:
-                hipLaunchKernelGGL(k_backward_weight<2>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-            case 3:
-                hipLaunchKernelGGL(k_backward_weight<3>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-            case 4:
-                hipLaunchKernelGGL(k_backward_weight<4>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-            default:
-                hipLaunchKernelGGL(k_backward_weight<5>, dim3(wblocks), dim3(256), 0, s, dz, M.acts, inputs, n, n_pad,
-                                   rows_per_block, map, M.part);
-                break;
-        }
-        const uint32_t count = (uint32_t)bw_count(l);
-        hipLaunchKernelGGL(k_reduce_partials, dim3((count + 255) / 256, (wblocks + BW_GROUP - 1) / BW_GROUP), dim3(256),
-                           0, s, M.part, wblocks, count, M.grad + layer_w_offset(l));
-        if (l > 0) {
-            _Float16* nxt = (dz == dzA) ? dzB : dzA;
-            hipLaunchKernelGGL(k_backward_data, dim3(grid_waves(n_pad)), dim3(256), 0, s, M.w16, l, dz, dz_stride,
-                               M.acts, n_pad, nxt);
-            dz = nxt;
-            dz_stride = HIDDEN;
-        }
+void train_step_batched(model& M, const float* inputs, const float* dirs, const float* weights,
+                        const std::vector<train_rows>& rows, const adam_opts& opt, bool apply, hipStream_t s) {
+    if (rows.size() != M.nets) throw std::runtime_error("one row set per network");
+    train_segs t = no_segs();
+    bool any = false;
+    for (uint32_t k = 0; k < M.nets; k++) {
+        const train_rows& r = rows[k];
+        train_seg& sg = t.s[k];
+        sg.base = r.base;
+        sg.n = r.n;
+        sg.n_src = r.map.n_src ? r.map.n_src : r.n;
+        sg.a = r.map.n_src ? r.map.a : 1u;
+        sg.b = r.map.n_src ? r.map.b : 0u;
+        if (r.n > sg.n_src) throw std::runtime_error("training batch larger than the sample set");
+        if (r.n && apply) M.step[k]++;
+        sg.c1 = 1.0f - std::pow(opt.b1, (float)M.step[k]);
+        sg.c2 = 1.0f - std::pow(opt.b2, (float)M.step[k]);
+        any = any || r.n;
     }
-    if (apply) {
-        M.step++;
-        const float c1 = 1.0f - std::pow(opt.b1, (float)M.step);
-        const float c2 = 1.0f - std::pow(opt.b2, (float)M.step);
-        adam_apply(M, n, opt, c1, c2, s);
+    if (!any) return;
+    uint32_t rows_total = 0;
+    for (uint32_t k = 0; k < M.nets; k++) {
+        t.s[k].idx_off = rows_total;
+        rows_total += t.s[k].n;
     }
+    if (rows_total > M.idx_cap) {  // grows once to the largest batch a model trains on
+        if (M.idx) GCHECK(hipFree(M.idx));
+        M.idx = nullptr;
+        GCHECK(hipMalloc(&M.idx, sizeof(uint32_t) * rows_total));
+        M.idx_cap = rows_total;
+    }
+    allow_lds();
+    const uint32_t blocks = train_blocks(M.nets);
+    hipLaunchKernelGGL(k_csum, dim3(SUM_BLOCKS, M.nets), dim3(256), 0, s, weights, t, M.csum_part, M.idx);
+    hipLaunchKernelGGL(k_train, dim3(blocks, M.nets), dim3(TR_WAVES * 64), TRAIN_LDS, s, M.pack, M.bias, inputs, dirs,
+                       weights, t, M.csum_part, M.idx, M.part, M.loss_part);
+    hipLaunchKernelGGL(k_adam_pack, dim3(ADAM_BLOCKS, M.nets), dim3(RED_QUADS * RED_SPLITS), 0, s, M.w32, M.m, M.v, M.pack,
+                       M.bias, M.grad, M.loss, M.part, M.loss_part, blocks, t, opt.lr, opt.b1, opt.b2, opt.eps,
+                       apply ? 1u : 2u);
     GCHECK(hipGetLastError());
 }
 
-void adam_apply(model& M, uint32_t n, const adam_opts& opt, float c1, float c2, hipStream_t s) {
-    hipLaunchKernelGGL(k_adam, dim3((N_PARAMS + 255) / 256), dim3(256), 0, s, M.w32, M.w16, M.m, M.v, M.grad,
-                       1.0f / (float)n, opt.lr, opt.b1, opt.b2, opt.eps, c1, c2, 1u);
-    GCHECK(hipGetLastError());
+void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
+                const adam_opts& opt, bool apply, hipStream_t s, sample_map map) {
+    if (n == 0) return;
+    std::vector<train_rows> rows(M.nets);
+    rows[0].n = n;
+    rows[0].map = map;
+    train_step_batched(M, inputs, dirs, weights, rows, opt, apply, s);
 }
 
 }  // namespace guide

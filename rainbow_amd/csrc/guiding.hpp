@@ -1,13 +1,18 @@
 // Learned path-guiding proposal: a small MLP maps frequency-encoded
 // (position, normal, outgoing direction) to an 8-lobe von Mises-Fisher
-// mixture over incident directions.  FP16 MFMA kernels with FP32
-// accumulation for batched inference, training forward/backward, and Adam.
-// No counterpart exists in the reference; the continuation sample of the
-// path integrator mixes this proposal with BSDF sampling (one-sample MIS).
+// mixture over incident directions.  FP16 MFMA kernels (CDNA4's
+// v_mfma_f32_16x16x32_f16) with FP32 accumulation for batched inference and
+// for one fused training step (forward, mixture gradient, backward, weight
+// gradients), FP32 Adam.  A model holds several networks (one per render job
+// of a batch); inference and training take the networks' rows in segments.
+// No counterpart exists in the reference; the continuation sample of the path
+// integrator mixes this proposal with BSDF sampling (one-sample MIS).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 
 namespace rbw {
 namespace guide {
@@ -23,6 +28,7 @@ constexpr int N_OUT = 40;
 constexpr int N_OUT_PAD = 48;
 constexpr float KAPPA_MIN = 0.01f;
 constexpr float KAPPA_MAX = 500.0f;
+constexpr uint32_t MAX_NETS = 32;  // networks of one model
 
 __host__ __device__ constexpr int layer_in(int l) { return l == 0 ? N_FEAT_PAD : HIDDEN; }
 __host__ __device__ constexpr int layer_out(int l) { return l == N_LAYERS - 1 ? N_OUT_PAD : HIDDEN; }
@@ -35,31 +41,38 @@ __host__ __device__ constexpr int layer_w_offset(int l) {
 __host__ __device__ constexpr int layer_b_offset(int l) { return layer_w_offset(l) + layer_out(l) * layer_in(l); }
 constexpr int N_PARAMS = layer_w_offset(N_LAYERS);  // 34160 padded floats (33128 live)
 
+// The kernels read each network's half weights as MFMA A fragments in fragment
+// order (64 lanes x 8 halfs, 1 KiB a fragment): the forward product W_l x of
+// every layer (66 fragments) and the backward product W_l^T dz of layers 1..5
+// (40 fragments); biases separately in FP32.
+constexpr int FWD_FRAGS = 28 + 4 * 8 + 6;  // layer 0: 4 x 7, hidden: 4 x 2 each, last: 3 x 2
+constexpr int BWD_FRAGS = 5 * 8;
+constexpr int FRAG_HALFS = 64 * 8;
+constexpr int PACK_HALFS = (FWD_FRAGS + BWD_FRAGS) * FRAG_HALFS;  // 54,272
+constexpr int N_BIAS = 5 * HIDDEN + N_OUT_PAD;                     // 368
+__host__ __device__ constexpr int bias_offset(int l) { return l * HIDDEN; }
+
 struct model {
-    _Float16* w16 = nullptr;   // N_PARAMS (biases included, unused in half)
-    float* w32 = nullptr;      // master weights
+    uint32_t nets = 0;
+    float* w32 = nullptr;      // [nets][N_PARAMS] master weights
     float* m = nullptr;        // Adam moments
     float* v = nullptr;
-    float* grad = nullptr;     // gradient accumulator
-    float* loss = nullptr;     // 1 float
-    uint32_t step = 0;
-    // activation scratch for training (capacity rows)
-    _Float16* acts = nullptr;  // inputs of layers 1..5: [5*64] halfs per row
-    float* part = nullptr;     // per-block partial weight gradients
-    _Float16* dz = nullptr;    // gradients: 2 x [64] halfs per row (ping-pong) + output [48]
-    float* out = nullptr;      // raw outputs, [48] floats per row
-    uint32_t cap = 0;
+    _Float16* pack = nullptr;  // [nets][PACK_HALFS] half weights in fragment order
+    float* bias = nullptr;     // [nets][N_BIAS]
+    float* grad = nullptr;     // [nets][N_PARAMS]: the mean gradient of the last training step
+    float* loss = nullptr;     // [nets]: the loss of the last step (before its update)
+    std::vector<uint32_t> step;  // Adam steps taken per network
+    // training scratch
+    float* part = nullptr;       // [nets * blocks][N_PARAMS] per-block gradient partial sums
+    float* loss_part = nullptr;  // [nets * blocks]
+    float* csum_part = nullptr;  // [nets][SUM_BLOCKS] partial sums of the records' weights
+    uint32_t* idx = nullptr;     // the training rows' mapped record indices (all networks)
+    uint32_t idx_cap = 0;
 };
 
 struct adam_opts {
     float lr = 1e-2f, b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
 };
-
-void model_init(model& M, uint32_t seed, hipStream_t s);      // allocates, He-uniform init
-void model_free(model& M);
-void model_ensure(model& M, uint32_t rows);                   // training scratch capacity
-void model_get(model& M, float* host_params, hipStream_t s);  // N_PARAMS floats (padded layout)
-void model_set(model& M, const float* host_params, hipStream_t s);
 
 // training rows r = 0..n-1 read sample (a * r + b) mod n_src of the collected set; with
 // gcd(a, n_src) = 1 and n <= n_src the rows are distinct samples
@@ -71,16 +84,33 @@ struct sample_map {
 };
 sample_map make_sample_map(uint32_t n_src, uint64_t seed);
 
-// inputs: n x 9 floats in [0,1] (device); out: n x N_OUT_PAD raw outputs (device)
+// one network's training rows: records base + map(r), r < n, of the collected arrays
+struct train_rows {
+    uint32_t base = 0, n = 0;
+    sample_map map;
+};
+
+void model_init(model& M, uint32_t seed, hipStream_t s, uint32_t nets = 1);  // allocates, He-uniform init
+void model_free(model& M);
+void model_get(model& M, float* host_params, hipStream_t s, uint32_t net = 0);  // N_PARAMS floats (padded layout)
+void model_set(model& M, const float* host_params, hipStream_t s, uint32_t net = 0);
+
+// inputs: n x 9 floats in [0,1] (device); out: n x N_OUT_PAD raw outputs (device); network 0
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s);
 // the same with the row count in device memory (at most n_max rows; no host synchronisation)
 void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s);
-// one training step on n samples: inputs n x 9, dirs n x 3, weights n (radiance / pdf), all device.
-// Leaves the gradient in M.grad (scaled to the mean loss) and, when apply, takes an Adam step.
+// every network at once: network k's rows are inputs/out rows [k * stride, k * stride + n_dev[k]),
+// n_dev[k] <= n_max (device memory)
+void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint32_t stride, uint32_t n_max,
+                      float* out, hipStream_t s);
+// one training step of every network k with rows[k].n > 0 on its rows: inputs x 9, dirs x 3,
+// weights (radiance / pdf), all device.  Leaves each network's mean gradient in M.grad and
+// its loss in M.loss and, when apply, takes an Adam step.  Three launches, no memsets.
+void train_step_batched(model& M, const float* inputs, const float* dirs, const float* weights,
+                        const std::vector<train_rows>& rows, const adam_opts& opt, bool apply, hipStream_t s);
+// network 0 on n rows through map (default: rows 0..n-1)
 void train_step(model& M, const float* inputs, const float* dirs, const float* weights, uint32_t n,
                 const adam_opts& opt, bool apply, hipStream_t s, sample_map map = sample_map());
-// Adam step on the accumulated gradient of n samples (bias corrections c1, c2)
-void adam_apply(model& M, uint32_t n, const adam_opts& opt, float c1, float c2, hipStream_t s);
 
 }  // namespace guide
 }  // namespace rbw

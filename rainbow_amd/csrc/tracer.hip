@@ -1454,30 +1454,15 @@ rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const 
         ctx->g_io_c.upload(c, n, s);
         ctx->timer.reset();
         ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
-        HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
         guide::adam_opts ao;
         ao.lr = learning_rate;
+        // gradient (and, with apply, the Adam step) in three launches; the network's mean
+        // gradient and loss stay in the model
         guide::train_step(ctx->gm, ctx->g_io_in.as<float>(), ctx->g_io_dir.as<float>(), ctx->g_io_c.as<float>(), n,
-                          ao, false, s);
-        if (grad || loss) {
-            std::vector<float> g(guide::N_PARAMS);
-            float l = 0.0f;
-            HIP_CHECK(hipMemcpyAsync(g.data(), ctx->gm.grad, sizeof(float) * guide::N_PARAMS, hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipMemcpyAsync(&l, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            if (grad)
-                for (int i = 0; i < guide::N_PARAMS; i++) grad[i] = g[i] / (float)n;
-            if (loss) *loss = l;
-        }
-        if (apply) {
-            // the Adam step of train_step(apply) on the gradient just computed
-            ctx->gm.step++;
-            const float c1 = 1.0f - std::pow(ao.b1, (float)ctx->gm.step);
-            const float c2 = 1.0f - std::pow(ao.b2, (float)ctx->gm.step);
-            guide::adam_apply(ctx->gm, n, ao, c1, c2, s);
-        }
+                          ao, apply != 0, s);
+        if (grad) HIP_CHECK(hipMemcpyAsync(grad, ctx->gm.grad, sizeof(float) * guide::N_PARAMS, hipMemcpyDeviceToHost, s));
+        if (loss) HIP_CHECK(hipMemcpyAsync(loss, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
         ctx->timer.end(s);
-        HIP_CHECK(hipMemsetAsync(ctx->gm.grad, 0, sizeof(float) * guide::N_PARAMS, s));
         HIP_CHECK(hipStreamSynchronize(s));
         ctx->guide_last_ms = ctx->timer.elapsed_last();
     });

@@ -91,41 +91,76 @@ def test_adam_first_step_closed_form():
 
 # ------------------------------------------------------------------ GPU kernels
 
+def test_fp16_oracle_is_close_to_the_exact_one():
+    """The FP16-rounded restatement against the float64 model: outputs within half
+    precision, and every layer's gradient within 1e-2 when the float64 backward uses the
+    same ReLU masks.  (Rounding the activations also flips ~1e-4 of the masks; records whose
+    mean-direction output is near zero carry gradients ~1/|m| large, so such flips move the
+    summed gradients by up to ~20 % -- a property of the half-precision model, which the
+    kernels share, not of the restatement.)"""
+    Ws, bs = G.init_params(4)
+    x, d, c = _batch(512, 9)
+    X = G.encode(x)
+    o64, acts, pre = G.forward(Ws, bs, X)
+    o16, acts16 = G.forward_f16(Ws, bs, x)
+    assert np.abs(o16 - o64).max() < 2e-2 * np.abs(o64).max()
+    loss16, dW16, db16 = G.grads_f16(Ws, bs, x, d, c)
+    _, dout = G.loss_and_grad_out(o64, d, c)
+    # float64 backward with the FP16 forward's masks (acts16[i] > 0 iff its pre-activation was)
+    dWs, dbs = G.backward(Ws, bs, acts, acts16[1:], dout)
+    for l in range(6):
+        a, b = dW16[l], dWs[l][:, :G.N_FEAT] if l == 0 else dWs[l]
+        assert np.linalg.norm(a - b) < 1e-2 * np.linalg.norm(b), l
+        assert np.linalg.norm(db16[l] - dbs[l]) < 1e-2 * np.linalg.norm(dbs[l]), l
+
+
 @pytest.mark.gpu
 def test_gpu_guide_forward_matches_oracle(tracer):
+    """inference against the FP16-rounded restatement (tight) and the float64 model (the
+    rounding's own size)"""
     Ws, bs = G.init_params(7)
     tracer.guide_set_params(G.to_padded(Ws, bs))
     x, _, _ = _batch(1000, 1)
     got = tracer.guide_forward(x)[:, :40]
+    ref16, _ = G.forward_f16(Ws, bs, x)
     ref, _, _ = G.forward(Ws, bs, G.encode(x))
     scale = np.abs(ref).max()
+    err16 = np.abs(got - ref16).max() / scale
     err = np.abs(got - ref).max() / scale
-    print("forward max error / output scale", err)
-    assert err < 2e-2            # FP16 operands, FP32 accumulation
+    print("forward max error / output scale: vs FP16 oracle", err16, "vs float64", err)
+    assert err16 < 2e-3
+    assert err < 2e-2
     assert np.corrcoef(got.ravel(), ref.ravel())[0, 1] > 0.9995
 
 
 @pytest.mark.gpu
-def test_gpu_guide_gradients_match_oracle(tracer):
+@pytest.mark.parametrize("n", [4096, 1000])
+def test_gpu_guide_gradients_match_oracle(tracer, n):
+    """the fused training step's loss and mean gradients of every layer against the
+    FP16-rounded restatement (relative error <= 1e-2 per layer; 1000 rows: a ragged last
+    chunk) and, loosely, the float64 model"""
     Ws, bs = G.init_params(11)
     tracer.guide_set_params(G.to_padded(Ws, bs))
-    x, d, c = _batch(4096, 2)
+    x, d, c = _batch(n, 2)
     loss, grad = tracer.guide_train(x, d, c, apply=False, want_grad=True)
+    loss16, dW16, db16 = G.grads_f16(Ws, bs, x, d, c)
     X = G.encode(x)
     o, acts, pre = G.forward(Ws, bs, X)
     ref_loss, dout = G.loss_and_grad_out(o, d, c)
     dWs, dbs = G.backward(Ws, bs, acts, pre, dout)
     gW, gb = G.from_padded(grad)
-    print("loss", loss, ref_loss)
+    print("loss", loss, "FP16 oracle", loss16, "float64", ref_loss)
+    assert abs(loss - loss16) < 1e-3 * abs(loss16) + 1e-4
     assert abs(loss - ref_loss) < 1e-2 * abs(ref_loss) + 1e-3
     for l in range(6):
-        for a, b, name in ((gW[l], dWs[l], "W"), (gb[l], dbs[l], "b")):
-            if l == 0 and name == "W":
-                a, b = a[:, :G.N_FEAT], b[:, :G.N_FEAT]
-            cos = (a * b).sum() / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
-            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            print(f"layer {l} {name}: cosine {cos:.5f} relative error {rel:.4f}")
-            assert cos > 0.995 and rel < 0.1, (l, name, cos, rel)
+        pairs = ((gW[l][:, :G.N_FEAT] if l == 0 else gW[l], dW16[l], dWs[l][:, :G.N_FEAT] if l == 0 else dWs[l], "W"),
+                 (gb[l], db16[l], dbs[l], "b"))
+        for a, b16, b64, name in pairs:
+            rel16 = np.linalg.norm(a - b16) / (np.linalg.norm(b16) + 1e-30)
+            rel = np.linalg.norm(a - b64) / (np.linalg.norm(b64) + 1e-30)
+            print(f"layer {l} {name}: relative error vs FP16 oracle {rel16:.2e}, vs float64 {rel:.2e}")
+            assert rel16 < 1e-2, (l, name, rel16)
+            assert rel < 0.3, (l, name, rel)  # ReLU masks flipped by the half rounding
 
 
 @pytest.mark.gpu
