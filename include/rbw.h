@@ -406,6 +406,14 @@ typedef struct rbw_batch_opts {
                                         first (the pool size stays as given) */
     uint32_t triangle_test;          /* RBW_TRI_* */
     uint32_t shadow_mode;            /* RBW_SHADOW_* */
+    /* NULL: the plain estimator.  Otherwise every job is rendered with learned guiding (the
+     * estimator of rbw_guiding_opts) by its own network, job i by network i of the context's
+     * job networks (rbw_guide_jobs_reset): all jobs share one persistent launch, their guided
+     * vertices are inferred by one launch over all networks, and with train set every network
+     * takes steps_per_pass Adam steps of up to `batch` of its job's training rows after every
+     * 8 iterations of the bounce loop, all networks in one batched training kernel.
+     * light_records must be 0 and scene_min/scene_max are ignored (each job's scene box). */
+    const rbw_guiding_opts* guiding;
 } rbw_batch_opts;
 
 typedef struct rbw_job_stats {
@@ -425,11 +433,28 @@ typedef struct rbw_batch_stats {
     uint32_t graph_launches;
     uint32_t block_size, pool_size, bvh_width;  /* the configuration that ran */
     uint32_t scene_triangles;        /* triangles of the merged scene */
+    uint64_t guide_samples;          /* guided batches: training rows the jobs offered */
+    uint32_t guide_steps;            /* guided batches: batched training steps taken */
+    uint32_t pad;
 } rbw_batch_stats;
 
 /* renders every job; job_stats (nullable) has n entries */
 rbw_status rbw_submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, const rbw_batch_opts* opts,
                            rbw_job_stats* job_stats, rbw_batch_stats* stats);
+/* The job networks of guided batches (no counterpart in the reference): n_networks networks
+ * of RBW_GUIDE_PARAMS parameters each (at most 32), He-uniform from seed + network index, Adam
+ * state cleared.  Job i of a guided rbw_submit_jobs call uses network i. */
+rbw_status rbw_guide_jobs_reset(rbw_ctx* ctx, uint32_t n_networks, uint32_t seed);
+rbw_status rbw_guide_jobs_get_params(rbw_ctx* ctx, uint32_t network, float* params /* RBW_GUIDE_PARAMS */);
+rbw_status rbw_guide_jobs_set_params(rbw_ctx* ctx, uint32_t network, const float* params);
+/* one training step of the first n_networks job networks in one batched launch: network k
+ * trains on rows[k] records (inputs x 9, dirs x 3, c), the records of the networks one
+ * after another; loss (nullable) receives n_networks losses and grad (nullable) the
+ * n_networks gradients (RBW_GUIDE_PARAMS each), as rbw_guide_train does for one network */
+rbw_status rbw_guide_jobs_train(rbw_ctx* ctx, uint32_t n_networks, const uint32_t* rows, const float* inputs,
+                                const float* dirs, const float* c, float learning_rate, uint32_t apply, float* loss,
+                                float* grad);
+
 /* the autotuner alone: fills block_size/pool_size/bvh_width of *opts with the
  * fastest candidate on a 1-spp probe of the jobs */
 rbw_status rbw_autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, uint32_t n, rbw_batch_opts* opts,

@@ -135,7 +135,7 @@ class rbw_job(C.Structure):
 class rbw_batch_opts(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("pool_size", C.c_uint32), ("bvh_width", C.c_uint32),
                 ("use_graph", C.c_uint32), ("autotune", C.c_uint32), ("triangle_test", C.c_uint32),
-                ("shadow_mode", C.c_uint32)]
+                ("shadow_mode", C.c_uint32), ("guiding", C.POINTER(rbw_guiding_opts))]
 
 
 class rbw_job_stats(C.Structure):
@@ -151,7 +151,8 @@ class rbw_batch_stats(C.Structure):
                 ("autotune_ms", C.c_double), ("samples", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("mis_rays", C.c_uint64), ("iterations", C.c_uint32),
                 ("graph_launches", C.c_uint32), ("block_size", C.c_uint32), ("pool_size", C.c_uint32),
-                ("bvh_width", C.c_uint32), ("scene_triangles", C.c_uint32)]
+                ("bvh_width", C.c_uint32), ("scene_triangles", C.c_uint32), ("guide_samples", C.c_uint64),
+                ("guide_steps", C.c_uint32), ("pad", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -185,6 +186,11 @@ RBW_SIGNATURES = {
     "rbw_guide_forward": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p]),
     "rbw_guide_train": (C.c_int32, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_float, C.c_uint32, f32p, f32p]),
     "rbw_guide_last_ms": (C.c_int32, [C.c_void_p, C.POINTER(C.c_double)]),
+    "rbw_guide_jobs_reset": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "rbw_guide_jobs_get_params": (C.c_int32, [C.c_void_p, C.c_uint32, f32p]),
+    "rbw_guide_jobs_set_params": (C.c_int32, [C.c_void_p, C.c_uint32, f32p]),
+    "rbw_guide_jobs_train": (C.c_int32, [C.c_void_p, C.c_uint32, u32p, f32p, f32p, f32p, C.c_float, C.c_uint32, f32p,
+                                         f32p]),
     "rbw_add_scene": (C.c_int32, [C.c_void_p, C.POINTER(rbw_scene_desc), C.POINTER(C.c_uint32)]),
     "rbw_remove_scene": (C.c_int32, [C.c_void_p, C.c_uint32]),
     "rbw_submit_jobs": (C.c_int32, [C.c_void_p, C.POINTER(rbw_job), C.c_uint32, C.POINTER(rbw_batch_opts),

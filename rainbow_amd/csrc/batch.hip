@@ -28,6 +28,8 @@
 #include <map>
 #include <vector>
 
+#include "guided.hpp"
+#include "guiding.hpp"
 #include "host.hpp"
 #include "path.hpp"
 
@@ -54,6 +56,7 @@ struct djob {
     uint32_t root8;
     uint32_t light_flags;  // k_lights_* of the job's scene
     uint32_t pad;
+    f3 gmin, ginv;         // guided jobs: the network query's position normalisation (scene box)
 };
 
 struct dctl {
@@ -78,12 +81,42 @@ struct dbatch {
     uint32_t* first_iter;           // per job
     uint32_t* last_iter;            // per job
     unsigned long long* counters;   // extension, shadow, MIS, light casts (striped)
+    // guided jobs (job j uses network j): this iteration's guided vertices per job, as slots
+    // (rows[j * pool + k], k < gcount[j]), and the training rows closed at retirement
+    uint32_t guided, train;
+    uint32_t* grows;
+    uint32_t* gcount;
+    float* t_in;                    // [j * t_cap + k] x 9
+    float* t_dir;                   // x 3
+    float* t_c;
+    uint32_t* t_count;              // per job (rows offered; at most t_cap kept)
+    uint32_t t_cap;
 };
 
 // slot state (U_STATE): 0 = free
 constexpr uint32_t ST_ALIVE = 1u;    // trace an extension ray this iteration
 constexpr uint32_t ST_CONNECT = 2u;  // next-event estimation rays pending
 constexpr uint32_t ST_DYING = 4u;    // finished: into the film at the next regeneration
+constexpr uint32_t ST_GUIDE = 8u;    // a guided vertex waits for its network outputs
+
+// appends one entry per lane with `take` to its job's list (counts[job], slots of list job at
+// list + job * stride): lanes of one job in a wave share one atomic; returns the lane's position
+__device__ __forceinline__ uint32_t append_by_job(bool take, uint32_t job, uint32_t* counts) {
+    uint64_t pending = __ballot(take);
+    uint32_t pos = 0;
+    const uint64_t below = __lane_id() == 0 ? 0ull : (~0ull >> (64 - __lane_id()));
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t jl = __shfl(job, leader);
+        const uint64_t same = __ballot(take && job == jl);
+        uint32_t base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(&counts[jl], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if (take && job == jl) pos = base + (uint32_t)__popcll(same & below);
+        pending &= ~same;
+    }
+    return pos;
+}
 
 __device__ __forceinline__ uint32_t find_job(const djob* jobs, uint32_t n, unsigned long long g) {
     uint32_t lo = 0, hi = n;  // jobs[lo].base <= g < jobs[hi].base
@@ -177,6 +210,8 @@ __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     fold_retire_records(B, B.ctl->iteration);
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
+    if (B.guided)
+        for (uint32_t j = t; j < B.njobs; j += 1024) B.gcount[j] = 0u;
     uint32_t sum = 0;
     for (uint32_t b = b0; b < b1; b++) sum += B.block_free[b];
     part[t] = sum;
@@ -227,6 +262,19 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
         film_add_sample(J.cam, J.film, (int32_t)(pixel % J.cam.width), (int32_t)(pixel / J.cam.width),
                         P.F(F_PX, slot), P.F(F_PY, slot), P.F3(F_VX, slot), false);
         st = 0u;
+    }
+    if (B.train) {
+        // the finished path's closed training records into its job's training rows
+        const dguide& G = base.g;
+        for (uint32_t b = 0; b < G.rec_depth; b++) {
+            const size_t ri = (size_t)slot * G.rec_depth + b;
+            float c = 0.0f;
+            if (dying && G.rec_flag[ri] == 2) c = record_target(P, G, slot, ri);
+            const bool take = c > 0.0f;
+            const uint32_t k = append_by_job(take, j, B.t_count);
+            if (take && k < B.t_cap) record_to_row(P, G, ri, c, B.t_in, B.t_dir, B.t_c, (size_t)j * B.t_cap + k);
+            if (dying) G.rec_flag[ri] = 0;
+        }
     }
     // the iteration in which a job's last path retired: the wave records its first
     // retiring job (folded by the next plan); lanes of another job, met only where
@@ -287,6 +335,73 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
     }
 }
 
+// guided jobs: the vertex up to the guided stage.  A guided vertex stores its surface
+// record, material and network query by slot and joins its job's list of guided slots;
+// the others shade as the plain estimator (shade_path).
+__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade_guided(dscene S, dpaths P,
+                                                                                         dparams base, dbatch B) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool to_guide = false;
+    uint32_t j = 0;
+    if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
+        bool alive = false, need_connect = false;
+        uint32_t gmat = 0;
+        surf gsi;
+        dscene Sj;
+        dparams pj;
+        j = P.U(U_JOB, slot);
+        const djob& J = B.jobs[j];
+        job_view(S, base, J, Sj, pj);
+        shade_path<true>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
+        if (to_guide) {
+            const dguide& G = base.g;
+            const size_t cap = G.cap;
+            const float v[G_COUNT] = {gsi.point.x, gsi.point.y, gsi.point.z, gsi.normal.x, gsi.normal.y, gsi.normal.z,
+                                      gsi.wo.x, gsi.wo.y, gsi.wo.z, gsi.sh.x.x, gsi.sh.x.y, gsi.sh.x.z,
+                                      gsi.sh.y.x, gsi.sh.y.y, gsi.sh.y.z, gsi.sh.z.x, gsi.sh.z.y, gsi.sh.z.z};
+#pragma unroll
+            for (int k = 0; k < G_COUNT; k++) G.surf[k * cap + slot] = v[k];
+            G.surf_mat[slot] = gmat;
+            float q[9];
+            guide_query(gsi, J.gmin, J.ginv, q);
+#pragma unroll
+            for (int k = 0; k < 9; k++) G.query[(size_t)slot * 9 + k] = q[k];
+            P.U(U_STATE, slot) = ST_GUIDE;
+        } else {
+            P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
+        }
+    }
+    const uint32_t k = append_by_job(to_guide, j, B.gcount);
+    if (to_guide) B.grows[(size_t)j * B.pool + k] = slot;
+}
+
+// guided jobs: the guided vertex (guided.hpp) of every slot that waits for it, with its job's
+// network outputs
+__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_guided(dscene S, dpaths P, dparams base,
+                                                                                   dbatch B) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= B.pool || P.U(U_STATE, slot) != ST_GUIDE) return;
+    const dguide& G = base.g;
+    const size_t cap = G.cap;
+    float v[G_COUNT];
+#pragma unroll
+    for (int k = 0; k < G_COUNT; k++) v[k] = G.surf[k * cap + slot];
+    surf si;
+    si.point = f3{v[G_PX], v[G_PY], v[G_PZ]};
+    si.normal = f3{v[G_NX], v[G_NY], v[G_NZ]};
+    si.wo = f3{v[G_WX], v[G_WY], v[G_WZ]};
+    si.sh.x = f3{v[G_XX], v[G_XY], v[G_XZ]};
+    si.sh.y = f3{v[G_YX], v[G_YY], v[G_YZ]};
+    si.sh.z = f3{v[G_ZX], v[G_ZY], v[G_ZZ]};
+    dscene Sj;
+    dparams pj;
+    job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
+    bool alive = false, need_connect = false;
+    guided_vertex(Sj, P, pj, slot, si, G.surf_mat[slot], G.mix + (size_t)slot * guide::N_OUT_PAD,
+                  G.query + (size_t)slot * 9, 0u, alive, need_connect);
+    P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
+}
+
 // next-event estimation rays; then the block's count of slots the next
 // regeneration will find free (free or finished)
 template <int W>
@@ -336,6 +451,9 @@ struct rbw_batch_state {
     // per batch
     dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, films, rgb;
     dctl* host_ctl = nullptr;  // pinned
+    // guided jobs: per-slot guided vertex data and training records, per-job lists and rows
+    dbuf g_surf, g_mat, g_query, g_mix, g_rec, g_flag, g_rows, g_count, t_in, t_dir, t_c, t_count;
+    uint32_t g_pool = 0, g_depth = 0, g_jobs = 0, g_tcap = 0;
     // captured bounce loop
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -370,6 +488,10 @@ struct batch_config {
     uint32_t tri_test = RBW_TRI_MOLLER_TRUMBORE, shadow = RBW_SHADOW_CLOSEST_VISIBLE;
     // RBW_SAMPLER_PER_SAMPLE_PCG or RBW_SAMPLER_STRATIFIED (a single render's sampler)
     uint32_t sampler = RBW_SAMPLER_PER_SAMPLE_PCG, strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
+    // guided jobs (job j: network j of `model`)
+    const rbw_guiding_opts* guiding = nullptr;
+    guide::model* model = nullptr;
+    uint32_t seed = 0;
 };
 
 void validate_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n) {
@@ -451,6 +573,8 @@ void ensure_pool(rbw_batch_state& b, uint32_t pool) {
 struct run_result {
     double ms = 0.0;
     uint32_t iterations = 0, launches = 0;
+    unsigned long long guide_samples = 0;
+    uint32_t guide_steps = 0;
     unsigned long long counters[4] = {0, 0, 0, 0};
     std::vector<double> launch_ms;  // end time of each launch from the start
     std::vector<uint32_t> first_iter, last_iter;
@@ -514,6 +638,67 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     base.shadow_mode = cfg.shadow;
     // grid: one thread per slot of the pool
     const uint32_t grid = nblocks;
+    // guided jobs: per-slot vertex data and training records, per-job lists and training rows
+    const bool guided = cfg.guiding != nullptr;
+    const bool train = guided && cfg.guiding->train;
+    B.guided = guided ? 1u : 0u;
+    B.train = train ? 1u : 0u;
+    B.grows = nullptr;
+    B.gcount = nullptr;
+    B.t_in = B.t_dir = B.t_c = nullptr;
+    B.t_count = nullptr;
+    B.t_cap = 0;
+    std::vector<uint32_t> host_tcount;
+    if (guided) {
+        const rbw_guiding_opts& go = *cfg.guiding;
+        const uint32_t pool = cfg.pool;
+        const uint32_t tcap = go.batch * 2;  // training rows kept per job between training points
+        if (b.g_pool != pool || b.g_depth < max_depth || b.g_jobs < nj || b.g_tcap < tcap) {
+            const uint32_t depth = std::max(max_depth, b.g_depth);
+            const uint32_t jobs = std::max(nj, b.g_jobs);
+            b.g_surf.ensure(size_t(G_COUNT) * pool * 4);
+            b.g_mat.ensure(size_t(pool) * 4);
+            b.g_query.ensure(size_t(pool) * 9 * 4);
+            b.g_mix.ensure(size_t(pool) * guide::N_OUT_PAD * 4);
+            b.g_rec.ensure(size_t(R_COUNT) * pool * depth * 4);
+            b.g_flag.ensure(size_t(pool) * depth * 4);
+            b.g_rows.ensure(size_t(jobs) * pool * 4);
+            b.g_count.ensure(size_t(jobs) * 4);
+            b.t_in.ensure(size_t(jobs) * tcap * 9 * 4);
+            b.t_dir.ensure(size_t(jobs) * tcap * 3 * 4);
+            b.t_c.ensure(size_t(jobs) * tcap * 4);
+            b.t_count.ensure(size_t(jobs) * 4);
+            b.g_pool = pool;
+            b.g_depth = depth;
+            b.g_jobs = jobs;
+            b.g_tcap = tcap;
+        }
+        HIP_CHECK(hipMemsetAsync(b.g_flag.p, 0, size_t(pool) * b.g_depth * 4, s));
+        HIP_CHECK(hipMemsetAsync(b.t_count.p, 0, size_t(nj) * 4, s));
+        dguide& G = base.g;
+        G.enable = 1;
+        G.alpha = go.mis_fraction;
+        G.train = train ? 1u : 0u;
+        G.light_records = 0;
+        G.depth = go.guide_depth ? go.guide_depth : 0xFFFFFFFFu;
+        G.surf = b.g_surf.as<float>();
+        G.surf_mat = b.g_mat.as<uint32_t>();
+        G.query = b.g_query.as<float>();
+        G.mix = b.g_mix.as<float>();
+        G.rec = b.g_rec.as<float>();
+        G.rec_flag = b.g_flag.as<uint32_t>();
+        G.rec_depth = b.g_depth;
+        G.cap = pool;
+        B.grows = b.g_rows.as<uint32_t>();
+        B.gcount = b.g_count.as<uint32_t>();
+        B.t_in = b.t_in.as<float>();
+        B.t_dir = b.t_dir.as<float>();
+        B.t_c = b.t_c.as<float>();
+        B.t_count = b.t_count.as<uint32_t>();
+        B.t_cap = b.g_tcap;
+        host_tcount.resize(nj);
+        guide::prepare_kernels();
+    }
 
     const uint32_t init_n = std::max(std::max(cfg.pool, nj), RBW_STRIPES * RBW_STRIPE_WORDS);
     hipLaunchKernelGGL(k_b_init, dim3(blocks(init_n, 256)), dim3(256), 0, s, B, P, total);
@@ -533,7 +718,19 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                            base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_SHADE, s);
-        hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (guided) {
+            hipLaunchKernelGGL(k_b_shade_guided, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+            if (tm) tm->end(s);
+            // every job's guided vertices through its network, in one launch
+            if (tm) tm->begin(RBW_KERNEL_GUIDE_INFER, s);
+            guide::forward_segments(*cfg.model, base.g.query, B.gcount, cfg.pool, cfg.pool, base.g.mix, s, B.grows,
+                                    nj);
+            if (tm) tm->end(s);
+            if (tm) tm->begin(RBW_KERNEL_SHADE_GUIDED, s);
+            hipLaunchKernelGGL(k_b_guided, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        } else {
+            hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
         hipLaunchKernelGGL(S.width == 8 ? k_b_connect<8> : k_b_connect<2>, dim3(grid), dim3(bs), trav_lds, s, S, P,
@@ -586,8 +783,39 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         HIP_CHECK(hipEventRecord(b.events[launches], s));
         if (launches % check_every == 0) {
             HIP_CHECK(hipMemcpyAsync(b.host_ctl, B.ctl, sizeof(dctl), hipMemcpyDeviceToHost, s));
+            if (train) HIP_CHECK(hipMemcpyAsync(host_tcount.data(), B.t_count, size_t(nj) * 4, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
-            if (b.host_ctl->cursor >= total && b.host_ctl->live == 0) break;
+            const bool done = b.host_ctl->cursor >= total && b.host_ctl->live == 0;
+            if (train) {
+                // every network with enough closed rows takes steps_per_pass Adam steps on up to
+                // `batch` of them, all networks in one batched step
+                const rbw_guiding_opts& go = *cfg.guiding;
+                std::vector<guide::train_rows> rows(cfg.model->nets);
+                bool any = false;
+                for (uint32_t j = 0; j < nj; j++) {
+                    const uint32_t avail = std::min(host_tcount[j], B.t_cap);
+                    rr.guide_samples += host_tcount[j];
+                    if (avail < 256) continue;
+                    rows[j].base = j * B.t_cap;
+                    rows[j].n = std::min(avail, go.batch);
+                    any = true;
+                }
+                if (any) {
+                    guide::adam_opts ao;
+                    ao.lr = go.learning_rate;
+                    for (uint32_t st = 0; st < go.steps_per_pass; st++) {
+                        for (uint32_t j = 0; j < nj; j++)
+                            if (rows[j].n)
+                                rows[j].map = guide::make_sample_map(std::min(host_tcount[j], B.t_cap),
+                                                                     (uint64_t(cfg.seed) << 32) ^ (uint64_t(launches) << 12) ^
+                                                                         (uint64_t(j) << 6) ^ st);
+                        guide::train_step_batched(*cfg.model, B.t_in, B.t_dir, B.t_c, rows, ao, true, s);
+                    }
+                    rr.guide_steps += go.steps_per_pass;
+                }
+                HIP_CHECK(hipMemsetAsync(B.t_count, 0, size_t(nj) * 4, s));
+            }
+            if (done) break;
             require(launches < max_launches, RBW_ERR_DEVICE, "batched bounce loop did not drain");
         }
     }
@@ -726,13 +954,38 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     std::memset(&o, 0, sizeof(o));
     o.use_graph = 1;
     if (opts) o = *opts;
+    const rbw_guiding_opts* go = o.guiding && o.guiding->enable ? o.guiding : nullptr;
+    if (go) {
+        require(ctx->gjobs.nets >= n, RBW_ERR_INVALID_ARGUMENT,
+                "a guided batch needs a network per job (rbw_guide_jobs_reset with at least as many networks as jobs)");
+        require(go->mis_fraction >= 0.0f && go->mis_fraction <= 1.0f, RBW_ERR_INVALID_ARGUMENT,
+                "guiding mis_fraction must be in [0, 1]");
+        require(!go->train || (go->batch >= 256 && go->batch <= (1u << 20)), RBW_ERR_INVALID_ARGUMENT,
+                "guiding batch must be in [256, 2^20]");
+        require(!go->light_records, RBW_ERR_UNSUPPORTED, "guided batches take no light-sample training records");
+        require(!o.autotune, RBW_ERR_UNSUPPORTED, "the autotuner probes plain batches only");
+    }
     double tune_ms = 0.0;
     if (o.autotune) autotune_jobs(ctx, jobs, n, &o, &tune_ms);
-    const batch_config cfg = config_of(&o);
+    batch_config cfg = config_of(&o);
+    if (go) {
+        cfg.guiding = go;
+        cfg.model = &ctx->gjobs;
+        cfg.seed = jobs[0].seed;
+    }
     std::vector<uint32_t> part;
     prepare_scene(ctx, jobs, n, part);
     rbw_batch_state& b = *ctx->batch;
     std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 0);
+    if (go) {
+        // each job's network query normalises positions by its scene's box
+        for (size_t i = 0; i < n; i++) {
+            const host_scene& h = *ctx->pool[jobs[i].scene];
+            hj[i].gmin = h.smin;
+            hj[i].ginv = f3{1.0f / std::max(h.smax.x - h.smin.x, 1e-6f), 1.0f / std::max(h.smax.y - h.smin.y, 1e-6f),
+                            1.0f / std::max(h.smax.z - h.smin.z, 1e-6f)};
+        }
+    }
     // Jobs with at least as many samples as the pool has slots fill it on their own: they run
     // alone, one after another in submission order, and the other jobs then share one
     // persistent launch.  (Measured on the 64-job workload: letting large jobs share the
@@ -743,7 +996,8 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
     {
         std::vector<size_t> small;
         for (size_t i = 0; i < n; i++) {
-            if (n > 1 && hj[i].count >= cfg.pool) groups.push_back({i});
+            // (guided jobs all share the launch: their networks are addressed by job index)
+            if (!go && n > 1 && hj[i].count >= cfg.pool) groups.push_back({i});
             else small.push_back(i);
         }
         if (!small.empty()) groups.push_back(small);
@@ -783,6 +1037,8 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
         rr.ms += r.ms;
         rr.iterations += r.iterations;
         rr.launches += r.launches;
+        rr.guide_samples += r.guide_samples;
+        rr.guide_steps += r.guide_steps;
         for (int k = 0; k < 4; k++) rr.counters[k] += r.counters[k];
     }
     // films -> caller
@@ -829,6 +1085,8 @@ void submit_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const rbw_batch_op
         stats->pool_size = cfg.pool;
         stats->bvh_width = cfg.width;
         stats->scene_triangles = b.merged_tris;
+        stats->guide_samples = rr.guide_samples;
+        stats->guide_steps = rr.guide_steps;
     }
 }
 

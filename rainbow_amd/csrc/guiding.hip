@@ -99,16 +99,20 @@ constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 64 * N_IN * 4;        //
 static_assert(INF_XS % 16 == 0, "aligned input rows");
 
 // R record tiles (16 rows each) per wave share every weight fragment read from LDS.
-// Network blockIdx.y: rows [net * stride, net * stride + n), n from n_dev[net] (or n_host).
+// Network blockIdx.y: rows [net * stride, net * stride + n), n from n_dev[net] (or n_host);
+// with `rows`, row r of the network is record rows[net * stride + r] of inputs and out.
 template <int R>
 __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __restrict__ pack,
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ inputs, uint32_t n_host,
                                                           const uint32_t* __restrict__ n_dev, uint32_t stride,
-                                                          float* __restrict__ out) {
+                                                          float* __restrict__ out,
+                                                          const uint32_t* __restrict__ rows = nullptr) {
     const uint32_t net = blockIdx.y;
     const uint32_t n = n_dev ? n_dev[net] : n_host;
-    if (n == 0) return;
+    // blocks past the network's rows leave before staging its weights (grids are sized for
+    // the most rows a network may have)
+    if (n == 0 || blockIdx.x * INF_WAVES * 16 * R >= n) return;
     extern __shared__ __align__(16) unsigned char smem[];
     _Float16* wl = reinterpret_cast<_Float16*>(smem);
     float* bl = reinterpret_cast<float*>(smem + size_t(FWD_FRAGS) * FRAG_HALFS * 2);
@@ -119,8 +123,9 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
         for (int q = threadIdx.x; q < N_BIAS; q += blockDim.x) bl[q] = bias[(size_t)net * N_BIAS + q];
     }
     __syncthreads();
-    const float* in = inputs + (size_t)net * stride * N_IN;
-    float* o = out + (size_t)net * stride * N_OUT_PAD;
+    const float* in = rows ? inputs : inputs + (size_t)net * stride * N_IN;
+    float* o = rows ? out : out + (size_t)net * stride * N_OUT_PAD;
+    const uint32_t* rl = rows ? rows + (size_t)net * stride : nullptr;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const uint32_t groups = (n + 16 * R - 1) / (16 * R);
@@ -133,7 +138,12 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
 #pragma unroll
         for (int k = 0; k < XPL; k++) {
             const size_t q = (size_t)grp * 16 * R * N_IN + lane + 64 * k;
-            xpre[k] = grp < groups && q < (size_t)n * N_IN ? in[q] : 0.0f;
+            if (grp < groups && q < (size_t)n * N_IN) {
+                const size_t r = q / N_IN;
+                xpre[k] = rl ? in[(size_t)rl[r] * N_IN + (q - r * N_IN)] : in[q];
+            } else {
+                xpre[k] = 0.0f;
+            }
         }
     };
     load_group(blockIdx.x * INF_WAVES + wave);
@@ -207,7 +217,8 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
                 v.y = acc[t][m][1] + bl[bias_offset(5) + c0 + 1];
                 v.z = acc[t][m][2] + bl[bias_offset(5) + c0 + 2];
                 v.w = acc[t][m][3] + bl[bias_offset(5) + c0 + 3];
-                *reinterpret_cast<float4*>(o + (size_t)row[t] * N_OUT_PAD + c0) = v;
+                const size_t orow = rl ? (size_t)rl[row[t]] : (size_t)row[t];
+                *reinterpret_cast<float4*>(o + orow * N_OUT_PAD + c0) = v;
             }
         }
     }
@@ -812,7 +823,11 @@ void model_get(model& M, float* host, hipStream_t s, uint32_t net) {
     GCHECK(hipStreamSynchronize(s));
 }
 
-static void allow_lds() {
+void prepare_kernels();
+static void allow_lds() { prepare_kernels(); }
+
+// the kernels' dynamic LDS above the default limit of a launch (before any graph capture)
+void prepare_kernels() {
     static bool done = false;
     if (done) return;
     GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -894,12 +909,13 @@ void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t 
 }
 
 void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint32_t stride, uint32_t n_max,
-                      float* out, hipStream_t s) {
+                      float* out, hipStream_t s, const uint32_t* rows, uint32_t nets) {
     if (n_max == 0) return;
+    if (nets == 0 || nets > M.nets) nets = M.nets;
     allow_lds();
-    const uint32_t per = (infer_blocks(n_max * M.nets) + M.nets - 1) / M.nets;
-    hipLaunchKernelGGL(k_infer<4>, dim3(per ? per : 1, M.nets), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
-                       inputs, 0u, n_dev, stride, out);
+    const uint32_t per = (infer_blocks(n_max * nets) + nets - 1) / nets;
+    hipLaunchKernelGGL(k_infer<4>, dim3(per ? per : 1, nets), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
+                       inputs, 0u, n_dev, stride, out, rows);
     GCHECK(hipGetLastError());
 }
 

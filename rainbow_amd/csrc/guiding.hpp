@@ -90,6 +90,8 @@ struct train_rows {
     sample_map map;
 };
 
+// sets the kernels' launch attributes; call once before capturing launches in a graph
+void prepare_kernels();
 void model_init(model& M, uint32_t seed, hipStream_t s, uint32_t nets = 1);  // allocates, He-uniform init
 void model_free(model& M);
 void model_get(model& M, float* host_params, hipStream_t s, uint32_t net = 0);  // N_PARAMS floats (padded layout)
@@ -99,10 +101,11 @@ void model_set(model& M, const float* host_params, hipStream_t s, uint32_t net =
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s);
 // the same with the row count in device memory (at most n_max rows; no host synchronisation)
 void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s);
-// every network at once: network k's rows are inputs/out rows [k * stride, k * stride + n_dev[k]),
-// n_dev[k] <= n_max (device memory)
+// networks 0..nets-1 (0: all) at once: network k's rows are inputs/out rows [k * stride,
+// k * stride + n_dev[k]), n_dev[k] <= n_max (device memory); with `rows`, network k's row r
+// is record rows[k * stride + r] of inputs and out
 void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint32_t stride, uint32_t n_max,
-                      float* out, hipStream_t s);
+                      float* out, hipStream_t s, const uint32_t* rows = nullptr, uint32_t nets = 0);
 // one training step of every network k with rows[k].n > 0 on its rows: inputs x 9, dirs x 3,
 // weights (radiance / pdf), all device.  Leaves each network's mean gradient in M.grad and
 // its loss in M.loss and, when apply, takes an Adam step.  Three launches, no memsets.

@@ -184,6 +184,8 @@ struct rbw_ctx {
         g_io_dir, g_io_c;
     uint32_t g_cap = 0, g_rec_cap = 0, g_t_cap = 0;
     double guide_last_ms = 0.0;  // network kernels of the last rbw_guide_forward / rbw_guide_train
+    // the job networks of guided batches (job i: network i)
+    rbw::guide::model gjobs;
     // batched jobs: scenes added with rbw_add_scene, and the scheduler
     std::vector<std::unique_ptr<rbw::host::host_scene>> pool;
     rbw_batch_state* batch = nullptr;  // owned; released by batch_free

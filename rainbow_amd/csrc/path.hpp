@@ -554,8 +554,9 @@ struct dguide {
     uint32_t* gq_count;
     float* query;                // 9 per row
     float* mix;                  // 48 per row (head outputs)
-    float* rec;                  // R_COUNT * (cap * max_depth) (SoA)
+    float* rec;                  // R_COUNT * (cap * rec_depth) (SoA)
     uint32_t* rec_flag;          // 0 none, 1 sampled, 2 closed
+    uint32_t rec_depth;          // records per slot (at least every path's max depth)
     uint32_t cap;
     // training batch
     float* t_in;                 // 9 per sample
@@ -839,8 +840,8 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
         // everything the path gathers from here on
         const uint32_t pend = P.U(U_PENDING, slot);
         if (pend != RBW_NONE) {
-            const size_t ri = (size_t)slot * prm.max_depth + pend;
-            const size_t rc = (size_t)P.cap * prm.max_depth;
+            const size_t ri = (size_t)slot * prm.g.rec_depth + pend;
+            const size_t rc = (size_t)P.cap * prm.g.rec_depth;
             prm.g.rec[R_MX * rc + ri] = value.x;
             prm.g.rec[R_MY * rc + ri] = value.y;
             prm.g.rec[R_MZ * rc + ri] = value.z;

@@ -29,6 +29,7 @@
 #include "../../include/rbw.h"
 #include "bvh.hpp"
 #include "device.hpp"
+#include "guided.hpp"
 #include "guiding.hpp"
 #include "volume.hpp"
 #include "image.hpp"
@@ -100,86 +101,16 @@ __global__ void k_shade(dscene S, dpaths P, dparams prm, const uint32_t* qin, co
             for (int k = 0; k < G_COUNT; k++) G.surf[k * cap + g] = v[k];
             G.surf_mat[g] = guide_mat;
             G.gq[g] = slot;
-            // MLP query: position in the scene box, shading normal and wo mapped to [0,1]
-            const f3 pn = (gsi.point - G.smin) * G.sinv;
-            const float qv[9] = {gclamp(pn.x, 0.0f, 1.0f), gclamp(pn.y, 0.0f, 1.0f), gclamp(pn.z, 0.0f, 1.0f),
-                                 0.5f * (gsi.sh.z.x + 1.0f), 0.5f * (gsi.sh.z.y + 1.0f), 0.5f * (gsi.sh.z.z + 1.0f),
-                                 0.5f * (gsi.wo.x + 1.0f), 0.5f * (gsi.wo.y + 1.0f), 0.5f * (gsi.wo.z + 1.0f)};
+            float qv[9];
+            guide_query(gsi, G.smin, G.sinv, qv);
 #pragma unroll
             for (int k = 0; k < 9; k++) G.query[(size_t)g * 9 + k] = qv[k];
         }
     }
 }
 
-// the vMF-mixture head of the guiding MLP (same parameterisation as the training loss)
-struct vmf_mixture {
-    f3 mu[guide::N_LOBES];
-    float kappa[guide::N_LOBES], w[guide::N_LOBES], norm[guide::N_LOBES];
-    __device__ void load(const float* o) {
-        float amax = -3.4e38f;
-        for (int j = 0; j < guide::N_LOBES; j++) amax = fmaxf(amax, o[5 * j + 4]);
-        float ws = 0.0f;
-        for (int j = 0; j < guide::N_LOBES; j++) {
-            w[j] = expf(o[5 * j + 4] - amax);
-            ws += w[j];
-            const f3 m = f3{o[5 * j], o[5 * j + 1], o[5 * j + 2]};
-            const float nm = fmaxf(sqrtf(m.x * m.x + m.y * m.y + m.z * m.z), 1e-12f);
-            mu[j] = f3{m.x / nm, m.y / nm, m.z / nm};
-            const float x = o[5 * j + 3];
-            const float sp = x > 20.0f ? x : log1pf(expf(x));
-            kappa[j] = fminf(sp + guide::KAPPA_MIN, guide::KAPPA_MAX);
-            norm[j] = kappa[j] / (2.0f * 3.14159265f * (-expm1f(-2.0f * kappa[j])));
-        }
-        for (int j = 0; j < guide::N_LOBES; j++) w[j] /= ws;
-    }
-    __device__ float pdf(f3 d) const {
-        float q = 0.0f;
-        for (int j = 0; j < guide::N_LOBES; j++)
-            q += w[j] * norm[j] * expf(kappa[j] * ((mu[j].x * d.x + mu[j].y * d.y + mu[j].z * d.z) - 1.0f));
-        return q;
-    }
-    // pick a lobe with u.x, then Jakob's vMF sampler around its mean
-    __device__ f3 sample(f2 u) const {
-        int j = 0;
-        float acc = 0.0f;
-        for (; j < guide::N_LOBES - 1; j++) {
-            if (u.x < acc + w[j]) break;
-            acc += w[j];
-        }
-        const float ux = fminf(fmaxf((u.x - acc) / fmaxf(w[j], 1e-20f), 0.0f), 0.99999994f);
-        const float k = kappa[j];
-        const float W = 1.0f + logf(ux + (1.0f - ux) * expf(-2.0f * k)) / k;
-        const float sr = sqrtf(fmaxf(0.0f, 1.0f - W * W));
-        const float phi = 2.0f * 3.14159265f * u.y;
-        const frame fr = frame_from_normal(mu[j]);
-        return local_to_world(fr, f3{sr * cosf(phi), sr * sinf(phi), W});
-    }
-};
-
-// sample from the guided technique: the vMF mixture with probability alpha, else
-// the BSDF; returns false when no direction results
-__device__ __forceinline__ bool guided_sample(const bsdf& fn, const surf& si, const vmf_mixture& mix, float alpha,
-                                              float sel, f2 u, f3& wi) {
-    if (sel < alpha) {
-        wi = mix.sample(u);
-        return true;
-    }
-    f3 f0;
-    float p0;
-    return fn.sample(si, u, wi, f0, p0);
-}
-
-// Guided vertex: next-event estimation whose second technique is the path's own
-// continuation, drawn from the guided density p_G = alpha q + (1 - alpha) p_bsdf.
-// The light sample is weighted by the power heuristic against p_G (the structure
-// of integrator.cpp:101-161 with the BSDF technique replaced); the continuation
-// ray, when it reaches an emitter, adds the emission weighted against the light
-// pdf (k_shade, bounce > 0).  At the last vertex there is no continuation and the
-// light sample takes weight 1.  Russian roulette as path_integrator.cpp:35-43.
-// The continuation direction becomes a training record, closed at the next vertex:
-// its target is everything the continuation brings back (emission weighted by its
-// MIS weight, plus all later bounces), so the guide learns the radiance that light
-// sampling does not already cover.
+// the guided vertex (guided.hpp) of each guide-queue row, its surface record and network
+// outputs stored by row
 __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, uint32_t* qout_count, uint32_t* cq,
                                uint32_t* cq_count) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,106 +131,8 @@ __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, 
         si.sh.x = f3{v[G_XX], v[G_XY], v[G_XZ]};
         si.sh.y = f3{v[G_YX], v[G_YY], v[G_YZ]};
         si.sh.z = f3{v[G_ZX], v[G_ZY], v[G_ZZ]};
-        const bsdf fn = make_bsdf(S, G.surf_mat[g], RBW_NONE);
-        vmf_mixture mix;
-        mix.load(G.mix + (size_t)g * guide::N_OUT_PAD);
-        sampler_state smp = load_sampler(P, prm, slot);
-        int bounce = (int)P.U(U_BOUNCE, slot);
-        f3 beta = P.F3(F_BX, slot);
-        const f3 wo_s = world_to_local(si.sh, si.wo);
-        const float a = G.alpha;
-        const bool last = (uint32_t)bounce + 1 >= prm.max_depth;
-        auto pdf_g = [&](f3 w, f3 w_s) { return a * mix.pdf(w) + (1.0f - a) * fn.pdf(wo_s, w_s); };
-        // light sample
-        const uint32_t ne = S.num_emitters;
-        const float u1 = smp.next();
-        uint32_t which = (uint32_t)floorf(u1 * (float)ne);
-        if (which > ne - 1) which = ne - 1;
-        const float pick = 1.0f / (float)ne;
-        const uint32_t lid = S.emitters[which];
-        const dentity L = S.ents[lid];
-        light_sample ls = sample_light(S, L, si.point, si.normal, smp.next2());
-        ls.pdf = ls.pdf * pick;
-        if (!is_black(ls.L) && ls.pdf > 0.0f) {
-            const f3 wi = world_to_local(si.sh, ls.wi);
-            f3 f = fn.evaluate(wo_s, wi);
-            f = f * fabsf(dot(ls.wi, si.sh.z));
-            if (!is_black(f)) {
-                f3 so, sd;
-                float slen;
-                spawn_ray_to(si.point, si.normal, ls.point, so, sd, slen);
-                const float w = last ? 1.0f : power_heuristic(ls.pdf, pdf_g(ls.wi, wi));
-                P.setF3(F_SOX, slot, so);
-                P.setF3(F_SDX, slot, sd);
-                P.F(F_SLEN, slot) = slen;
-                P.setF3(F_SFX, slot, f * ls.L * w / ls.pdf);
-                if (G.train && G.light_records) {
-                    // training record of the light sample: its MIS-weighted radiance / pdf
-                    P.F(F_MFX, slot) = w * (0.2126f * ls.L.x + 0.7152f * ls.L.y + 0.0722f * ls.L.z) / ls.pdf;
-                    P.F(F_MFY, slot) = __uint_as_float(g);
-                }
-                P.U(U_SENT, slot) = lid;
-                P.setF3(F_CBX, slot, beta);
-                P.U(U_CFLAGS, slot) = C_SHADOW | ((uint32_t)bounce << 8);
-                need_connect = true;
-            }
-        }
-        // continuation from the guided technique
-        bool cont = !last;
-        f3 wi = f3{0.0f, 0.0f, 0.0f};
-        float pdf = 0.0f;
-        if (cont) {
-            const float sel = smp.next();
-            const f2 u = smp.next2();
-            cont = guided_sample(fn, si, mix, a, sel, u, wi);
-        }
-        if (cont) {
-            const f3 wi_s = world_to_local(si.sh, wi);
-            const f3 f = fn.evaluate(wo_s, wi_s);
-            pdf = pdf_g(wi, wi_s);
-            if (is_black(f) || !(pdf > 0.0f)) {
-                cont = false;
-            } else {
-                beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
-                f3 no, nd;
-                spawn_ray(si.point, si.normal, wi, no, nd);
-                P.setF3(F_OX, slot, no);
-                P.setF3(F_DX, slot, nd);
-                // what the next vertex needs to weight emission it finds against light sampling
-                P.setF3(F_MPX, slot, si.point);
-                P.setF3(F_MNX, slot, si.normal);
-                P.setF3(F_MWX, slot, wi);
-                P.F(F_MPDF, slot) = pdf;
-            }
-        }
-        if (cont) {
-            const float mc = spectrum_max(beta * 1.0f);
-            if (mc < prm.rr_threshold && bounce > 3) {
-                const float qq = gmax(0.05f, 1.0f - mc);
-                if (smp.next() < qq) cont = false;
-                else beta = beta / (1.0f - qq);
-            }
-        }
-        const int b0 = bounce;
-        bounce += 1;
-        alive = cont;
-        if (alive && G.train) {
-            const size_t ri = (size_t)slot * prm.max_depth + (uint32_t)b0;
-            const size_t rc = (size_t)P.cap * prm.max_depth;
-            for (int k = 0; k < 9; k++) G.rec[(R_Q0 + k) * rc + ri] = G.query[(size_t)g * 9 + k];
-            G.rec[R_DX * rc + ri] = wi.x;
-            G.rec[R_DY * rc + ri] = wi.y;
-            G.rec[R_DZ * rc + ri] = wi.z;
-            G.rec[R_PDF * rc + ri] = pdf;
-            G.rec[R_BX * rc + ri] = beta.x;
-            G.rec[R_BY * rc + ri] = beta.y;
-            G.rec[R_BZ * rc + ri] = beta.z;
-            G.rec_flag[ri] = 1;
-            P.U(U_PENDING, slot) = (uint32_t)b0;
-        }
-        P.setF3(F_BX, slot, beta);
-        P.U(U_BOUNCE, slot) = (uint32_t)bounce;
-        store_sampler(P, slot, smp);
+        guided_vertex(S, P, prm, slot, si, G.surf_mat[g], G.mix + (size_t)g * guide::N_OUT_PAD,
+                      G.query + (size_t)g * 9, g, alive, need_connect);
     }
     const uint32_t ai = block_append(qout_count, alive);
     if (alive) qout[ai] = slot;
@@ -313,8 +146,7 @@ __global__ void k_shade_guided(dscene S, dpaths P, dparams prm, uint32_t* qout, 
 constexpr uint32_t COLLECT_PER_THREAD = 8;
 __global__ __launch_bounds__(256) void k_collect(dpaths P, dparams prm, uint32_t n) {
     const dguide& G = prm.g;
-    const size_t rc = (size_t)P.cap * prm.max_depth;
-    const size_t total = (size_t)n * prm.max_depth;
+    const size_t total = (size_t)n * G.rec_depth;
     const size_t first = (size_t)blockIdx.x * blockDim.x * COLLECT_PER_THREAD + threadIdx.x;
     float c[COLLECT_PER_THREAD];
     uint32_t takes = 0, mask = 0;
@@ -323,17 +155,8 @@ __global__ __launch_bounds__(256) void k_collect(dpaths P, dparams prm, uint32_t
         const size_t ri = first + (size_t)j * blockDim.x;
         c[j] = 0.0f;
         if (ri < total && G.rec_flag[ri] == 2) {
-            const uint32_t slot = (uint32_t)(ri / prm.max_depth);
-            const f3 vf = P.F3(F_VX, slot);
-            const f3 m = f3{G.rec[R_MX * rc + ri], G.rec[R_MY * rc + ri], G.rec[R_MZ * rc + ri]};
-            const f3 b = f3{G.rec[R_BX * rc + ri], G.rec[R_BY * rc + ri], G.rec[R_BZ * rc + ri]};
-            const f3 d = vf - m;
-            const float Lx = b.x > 0.0f ? d.x / b.x : 0.0f, Ly = b.y > 0.0f ? d.y / b.y : 0.0f,
-                        Lz = b.z > 0.0f ? d.z / b.z : 0.0f;
-            const float lum = 0.2126f * Lx + 0.7152f * Ly + 0.0722f * Lz;
-            const float pdf = G.rec[R_PDF * rc + ri];
-            c[j] = pdf > 0.0f ? lum / pdf : 0.0f;
-            if (c[j] > 0.0f && isfinite(c[j])) {
+            c[j] = record_target(P, G, (uint32_t)(ri / G.rec_depth), ri);
+            if (c[j] > 0.0f) {
                 mask |= 1u << j;
                 takes++;
             }
@@ -345,13 +168,7 @@ __global__ __launch_bounds__(256) void k_collect(dpaths P, dparams prm, uint32_t
     for (uint32_t j = 0; j < COLLECT_PER_THREAD; j++) {
         if (!((mask >> j) & 1u)) continue;
         const size_t ri = first + (size_t)j * blockDim.x;
-        if (k < G.t_cap) {
-            for (int q = 0; q < 9; q++) G.t_in[(size_t)k * 9 + q] = G.rec[(R_Q0 + q) * rc + ri];
-            G.t_dir[(size_t)k * 3 + 0] = G.rec[R_DX * rc + ri];
-            G.t_dir[(size_t)k * 3 + 1] = G.rec[R_DY * rc + ri];
-            G.t_dir[(size_t)k * 3 + 2] = G.rec[R_DZ * rc + ri];
-            G.t_c[k] = c[j];
-        }
+        if (k < G.t_cap) record_to_row(P, G, ri, c[j], G.t_in, G.t_dir, G.t_c, k);
         k++;
     }
 }
@@ -793,6 +610,7 @@ rbw_ctx::~rbw_ctx() {
     (void)hipSetDevice(device);
     batch_free(this);
     guide::model_free(gm);
+    guide::model_free(gjobs);
     if (host_count) (void)hipHostFree(host_count);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -955,6 +773,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         dp.g.train = go->train ? 1u : 0u;
         dp.g.light_records = go->light_records ? 1u : 0u;
         dp.g.depth = go->guide_depth ? go->guide_depth : pr.max_depth;
+        dp.g.rec_depth = pr.max_depth;
         f3 smin = ctx->scene_min, smax = ctx->scene_max;
         if (go->scene_min[0] != 0.0f || go->scene_min[1] != 0.0f || go->scene_min[2] != 0.0f ||
             go->scene_max[0] != 0.0f || go->scene_max[1] != 0.0f || go->scene_max[2] != 0.0f) {
@@ -1463,6 +1282,72 @@ rbw_status rbw_guide_train(rbw_ctx* ctx, uint32_t n, const float* inputs, const 
         if (grad) HIP_CHECK(hipMemcpyAsync(grad, ctx->gm.grad, sizeof(float) * guide::N_PARAMS, hipMemcpyDeviceToHost, s));
         if (loss) HIP_CHECK(hipMemcpyAsync(loss, ctx->gm.loss, sizeof(float), hipMemcpyDeviceToHost, s));
         ctx->timer.end(s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        ctx->guide_last_ms = ctx->timer.elapsed_last();
+    });
+}
+
+rbw_status rbw_guide_jobs_reset(rbw_ctx* ctx, uint32_t n_networks, uint32_t seed) {
+    return guarded([&] {
+        require(ctx != nullptr, RBW_ERR_INVALID_ARGUMENT, "context is null");
+        require(n_networks >= 1 && n_networks <= guide::MAX_NETS, RBW_ERR_INVALID_ARGUMENT,
+                "n_networks must be in [1, 32]");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        guide::model_init(ctx->gjobs, seed, ctx->stream, n_networks);
+    });
+}
+
+rbw_status rbw_guide_jobs_get_params(rbw_ctx* ctx, uint32_t network, float* params) {
+    return guarded([&] {
+        require(ctx != nullptr && params != nullptr, RBW_ERR_INVALID_ARGUMENT, "null argument");
+        require(network < ctx->gjobs.nets, RBW_ERR_INVALID_ARGUMENT, "no such job network");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        guide::model_get(ctx->gjobs, params, ctx->stream, network);
+    });
+}
+
+rbw_status rbw_guide_jobs_set_params(rbw_ctx* ctx, uint32_t network, const float* params) {
+    return guarded([&] {
+        require(ctx != nullptr && params != nullptr, RBW_ERR_INVALID_ARGUMENT, "null argument");
+        require(network < ctx->gjobs.nets, RBW_ERR_INVALID_ARGUMENT, "no such job network");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        guide::model_set(ctx->gjobs, params, ctx->stream, network);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+rbw_status rbw_guide_jobs_train(rbw_ctx* ctx, uint32_t n_networks, const uint32_t* rows, const float* inputs,
+                                const float* dirs, const float* c, float learning_rate, uint32_t apply, float* loss,
+                                float* grad) {
+    return guarded([&] {
+        require(ctx != nullptr && rows != nullptr, RBW_ERR_INVALID_ARGUMENT, "null argument");
+        require(n_networks >= 1 && n_networks <= ctx->gjobs.nets, RBW_ERR_INVALID_ARGUMENT,
+                "n_networks must be in [1, the job networks]");
+        std::vector<guide::train_rows> tr(ctx->gjobs.nets);
+        uint64_t total = 0;
+        for (uint32_t k = 0; k < n_networks; k++) {
+            tr[k].base = uint32_t(total);
+            tr[k].n = rows[k];
+            total += rows[k];
+        }
+        require(total > 0 && total <= (1u << 22), RBW_ERR_INVALID_ARGUMENT, "1 to 2^22 rows in all");
+        require(inputs && dirs && c, RBW_ERR_INVALID_ARGUMENT, "null arrays");
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ctx->g_io_in.upload(inputs, size_t(total) * 9, s);
+        ctx->g_io_dir.upload(dirs, size_t(total) * 3, s);
+        ctx->g_io_c.upload(c, total, s);
+        ctx->timer.reset();
+        ctx->timer.begin(RBW_KERNEL_GUIDE_TRAIN, s);
+        guide::adam_opts ao;
+        ao.lr = learning_rate;
+        guide::train_step_batched(ctx->gjobs, ctx->g_io_in.as<float>(), ctx->g_io_dir.as<float>(),
+                                  ctx->g_io_c.as<float>(), tr, ao, apply != 0, s);
+        ctx->timer.end(s);
+        if (grad)
+            HIP_CHECK(hipMemcpyAsync(grad, ctx->gjobs.grad, sizeof(float) * guide::N_PARAMS * n_networks,
+                                     hipMemcpyDeviceToHost, s));
+        if (loss) HIP_CHECK(hipMemcpyAsync(loss, ctx->gjobs.loss, sizeof(float) * n_networks, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         ctx->guide_last_ms = ctx->timer.elapsed_last();
     });

@@ -625,21 +625,25 @@ def narrow_aperture_room(level=8, width=512, height=512, gap=0.3, radiance=50.0,
     return sc, cam
 
 
-def room_variants(n=16, seed=4, res=256):
+def room_variants(n=16, seed=4, res=256, tris=(1e5, 1.3e6)):
     """Variants of the narrow-aperture room: gap U[0.1, 0.5], a seeded fBm-displaced grid
-    sphere of 1e5-1.3e6 triangles (log-uniform) with per-triangle Lambert / GGX materials
-    (alpha U[0.02, 0.6]), the light aimed 45 degrees up through the gap; res x res.
-    Returns a list of dicts {scene, camera, gap, triangles, seed}."""
+    sphere of tris[0]-tris[1] triangles (log-uniform) with per-triangle Lambert / GGX
+    materials (alpha U[0.02, 0.6]), the light aimed 45 degrees up through the gap; res x res,
+    or, for res = (lo, hi), a square resolution log-uniform in [lo, hi] per variant (drawn
+    from a generator of its own, so gaps and meshes do not depend on it).
+    Returns a list of dicts {scene, camera, gap, triangles, seed, res}."""
     rng = np.random.default_rng(seed)
+    rres = np.random.default_rng(seed + 1000)
     out = []
     for i in range(n):
         gap = float(rng.uniform(0.1, 0.5))
-        t = int(round(math.exp(rng.uniform(math.log(1e5), math.log(1.3e6)))))
-        sc, cam = narrow_aperture_room(level=8, width=res, height=res, gap=gap, light_tilt=45.0, mesh=False)
+        t = int(round(math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1])))))
+        r = res if np.isscalar(res) else int(round(math.exp(rres.uniform(math.log(res[0]), math.log(res[1])))))
+        sc, cam = narrow_aperture_room(level=8, width=r, height=r, gap=gap, light_tilt=45.0, mesh=False)
         P, F, N = fbm_displaced_grid_sphere(t, seed=2000 + i, center=(-0.2, -0.45, -0.3), radius=0.45)
         tm = per_triangle_materials(sc, len(F), 2000 + i, alpha_range=(0.02, 0.6))
         sc.add_mesh(P, F, tm[0], normals=N, tri_material=tm)
-        out.append({"scene": sc, "camera": cam, "gap": gap, "triangles": len(F), "seed": 2000 + i})
+        out.append({"scene": sc, "camera": cam, "gap": gap, "triangles": len(F), "seed": 2000 + i, "res": r})
     return out
 
 

@@ -153,6 +153,33 @@ class Tracer:
     def remove_scene(self, sid):
         _check(self.lib.rbw_remove_scene(self.h, sid))
 
+    # -- the job networks of guided batches (job i: network i) ---------------
+    def guide_jobs_reset(self, n_networks, seed=0):
+        _check(self.lib.rbw_guide_jobs_reset(self.h, n_networks, seed))
+
+    def guide_jobs_params(self, network):
+        out = np.zeros(abi.RBW_GUIDE_PARAMS, np.float32)
+        _check(self.lib.rbw_guide_jobs_get_params(self.h, network, abi.ptr(out, abi.f32p)))
+        return out
+
+    def guide_jobs_set_params(self, network, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _check(self.lib.rbw_guide_jobs_set_params(self.h, network, abi.ptr(p, abi.f32p)))
+
+    def guide_jobs_train(self, batches, lr=1e-2, apply=False):
+        """one batched training step: batches[k] = (inputs n_k x 9, dirs n_k x 3, c n_k) of
+        network k; returns (losses, gradients n_networks x RBW_GUIDE_PARAMS)"""
+        rows = np.array([len(b[2]) for b in batches], np.uint32)
+        x = np.ascontiguousarray(np.concatenate([b[0] for b in batches]), np.float32)
+        d = np.ascontiguousarray(np.concatenate([b[1] for b in batches]), np.float32)
+        c = np.ascontiguousarray(np.concatenate([b[2] for b in batches]), np.float32)
+        loss = np.zeros(len(batches), np.float32)
+        grad = np.zeros((len(batches), abi.RBW_GUIDE_PARAMS), np.float32)
+        _check(self.lib.rbw_guide_jobs_train(self.h, len(batches), abi.ptr(rows, abi.u32p), abi.ptr(x, abi.f32p),
+                                             abi.ptr(d, abi.f32p), abi.ptr(c, abi.f32p), lr, int(apply),
+                                             abi.ptr(loss, abi.f32p), abi.ptr(grad, abi.f32p)))
+        return loss, grad
+
     def submit_jobs(self, jobs, opts=None, images=True):
         """jobs: list of dicts {scene, camera, spp, max_depth, rr_threshold=1, seed=0}.
         Returns (images or None, per-job stats, batch stats)."""
@@ -193,9 +220,13 @@ class Tracer:
         return o, ms.value
 
 
-def batch_opts(block_size=0, pool_size=0, use_graph=True, autotune=False, **kw):
+def batch_opts(block_size=0, pool_size=0, use_graph=True, autotune=False, guiding=None, **kw):
+    """guiding: an rbw_guiding_opts (kept alive by the returned options) for a guided batch"""
     o = abi.rbw_batch_opts(block_size=block_size, pool_size=pool_size, bvh_width=0, use_graph=int(use_graph),
                            autotune=int(autotune))
+    if guiding is not None:
+        o._guiding_ref = guiding
+        o.guiding = C.pointer(guiding)
     for k, v in kw.items():
         setattr(o, k, v)
     return o

@@ -1,0 +1,124 @@
+"""Learned guiding in batched jobs (no counterpart in the reference): one network per
+job, all jobs in one persistent-slot launch, every job's guided vertices inferred by one
+launch over all networks, and all networks trained by one batched training kernel.
+
+* the batched training step's per-network losses and gradients against the FP16-rounded
+  oracle (oracle/guiding_ref.py), networks with different parameters and ragged row counts;
+* per job, the guided batch's image mean equals the plain estimator's (unbiased) within
+  stated standard errors, while the networks train online;
+* after online training, the guided batch's relMSE at equal spp is below the plain
+  batch's over the jobs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import guiding_ref as G
+from rainbow_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 9))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = rng.random(n) + 0.1
+    return x, d, c
+
+
+def test_gpu_job_networks_gradients_match_oracle(tracer):
+    sizes = [1000, 2048, 777]
+    tracer.guide_jobs_reset(4, seed=3)
+    params, batches = [], []
+    for k, n in enumerate(sizes):
+        Ws, bs = G.init_params(20 + k)
+        tracer.guide_jobs_set_params(k, G.to_padded(Ws, bs))
+        params.append((Ws, bs))
+        batches.append(_batch(n, 30 + k))
+    loss, grad = tracer.guide_jobs_train(batches)
+    for k, ((Ws, bs), (x, d, c)) in enumerate(zip(params, batches)):
+        loss16, dW16, db16 = G.grads_f16(Ws, bs, x, d, c)
+        gW, gb = G.from_padded(grad[k])
+        print(f"network {k}: loss {loss[k]:.6f} FP16 oracle {loss16:.6f}")
+        assert abs(loss[k] - loss16) < 1e-3 * abs(loss16) + 1e-4
+        for l in range(6):
+            a = gW[l][:, :G.N_FEAT] if l == 0 else gW[l]
+            rel = np.linalg.norm(a - dW16[l]) / np.linalg.norm(dW16[l])
+            relb = np.linalg.norm(gb[l] - db16[l]) / np.linalg.norm(db16[l])
+            assert rel < 1e-2 and relb < 1e-2, (k, l, rel, relb)
+    # networks keep their own parameters: an Adam step moves each by its own gradient
+    before = [tracer.guide_jobs_params(k) for k in range(3)]
+    tracer.guide_jobs_train(batches, lr=1e-3, apply=True)
+    for k in range(3):
+        step = tracer.guide_jobs_params(k) - before[k]
+        g = grad[k]
+        live = np.abs(g) > 1e-6 * np.abs(g).max()
+        # the first Adam step is -lr * sign(g) where the gradient is well above eps
+        assert np.mean(np.sign(step[live]) == -np.sign(g[live])) > 0.99, k
+
+
+@pytest.fixture(scope="module")
+def rooms(tracer):
+    """four small narrow-aperture rooms (gap 0.1-0.5, 2e4-6e4 triangles), 48x48, pooled"""
+    rs = scenes.room_variants(n=4, seed=11, res=48, tris=(2e4, 6e4))
+    ids = [tracer.add_scene(r["scene"]) for r in rs]
+    yield rs, ids
+    for i in ids:
+        tracer.remove_scene(i)
+
+
+def _jobs(rs, ids, spp, seed0):
+    return [{"scene": ids[i], "camera": r["camera"], "spp": spp, "max_depth": 8, "seed": seed0 + 97 * i}
+            for i, r in enumerate(rs)]
+
+
+def test_gpu_guided_batch_unbiased_per_job(tracer, rooms):
+    from rainbow_amd.tracer import batch_opts, guiding_opts
+    rs, ids = rooms
+    ref, _, _ = tracer.submit_jobs(_jobs(rs, ids, 4096, 1), batch_opts())
+    tracer.guide_jobs_reset(len(rs), seed=5)
+    go = guiding_opts(train=True, steps_per_pass=2, batch=1 << 14)
+    chunks = []
+    steps = 0
+    for k in range(8):
+        imgs, _, st = tracer.submit_jobs(_jobs(rs, ids, 64, 1000 + 10 * k), batch_opts(guiding=go))
+        chunks.append(imgs)
+        steps += st["guide_steps"]
+    assert steps > 0
+    for i in range(len(rs)):
+        means = np.array([float(c[i].mean()) for c in chunks])
+        m, se = means.mean(), means.std(ddof=1) / np.sqrt(len(means))
+        mr = float(ref[i].mean())
+        print(f"job {i}: guided {m:.5f} +- {se:.1e}, plain 4096 spp {mr:.5f}")
+        assert abs(m - mr) < 4 * se * np.sqrt(1 + 512 / 4096) + 1e-6, i
+
+
+def test_gpu_guided_batch_reduces_error(tracer, rooms):
+    """after 24 training submissions (4 Adam steps of up to 2^16 rows per network after every
+    8 bounce iterations), the guided batch has lower relMSE than the plain batch at equal spp,
+    summed over the jobs; every job's image mean matches the judge's"""
+    from rainbow_amd.tracer import batch_opts, guiding_opts
+    rs, ids = rooms
+    judge, _, _ = tracer.submit_jobs(_jobs(rs, ids, 8192, 7), batch_opts())
+    plain, _, _ = tracer.submit_jobs(_jobs(rs, ids, 256, 11), batch_opts())
+    tracer.guide_jobs_reset(len(rs), seed=9)
+    for k in range(24):
+        tracer.submit_jobs(_jobs(rs, ids, 16, 2000 + k), batch_opts(guiding=guiding_opts(train=True, batch=1 << 16)))
+    guided, _, _ = tracer.submit_jobs(_jobs(rs, ids, 256, 13), batch_opts(guiding=guiding_opts(train=False)))
+    ep = sum(scenes.rel_mse(p, j) for p, j in zip(plain, judge))
+    eg = sum(scenes.rel_mse(g, j) for g, j in zip(guided, judge))
+    print("relMSE over jobs: plain", ep, "guided", eg, "ratio", eg / ep)
+    assert eg < ep
+    for g, j in zip(guided, judge):
+        assert abs(float(g.mean()) - float(j.mean())) < 0.02 * float(j.mean())
+
+
+def test_gpu_guided_batch_needs_a_network_per_job(tracer, rooms):
+    from rainbow_amd.tracer import RbwError, batch_opts, guiding_opts
+    rs, ids = rooms
+    tracer.guide_jobs_reset(2, seed=1)
+    with pytest.raises(RbwError):
+        tracer.submit_jobs(_jobs(rs, ids, 1, 1), batch_opts(guiding=guiding_opts()))
