@@ -277,13 +277,22 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
         }
     }
     // the iteration in which a job's last path retired: the wave records its first
-    // retiring job (folded by the next plan); lanes of another job, met only where
-    // the job queue moves on, use an atomic
+    // retiring job (folded by the next plan); each other job with retiring lanes in the
+    // wave takes one atomic from its first lane.  (Slots regenerate in place, so a wave
+    // of a shared launch holds paths of several jobs; an atomic per lane on the jobs'
+    // few counters serialised the kernel.)
     {
-        const int leader = __ffsll((unsigned long long)__ballot(dying)) - 1;
+        uint64_t pending = __ballot(dying);
+        const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t jl = __shfl(j, leader < 0 ? 0 : leader);
         if (leader >= 0 && (int)__lane_id() == leader) B.wave_job[slot >> 6] = jl;
-        if (dying && j != jl) atomicMax(&B.last_iter[j], iter);
+        pending &= ~__ballot(dying && j == jl);
+        while (pending) {
+            const int first = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t jo = __shfl(j, first);
+            if ((int)__lane_id() == first) atomicMax(&B.last_iter[jo], iter);
+            pending &= ~__ballot(dying && j == jo);
+        }
     }
     uint32_t nfree = 0;
     const bool free_slot = in && st == 0u;

@@ -99,64 +99,92 @@ constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 64 * N_IN * 4;        //
 static_assert(INF_XS % 16 == 0, "aligned input rows");
 
 // R record tiles (16 rows each) per wave share every weight fragment read from LDS.
-// Network blockIdx.y: rows [net * stride, net * stride + n), n from n_dev[net] (or n_host);
-// with `rows`, row r of the network is record rows[net * stride + r] of inputs and out.
+// Network k (k < nets) has rows [k * stride, k * stride + n_k), n_k from n_dev[k] (or n_host
+// with one network); with `rows`, row r of network k is record rows[k * stride + r] of
+// inputs and out.  The networks' rows are cut into steps of INF_WAVES groups of 16 R rows
+// (a step never spans two networks) and each block takes a contiguous range of the steps
+// of all networks, so the work spreads over the whole grid however unevenly the networks'
+// rows fall; a block stages a network's weights in LDS when its range enters that network.
+constexpr uint32_t INF_STEP_ROWS = INF_WAVES * 64;  // R = 4
 template <int R>
 __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __restrict__ pack,
                                                           const float* __restrict__ bias,
                                                           const float* __restrict__ inputs, uint32_t n_host,
-                                                          const uint32_t* __restrict__ n_dev, uint32_t stride,
-                                                          float* __restrict__ out,
+                                                          const uint32_t* __restrict__ n_dev, uint32_t nets,
+                                                          uint32_t stride, float* __restrict__ out,
                                                           const uint32_t* __restrict__ rows = nullptr) {
-    const uint32_t net = blockIdx.y;
-    const uint32_t n = n_dev ? n_dev[net] : n_host;
-    // blocks past the network's rows leave before staging its weights (grids are sized for
-    // the most rows a network may have)
-    if (n == 0 || blockIdx.x * INF_WAVES * 16 * R >= n) return;
+    static_assert(INF_WAVES * 16 * R == INF_STEP_ROWS, "step size");
+    static_assert(MAX_NETS <= 64, "one network per lane");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    // lane k: the steps of networks 0..k (an inclusive scan over the wave)
+    uint32_t c = 0;
+    if ((uint32_t)lane < nets) {
+        const uint32_t nk = n_dev ? n_dev[lane] : n_host;
+        c = (nk + INF_STEP_ROWS - 1) / INF_STEP_ROWS;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)c, off);
+        if (lane >= off) c += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)c, 63);
+    const uint32_t s0 = (uint32_t)((uint64_t)blockIdx.x * total / gridDim.x);
+    const uint32_t s1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * total / gridDim.x);
+    if (s0 >= s1) return;
     extern __shared__ __align__(16) unsigned char smem[];
     _Float16* wl = reinterpret_cast<_Float16*>(smem);
     float* bl = reinterpret_cast<float*>(smem + size_t(FWD_FRAGS) * FRAG_HALFS * 2);
-    {
-        const h8* src = reinterpret_cast<const h8*>(pack + (size_t)net * PACK_HALFS);
-        h8* dst = reinterpret_cast<h8*>(wl);
-        for (int q = threadIdx.x; q < FWD_FRAGS * 64; q += blockDim.x) dst[q] = src[q];
-        for (int q = threadIdx.x; q < N_BIAS; q += blockDim.x) bl[q] = bias[(size_t)net * N_BIAS + q];
-    }
-    __syncthreads();
-    const float* in = rows ? inputs : inputs + (size_t)net * stride * N_IN;
-    float* o = rows ? out : out + (size_t)net * stride * N_OUT_PAD;
-    const uint32_t* rl = rows ? rows + (size_t)net * stride : nullptr;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int li = lane & 15, g = lane >> 4;
-    const uint32_t groups = (n + 16 * R - 1) / (16 * R);
-    // the wave's group of 16 R rows is staged in LDS (coalesced loads, the next group's in
+    // the first step of network k and the network of step st (both wave-uniform)
+    auto first_step = [&](uint32_t k) { return k == 0 ? 0u : (uint32_t)__shfl((int)c, (int)k - 1); };
+    auto net_of = [&](uint32_t st) {
+        return (uint32_t)__popcll(__ballot((uint32_t)lane + 1 < nets && c <= st));
+    };
+    // the wave's 16 R rows of a step are staged in LDS (coalesced loads, the next step's in
     // flight while this one computes)
     float* xs = reinterpret_cast<float*>(smem + INF_XS) + wave * 64 * N_IN;
     constexpr int XPL = 16 * R * N_IN / 64;  // floats per lane
     float xpre[XPL];
-    auto load_group = [&](uint32_t grp) {
+    auto load_step = [&](uint32_t st) {
+        const uint32_t k = st < s1 ? net_of(st) : 0u;
+        const uint32_t nk = st < s1 ? (n_dev ? n_dev[k] : n_host) : 0u;
+        const uint32_t r0 = (st - first_step(k)) * INF_STEP_ROWS + (uint32_t)wave * 16 * R;
+        const float* in = rows ? inputs : inputs + (size_t)k * stride * N_IN;
 #pragma unroll
-        for (int k = 0; k < XPL; k++) {
-            const size_t q = (size_t)grp * 16 * R * N_IN + lane + 64 * k;
-            if (grp < groups && q < (size_t)n * N_IN) {
-                const size_t r = q / N_IN;
-                xpre[k] = rl ? in[(size_t)rl[r] * N_IN + (q - r * N_IN)] : in[q];
+        for (int q = 0; q < XPL; q++) {
+            const uint32_t e = (uint32_t)lane + 64u * q, r = r0 + e / N_IN;
+            if (st < s1 && r < nk) {
+                const size_t rec = rows ? (size_t)rows[(size_t)k * stride + r] : (size_t)r;
+                xpre[q] = in[rec * N_IN + e % N_IN];
             } else {
-                xpre[k] = 0.0f;
+                xpre[q] = 0.0f;
             }
         }
     };
-    load_group(blockIdx.x * INF_WAVES + wave);
-    for (uint32_t grp = blockIdx.x * INF_WAVES + wave; grp < groups; grp += gridDim.x * INF_WAVES) {
+    load_step(s0);
+    uint32_t staged = 0xFFFFFFFFu;
+    for (uint32_t st = s0; st < s1; st++) {
+        const uint32_t net = net_of(st);
+        if (net != staged) {
+            __syncthreads();  // every wave is done with the previous network's weights
+            const h8* src = reinterpret_cast<const h8*>(pack + (size_t)net * PACK_HALFS);
+            h8* dst = reinterpret_cast<h8*>(wl);
+            for (int q = threadIdx.x; q < FWD_FRAGS * 64; q += blockDim.x) dst[q] = src[q];
+            for (int q = threadIdx.x; q < N_BIAS; q += blockDim.x) bl[q] = bias[(size_t)net * N_BIAS + q];
+            __syncthreads();
+            staged = net;
+        }
+        const uint32_t n = n_dev ? n_dev[net] : n_host;
+        float* o = rows ? out : out + (size_t)net * stride * N_OUT_PAD;
+        const uint32_t* rl = rows ? rows + (size_t)net * stride : nullptr;
+        const uint32_t r0 = (st - first_step(net)) * INF_STEP_ROWS + (uint32_t)wave * 16 * R;
 #pragma unroll
         for (int k = 0; k < XPL; k++) xs[lane + 64 * k] = xpre[k];
         wave_sync();
-        load_group(grp + gridDim.x * INF_WAVES);
+        load_step(st + 1);
         uint32_t row[R];
 #pragma unroll
-        for (int t = 0; t < R; t++) {
-            row[t] = grp * 16 * R + 16 * t + li;
-        }
+        for (int t = 0; t < R; t++) row[t] = r0 + 16 * t + li;
         f4 acc[R][4];
         h8 X[R][2];
 #pragma unroll
@@ -887,24 +915,23 @@ void model_init(model& M, uint32_t seed, hipStream_t s, uint32_t nets) {
 
 // persistent inference blocks: two fit a CU's LDS
 static uint32_t infer_blocks(uint32_t rows) {
-    const uint32_t groups = (rows + 63) / 64;
-    const uint32_t want = (groups + INF_WAVES - 1) / INF_WAVES;
+    const uint32_t want = (rows + INF_STEP_ROWS - 1) / INF_STEP_ROWS;
     return want < 512 ? (want ? want : 1) : 512;
 }
 
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s) {
     if (n == 0) return;
     allow_lds();
-    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n), 1), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs,
-                       n, nullptr, 0u, out);
+    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs, n,
+                       nullptr, 1u, 0u, out);
     GCHECK(hipGetLastError());
 }
 
 void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s) {
     if (n_max == 0) return;
     allow_lds();
-    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n_max), 1), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
-                       inputs, 0u, n_dev, 0u, out);
+    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n_max)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
+                       inputs, 0u, n_dev, 1u, 0u, out);
     GCHECK(hipGetLastError());
 }
 
@@ -913,9 +940,9 @@ void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint
     if (n_max == 0) return;
     if (nets == 0 || nets > M.nets) nets = M.nets;
     allow_lds();
-    const uint32_t per = (infer_blocks(n_max * nets) + nets - 1) / nets;
-    hipLaunchKernelGGL(k_infer<4>, dim3(per ? per : 1, nets), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
-                       inputs, 0u, n_dev, stride, out, rows);
+    // (the rows of all networks together number at most n_max * nets)
+    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks((uint32_t)std::min<uint64_t>(uint64_t(n_max) * nets, 1u << 30))),
+                       dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs, 0u, n_dev, nets, stride, out, rows);
     GCHECK(hipGetLastError());
 }
 
