@@ -413,7 +413,8 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_gui
 
 // next-event estimation rays; then the block's count of slots the next
 // regeneration will find free (free or finished)
-template <int W>
+// FAST: every job's scene has few planar lights (connect_path's occlusion-only form)
+template <int W, bool FAST>
 __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_c
             dscene Sj;
             dparams pj;
             job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-            connect_path<W, false, true>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
+            connect_path<W, false, true, FAST>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
             st &= ~ST_CONNECT;
             P.U(U_STATE, slot) = st;
         }
@@ -717,6 +718,12 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     const size_t trav_lds = lds_stack_bytes(bs);
     // per-kernel HIP events only without graph capture (kernel_ms of rbw_stats)
     kernel_timer* tm = cfg.graph ? nullptr : timer;
+    // the occlusion-only connect kernel when every job's scene has few planar lights
+    bool fast = base.debug == nullptr;
+    for (const djob& J : hjobs) fast = fast && (J.light_flags & (k_lights_few | k_lights_planar)) == (k_lights_few | k_lights_planar);
+    void (*connect_kernel)(dscene, dpaths, dparams, dbatch) =
+        S.width == 8 ? (fast ? k_b_connect<8, true> : k_b_connect<8, false>)
+                     : (fast ? k_b_connect<2, true> : k_b_connect<2, false>);
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
@@ -742,8 +749,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
-        hipLaunchKernelGGL(S.width == 8 ? k_b_connect<8> : k_b_connect<2>, dim3(grid), dim3(bs), trav_lds, s, S, P,
-                           base, B);
+        hipLaunchKernelGGL(connect_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
         if (tm) tm->end(s);
     };
     if (b.exec) {

@@ -975,14 +975,18 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
 }
 
 // shadow ray, MIS ray and light pdf of next-event estimation for the path in `slot`;
-// value += beta * L (integrator.cpp:101-201)
-template <int W, bool COUNT, bool LDS = false>
+// value += beta * L (integrator.cpp:101-201).  FAST: the scene's lights are planar and
+// few (k_lights_planar | k_lights_few) and no debug hits are recorded, so both rays are
+// occlusion queries (any-hit walks) and the closest-hit walks are not instantiated.  The
+// MIS ray's vertex (point, normal, direction) is read again after its walk rather than
+// held in registers across it.
+template <int W, bool COUNT, bool LDS = false, bool FAST = false>
 __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                              trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
                                              uint32_t& nlight, bool* lit = nullptr) {
     const uint32_t flags = P.U(U_CFLAGS, slot);
     const int bounce = (int)(flags >> 8);
-    uint32_t* dbg = debug_slot(P, prm, slot, bounce);
+    uint32_t* dbg = FAST ? nullptr : debug_slot(P, prm, slot, bounce);
     f3 L = f3{0.0f, 0.0f, 0.0f};
     if (flags & C_SHADOW) {
         nshadow = 1;
@@ -991,7 +995,8 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         const float slen = P.F(F_SLEN, slot);
         // a planar light cannot be hit before the sample point on it, so the closest visible
         // hit is the light or none exactly when no other visible surface is in the way
-        const bool any = prm.shadow_mode == RBW_SHADOW_ANY_HIT || (!dbg && (S.light_flags & k_lights_planar));
+        const bool any =
+            FAST || prm.shadow_mode == RBW_SHADOW_ANY_HIT || (!dbg && (S.light_flags & k_lights_planar));
         hitrec h;
         if (any)
             h = traverse<W, true, true, COUNT, LDS>(S, so, sd, slen, prm.tri_test, lid, cnt);
@@ -1009,29 +1014,39 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
     if (flags & C_MIS) {
         nmis = 1;
         // find_emitter (integrator.cpp:49-82)
-        const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
-        f3 eo, ed;
-        spawn_ray(mp, mn, mw, eo, ed);
-        hitrec h;
-        if (!dbg && (S.light_flags & k_lights_few)) {
-            // the closest visible hit is an emitter exactly when the nearest visible emitter
-            // triangle along the ray has no other visible surface before it: test the few
-            // emitter triangles, then walk the tree only for rays that reach one, as an
-            // occlusion query up to it
-            h = nearest_emitter_hit(S, eo, ed, prm.tri_test);
-            if (h.prim != RBW_MISS) {
-                const uint32_t le = S.tri_entity[h.prim];
-                const local_ray lr = make_local_ray(eo, ed);
-                const hitrec o = traverse<W, true, true, COUNT, LDS>(S, eo, ed, h.t * lr.L1, prm.tri_test, le, cnt);
-                if (o.prim != RBW_MISS) h.prim = RBW_MISS;
+        uint32_t prim;
+        {
+            const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
+            f3 eo, ed;
+            spawn_ray(mp, mn, mw, eo, ed);
+            hitrec h;
+            if (FAST || (!dbg && (S.light_flags & k_lights_few))) {
+                // the closest visible hit is an emitter exactly when the nearest visible
+                // emitter triangle along the ray has no other visible surface before it: test
+                // the few emitter triangles, then walk the tree only for rays that reach one,
+                // as an occlusion query up to it
+                h = nearest_emitter_hit(S, eo, ed, prm.tri_test);
+                if (h.prim != RBW_MISS) {
+                    const uint32_t le = S.tri_entity[h.prim];
+                    const local_ray lr = make_local_ray(eo, ed);
+                    const hitrec o =
+                        traverse<W, true, true, COUNT, LDS>(S, eo, ed, h.t * lr.L1, prm.tri_test, le, cnt);
+                    if (o.prim != RBW_MISS) h.prim = RBW_MISS;
+                }
+            } else {
+                h = traverse<W, true, false, COUNT, LDS>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
             }
-        } else {
-            h = traverse<W, true, false, COUNT, LDS>(S, eo, ed, __builtin_inff(), prm.tri_test, RBW_NONE, cnt);
+            prim = h.prim;
         }
-        if (dbg) dbg[2] = h.prim;
-        if (h.prim != RBW_MISS) {
+        if (dbg) dbg[2] = prim;
+        if (prim != RBW_MISS) {
+            // the vertex again from the path state (a compiler barrier keeps the loads here)
+            __asm__ volatile("" ::: "memory");
+            const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
+            f3 eo, ed;
+            spawn_ray(mp, mn, mw, eo, ed);
             uint32_t he;
-            const surf hs = hit_surface(S, h.prim, eo, ed, he);
+            const surf hs = hit_surface(S, prim, eo, ed, he);
             const dentity E = S.ents[he];
             if (E.flags & RBW_ENTITY_EMITTER) {
                 nlight = 1;
