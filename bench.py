@@ -165,6 +165,43 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
     tp, tg = _time_to(pc, threshold), _time_to(gc, threshold)
     ac, aspp = run(True, 0) if guide_depth != 0 else (gc, gspp)
     ta = _time_to(ac, threshold)
+
+    # The ceiling of this guide: a network fitted offline (800 Adam steps of 2^18 records from
+    # 1600 spp of training renders, learning rate 1e-2 then 3e-3 then 1e-3), frozen, timed
+    # without its training against the same judge.
+    def offline(gd, alpha):
+        tr.guide_reset(11)
+        for n in range(200):
+            lr = 1e-2 if n < 100 else (3e-3 if n < 150 else 1e-3)
+            go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr)
+            tr.render(cam, render_params(spp=8, max_depth=depth, seed=7 * 100000 + n, guiding=C.pointer(go)),
+                      stats=False)
+        acc = np.zeros_like(ref)
+        elapsed, curve, n = 0.0, [], 0
+        while n * chunk < max_spp:
+            go = guiding_opts(mis_fraction=alpha, train=False, guide_depth=gd)
+            t = time.perf_counter()
+            img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=3 * 100000 + n, guiding=C.pointer(go)),
+                            stats=False)["image"]
+            elapsed += time.perf_counter() - t
+            acc += img
+            n += 1
+            curve.append((elapsed * 1e3, scenes.rel_mse(acc / n, ref)))
+            if curve[-1][1] <= threshold * 0.8:
+                break
+        to = _time_to(curve, threshold)
+        # relMSE at the plain run's spp count, and the cost of a guided sample
+        return {"guided_ms_to_threshold": None if to is None else round(to, 1),
+                "speedup": None if (tp is None or to is None) else round(tp / to, 3), "guided_spp_run": n * chunk,
+                "ms_per_spp_vs_plain": round((elapsed / (n * chunk)) / (pc[-1][0] * 1e-3 / pspp), 3)}
+
+    ceiling = {"fit": "offline: 200 training renders of 8 spp, 4 Adam steps of 2^18 records each "
+                      "(lr 1e-2, 3e-3, 1e-3), then frozen; time excludes the fit"}
+    if full:
+        for gd in (1, 0):
+            for alpha in (0.5, 0.75):
+                ceiling[f"guide_depth_{gd}_alpha_{alpha}"] = offline(gd, alpha)
+        print(f"# guided leg: offline-guide ceiling {ceiling}", file=sys.stderr, flush=True)
     return {
         "workload": f"narrow-aperture room, displaced icosphere level 8, {res}x{res}, depth 8, "
                     f"beam through a 0.3x0.3 gap onto the ceiling",
@@ -178,7 +215,51 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
         "every_vertex_guided": {"guided_ms_to_threshold": None if ta is None else round(ta, 1),
                                 "speedup": None if (tp is None or ta is None) else round(tp / ta, 3),
                                 "guided_spp_run": aspp},
+        "offline_guide_ceiling": ceiling if full else None,
+        "variance_by_depth": variance_by_depth(tr) if full else None,
     }
+
+
+def variance_by_depth(tr, res=256, judge_spp=4096, spp=256):
+    """Where the narrow-aperture room's variance lives: for max depth D, the per-sample
+    variance of the plain estimator (mean over pixels of the squared error at `spp` times
+    spp, less the judge's noise; judges of 2 x judge_spp spp at the same depth), and the
+    ratio of a guided estimator's to it, the guide fitted offline at that depth (200
+    training renders of 8 spp) and frozen, at the first vertex or at every vertex, mixing
+    fraction 0.5 and 0.9.  Depth 1 is direct light only (next-event estimation), depth 2 adds
+    the first indirect bounce, which a guide at the first vertex steers."""
+    import numpy as np
+    from rainbow_amd import scenes
+    from rainbow_amd.tracer import guiding_opts, render_params
+    sc, cam = scenes.narrow_aperture_room(level=8, width=res, height=res, light_tilt=45.0)
+    tr.upload(sc)
+    out = {"workload": f"narrow-aperture room, level 8, {res}x{res}", "judge_spp": 2 * judge_spp, "spp": spp}
+    for depth in (1, 2, 3, 8):
+        r1 = tr.render(cam, render_params(spp=judge_spp, max_depth=depth, seed=1 << 20), stats=False)["image"]
+        r2 = tr.render(cam, render_params(spp=judge_spp, max_depth=depth, seed=1 << 21), stats=False)["image"]
+        r1, r2 = r1.astype(np.float64), r2.astype(np.float64)
+        ref = 0.5 * (r1 + r2)
+        noise = float(np.mean((r1 - r2) ** 2)) / 4
+
+        def var(img):
+            return (float(np.mean((img - ref) ** 2)) - noise) * spp
+
+        vp = var(tr.render(cam, render_params(spp=spp, max_depth=depth, seed=77), stats=False)["image"])
+        row = {"mean": round(float(ref.mean()), 5), "plain_variance": float(f"{vp:.4g}")}
+        for gd in (1, 0):
+            tr.guide_reset(7)
+            for n in range(200):
+                lr = 1e-2 if n < 100 else (3e-3 if n < 150 else 1e-3)
+                go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr)
+                tr.render(cam, render_params(spp=8, max_depth=depth, seed=5000 + n, guiding=C.pointer(go)), stats=False)
+            for alpha in (0.5, 0.9):
+                go = guiding_opts(mis_fraction=alpha, train=False, guide_depth=gd)
+                img = tr.render(cam, render_params(spp=spp, max_depth=depth, seed=99, guiding=C.pointer(go)),
+                                stats=False)["image"]
+                row[f"guide_depth_{gd}_alpha_{alpha}_ratio"] = round(var(img) / vp, 3)
+        out[f"depth_{depth}"] = row
+    print(f"# guided leg: variance by depth {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def volume_leg(cpu=True, steps=2):
@@ -297,7 +378,7 @@ def integrator_variants_leg(sc, cam, spp, depth, cpu=True, steps=2):
     return out
 
 
-def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=2048, threshold=1e-3, chunk=8, max_spp=4096, train_calls=32,
+def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=1024, threshold=1e-3, chunk=8, max_spp=4096, train_calls=32,
                     guide_depth=1):
     """The guided-plus-batched workload: 16 narrow-aperture room variants (gap 0.1-0.5,
     1e5-1.3e6 triangles, GGX alpha 0.02-0.6, 256^2-1024^2) submitted as ONE batch of jobs,
