@@ -511,6 +511,26 @@ def batch_leg(n_jobs=64, autotune=True):
         single_ms += st["render_ms"]
     samples = bst["samples"]
     per_job = [round(x["msamples_per_s"], 1) for x in jst]
+    # the jobs that share the persistent launch (fewer samples than the pool has slots):
+    # as one submission against one render call each
+    pool = bst["pool_size"]
+    small = [k for k, j in enumerate(jobs)
+             if j["camera"].width * j["camera"].height * j["spp"] < pool]
+    small_out = None
+    if len(small) > 1:
+        sj = [js[k] for k in small]
+        tr.submit_jobs(sj, opts, images=False)
+        _, _, sb = tr.submit_jobs(sj, opts, images=False)
+        one_ms = 0.0
+        for k in small:
+            j = jobs[k]
+            tr.upload(j["scene"])
+            one_ms += tr.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"]),
+                                stats=True)["stats"]["render_ms"]
+        small_out = {"jobs": len(small), "samples": sb["samples"], "batch_ms": round(sb["render_ms"], 1),
+                     "one_at_a_time_ms": round(one_ms, 1), "gain": round(one_ms / sb["render_ms"], 3)}
+        print(f"# batch: {len(small)} pool-sharing jobs: batch {sb['render_ms']:.1f} ms, one at a time "
+              f"{one_ms:.1f} ms", file=sys.stderr, flush=True)
     return {
         "workload": f"{n_jobs} jobs, seed 2: grid-sphere mesh 1e4-2e6 tris (log-uniform) in the Cornell box, "
                     "128^2-1024^2, 4-256 spp, depth 3-12, per-triangle Lambert/GGX",
@@ -524,6 +544,8 @@ def batch_leg(n_jobs=64, autotune=True):
         "iterations": bst["iterations"], "graph_launches": bst["graph_launches"],
         "block_size": bst["block_size"], "pool_size": bst["pool_size"], "bvh_width": bst["bvh_width"],
         "autotune_ms": round(tuned_ms, 1),
+        "tuned": {"pool_size": opts.pool_size, "block_size": opts.block_size, "bvh_width": opts.bvh_width},
+        "pool_sharing_jobs": small_out,
     }
 
 

@@ -498,6 +498,8 @@ struct batch_config {
     uint32_t tri_test = RBW_TRI_MOLLER_TRUMBORE, shadow = RBW_SHADOW_CLOSEST_VISIBLE;
     // RBW_SAMPLER_PER_SAMPLE_PCG or RBW_SAMPLER_STRATIFIED (a single render's sampler)
     uint32_t sampler = RBW_SAMPLER_PER_SAMPLE_PCG, strata_x = 1, strata_y = 1, dims_1d = 0, dims_2d = 0;
+    // the autotuner's probes: stop after this many graph launches (0: run to the end)
+    uint32_t probe_launches = 0;
     // guided jobs (job j: network j of `model`)
     const rbw_guiding_opts* guiding = nullptr;
     guide::model* model = nullptr;
@@ -583,6 +585,7 @@ void ensure_pool(rbw_batch_state& b, uint32_t pool) {
 struct run_result {
     double ms = 0.0;
     uint32_t iterations = 0, launches = 0;
+    unsigned long long started = 0;  // camera samples started (all of them unless cut short)
     unsigned long long guide_samples = 0;
     uint32_t guide_steps = 0;
     unsigned long long counters[4] = {0, 0, 0, 0};
@@ -830,7 +833,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                 }
                 HIP_CHECK(hipMemsetAsync(B.t_count, 0, size_t(nj) * 4, s));
             }
-            if (done) break;
+            if (done || (cfg.probe_launches && launches >= cfg.probe_launches)) break;
             require(launches < max_launches, RBW_ERR_DEVICE, "batched bounce loop did not drain");
         }
     }
@@ -839,6 +842,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     HIP_CHECK(hipStreamSynchronize(s));
     rr.launches = launches;
     rr.iterations = b.host_ctl->iteration;
+    rr.started = b.host_ctl->cursor < total ? b.host_ctl->cursor : total;
     float ms = 0.0f;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, b.events[launches]));
     rr.ms = ms;
@@ -921,38 +925,92 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
     prepare_scene(ctx, jobs, n, part);
     batch_config cfg = config_of(inout);
     const auto t0 = std::chrono::steady_clock::now();
-    // probe: every job at 1 spp; coordinate search over BVH width, then block size
-    auto probe = [&](uint32_t block, uint32_t pool, uint32_t width) {
+    // Probes run the real jobs at their real spp through the real schedule, each launch cut
+    // after k_probe_launches graph launches (the first 24 bounce iterations: past the
+    // camera-ray ramp of the deepest jobs); a probe's rate is the camera samples it started
+    // per millisecond.  Jobs that fill the pool run alone (submit_jobs), so the pool decides
+    // which jobs share the launch as well as its size: each candidate pool is scored by the
+    // estimated time of the whole batch, the shared jobs' samples at their probed rate plus,
+    // for the jobs that run alone under some candidates and share under others, their
+    // samples at the rate each one reaches alone.  Jobs alone under every candidate do not
+    // change the choice.
+    constexpr uint32_t k_probe_launches = 12;
+    const std::vector<djob> all = make_jobs(ctx, jobs, n, part, 0);
+    auto probe = [&](const std::vector<size_t>& idx, uint32_t block, uint32_t pool, uint32_t width) {
+        std::vector<djob> sub;
+        unsigned long long base = 0;
+        for (size_t i : idx) {
+            djob d = all[i];
+            d.base = base;
+            base += d.count;
+            sub.push_back(d);
+        }
         batch_config c = cfg;
         c.block = block;
         c.pool = pool;
         c.width = width;
-        std::vector<djob> hj = make_jobs(ctx, jobs, n, part, 1);
+        c.probe_launches = k_probe_launches;
         run_result rr;
-        run_batch(ctx, ctx->batch->scene.view(c.width), hj, c, rr, false);
-        return rr.ms;
+        run_batch(ctx, ctx->batch->scene.view(c.width), sub, c, rr, false);
+        return double(rr.started) / std::max(rr.ms, 1e-3);  // samples per ms
     };
-    // coordinate search: BVH width, then block size
-    uint32_t best_block = 256, best_pool = cfg.pool, best_width = k_default_width;
-    double best = 1e300;
-    for (uint32_t width : {2u, 8u}) {
-        const double t = probe(best_block, best_pool, width);
-        if (t < best) {
-            best = t;
-            best_width = width;
+    auto shared_of = [&](uint32_t pool) {
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < n; i++)
+            if (n == 1 || all[i].count < pool) idx.push_back(i);
+        return idx;
+    };
+    auto alone_pool = [&](size_t i, uint32_t pool) {
+        const unsigned long long want = (all[i].count + 1023) & ~1023ull;
+        return std::max<uint32_t>(pool, (uint32_t)std::min<unsigned long long>(k_render_pool, want));
+    };
+    // coordinate search on the shared launch at the given pool: BVH width, then block size
+    uint32_t best_block = cfg.block, best_pool = cfg.pool, best_width = k_default_width;
+    {
+        std::vector<size_t> idx = shared_of(cfg.pool);
+        if (idx.empty()) idx.push_back(0);
+        const uint32_t pool = idx.size() == 1 && n > 1 ? alone_pool(idx[0], cfg.pool) : cfg.pool;
+        double best = -1.0;
+        for (uint32_t width : {2u, 8u}) {
+            const double r = probe(idx, best_block, pool, width);
+            if (r > best) {
+                best = r;
+                best_width = width;
+            }
+        }
+        for (uint32_t block : {64u, 128u, 256u, 512u}) {
+            if (block == best_block) continue;
+            const double r = probe(idx, block, pool, best_width);
+            if (r > best) {
+                best = r;
+                best_block = block;
+            }
         }
     }
-    for (uint32_t block : {64u, 128u, 512u}) {
-        const double t = probe(block, best_pool, best_width);
-        if (t < best) {
-            best = t;
-            best_block = block;
+    // the pool: 2^20, 2^21 or 2^22 slots
+    if (n > 1) {
+        const uint32_t cands[3] = {1u << 20, 1u << 21, 1u << 22};
+        std::vector<double> alone_rate(n, 0.0);
+        for (size_t i = 0; i < n; i++)
+            if (all[i].count >= cands[0] && all[i].count < cands[2])
+                alone_rate[i] = probe({i}, best_block, alone_pool(i, cands[0]), best_width);
+        double best_t = 1e300;
+        for (uint32_t pool : cands) {
+            const std::vector<size_t> idx = shared_of(pool);
+            double t = 0.0;
+            if (!idx.empty()) {
+                unsigned long long shared = 0;
+                for (size_t i : idx) shared += all[i].count;
+                t += double(shared) / probe(idx, best_block, pool, best_width);
+            }
+            for (size_t i = 0; i < n; i++)
+                if (all[i].count >= pool && all[i].count < cands[2]) t += double(all[i].count) / alone_rate[i];
+            if (t < best_t) {
+                best_t = t;
+                best_pool = pool;
+            }
         }
     }
-    // (the pool is not searched: at 1 spp every job is smaller than the pool and shares one
-    // launch, while in the real run the jobs that fill the pool run alone; the shared
-    // launch's pool stays 2^21, measured best for the 64-job workload: 384 Msamples/s
-    // against 376 at 2^22 and 353 at 2^23)
     inout->block_size = best_block;
     inout->pool_size = best_pool;
     inout->bvh_width = best_width;
