@@ -34,6 +34,7 @@ namespace guide {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 #define GCHECK(expr)                                                                                \
     do {                                                                                            \
@@ -58,7 +59,11 @@ __device__ __forceinline__ float cos_rev(float t) { return __builtin_amdgcn_cosf
 
 // Layer 0's B fragment for K block kb of one record (inputs x at stride 1): lane group g
 // holds input v at four consecutive frequencies k0..k0+3, sin(2^k pi x) = sin(2 pi t) with
-// t = 2^(k-1) x, each frequency's t the previous one's doubled (exact in FP32)
+// t = 2^(k-1) x.  The first frequency's sine and cosine come from the hardware units
+// (v_sin_f32 / v_cos_f32 take revolutions), the next three by the double-angle identities
+// sin 2a = 2 sin a cos a, cos 2a = 1 - 2 sin^2 a: two transcendentals per four frequencies
+// instead of eight.  Three doublings scale the units' ~1e-6 error by at most 8, far below
+// the half precision the features are rounded to.
 __device__ __forceinline__ h8 encode_frag(float xv, int kb, int g) {
     h8 b;
     const int Q = 4 * kb + g;
@@ -68,13 +73,19 @@ __device__ __forceinline__ h8 encode_frag(float xv, int kb, int g) {
         return b;
     }
     const int k0 = 4 * (Q % 3);
-    float t = xv * (float)(1 << k0) * 0.5f;
+    const float t = xv * (float)(1 << k0) * 0.5f;
+    const float f = t - floorf(t);
+    float sn = sin_rev(f), cs = cos_rev(f);
+    b[0] = (_Float16)sn;
+    b[1] = (_Float16)cs;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const float f = t - floorf(t);
-        b[2 * q] = (_Float16)sin_rev(f);
-        b[2 * q + 1] = (_Float16)cos_rev(f);
-        t = t * 2.0f;
+    for (int q = 1; q < 4; q++) {
+        const float s2 = sn + sn;
+        const float s_next = s2 * cs;
+        cs = __builtin_fmaf(-s2, sn, 1.0f);
+        sn = s_next;
+        b[2 * q] = (_Float16)sn;
+        b[2 * q + 1] = (_Float16)cs;
     }
     return b;
 }
@@ -93,9 +104,11 @@ __device__ __forceinline__ void wave_sync() {
 
 // ------------------------------------------------------------------ inference
 
-constexpr int INF_WAVES = 4;
+constexpr int INF_WAVES = 16;     // waves per block (one block per CU)
+constexpr int INF_R = 2;          // 16-record tiles per wave
+constexpr uint32_t INF_MAX_BLOCKS = 256;
 constexpr size_t INF_XS = size_t(FWD_FRAGS) * FRAG_HALFS * 2 + N_BIAS * 4;  // per-wave input rows
-constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 64 * N_IN * 4;        // 78,272 bytes
+constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 16 * INF_R * N_IN * 4;
 static_assert(INF_XS % 16 == 0, "aligned input rows");
 
 // R record tiles (16 rows each) per wave share every weight fragment read from LDS.
@@ -105,7 +118,7 @@ static_assert(INF_XS % 16 == 0, "aligned input rows");
 // (a step never spans two networks) and each block takes a contiguous range of the steps
 // of all networks, so the work spreads over the whole grid however unevenly the networks'
 // rows fall; a block stages a network's weights in LDS when its range enters that network.
-constexpr uint32_t INF_STEP_ROWS = INF_WAVES * 64;  // R = 4
+constexpr uint32_t INF_STEP_ROWS = INF_WAVES * 16 * INF_R;
 template <int R>
 __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __restrict__ pack,
                                                           const float* __restrict__ bias,
@@ -142,8 +155,9 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
     };
     // the wave's 16 R rows of a step are staged in LDS (coalesced loads, the next step's in
     // flight while this one computes)
-    float* xs = reinterpret_cast<float*>(smem + INF_XS) + wave * 64 * N_IN;
-    constexpr int XPL = 16 * R * N_IN / 64;  // floats per lane
+    constexpr int XW = 16 * R * N_IN;          // floats per wave
+    constexpr int XPL = (XW + 63) / 64;        // per lane
+    float* xs = reinterpret_cast<float*>(smem + INF_XS) + wave * XW;
     float xpre[XPL];
     auto load_step = [&](uint32_t st) {
         const uint32_t k = st < s1 ? net_of(st) : 0u;
@@ -153,7 +167,7 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
 #pragma unroll
         for (int q = 0; q < XPL; q++) {
             const uint32_t e = (uint32_t)lane + 64u * q, r = r0 + e / N_IN;
-            if (st < s1 && r < nk) {
+            if (st < s1 && e < (uint32_t)XW && r < nk) {
                 const size_t rec = rows ? (size_t)rows[(size_t)k * stride + r] : (size_t)r;
                 xpre[q] = in[rec * N_IN + e % N_IN];
             } else {
@@ -179,7 +193,8 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
         const uint32_t* rl = rows ? rows + (size_t)net * stride : nullptr;
         const uint32_t r0 = (st - first_step(net)) * INF_STEP_ROWS + (uint32_t)wave * 16 * R;
 #pragma unroll
-        for (int k = 0; k < XPL; k++) xs[lane + 64 * k] = xpre[k];
+        for (int k = 0; k < XPL; k++)
+            if (lane + 64 * k < XW) xs[lane + 64 * k] = xpre[k];
         wave_sync();
         load_step(st + 1);
         uint32_t row[R];
@@ -187,10 +202,13 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
         for (int t = 0; t < R; t++) row[t] = r0 + 16 * t + li;
         f4 acc[R][4];
         h8 X[R][2];
+        // every layer's sums start from its biases
 #pragma unroll
-        for (int t = 0; t < R; t++)
+        for (int m = 0; m < 4; m++) {
+            const f4 b0 = *reinterpret_cast<const f4*>(bl + bias_offset(0) + 16 * m + 4 * g);
 #pragma unroll
-            for (int m = 0; m < 4; m++) acc[t][m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < R; t++) acc[t][m] = b0;
+        }
         // layer 0: the encoding
 #pragma unroll 1
         for (int kb = 0; kb < 7; kb++) {
@@ -208,23 +226,25 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
         // hidden layers (rolled: unrolled, every layer's fragment loads are hoisted together)
 #pragma unroll 1
         for (int l = 1; l < N_LAYERS; l++) {
+            // ReLU on half pairs (rounding commutes with max(., 0)) into the next B fragments
 #pragma unroll
             for (int t = 0; t < R; t++)
 #pragma unroll
                 for (int m = 0; m < 4; m++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const float z = acc[t][m][j] + bl[bias_offset(l - 1) + 16 * m + 4 * g + j];
-                        X[t][m >> 1][(m & 1) * 4 + j] = (_Float16)(z > 0.0f ? z : 0.0f);
+                    for (int j = 0; j < 4; j += 2) {
+                        h2 p = {(_Float16)acc[t][m][j], (_Float16)acc[t][m][j + 1]};
+                        p = __builtin_elementwise_max(p, h2{(_Float16)0.0f, (_Float16)0.0f});
+                        X[t][m >> 1][(m & 1) * 4 + j] = p.x;
+                        X[t][m >> 1][(m & 1) * 4 + j + 1] = p.y;
                     }
             const int mt = l == N_LAYERS - 1 ? 3 : 4;
 #pragma unroll
-            for (int t = 0; t < R; t++)
-#pragma unroll
-                for (int m = 0; m < 4; m++) acc[t][m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
             for (int m = 0; m < 4; m++) {
                 if (m >= mt) break;
+                const f4 bm = *reinterpret_cast<const f4*>(bl + bias_offset(l) + 16 * m + 4 * g);
+#pragma unroll
+                for (int t = 0; t < R; t++) acc[t][m] = bm;
 #pragma unroll
                 for (int kb = 0; kb < 2; kb++) {
                     const h8 a = frag(wl, fwd_frag(l, m, kb), lane);
@@ -233,7 +253,7 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
                 }
             }
         }
-        // raw outputs (48 columns, 40 live) with bias
+        // raw outputs (48 columns, 40 live; biases included)
 #pragma unroll
         for (int t = 0; t < R; t++) {
             if (row[t] >= n) continue;
@@ -241,10 +261,10 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_infer(const _Float16* __rest
             for (int m = 0; m < 3; m++) {
                 const int c0 = 16 * m + 4 * g;
                 float4 v;
-                v.x = acc[t][m][0] + bl[bias_offset(5) + c0];
-                v.y = acc[t][m][1] + bl[bias_offset(5) + c0 + 1];
-                v.z = acc[t][m][2] + bl[bias_offset(5) + c0 + 2];
-                v.w = acc[t][m][3] + bl[bias_offset(5) + c0 + 3];
+                v.x = acc[t][m][0];
+                v.y = acc[t][m][1];
+                v.z = acc[t][m][2];
+                v.w = acc[t][m][3];
                 const size_t orow = rl ? (size_t)rl[row[t]] : (size_t)row[t];
                 *reinterpret_cast<float4*>(o + orow * N_OUT_PAD + c0) = v;
             }
@@ -478,8 +498,9 @@ __global__ __launch_bounds__(TR_WAVES * 64) __attribute__((amdgpu_waves_per_eu(T
         const float* xr = xin + (16 * wave + li) * N_IN;
         h8 X[6][2];  // X[l]: the input of layer l (l = 1..5) as B fragments
         f4 acc[4];
+        // every layer's sums start from its biases
 #pragma unroll
-        for (int m = 0; m < 4; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int m = 0; m < 4; m++) acc[m] = *reinterpret_cast<const f4*>(bl + bias_offset(0) + 16 * m + 4 * g);
 #pragma unroll
         for (int kb = 0; kb < 7; kb++) {
             const int v = encode_input(kb, g);
@@ -489,30 +510,34 @@ __global__ __launch_bounds__(TR_WAVES * 64) __attribute__((amdgpu_waves_per_eu(T
         }
 #pragma unroll
         for (int l = 1; l < N_LAYERS; l++) {
+            // ReLU on half pairs (rounding commutes with max(., 0))
 #pragma unroll
             for (int m = 0; m < 4; m++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const float z = acc[m][j] + bl[bias_offset(l - 1) + 16 * m + 4 * g + j];
-                    X[l][m >> 1][(m & 1) * 4 + j] = (_Float16)(z > 0.0f ? z : 0.0f);
+                for (int j = 0; j < 4; j += 2) {
+                    h2 p = {(_Float16)acc[m][j], (_Float16)acc[m][j + 1]};
+                    p = __builtin_elementwise_max(p, h2{(_Float16)0.0f, (_Float16)0.0f});
+                    X[l][m >> 1][(m & 1) * 4 + j] = p.x;
+                    X[l][m >> 1][(m & 1) * 4 + j + 1] = p.y;
                 }
             const int mt = l == N_LAYERS - 1 ? 3 : 4;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
                 if (m < mt) {
+                    acc[m] = *reinterpret_cast<const f4*>(bl + bias_offset(l) + 16 * m + 4 * g);
 #pragma unroll
                     for (int kb = 0; kb < 2; kb++) acc[m] = mfma32(frag(wl, fwd_frag(l, m, kb), lane), X[l][kb], acc[m]);
                 }
             }
         }
-        // ---- head: raw outputs to the wave's staging rows, then four lanes per record
+        // ---- head: raw outputs (biases included) to the wave's staging rows, then four lanes
+        // per record
 #pragma unroll
         for (int m = 0; m < 3; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int col = 16 * m + 4 * g + j;
-                if (col < N_OUT) hs[li * N_OUT + col] = acc[m][j] + bl[bias_offset(5) + col];
+                if (col < N_OUT) hs[li * N_OUT + col] = acc[m][j];
             }
         wave_sync();
         {
@@ -858,7 +883,7 @@ static void allow_lds() { prepare_kernels(); }
 void prepare_kernels() {
     static bool done = false;
     if (done) return;
-    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+    GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<INF_R>), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(INF_LDS)));
     GCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train), hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(TRAIN_LDS)));
@@ -916,13 +941,13 @@ void model_init(model& M, uint32_t seed, hipStream_t s, uint32_t nets) {
 // persistent inference blocks: two fit a CU's LDS
 static uint32_t infer_blocks(uint32_t rows) {
     const uint32_t want = (rows + INF_STEP_ROWS - 1) / INF_STEP_ROWS;
-    return want < 512 ? (want ? want : 1) : 512;
+    return want < INF_MAX_BLOCKS ? (want ? want : 1) : INF_MAX_BLOCKS;
 }
 
 void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t s) {
     if (n == 0) return;
     allow_lds();
-    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs, n,
+    hipLaunchKernelGGL(k_infer<INF_R>, dim3(infer_blocks(n)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs, n,
                        nullptr, 1u, 0u, out);
     GCHECK(hipGetLastError());
 }
@@ -930,7 +955,7 @@ void forward(model& M, const float* inputs, uint32_t n, float* out, hipStream_t 
 void forward_dev(model& M, const float* inputs, const uint32_t* n_dev, uint32_t n_max, float* out, hipStream_t s) {
     if (n_max == 0) return;
     allow_lds();
-    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks(n_max)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
+    hipLaunchKernelGGL(k_infer<INF_R>, dim3(infer_blocks(n_max)), dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias,
                        inputs, 0u, n_dev, 1u, 0u, out);
     GCHECK(hipGetLastError());
 }
@@ -941,7 +966,7 @@ void forward_segments(model& M, const float* inputs, const uint32_t* n_dev, uint
     if (nets == 0 || nets > M.nets) nets = M.nets;
     allow_lds();
     // (the rows of all networks together number at most n_max * nets)
-    hipLaunchKernelGGL(k_infer<4>, dim3(infer_blocks((uint32_t)std::min<uint64_t>(uint64_t(n_max) * nets, 1u << 30))),
+    hipLaunchKernelGGL(k_infer<INF_R>, dim3(infer_blocks((uint32_t)std::min<uint64_t>(uint64_t(n_max) * nets, 1u << 30))),
                        dim3(INF_WAVES * 64), INF_LDS, s, M.pack, M.bias, inputs, 0u, n_dev, nets, stride, out, rows);
     GCHECK(hipGetLastError());
 }
