@@ -402,8 +402,10 @@ typedef struct rbw_batch_opts {
                                         least a single render's 2^23 slots; the rest share the launch */
     uint32_t bvh_width;              /* 0 (default: 2), 2 or 8; the autotuner tries both */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
-    uint32_t autotune;               /* 1: time candidate BVH widths and block sizes on a 1-spp probe
-                                        first (the pool size stays as given) */
+    uint32_t autotune;               /* 1: first probe the jobs themselves (their real spp, each probe
+                                        cut after 12 graph launches) over BVH widths 2/8, block sizes
+                                        64..512 and pools of 2^20..2^22 slots, and run the batch with
+                                        the configuration of the least estimated batch time */
     uint32_t triangle_test;          /* RBW_TRI_* */
     uint32_t shadow_mode;            /* RBW_SHADOW_* */
     /* NULL: the plain estimator.  Otherwise every job is rendered with learned guiding (the
