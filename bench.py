@@ -113,7 +113,7 @@ def _err_at(curve, t_ms):
     return curve[-1][1] if abs(t_ms - curve[-1][0]) < 1e-9 else None
 
 
-def guided_leg(full, threshold=1e-3, guide_depth=1):
+def guided_leg(full, threshold=1e-3, guide_depth=1, product_targets=True, analysis=False):
     """Time to relMSE <= threshold, plain vs guided, on the narrow-aperture room.
 
     The room is lit only through a 0.3x0.3 gap in its +x wall by a 0.5x0.5 quad light
@@ -123,7 +123,9 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
     kernels: tracing, MLP inference, training), relMSE computed between calls.  The guided
     estimator guides the first guide_depth vertices of each path and shades later ones as
     the plain tracer (0: every vertex); the leg also reports the every-vertex schedule
-    against the same judge.
+    against the same judge.  product_targets: the network is trained on f |cos| L_i (else
+    L_i).  analysis (full size only): also the offline-fitted guide's ceiling and the
+    variance by path depth (several minutes more).
     """
     import numpy as np
     from rainbow_amd import scenes
@@ -146,7 +148,8 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
         while n * chunk < max_spp:
             kw = {}
             if guided:
-                go = guiding_opts(train=n < train_calls, steps_per_pass=4, batch=1 << 18, guide_depth=gd)
+                go = guiding_opts(train=n < train_calls, steps_per_pass=4, batch=1 << 18, guide_depth=gd,
+                                  product_targets=product_targets)
                 kw["guiding"] = C.pointer(go)
             t = time.perf_counter()
             img = tr.render(cam, render_params(spp=chunk, max_depth=depth, seed=(2 if guided else 1) * 100000 + n,
@@ -173,7 +176,8 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
         tr.guide_reset(11)
         for n in range(200):
             lr = 1e-2 if n < 100 else (3e-3 if n < 150 else 1e-3)
-            go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr)
+            go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr,
+                              product_targets=product_targets)
             tr.render(cam, render_params(spp=8, max_depth=depth, seed=7 * 100000 + n, guiding=C.pointer(go)),
                       stats=False)
         acc = np.zeros_like(ref)
@@ -197,7 +201,7 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
 
     ceiling = {"fit": "offline: 200 training renders of 8 spp, 4 Adam steps of 2^18 records each "
                       "(lr 1e-2, 3e-3, 1e-3), then frozen; time excludes the fit"}
-    if full:
+    if full and analysis:
         for gd in (1, 0):
             for alpha in (0.5, 0.75):
                 ceiling[f"guide_depth_{gd}_alpha_{alpha}"] = offline(gd, alpha)
@@ -212,15 +216,16 @@ def guided_leg(full, threshold=1e-3, guide_depth=1):
         "plain_spp_run": pspp, "guided_spp_run": gspp,
         "guided_training": f"online, first {train_calls} calls of {chunk} spp, 4 Adam steps of 2^18 per pass",
         "guided_vertices": "all" if guide_depth == 0 else f"the first {guide_depth}",
+        "training_targets": "f |cos| L_i / pdf" if product_targets else "L_i / pdf",
         "every_vertex_guided": {"guided_ms_to_threshold": None if ta is None else round(ta, 1),
                                 "speedup": None if (tp is None or ta is None) else round(tp / ta, 3),
                                 "guided_spp_run": aspp},
-        "offline_guide_ceiling": ceiling if full else None,
-        "variance_by_depth": variance_by_depth(tr) if full else None,
+        "offline_guide_ceiling": ceiling if full and analysis else None,
+        "variance_by_depth": variance_by_depth(tr, product_targets) if full and analysis else None,
     }
 
 
-def variance_by_depth(tr, res=256, judge_spp=4096, spp=256):
+def variance_by_depth(tr, product_targets, res=256, judge_spp=4096, spp=256):
     """Where the narrow-aperture room's variance lives: for max depth D, the per-sample
     variance of the plain estimator (mean over pixels of the squared error at `spp` times
     spp, less the judge's noise; judges of 2 x judge_spp spp at the same depth), and the
@@ -250,7 +255,8 @@ def variance_by_depth(tr, res=256, judge_spp=4096, spp=256):
             tr.guide_reset(7)
             for n in range(200):
                 lr = 1e-2 if n < 100 else (3e-3 if n < 150 else 1e-3)
-                go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr)
+                go = guiding_opts(train=True, steps_per_pass=4, batch=1 << 18, guide_depth=gd, learning_rate=lr,
+                                  product_targets=product_targets)
                 tr.render(cam, render_params(spp=8, max_depth=depth, seed=5000 + n, guiding=C.pointer(go)), stats=False)
             for alpha in (0.5, 0.9):
                 go = guiding_opts(mis_fraction=alpha, train=False, guide_depth=gd)
@@ -379,7 +385,7 @@ def integrator_variants_leg(sc, cam, spp, depth, cpu=True, steps=2):
 
 
 def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=1024, threshold=1e-3, chunk=8, max_spp=4096, train_calls=32,
-                    guide_depth=1):
+                    guide_depth=1, product_targets=True, pool=0, steps_per_pass=1):
     """The guided-plus-batched workload: 16 narrow-aperture room variants (gap 0.1-0.5,
     1e5-1.3e6 triangles, GGX alpha 0.02-0.6, 256^2-1024^2) submitted as ONE batch of jobs,
     guided (one network per job, all trained online by one batched kernel, the first
@@ -388,7 +394,9 @@ def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=1024, threshold=1e-3, chunk
     training).  A job's relMSE is measured against a judge of 2 x judge_spp spp (two
     independent halves R1, R2), less the judge's own noise, mean((R1 - R2)^2 / (R^2 + 0.01)) / 4,
     so a modest judge does not bias the estimate.  Reports each job's time to relMSE <=
-    threshold and the time until every job is below it."""
+    threshold and the time until every job is below it.  pool: path slots of both schedules
+    (0: the library's default); steps_per_pass: Adam steps per training point (every 8 bounce
+    iterations)."""
     import numpy as np
     from rainbow_amd import scenes
     from rainbow_amd.tracer import Tracer, batch_opts, guiding_opts
@@ -419,10 +427,12 @@ def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=1024, threshold=1e-3, chunk
         if guided:
             tr.guide_jobs_reset(n, seed=7)
         while calls * chunk < max_spp:
-            o = batch_opts()
+            o = batch_opts(pool_size=pool)
             if guided:
-                o = batch_opts(guiding=guiding_opts(train=calls < train_calls, steps_per_pass=1, batch=1 << 16,
-                                                    guide_depth=guide_depth))
+                o = batch_opts(pool_size=pool,
+                               guiding=guiding_opts(train=calls < train_calls, steps_per_pass=steps_per_pass,
+                                                    batch=1 << 16, guide_depth=guide_depth,
+                                                    product_targets=product_targets))
             t = time.perf_counter()
             imgs, _, st = tr.submit_jobs(jobs(chunk, (2 if guided else 1) * 100000 + 1000 * calls), o)
             elapsed += time.perf_counter() - t
@@ -461,8 +471,10 @@ def rooms_batch_leg(n=16, res=(256, 1024), judge_spp=1024, threshold=1e-3, chunk
                        f"{res[0]}^2-{res[1]}^2) in one submission, depth {depth}, one network per job",
            "judge": f"2 x {judge_spp} spp per job, relMSE less the judge's noise", "threshold": threshold,
            "guided_vertices": "all" if guide_depth == 0 else f"the first {guide_depth}",
-           "training": f"online in the first {train_calls} calls of {chunk} spp: 1 batched Adam step (2^16 rows per "
-                       f"network) after every 8 bounce iterations ({gsteps} steps)",
+           "training": f"online in the first {train_calls} calls of {chunk} spp: {steps_per_pass} batched Adam "
+                       f"step(s) (2^16 rows per network) after every 8 bounce iterations ({gsteps} steps), targets "
+                       + ("f |cos| L_i / pdf" if product_targets else "L_i / pdf"),
+           "pool_size": pool or "default",
            "plain_ms_all_jobs": None if tp_all is None else round(tp_all, 1),
            "guided_ms_all_jobs": None if tg_all is None else round(tg_all, 1),
            "speedup_all_jobs": None if (tp_all is None or tg_all is None) else round(tp_all / tg_all, 3),
@@ -591,6 +603,10 @@ def main():
                     help="guided leg: guide the first N vertices of a path, the plain estimator after (0: all)")
     ap.add_argument("--mlp", action=argparse.BooleanOptionalAction, default=True,
                     help="guiding-network kernels (training step, inference) at 2^18 records vs the MFMA peak")
+    ap.add_argument("--product-targets", action=argparse.BooleanOptionalAction, default=True,
+                    help="guided legs: train the networks on f |cos| L_i (else L_i)")
+    ap.add_argument("--guided-analysis", action="store_true",
+                    help="guided leg: also the offline-fitted guide's ceiling and the variance by path depth")
     ap.add_argument("--guided", default="full", choices=["reduced", "full", "off"],
                     help="guided-vs-plain time-to-relMSE leg: 512^2 with a 65536-spp judge (full, the "
                          "configured workload, ~90 s), 256^2 with a 16384-spp judge (reduced), or skipped")
@@ -704,7 +720,8 @@ def main():
     if args.batch:
         out["batch"] = batch_leg()
     if args.guided != "off":
-        out["guided"] = guided_leg(args.guided == "full", guide_depth=args.guide_depth)
+        out["guided"] = guided_leg(args.guided == "full", guide_depth=args.guide_depth,
+                                   product_targets=args.product_targets, analysis=args.guided_analysis)
     if args.volume:
         out["volume"] = volume_leg(cpu=not args.no_cpu)
     if args.mlp:
@@ -712,7 +729,7 @@ def main():
     if args.variants:
         out["integrator_variants"] = integrator_variants_leg(sc, cam, spp, depth, cpu=not args.no_cpu)
     if args.rooms:
-        out["room_variants"] = rooms_batch_leg(guide_depth=args.guide_depth)
+        out["room_variants"] = rooms_batch_leg(guide_depth=args.guide_depth, product_targets=args.product_targets)
     if not args.no_cpu:
         cb = cpu_baseline(sc, cam, spp, depth)
         out["cpu_baseline"] = cb

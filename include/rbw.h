@@ -212,6 +212,9 @@ typedef struct rbw_guiding_opts {
                                         radiance), so the guide learns all incident light */
     uint32_t guide_depth;            /* vertices 0 .. guide_depth-1 of a path are guided; later ones
                                         use the plain estimator (light + BSDF samples); 0 = all */
+    uint32_t product_targets;        /* 1: a training record's weight is f(wo, wi) |cos| L_i / pdf (the
+                                        product the continuation's contribution follows), so the guide
+                                        learns BSDF x cosine x incident radiance; 0: L_i / pdf */
 } rbw_guiding_opts;
 
 typedef struct rbw_render_params {

@@ -82,7 +82,7 @@ class rbw_guiding_opts(C.Structure):
     _fields_ = [("enable", C.c_uint32), ("mis_fraction", C.c_float), ("train", C.c_uint32),
                 ("learning_rate", C.c_float), ("batch", C.c_uint32), ("steps_per_pass", C.c_uint32),
                 ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3), ("light_records", C.c_uint32),
-                ("guide_depth", C.c_uint32)]
+                ("guide_depth", C.c_uint32), ("product_targets", C.c_uint32)]
 
 
 class rbw_render_params(C.Structure):

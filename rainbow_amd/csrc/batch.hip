@@ -384,12 +384,9 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
     if (to_guide) B.grows[(size_t)j * B.pool + k] = slot;
 }
 
-// guided jobs: the guided vertex (guided.hpp) of every slot that waits for it, with its job's
-// network outputs
-__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_guided(dscene S, dpaths P, dparams base,
-                                                                                   dbatch B) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= B.pool || P.U(U_STATE, slot) != ST_GUIDE) return;
+// the guided vertex of the slot's path with its job's network outputs
+__device__ __forceinline__ void guided_slot(const dscene& S, const dpaths& P, const dparams& base, const dbatch& B,
+                                            uint32_t slot) {
     const dguide& G = base.g;
     const size_t cap = G.cap;
     float v[G_COUNT];
@@ -409,6 +406,32 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_gui
     guided_vertex(Sj, P, pj, slot, si, G.surf_mat[slot], G.mix + (size_t)slot * guide::N_OUT_PAD,
                   G.query + (size_t)slot * 9, 0u, alive, need_connect);
     P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
+}
+
+// guided jobs: the guided vertex (guided.hpp) of every slot that waits for it, with its job's
+// network outputs.  The threads walk the jobs' lists of guided slots (rows of job 0, then
+// job 1, ...) rather than the pool: the guided vertices of an iteration are a scattered
+// minority of the slots (with the first vertex guided, the paths regenerated in it), so a
+// walk over the pool would run each wave's few of them with most lanes idle.
+__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_guided(dscene S, dpaths P, dparams base,
+                                                                                   dbatch B) {
+    __shared__ uint32_t first_row[guide::MAX_NETS + 1];
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (uint32_t j = 0; j < B.njobs; j++) {
+            first_row[j] = a;
+            const uint32_t c = B.gcount[j];
+            a += c < B.pool ? c : B.pool;
+        }
+        first_row[B.njobs] = a;
+    }
+    __syncthreads();
+    const uint32_t rows = first_row[B.njobs];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        uint32_t j = 0;
+        while (first_row[j + 1] <= r) j++;
+        guided_slot(S, P, base, B, B.grows[(size_t)j * B.pool + (r - first_row[j])]);
+    }
 }
 
 // next-event estimation rays; then the block's count of slots the next
@@ -694,6 +717,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         G.train = train ? 1u : 0u;
         G.light_records = 0;
         G.depth = go.guide_depth ? go.guide_depth : 0xFFFFFFFFu;
+        G.product = go.product_targets ? 1u : 0u;
         G.surf = b.g_surf.as<float>();
         G.surf_mat = b.g_mat.as<uint32_t>();
         G.query = b.g_query.as<float>();
@@ -746,7 +770,8 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                                     nj);
             if (tm) tm->end(s);
             if (tm) tm->begin(RBW_KERNEL_SHADE_GUIDED, s);
-            hipLaunchKernelGGL(k_b_guided, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+            // a grid-stride walk over the guided rows: 8192 blocks cover an iteration's typical rows
+            hipLaunchKernelGGL(k_b_guided, dim3(std::min<uint32_t>(grid, 8192u)), dim3(bs), 0, s, S, P, base, B);
         } else {
             hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
         }

@@ -138,7 +138,8 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
             P.setF3(F_SFX, slot, f * ls.L * w / ls.pdf);
             if (G.train && G.light_records) {
                 // training record of the light sample: its MIS-weighted radiance / pdf
-                P.F(F_MFX, slot) = w * (0.2126f * ls.L.x + 0.7152f * ls.L.y + 0.0722f * ls.L.z) / ls.pdf;
+                const float fl = G.product ? 0.2126f * f.x + 0.7152f * f.y + 0.0722f * f.z : 1.0f;
+                P.F(F_MFX, slot) = fl * w * (0.2126f * ls.L.x + 0.7152f * ls.L.y + 0.0722f * ls.L.z) / ls.pdf;
                 P.F(F_MFY, slot) = __uint_as_float(light_tag);
             }
             P.U(U_SENT, slot) = lid;
@@ -150,7 +151,7 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
     // continuation from the guided technique
     bool cont = !last;
     f3 wi = f3{0.0f, 0.0f, 0.0f};
-    float pdf = 0.0f;
+    float pdf = 0.0f, rec_pdf = 0.0f;
     if (cont) {
         const float sel = smp.next();
         const f2 u = smp.next2();
@@ -163,7 +164,10 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
         if (is_black(f) || !(pdf > 0.0f)) {
             cont = false;
         } else {
-            beta = beta * (f * fabsf(dot(wi, si.sh.z)) / pdf);
+            const f3 fc = f * fabsf(dot(wi, si.sh.z));
+            // the record's weight is L_i / rec_pdf: with product targets rec_pdf carries 1 / (f |cos|)
+            rec_pdf = G.product ? pdf / fmaxf(0.2126f * fc.x + 0.7152f * fc.y + 0.0722f * fc.z, 1e-30f) : pdf;
+            beta = beta * (fc / pdf);
             f3 no, nd;
             spawn_ray(si.point, si.normal, wi, no, nd);
             P.setF3(F_OX, slot, no);
@@ -193,7 +197,7 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
         G.rec[R_DX * rc + ri] = wi.x;
         G.rec[R_DY * rc + ri] = wi.y;
         G.rec[R_DZ * rc + ri] = wi.z;
-        G.rec[R_PDF * rc + ri] = pdf;
+        G.rec[R_PDF * rc + ri] = rec_pdf;
         G.rec[R_BX * rc + ri] = beta.x;
         G.rec[R_BY * rc + ri] = beta.y;
         G.rec[R_BZ * rc + ri] = beta.z;

@@ -566,6 +566,7 @@ struct dguide {
     uint32_t t_cap;
     uint32_t light_records;      // also train on unoccluded light samples
     uint32_t depth;              // vertices below this bounce are guided (>= max_depth: all)
+    uint32_t product;            // records weigh f |cos| L_i / pdf (else L_i / pdf)
 };
 
 struct dparams {

@@ -773,6 +773,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         dp.g.train = go->train ? 1u : 0u;
         dp.g.light_records = go->light_records ? 1u : 0u;
         dp.g.depth = go->guide_depth ? go->guide_depth : pr.max_depth;
+        dp.g.product = go->product_targets ? 1u : 0u;
         dp.g.rec_depth = pr.max_depth;
         f3 smin = ctx->scene_min, smax = ctx->scene_max;
         if (go->scene_min[0] != 0.0f || go->scene_min[1] != 0.0f || go->scene_min[2] != 0.0f ||

@@ -233,10 +233,10 @@ def batch_opts(block_size=0, pool_size=0, use_graph=True, autotune=False, guidin
 
 
 def guiding_opts(mis_fraction=0.5, train=True, learning_rate=1e-2, batch=1 << 18, steps_per_pass=4,
-                 light_records=False, guide_depth=0):
+                 light_records=False, guide_depth=0, product_targets=False):
     return abi.rbw_guiding_opts(enable=1, mis_fraction=mis_fraction, train=int(train), learning_rate=learning_rate,
                                 batch=batch, steps_per_pass=steps_per_pass, light_records=int(light_records),
-                                guide_depth=guide_depth)
+                                guide_depth=guide_depth, product_targets=int(product_targets))
 
 
 def render_params(spp=16, max_depth=5, rr_threshold=1.0, seed=0, **kw):
