@@ -589,6 +589,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="icosphere", choices=["icosphere", "cornell"])
+    ap.add_argument("--bvh-width", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="BVH width of the headline renders (0: the library's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-counters", action="store_true", help="skip the instrumented (visit-count) run")
     ap.add_argument("--batch", action=argparse.BooleanOptionalAction, default=True,
@@ -639,11 +641,12 @@ def main():
     ntris = sum(len(i) for i in sc.indices)
     tr = Tracer(0)
     tr.upload(sc)
-    params = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0)
+    params = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, bvh_width=args.bvh_width)
 
     counted = None
     if not args.no_counters:
-        cp = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, count_visits=1)
+        cp = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, count_visits=1,
+                           bvh_width=args.bvh_width)
         counted = tr.render(cam, cp)["stats"]
         print(f"# instrumented run: {counted['node_visits']} node visits, {counted['prim_tests']} prim tests",
               file=sys.stderr, flush=True)
@@ -675,9 +678,11 @@ def main():
     if counted:
         # per-kernel HIP-event times: one more render of the same workload with the bounce
         # loop launched kernel by kernel (the timed steps replay it as a hipGraph)
-        tp = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, time_kernels=1)
+        tp = render_params(spp=spp, max_depth=depth, rr_threshold=1.0, seed=0, time_kernels=1,
+                           bvh_width=args.bvh_width)
         timed = tr.render(cam, tp)["stats"]
-        kb = roofline.kernel_bytes(counted)
+        width = counted["bvh_width"]
+        kb = roofline.kernel_bytes(counted, width)
         last = timed
         kms = {k: last["kernel_ms"][k] for k in ("extend", "shade", "connect")}
         dom = max(kms, key=kms.get)
@@ -685,7 +690,8 @@ def main():
         achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(f"k_b_{dom}<2>") if args.workload == "icosphere" else None,
+                "traffic": profiled_traffic(f"k_b_{dom}<{width}>") if args.workload == "icosphere" else None,
+                "bvh_width": width,
                 "algorithmic_bytes_per_launch": int(kb[dom] / max(1, launches)),
                 "avg_launch_ms": kms[dom] / max(1, launches),
                 "kernel_ms": {k: round(v, 3) for k, v in kms.items()},

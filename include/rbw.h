@@ -235,7 +235,7 @@ typedef struct rbw_render_params {
     uint32_t count_visits;           /* nonzero: count BVH node visits and primitive tests */
     const rbw_guiding_opts* guiding; /* NULL = plain BSDF continuation (the reference's) */
     uint32_t time_kernels;           /* nonzero: per-kernel HIP-event times in rbw_stats::kernel_ms */
-    uint32_t bvh_width;              /* 0 (default: 2), 2 (binary BVH) or 8 (eight-wide BVH) */
+    uint32_t bvh_width;              /* 0 (default: 4), 2 (binary BVH), 4 (four-wide BVH) or 8 (eight-wide BVH) */
     /* RBW_SAMPLER_STRATIFIED: strata_x * strata_y must equal spp; stratified 1D and 2D
      * dimensions (at most 65535 each) */
     uint32_t strata_x, strata_y, dims_1d, dims_2d;
@@ -285,6 +285,8 @@ typedef struct rbw_stats {
     uint64_t guide_samples;          /* guided + train: training samples collected */
     uint64_t guide_steps;            /* guided + train: Adam steps taken */
     double guide_loss;               /* guided + train: loss of the last step, before its update */
+    uint32_t bvh_width;              /* the BVH width the render walked (2, 4 or 8) */
+    uint32_t pad;
 } rbw_stats;
 
 typedef struct rbw_device_opts {
@@ -325,7 +327,7 @@ rbw_status rbw_render(rbw_ctx* ctx, const rbw_camera_desc* camera,
 rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
                           const float* directions, const float* tmax,
                           uint32_t visible_only, uint32_t triangle_test,
-                          uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
+                          uint32_t bvh_width /* 0 (default), 2, 4 or 8 */, rbw_hit* out);
 
 /* rbw_trace_rays without any device allocation: the same queries, with the rays and hit
  * records staged in a fixed device buffer that the library holds per device from its load
@@ -338,7 +340,7 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins,
 #define RBW_STATIC_MAX_RAYS 262144u
 rbw_status rbw_trace_rays_static(rbw_ctx* ctx, uint32_t n, const float* origins, const float* directions,
                                  const float* tmax, uint32_t visible_only, uint32_t triangle_test,
-                                 uint32_t bvh_width /* 0 (default), 2 or 8 */, rbw_hit* out);
+                                 uint32_t bvh_width /* 0 (default), 2, 4 or 8 */, rbw_hit* out);
 
 /* Diagnostic: the draws the sampler of `params` (per-sample pcg32 or stratified)
  * gives the spp samples of pixel p (y*width + x): per sample, n1 1D draws then n2 2D
@@ -403,7 +405,7 @@ typedef struct rbw_batch_opts {
     uint32_t pool_size;              /* path slots of the shared launch; 0 = default 2^21.  Jobs with at
                                         least this many samples run alone, one after another, with at
                                         least a single render's 2^23 slots; the rest share the launch */
-    uint32_t bvh_width;              /* 0 (default: 2), 2 or 8; the autotuner tries both */
+    uint32_t bvh_width;              /* 0 (default: 4), 2, 4 or 8; the autotuner tries all three */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
     uint32_t autotune;               /* 1: first probe the jobs themselves (their real spp, each probe
                                         cut after 12 graph launches) over BVH widths 2/8, block sizes

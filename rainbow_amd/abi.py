@@ -105,7 +105,8 @@ class rbw_stats(C.Structure):
                 ("extend_node_visits", C.c_uint64), ("extend_prim_tests", C.c_uint64),
                 ("kernel_ms", C.c_double * RBW_KERNEL_COUNT),
                 ("kernel_launches", C.c_uint64 * RBW_KERNEL_COUNT),
-                ("guide_samples", C.c_uint64), ("guide_steps", C.c_uint64), ("guide_loss", C.c_double)]
+                ("guide_samples", C.c_uint64), ("guide_steps", C.c_uint64), ("guide_loss", C.c_double),
+                ("bvh_width", C.c_uint32), ("pad", C.c_uint32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("kernel_ms", "kernel_launches")}

@@ -55,7 +55,7 @@ struct djob {
     float rr_threshold;
     uint32_t root8;
     uint32_t light_flags;  // k_lights_* of the job's scene
-    uint32_t pad;
+    uint32_t root4;
     f3 gmin, ginv;         // guided jobs: the network query's position normalisation (scene box)
 };
 
@@ -132,6 +132,7 @@ __device__ __forceinline__ uint32_t find_job(const djob* jobs, uint32_t n, unsig
 __device__ __forceinline__ void job_view(const dscene& S, const dparams& base, const djob& J, dscene& Sj, dparams& pj) {
     Sj = S;
     Sj.root = J.root;
+    Sj.root4 = J.root4;
     Sj.root8 = J.root8;
     Sj.emitters = S.emitters + J.em_first;
     Sj.num_emitters = J.em_count;
@@ -476,7 +477,7 @@ struct rbw_batch_state {
     // merged scene of the last submission
     std::vector<uint32_t> scene_ids;
     dev_scene scene;
-    std::vector<uint32_t> roots, roots8, em_first, em_count, light_flags;
+    std::vector<uint32_t> roots, roots4, roots8, em_first, em_count, light_flags;
     uint32_t merged_tris = 0;
     // path pool
     dbuf pf, pu, prng;
@@ -586,7 +587,7 @@ void prepare_scene(rbw_ctx* ctx, const rbw_job* jobs, size_t n, std::vector<uint
     std::vector<const host_scene*> parts;
     for (uint32_t id : ids) parts.push_back(ctx->pool[id].get());
     host_scene merged;
-    merge_scenes(parts, merged, b.roots, b.roots8, b.em_first, b.em_count);
+    merge_scenes(parts, merged, b.roots, b.roots4, b.roots8, b.em_first, b.em_count);
     b.light_flags.clear();
     for (const host_scene* h : parts) b.light_flags.push_back(h->light_flags);
     b.scene.upload(merged, ctx->stream);
@@ -749,16 +750,18 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     bool fast = base.debug == nullptr;
     for (const djob& J : hjobs) fast = fast && (J.light_flags & (k_lights_few | k_lights_planar)) == (k_lights_few | k_lights_planar);
     void (*connect_kernel)(dscene, dpaths, dparams, dbatch) =
-        S.width == 8 ? (fast ? k_b_connect<8, true> : k_b_connect<8, false>)
-                     : (fast ? k_b_connect<2, true> : k_b_connect<2, false>);
+        S.width == 8   ? (fast ? k_b_connect<8, true> : k_b_connect<8, false>)
+        : S.width == 4 ? (fast ? k_b_connect<4, true> : k_b_connect<4, false>)
+                       : (fast ? k_b_connect<2, true> : k_b_connect<2, false>);
+    void (*extend_kernel)(dscene, dpaths, dparams, dbatch) =
+        S.width == 8 ? k_b_extend<8> : (S.width == 4 ? k_b_extend<4> : k_b_extend<2>);
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
         hipLaunchKernelGGL(k_b_regen, dim3(ngroups), dim3(k_group), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
-        hipLaunchKernelGGL(S.width == 8 ? k_b_extend<8> : k_b_extend<2>, dim3(grid), dim3(bs), trav_lds, s, S, P,
-                           base, B);
+        hipLaunchKernelGGL(extend_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_SHADE, s);
         if (guided) {
@@ -922,6 +925,7 @@ std::vector<djob> make_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, const s
         d.film.nan = reinterpret_cast<uint32_t*>(f + np * 28);
         const uint32_t p = part_of_job[i];
         d.root = b.roots[p];
+        d.root4 = b.roots4[p];
         d.root8 = b.roots8[p];
         d.em_first = b.em_first[p];
         d.em_count = b.em_count[p];
@@ -996,7 +1000,7 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
         if (idx.empty()) idx.push_back(0);
         const uint32_t pool = idx.size() == 1 && n > 1 ? alone_pool(idx[0], cfg.pool) : cfg.pool;
         double best = -1.0;
-        for (uint32_t width : {2u, 8u}) {
+        for (uint32_t width : {2u, 4u, 8u}) {
             const double r = probe(idx, best_block, pool, width);
             if (r > best) {
                 best = r;
@@ -1205,6 +1209,7 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
     J.film.weight = reinterpret_cast<uint32_t*>(f + np * 24);
     J.film.nan = reinterpret_cast<uint32_t*>(f + np * 28);
     J.root = 0;
+    J.root4 = 0;
     J.root8 = 0;
     J.em_first = 0;
     J.em_count = ctx->main.num_emitters;
@@ -1248,6 +1253,7 @@ void render_scheduled(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render
         float resolve_ms = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&resolve_ms, ea, eb));
         stats->render_ms = rr.ms + resolve_ms;
+        stats->bvh_width = cfg.width;
         stats->samples = J.count;
         stats->camera_rays = J.count;
         stats->extension_rays = rr.counters[0];

@@ -217,6 +217,69 @@ inline float step_of(uint8_t e) {
 
 }  // namespace
 
+void build_bvh4(const bvh_build_result& bin, bvh4_result& out) {
+    const std::vector<build_node>& t = bin.tree;
+    out = bvh4_result();
+    if (t.empty()) throw std::runtime_error("empty tree");
+    out.nodes.reserve(t.size() / 3 + 1);
+    auto leaf_ref = [&](int32_t b) {
+        return int32_t(uint32_t(k_leaf_bit) | ((t[b].count - 1) << 27) | t[b].first);
+    };
+    // depth-first: a node's subtree follows it; pushed = stack entries its ancestors may leave
+    struct maker {
+        const std::vector<build_node>& t;
+        bvh4_result& out;
+        const decltype(leaf_ref)& leaf;
+        uint32_t make(int32_t b, uint32_t level, uint32_t pushed) {
+            out.depth = std::max(out.depth, level);
+            const uint32_t id = uint32_t(out.nodes.size());
+            out.nodes.emplace_back();
+            int32_t ch[4];
+            int nch = 0;
+            if (t[b].left < 0) {
+                ch[nch++] = b;
+            } else {
+                ch[nch++] = t[b].left;
+                ch[nch++] = t[b].right;
+                while (nch < 4) {
+                    int best = -1;
+                    float best_area = -1.0f;
+                    for (int k = 0; k < nch; k++)
+                        if (t[ch[k]].left >= 0 && area(t[ch[k]].box) > best_area) {
+                            best_area = area(t[ch[k]].box);
+                            best = k;
+                        }
+                    if (best < 0) break;
+                    const int32_t o = ch[best];
+                    ch[best] = t[o].left;
+                    ch[nch++] = t[o].right;
+                }
+            }
+            const uint32_t below = pushed + uint32_t(nch - 1);
+            out.stack = std::max(out.stack, below);
+            bvh4_node n;
+            std::memset(&n, 0, sizeof(n));
+            for (int k = 0; k < 4; k++) {
+                if (k >= nch) {
+                    n.lox[k] = n.loy[k] = n.loz[k] = FLT_MAX;
+                    n.hix[k] = n.hiy[k] = n.hiz[k] = -FLT_MAX;
+                    n.child[k] = k_empty_child;
+                    continue;
+                }
+                const aabb& bx = t[ch[k]].box;
+                n.lox[k] = bx.lo[0]; n.hix[k] = bx.hi[0];
+                n.loy[k] = bx.lo[1]; n.hiy[k] = bx.hi[1];
+                n.loz[k] = bx.lo[2]; n.hiz[k] = bx.hi[2];
+                n.child[k] = t[ch[k]].left >= 0 ? int32_t(make(ch[k], level + 1, below)) : leaf(ch[k]);
+            }
+            out.nodes[id] = n;
+            return id;
+        }
+    };
+    maker m{t, out, leaf_ref};
+    m.make(0, 1, 0);
+}
+
 void build_bvh8(const bvh_build_result& bin, bvh8_result& out) {
     const std::vector<build_node>& t = bin.tree;
     out = bvh8_result();

@@ -32,6 +32,20 @@ constexpr int32_t k_leaf_bit = int32_t(0x80000000u);
 constexpr uint32_t k_leaf_max = 16;
 constexpr int32_t k_empty_child = int32_t(0xFFFFFFFFu);  // leaf flag with count 16 at slot 2^27-1: never hit
 
+// Four-wide BVH, one 128-byte cache line per node (full-precision child boxes, SoA by
+// plane, so a visit is seven aligned 16-byte loads of one line):
+//   lox, hix, loy, hiy, loz, hiz   the four child boxes
+//   child                          child refs in the bvh2 encoding (inner node index,
+//                                  leaf bit | count - 1 | first slot, or k_empty_child)
+// Leaves are the binary tree's leaves and use its leaf slots, so the two trees share
+// one copy of the primitives.
+struct bvh4_node {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(bvh4_node) == 128, "bvh4 node is one 128-byte line");
+
 // Eight-wide BVH with 8-bit quantised child boxes, 80 bytes per node (five
 // 16-byte loads), after Ylitie, Karras and Laine, "Efficient Incoherent Ray
 // Traversal on GPUs Through Compressed Wide BVHs", HPG 2017; the field coding
@@ -95,6 +109,17 @@ struct bvh_build_result {
 
 // bounds: conservative primitive boxes.  Builds the binned-SAH tree and the bvh2 layout.
 void build_bvh(const std::vector<aabb>& bounds, bvh_build_result& out, uint32_t max_leaf = 4);
+
+struct bvh4_result {
+    std::vector<bvh4_node> nodes;   // root at index 0, depth-first order
+    uint32_t depth = 0;             // levels of nodes
+    uint32_t stack = 0;             // most entries a walk's stack can hold (3 per level at most)
+};
+
+// Collapses the binary SAH tree into the four-wide layout: each node takes a binary
+// node's children and opens its largest inner child until it has four.  Binary leaves
+// stay leaves (same slots as the binary tree's).
+void build_bvh4(const bvh_build_result& bin, bvh4_result& out);
 
 // Collapses the binary SAH tree into the eight-wide layout: each wide node
 // takes a binary node's children and keeps opening its largest inner child

@@ -66,13 +66,16 @@ struct dscene {
     uint32_t num_tris;
     uint32_t num_spheres;
     uint32_t root;           // BVH root node (a part's root inside a merged scene)
+    // four-wide BVH over the same primitives and leaf slots (8 float4 per node)
+    const float4* nodes4;
+    uint32_t root4;
     // eight-wide BVH over the same primitives, with its own leaf slots
     const float4* nodes8;    // 5 float4 per node
     const float4* prim8_a;
     const float4* prim8_b;
     const float4* prim8_c;
     uint32_t root8;
-    uint32_t width;          // 2 or 8: the tree traversals use
+    uint32_t width;          // 2, 4 or 8: the tree traversals use
     uint32_t light_flags;    // k_lights_* : what the connect stage may shortcut (set per scene)
     // media (volume path integrator only)
     const dmedium* media;

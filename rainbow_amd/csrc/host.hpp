@@ -115,7 +115,8 @@ struct kernel_timer {
 // the device copy holds.  Built once per rbw_upload_scene / rbw_add_scene.
 struct host_scene {
     std::vector<float4> nodes;      // 4 per bvh2 node
-    std::vector<float4> pa, pb, pc; // leaf slots
+    std::vector<float4> pa, pb, pc; // leaf slots (of the bvh2 and the bvh4)
+    std::vector<float4> nodes4;     // 8 per bvh4 node
     std::vector<float4> nodes8;     // 5 per bvh8 node
     std::vector<float4> pa8, pb8, pc8;  // leaf slots of the bvh8
     std::vector<float> pos, nrm, uv;
@@ -128,7 +129,7 @@ struct host_scene {
     std::vector<float> grid;
     uint32_t nt = 0, ns = 0, nv = 0;
     bool has_nrm = false, has_uv = false, has_tri_material = false;
-    uint32_t depth = 0, depth8 = 0;
+    uint32_t depth = 0, depth8 = 0, stack4 = 0;
     uint32_t light_flags = 0;  // k_lights_*
     f3 smin{0, 0, 0}, smax{0, 0, 0};
     size_t bytes() const;
@@ -139,14 +140,15 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& out);
 
 // One scene holding several: every index rebased (nodes, slots, primitives,
 // vertices, entities, materials), each part's BVHs kept intact under their own
-// roots.  roots/roots8/em_first/em_count describe part i inside the union.
+// roots.  roots/roots4/roots8/em_first/em_count describe part i inside the union.
 void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& out, std::vector<uint32_t>& roots,
-                  std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count);
+                  std::vector<uint32_t>& roots4, std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first,
+                  std::vector<uint32_t>& em_count);
 
-constexpr uint32_t k_default_width = 2;  // BVH width of a render that does not name one (the faster on the icosphere)
+constexpr uint32_t k_default_width = 4;  // BVH width of a render that does not name one (the fastest on the icosphere)
 
 struct dev_scene {
-    dbuf nodes, prim_a, prim_b, prim_c, nodes8, prim8_a, prim8_b, prim8_c, pos, nrm, uv, idx, tri_entity,
+    dbuf nodes, prim_a, prim_b, prim_c, nodes4, nodes8, prim8_a, prim8_b, prim8_c, pos, nrm, uv, idx, tri_entity,
         tri_material, spheres, sphere_entity, ents, mats, emitters, media, ent_media, grid;
     uint32_t nt = 0, ns = 0, num_emitters = 0, num_media = 0, light_flags = 0;
     bool has_uv = false, has_tri_material = false, valid = false;
@@ -154,7 +156,7 @@ struct dev_scene {
     dscene view(uint32_t width = k_default_width) const;
 };
 
-// 0 -> the default width; otherwise 2 or 8
+// 0 -> the default width; otherwise 2, 4 or 8
 uint32_t resolve_width(uint32_t requested);
 
 }  // namespace host

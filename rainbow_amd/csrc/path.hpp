@@ -29,12 +29,18 @@ inline size_t lds_stack_bytes(uint32_t block) { return size_t(RBW_LDS_STACK) * b
 // The shadow/MIS kernels are latency bound on dependent node fetches: with the binary
 // walk six waves per SIMD (at most 80 VGPRs, a few spilled) beat the five its natural
 // 95 VGPRs allow; the wide walk's larger state is left unconstrained (W: the width).
-#define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 6 : 1)))
+#define RBW_CONNECT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 6 : (W == 4 ? RBW_CONNECT4_WAVES : 1))))
 // The shading kernel keeps four waves per SIMD (at most 128 VGPRs).
 #define RBW_SHADE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 // The binary walk of extension rays: seven waves per SIMD (at most 72 VGPRs, 4 spilled)
 // beat the six of its natural 79 VGPRs; eight (64 VGPRs) spill 11 and run slower.
-#define RBW_EXTEND_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 7 : 1)))
+#define RBW_EXTEND_OCCUPANCY __attribute__((amdgpu_waves_per_eu(W == 2 ? 7 : (W == 4 ? RBW_EXTEND4_WAVES : 1))))
+#ifndef RBW_EXTEND4_WAVES
+#define RBW_EXTEND4_WAVES 6
+#endif
+#ifndef RBW_CONNECT4_WAVES
+#define RBW_CONNECT4_WAVES 6
+#endif
 
 // ------------------------------------------------------------------ traversal
 
@@ -398,12 +404,143 @@ __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, flo
     return st.best;
 }
 
+// closest hit (or closest visible hit) through the four-wide BVH: the walk of traverse2
+// with four child boxes per node (one 128-byte line), hit children visited nearest
+// first and the others pushed far to near; postponed leaves as there
+template <bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS>
+__device__ __forceinline__ hitrec traverse4(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
+                                            uint32_t target_entity, trace_counters& cnt) {
+    const local_ray lr = make_local_ray(o_w, d_w);
+    wt_setup wt;
+    if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+    trav_state st;
+    st.best.prim = RBW_MISS;
+    st.best.t = 0.0f;
+    st.best.b1 = st.best.b2 = 0.0f;
+    st.best_inst_entity = RBW_NONE;
+    st.len_w = len_w;
+    st.thr_local = len_w * lr.L0;
+    uint32_t stack[RBW_STACK];
+    int sp = 0;
+    uint32_t* ring = rbw_lds_stack + threadIdx.x;
+    const uint32_t ring_stride = blockDim.x;
+    auto push = [&](uint32_t v) {
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * ring_stride;
+            if (sp >= RBW_LDS_STACK) stack[sp - RBW_LDS_STACK] = *e;
+            *e = v;
+        } else {
+            stack[sp] = v;
+        }
+        sp++;
+    };
+    auto pop = [&]() -> uint32_t {
+        sp--;
+        if (LDS) {
+            uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * ring_stride;
+            const uint32_t v = *e;
+            if (sp >= RBW_LDS_STACK) *e = stack[sp - RBW_LDS_STACK];
+            return v;
+        }
+        return stack[sp];
+    };
+    int32_t cur = (int32_t)S.root4;
+    const f3 o = lr.o;
+    const f3 inv = lr.inv;
+    const f3 oi = f3{o.x * inv.x, o.y * inv.y, o.z * inv.z};
+    bool done = false, has_leaf = false;
+    int32_t leaf = 0;
+    auto advance = [&]() {
+        if (sp == 0) done = true;
+        else cur = (int32_t)pop();
+    };
+    for (;;) {
+        for (;;) {
+            if (!done && !has_leaf && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
+            }
+            const bool walk = !done && cur >= 0;
+            if (__ballot(walk) == 0) break;
+            if (walk) {
+                const float4* n = S.nodes4 + 8 * (size_t)cur;
+                const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+                const int4 ch = *reinterpret_cast<const int4*>(n + 6);
+                if (COUNT) cnt.nodes++;
+                const float tmax = st.thr_local * 1.0000002f;
+                float d[4];
+                int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
+                const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
+                const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
+                const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+                uint32_t nhit = 0;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float x0 = __builtin_fmaf(blx[c], inv.x, -oi.x), x1 = __builtin_fmaf(bhx[c], inv.x, -oi.x);
+                    const float y0 = __builtin_fmaf(bly[c], inv.y, -oi.y), y1 = __builtin_fmaf(bhy[c], inv.y, -oi.y);
+                    const float z0 = __builtin_fmaf(blz[c], inv.z, -oi.z), z1 = __builtin_fmaf(bhz[c], inv.z, -oi.z);
+                    const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+                    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+                    const bool hit = r[c] != k_empty_child && tmin <= tfar * 1.0000004f;
+                    d[c] = hit ? tmin : __builtin_inff();
+                    nhit += hit ? 1u : 0u;
+                }
+                // sorting network on (distance, child): misses (infinite) go last
+                auto cswap = [&](int i, int j) {
+                    const bool sw = d[j] < d[i];
+                    const float di = d[i], dj = d[j];
+                    const int32_t ri = r[i], rj = r[j];
+                    d[i] = sw ? dj : di;
+                    d[j] = sw ? di : dj;
+                    r[i] = sw ? rj : ri;
+                    r[j] = sw ? ri : rj;
+                };
+                cswap(0, 1);
+                cswap(2, 3);
+                cswap(0, 2);
+                cswap(1, 3);
+                cswap(1, 2);
+                if (nhit == 0) {
+                    advance();
+                } else {
+                    cur = r[0];
+                    if (nhit > 3) push((uint32_t)r[3]);
+                    if (nhit > 2) push((uint32_t)r[2]);
+                    if (nhit > 1) push((uint32_t)r[1]);
+                }
+            }
+            if (__ballot(!(has_leaf || done)) == 0) break;
+        }
+        while (has_leaf) {
+            has_leaf = false;
+            const uint32_t first = (uint32_t)leaf & 0x07FFFFFFu;
+            const uint32_t count = (((uint32_t)leaf >> 27) & 15u) + 1u;
+            if (test_leaf<VISIBLE_ONLY, ANY_HIT, COUNT>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt,
+                                                        tri_test, target_entity, cnt, st))
+                return st.best;
+            if (!done && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
+            }
+        }
+        if (done) break;
+    }
+    return st.best;
+}
+
 // W: the BVH width the kernel is instantiated for (each kernel carries one walk)
 // LDS: the kernel provides the walk's LDS stack (lds_stack_bytes of dynamic LDS)
 template <int W, bool VISIBLE_ONLY, bool ANY_HIT, bool COUNT, bool LDS = false>
 __device__ __forceinline__ hitrec traverse(const dscene& S, f3 o_w, f3 d_w, float len_w, uint32_t tri_test,
                                            uint32_t target_entity, trace_counters& cnt) {
     if (W == 8) return traverse8<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
+    if (W == 4) return traverse4<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
     return traverse2<VISIBLE_ONLY, ANY_HIT, COUNT, LDS>(S, o_w, d_w, len_w, tri_test, target_entity, cnt);
 }
 

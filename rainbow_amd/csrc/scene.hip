@@ -165,12 +165,13 @@ void fill_slots(const rbw_scene_desc* d, const host_scene& h, const std::vector<
 
 uint32_t resolve_width(uint32_t requested) {
     if (requested == 0) return k_default_width;
-    require(requested == 2 || requested == 8, RBW_ERR_INVALID_ARGUMENT, "bvh_width must be 0, 2 or 8");
+    require(requested == 2 || requested == 4 || requested == 8, RBW_ERR_INVALID_ARGUMENT,
+            "bvh_width must be 0, 2, 4 or 8");
     return requested;
 }
 
 size_t host_scene::bytes() const {
-    return (nodes.size() + nodes8.size()) * sizeof(float4) + 3 * (pa.size() + pa8.size()) * sizeof(float4) +
+    return (nodes.size() + nodes4.size() + nodes8.size()) * sizeof(float4) + 3 * (pa.size() + pa8.size()) * sizeof(float4) +
            (pos.size() + nrm.size() + uv.size()) * 4 +
            (idx.size() + tri_entity.size() + tri_material.size() + sphere_entity.size() + emitters.size()) * 4 +
            spheres.size() * sizeof(float4) + ents.size() * sizeof(dentity) + mats.size() * sizeof(dmaterial);
@@ -299,6 +300,9 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
     bvh_build_result bvh;
     build_bvh(boxes, bvh, 4);
     require(bvh.depth < k_stack_depth, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    bvh4_result bvh4;
+    build_bvh4(bvh, bvh4);
+    require(bvh4.stack < k_stack_depth, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     bvh8_result bvh8;
     build_bvh8(bvh, bvh8);
     require(bvh8.depth < k_stack_depth, RBW_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
@@ -306,6 +310,9 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
     fill_slots(d, h, bvh8.order, h.pa8, h.pb8, h.pc8);
     h.nodes.resize(bvh.nodes2.size() * 4);
     std::memcpy(h.nodes.data(), bvh.nodes2.data(), bvh.nodes2.size() * sizeof(bvh2_node));
+    h.nodes4.resize(bvh4.nodes.size() * 8);
+    std::memcpy(h.nodes4.data(), bvh4.nodes.data(), bvh4.nodes.size() * sizeof(bvh4_node));
+    h.stack4 = bvh4.stack;
     h.nodes8.resize(bvh8.nodes.size() * 5);
     std::memcpy(h.nodes8.data(), bvh8.nodes.data(), bvh8.nodes.size() * sizeof(bvh8_node));
     h.depth8 = bvh8.depth;
@@ -345,11 +352,12 @@ void build_host_scene(const rbw_scene_desc* d, host_scene& h) {
 }
 
 void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, std::vector<uint32_t>& roots,
-                  std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first, std::vector<uint32_t>& em_count) {
+                  std::vector<uint32_t>& roots4, std::vector<uint32_t>& roots8, std::vector<uint32_t>& em_first,
+                  std::vector<uint32_t>& em_count) {
     u = host_scene();
     u.light_flags = k_lights_few | k_lights_planar;
     for (const host_scene* p : parts) u.light_flags &= p->light_flags;
-    uint64_t T = 0, Sp = 0, V = 0, N = 0, L = 0, E = 0, M = 0, N8 = 0, L8 = 0;
+    uint64_t T = 0, Sp = 0, V = 0, N = 0, L = 0, E = 0, M = 0, N4 = 0, N8 = 0, L8 = 0;
     bool any_nrm = false, any_uv = false, any_tm = false;
     for (const host_scene* p : parts) {
         T += p->nt;
@@ -357,6 +365,7 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         V += p->nv;
         N += p->nodes.size() / 4;
         L += p->pa.size();
+        N4 += p->nodes4.size() / 8;
         N8 += p->nodes8.size() / 5;
         L8 += p->pa8.size();
         E += p->ents.size();
@@ -367,7 +376,7 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
     }
     require(T + Sp < (1ull << 29), RBW_ERR_UNSUPPORTED, "merged scene has too many primitives");
     require(L < (1ull << 27), RBW_ERR_UNSUPPORTED, "merged scene has too many leaf slots");
-    require(N < (1ull << 31) && N8 < (1ull << 32) && L8 < (1ull << 32) && V < (1ull << 32) && E < (1ull << 31) &&
+    require(N < (1ull << 31) && N4 < (1ull << 31) && N8 < (1ull << 32) && L8 < (1ull << 32) && V < (1ull << 32) && E < (1ull << 31) &&
                 M < (1ull << 31),
             RBW_ERR_UNSUPPORTED, "merged scene too large");
     u.nt = uint32_t(T);
@@ -390,19 +399,22 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
     u.sphere_entity.reserve(Sp);
     u.ents.reserve(E);
     u.mats.reserve(M);
+    u.nodes4.reserve(N4 * 8);
     u.nodes8.reserve(N8 * 5);
     u.pa8.reserve(L8);
     u.pb8.reserve(L8);
     u.pc8.reserve(L8);
     roots.clear();
+    roots4.clear();
     roots8.clear();
     em_first.clear();
     em_count.clear();
-    uint32_t tb = 0, sb = 0, vb = 0, nb = 0, lb = 0, eb = 0, mb = 0, n8b = 0, l8b = 0;
+    uint32_t tb = 0, sb = 0, vb = 0, nb = 0, lb = 0, eb = 0, mb = 0, n4b = 0, n8b = 0, l8b = 0;
     u.smin = f3{3.4e38f, 3.4e38f, 3.4e38f};
     u.smax = f3{-3.4e38f, -3.4e38f, -3.4e38f};
     for (const host_scene* p : parts) {
         roots.push_back(nb);
+        roots4.push_back(n4b);
         roots8.push_back(n8b);
         em_first.push_back(uint32_t(u.emitters.size()));
         em_count.push_back(uint32_t(p->emitters.size()));
@@ -424,6 +436,24 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
             float4 q[4];
             std::memcpy(q, &nd, sizeof(nd));
             for (int k = 0; k < 4; k++) u.nodes.push_back(q[k]);
+        }
+        // four-wide nodes: inner children + n4b, leaves' first slot + lb (the binary tree's slots)
+        for (size_t i = 0; i < p->nodes4.size(); i += 8) {
+            bvh4_node nd;
+            std::memcpy(&nd, &p->nodes4[i], sizeof(nd));
+            for (int c = 0; c < 4; c++) {
+                const int32_t ch = nd.child[c];
+                if (ch == k_empty_child) continue;
+                if (ch >= 0) {
+                    nd.child[c] = ch + int32_t(n4b);
+                } else {
+                    const uint32_t w = uint32_t(ch);
+                    nd.child[c] = int32_t((w & 0xF8000000u) | ((w & 0x07FFFFFFu) + lb));
+                }
+            }
+            float4 q[8];
+            std::memcpy(q, &nd, sizeof(nd));
+            for (int k = 0; k < 8; k++) u.nodes4.push_back(q[k]);
         }
         // wide nodes: first inner child + n8b, first leaf slot + l8b
         for (size_t i = 0; i < p->nodes8.size(); i += 5) {
@@ -480,6 +510,7 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         for (uint32_t e : p->emitters) u.emitters.push_back(e + eb);
         u.depth = std::max(u.depth, p->depth);
         u.depth8 = std::max(u.depth8, p->depth8);
+        u.stack4 = std::max(u.stack4, p->stack4);
         u.smin = f3{std::min(u.smin.x, p->smin.x), std::min(u.smin.y, p->smin.y), std::min(u.smin.z, p->smin.z)};
         u.smax = f3{std::max(u.smax.x, p->smax.x), std::max(u.smax.y, p->smax.y), std::max(u.smax.z, p->smax.z)};
         tb += p->nt;
@@ -487,6 +518,7 @@ void merge_scenes(const std::vector<const host_scene*>& parts, host_scene& u, st
         vb += p->nv;
         nb += uint32_t(p->nodes.size() / 4);
         lb += uint32_t(p->pa.size());
+        n4b += uint32_t(p->nodes4.size() / 8);
         n8b += uint32_t(p->nodes8.size() / 5);
         l8b += uint32_t(p->pa8.size());
         eb += uint32_t(p->ents.size());
@@ -500,6 +532,7 @@ void dev_scene::upload(const host_scene& h, hipStream_t s) {
     prim_a.upload(h.pa.data(), h.pa.size(), s);
     prim_b.upload(h.pb.data(), h.pb.size(), s);
     prim_c.upload(h.pc.data(), h.pc.size(), s);
+    nodes4.upload(h.nodes4.data(), h.nodes4.size(), s);
     nodes8.upload(h.nodes8.data(), h.nodes8.size(), s);
     prim8_a.upload(h.pa8.data(), h.pa8.size(), s);
     prim8_b.upload(h.pb8.data(), h.pb8.size(), s);
@@ -553,6 +586,8 @@ dscene dev_scene::view(uint32_t width) const {
     S.num_tris = nt;
     S.num_spheres = ns;
     S.root = 0;
+    S.nodes4 = nodes4.as<float4>();
+    S.root4 = 0;
     S.nodes8 = nodes8.as<float4>();
     S.prim8_a = prim8_a.as<float4>();
     S.prim8_b = prim8_b.as<float4>();

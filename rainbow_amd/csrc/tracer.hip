@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void k_direct(dscene S, dpaths P, dparams prm,
 // media = true) and the medium switch at entities with media (medium.cpp:24-33);
 // Russian roulette as path_integrator.cpp:35-43.
 template <int W>
-__global__ __launch_bounds__(256) void k_volume(dscene S, dpaths P, dparams prm, const uint32_t* qin,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_volume(dscene S, dpaths P, dparams prm, const uint32_t* qin,
                                                const uint32_t* qin_count, uint32_t* qout, uint32_t* qout_count,
                                                unsigned long long* counters) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -861,8 +861,9 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             HIP_CHECK(hipMemsetAsync(counts + 2, 0, 4, s));
             if (timed) ctx->timer.begin(RBW_KERNEL_EXTEND, s);
             {
-                auto kx = S.width == 8 ? (count ? k_extend<8, true> : k_extend<8, false>)
-                                       : (count ? k_extend<2, true> : k_extend<2, false>);
+                auto kx = S.width == 8   ? (count ? k_extend<8, true> : k_extend<8, false>)
+                          : S.width == 4 ? (count ? k_extend<4, true> : k_extend<4, false>)
+                                         : (count ? k_extend<2, true> : k_extend<2, false>);
                 hipLaunchKernelGGL(kx, dim3(blocks(active, BS)), dim3(BS), lds_stack_bytes(BS), s, S,
                                    P, dp, qin, counts + cin, counters);
             }
@@ -872,11 +873,13 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
                 // their shadow, emitter-finding and beam rays in one kernel
                 if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
                 if (direct)
-                    hipLaunchKernelGGL(S.width == 8 ? k_direct<8> : k_direct<2>, dim3(blocks(active, BS)), dim3(BS),
+                    hipLaunchKernelGGL(S.width == 8 ? k_direct<8> : (S.width == 4 ? k_direct<4> : k_direct<2>),
+                                       dim3(blocks(active, BS)), dim3(BS),
                                        lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout,
                                        counters);
                 else
-                    hipLaunchKernelGGL(S.width == 8 ? k_volume<8> : k_volume<2>, dim3(blocks(active, BS)), dim3(BS),
+                    hipLaunchKernelGGL(S.width == 8 ? k_volume<8> : (S.width == 4 ? k_volume<4> : k_volume<2>),
+                                       dim3(blocks(active, BS)), dim3(BS),
                                        lds_stack_bytes(BS), s, S, P, dp, qin, counts + cin, qout, counts + cout,
                                        counters);
                 if (timed) ctx->timer.end(s);
@@ -912,8 +915,9 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
             }
             if (timed) ctx->timer.begin(RBW_KERNEL_CONNECT, s);
             {
-                auto kc = S.width == 8 ? (count ? k_connect<8, true> : k_connect<8, false>)
-                                       : (count ? k_connect<2, true> : k_connect<2, false>);
+                auto kc = S.width == 8   ? (count ? k_connect<8, true> : k_connect<8, false>)
+                          : S.width == 4 ? (count ? k_connect<4, true> : k_connect<4, false>)
+                                         : (count ? k_connect<2, true> : k_connect<2, false>);
                 hipLaunchKernelGGL(kc, dim3(blocks(active, BS)), dim3(BS), lds_stack_bytes(BS), s, S,
                                    P, dp, cqp, counts + 2, counters);
             }
@@ -974,6 +978,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         float ms = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
         stats->render_ms = ms;
+        stats->bvh_width = S.width;
         stats->samples = npix * pr.spp;
         stats->camera_rays = stats->samples;
         stats->extension_rays = ext_rays;
@@ -1157,7 +1162,7 @@ rbw_status rbw_trace_rays(rbw_ctx* ctx, uint32_t n, const float* origins, const 
             d.upload(directions, size_t(n) * 3, s);
             t.upload(tmax, n, s);
             h.ensure(size_t(n) * sizeof(rbw_hit));
-            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256), lds_stack_bytes(256), s,
+            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : (width == 4 ? k_trace_rays<4> : k_trace_rays<2>), dim3(blocks(n, 256)), dim3(256), lds_stack_bytes(256), s,
                                ctx->main.view(width), n, o.as<float>(),
                                d.as<float>(), t.as<float>(), visible_only, triangle_test, h.as<rbw_hit>());
             HIP_CHECK(hipGetLastError());
@@ -1197,7 +1202,7 @@ rbw_status rbw_trace_rays_static(rbw_ctx* ctx, uint32_t n, const float* origins,
             HIP_CHECK(hipMemcpyAsync(o, origins, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
             HIP_CHECK(hipMemcpyAsync(d, directions, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
             HIP_CHECK(hipMemcpyAsync(t, tmax, size_t(n) * sizeof(float), hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : k_trace_rays<2>, dim3(blocks(n, 256)), dim3(256),
+            hipLaunchKernelGGL(width == 8 ? k_trace_rays<8> : (width == 4 ? k_trace_rays<4> : k_trace_rays<2>), dim3(blocks(n, 256)), dim3(256),
                                lds_stack_bytes(256), s, ctx->main.view(width), n, o, d, t, visible_only, triangle_test,
                                h);
             HIP_CHECK(hipGetLastError());

@@ -3,8 +3,9 @@
 Per unit of work each kernel must move at least these bytes between HBM and
 the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
 
-  node visit      64 B   one bvh2 node (two child boxes + child refs); 80 B for a
-                         bvh8 node (eight 8-bit quantised child boxes)
+  node visit      64 B   one bvh2 node (two child boxes + child refs); 128 B for a
+                         bvh4 node (four child boxes + refs, one cache line); 80 B
+                         for a bvh8 node (eight 8-bit quantised child boxes)
   primitive test  48 B   one leaf slot (three float4: p0|id, p1, p2)
   extend ray      36 B   queue index 4 + ray origin/direction 24 (read)
                          + hit prim/t 8 (write)
@@ -19,6 +20,7 @@ the CUs (SoA path state, BVH2 layout of csrc/bvh.hpp):
   light cast      96 B   two light triangles' vertex positions and indices
 """
 S_NODE = 64
+S_NODE4 = 128
 S_NODE8 = 80
 S_PRIM = 48
 B_EXTEND_RAY = 36
@@ -33,7 +35,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 def kernel_bytes(stats, width=2):
     """Algorithmic bytes of a render per kernel kind, from an instrumented run's counters
     (width: the BVH the instrumented run walked)."""
-    node = S_NODE8 if width == 8 else S_NODE
+    node = {8: S_NODE8, 4: S_NODE4}.get(width, S_NODE)
     ext_nodes = stats["extend_node_visits"]
     ext_prims = stats["extend_prim_tests"]
     con_nodes = stats["node_visits"] - ext_nodes

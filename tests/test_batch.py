@@ -95,7 +95,7 @@ def test_batch_shared_scene_and_autotune(tracer, small_batch):
           {"scene": ids[1], "camera": jobs[1]["camera"], "spp": 1, "max_depth": 6, "seed": 3}]
     o, ms = tracer.autotune_jobs(js)
     assert o.block_size in (64, 128, 256, 512) and o.pool_size in (1 << 20, 1 << 21, 1 << 22)
-    assert o.bvh_width in (2, 8) and ms > 0
+    assert o.bvh_width in (2, 4, 8) and ms > 0
     imgs, _, st = tracer.submit_jobs(js, batch_opts(autotune=True))
     assert st["autotune_ms"] > 0
     ref, _, _ = tracer.submit_jobs(js, batch_opts())
@@ -104,17 +104,18 @@ def test_batch_shared_scene_and_autotune(tracer, small_batch):
 
 
 @pytest.mark.gpu
-def test_batch_wide_bvh_matches_single_renders(tracer, small_batch):
-    """the eight-wide BVH in the merged scene (rebased wide nodes and leaf slots) against
-    single renders with the same width"""
+@pytest.mark.parametrize("width", [4, 8])
+def test_batch_wide_bvh_matches_single_renders(tracer, small_batch, width):
+    """the four- and eight-wide BVHs in the merged scene (rebased wide nodes and leaf slots)
+    against single renders with the same width"""
     from rainbow_amd.tracer import render_params
     jobs, ids = small_batch
-    imgs, _, st = _submit(tracer, jobs, ids, bvh_width=8)
-    assert st["bvh_width"] == 8
+    imgs, _, st = _submit(tracer, jobs, ids, bvh_width=width)
+    assert st["bvh_width"] == width
     for i, j in enumerate(jobs):
         tracer.upload(j["scene"])
         single = tracer.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"],
-                                                          bvh_width=8), stats=False)["image"]
+                                                          bvh_width=width), stats=False)["image"]
         assert np.array_equal(imgs[i], single), i
 
 
@@ -137,8 +138,8 @@ def test_batch_64_jobs_match_single_renders(tracer):
     """The batched-jobs workload at its configured size: 64 seeded jobs (seed 2) of
     1e4-2e6 triangles (25.6 M in all), 128^2-1024^2, 4-256 spp, depth 3-12 in one
     submission (the autotuner's choice of schedule).  Every job's image equals its single
-    rbw_render (oracle-pinned by the parity tests) bit for bit, and every job's samples
-    are accounted for."""
+    rbw_render with the BVH width the batch ran (oracle-pinned by the parity tests) bit for
+    bit, and every job's samples are accounted for."""
     import time
     from rainbow_amd.tracer import batch_opts, render_params
     t0 = time.perf_counter()
@@ -153,8 +154,8 @@ def test_batch_64_jobs_match_single_renders(tracer):
         assert bst["samples"] == sum(j["camera"].width * j["camera"].height * j["spp"] for j in jobs) == 1099077143
         for i, j in enumerate(jobs):
             tracer.upload(j["scene"])
-            single = tracer.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"]),
-                                   stats=False)["image"]
+            single = tracer.render(j["camera"], render_params(spp=j["spp"], max_depth=j["max_depth"], seed=j["seed"],
+                                                              bvh_width=bst["bvh_width"]), stats=False)["image"]
             assert np.isfinite(single).all() and single.mean() > 0, i
             assert np.array_equal(imgs[i], single), (i, np.abs(imgs[i] - single).max())
             assert jstats[i]["samples"] == j["camera"].width * j["camera"].height * j["spp"]

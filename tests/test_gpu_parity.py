@@ -47,7 +47,7 @@ def cornell(tracer):
     return sc, cam, orc.OracleScene(sc)
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("visible", [False, True])
 def test_trace_rays_cornell(tracer, cornell, visible, width):
     sc, cam, os_ = cornell
@@ -113,7 +113,7 @@ def test_render_cornell_replay(tracer, cornell):
     assert frac < 1e-3
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_render_cornell_per_sample_pcg(tracer, cornell, width):
     sc, cam, os_ = cornell
     orc = _oracle()
@@ -144,14 +144,15 @@ def icosphere(tracer):
     return sc, cam, orc.OracleScene(sc)
 
 
-def test_render_icosphere_widths_agree(tracer, icosphere):
-    """the persistent-slot render with the binary and the eight-wide BVH: the same image
-    up to the rare ties of equal-distance hits (shared edges) that the walk order breaks"""
+@pytest.mark.parametrize("width", [4, 8])
+def test_render_icosphere_widths_agree(tracer, icosphere, width):
+    """the persistent-slot render with the binary and the four- or eight-wide BVH: the same
+    image up to the rare ties of equal-distance hits (shared edges) that the walk order breaks"""
     sc, cam, os_ = icosphere
     a = tracer.render(cam, render_params(spp=4, max_depth=8, seed=11, bvh_width=2))["image"]
-    b = tracer.render(cam, render_params(spp=4, max_depth=8, seed=11, bvh_width=8))["image"]
+    b = tracer.render(cam, render_params(spp=4, max_depth=8, seed=11, bvh_width=width))["image"]
     frac = pixel_mismatch(a, b).mean()
-    print("pixels differing between widths", frac)
+    print("pixels differing between widths 2 and", width, frac)
     assert frac < 1e-3
 
 

@@ -18,7 +18,7 @@ from test_oracle_pins import (RR_CASES, check_furnace, check_sphere_hits, closed
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_gpu_sphere_closed_forms(tracer, width):
     tracer.upload(sphere_scene())
     for name, o, d, want in sphere_ray_sets():

@@ -27,7 +27,7 @@ def icosphere_scene(tracer):
     return sc
 
 
-@pytest.mark.parametrize("width", [2, 8])
+@pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("visible", [False, True])
 def test_static_trace_matches_trace(tracer, icosphere_scene, width, visible):
     for n, seed in ((abi.RBW_STATIC_MAX_RAYS, 1), (37_501, 2)):
