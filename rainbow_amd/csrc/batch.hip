@@ -189,14 +189,33 @@ __global__ void k_b_init(dbatch B, dpaths P, unsigned long long total) {
 
 // Folds the last regeneration's per-wave retire records into the jobs' last
 // iterations (plain stores: every record of one regeneration carries the same
-// iteration), with the threads of one block.
+// iteration), with the threads of one block: four records per 16-byte load, the loads
+// of a thread's next groups in flight together, and a job stored once per run of
+// records that name it (a render's records mostly name one job).
 __device__ __forceinline__ void fold_retire_records(const dbatch& B, uint32_t iter) {
-    for (uint32_t w = threadIdx.x; w < B.nwaves; w += blockDim.x) {
-        const uint32_t j = B.wave_job[w];
-        if (j != RBW_NONE) {
+    uint32_t last = RBW_NONE;
+    auto fold = [&](uint32_t j) {
+        if (j != RBW_NONE && j != last) {
             B.last_iter[j] = iter;
-            B.wave_job[w] = RBW_NONE;
+            last = j;
         }
+    };
+    uint4* rec = reinterpret_cast<uint4*>(B.wave_job);  // device allocations are 256-byte aligned
+    const uint32_t n4 = B.nwaves / 4;
+    const uint4 none = make_uint4(RBW_NONE, RBW_NONE, RBW_NONE, RBW_NONE);
+#pragma unroll 4
+    for (uint32_t q = threadIdx.x; q < n4; q += blockDim.x) {
+        const uint4 w = rec[q];
+        if ((w.x & w.y & w.z & w.w) == RBW_NONE) continue;  // RBW_NONE is all ones
+        fold(w.x);
+        fold(w.y);
+        fold(w.z);
+        fold(w.w);
+        rec[q] = none;
+    }
+    for (uint32_t w = 4 * n4 + threadIdx.x; w < B.nwaves; w += blockDim.x) {
+        fold(B.wave_job[w]);
+        B.wave_job[w] = RBW_NONE;
     }
 }
 

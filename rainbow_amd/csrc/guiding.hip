@@ -104,8 +104,14 @@ __device__ __forceinline__ void wave_sync() {
 
 // ------------------------------------------------------------------ inference
 
-constexpr int INF_WAVES = 16;     // waves per block (one block per CU)
-constexpr int INF_R = 2;          // 16-record tiles per wave
+#ifndef RBW_INFER_WAVES
+#define RBW_INFER_WAVES 16
+#endif
+#ifndef RBW_INFER_TILES
+#define RBW_INFER_TILES 2
+#endif
+constexpr int INF_WAVES = RBW_INFER_WAVES;  // waves per block (one block per CU)
+constexpr int INF_R = RBW_INFER_TILES;      // 16-record tiles per wave
 constexpr uint32_t INF_MAX_BLOCKS = 256;
 constexpr size_t INF_XS = size_t(FWD_FRAGS) * FRAG_HALFS * 2 + N_BIAS * 4;  // per-wave input rows
 constexpr size_t INF_LDS = INF_XS + size_t(INF_WAVES) * 16 * INF_R * N_IN * 4;
@@ -313,7 +319,10 @@ __global__ __launch_bounds__(256) void k_csum(const float* __restrict__ c, train
     if (threadIdx.x == 0) csum_part[blockIdx.y * SUM_BLOCKS + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-constexpr int TR_WAVES = 4;
+#ifndef RBW_TRAIN_WAVES
+#define RBW_TRAIN_WAVES 8
+#endif
+constexpr int TR_WAVES = RBW_TRAIN_WAVES;
 constexpr int CHUNK = 16 * TR_WAVES;  // records per block iteration
 constexpr int XB_PITCH = CHUNK + 8;   // halfs per exchange row
 constexpr size_t L_FRAGS = 0;
@@ -592,8 +601,9 @@ __global__ __launch_bounds__(TR_WAVES * 64) __attribute__((amdgpu_waves_per_eu(T
                     }
             }
             __syncthreads();
+            // the weight gradients' K dimension: the chunk's records, 32 per product
 #pragma unroll
-            for (int kb = 0; kb < 2; kb++) {
+            for (int kb = 0; kb < CHUNK / 32; kb++) {
                 const int kofs = 32 * kb + 8 * g;
                 if (l >= 1) {
                     const h8 B = lds_h8(xb_x + (16 * nq + li) * XB_PITCH + kofs);
