@@ -562,7 +562,7 @@ def batch_leg(n_jobs=64, autotune=True):
 
 
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                       "tracer_persistent_bvh2_icosphere_1024_64spp.md")
+                       "tracer_persistent_bvh4_icosphere_1024_64spp.md")
 
 
 def profiled_traffic(kernel):
