@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "guided.hpp"
@@ -81,6 +82,7 @@ struct dbatch {
     uint32_t* first_iter;           // per job
     uint32_t* last_iter;            // per job
     unsigned long long* counters;   // extension, shadow, MIS, light casts (striped)
+    uint32_t* fetch;                // next slot of the persistent extension walk (zeroed by the plan)
     // guided jobs (job j uses network j): this iteration's guided vertices per job, as slots
     // (rows[j * pool + k], k < gcount[j]), and the training rows closed at retirement
     uint32_t guided, train;
@@ -227,6 +229,7 @@ __global__ __launch_bounds__(1024) void k_b_fold(dbatch B) { fold_retire_records
 __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     __shared__ uint32_t part[1024];
     const uint32_t t = threadIdx.x, n = B.ngroups;
+    if (t == 0) B.fetch[0] = 0u;
     fold_retire_records(B, B.ctl->iteration);
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
@@ -333,6 +336,134 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
         st = ST_ALIVE;
     }
     if (in && (gen || dying)) P.U(U_STATE, slot) = st;
+}
+
+// The extension walks of every live slot on the four-wide BVH, persistent: each wave takes
+// ranges of k_fetch_chunk consecutive slots from one counter and hands them to its lanes as
+// their walks end, so the lanes keep walking instead of waiting for the wave's longest walk
+// (one slot per lane for the whole launch kept ~20 % of the lanes busy).  A slot's walk is
+// traverse4's, so its hit is the same.  One wave per block; launch at most the resident
+// waves (more are harmless: they find the counter past the pool and leave).
+constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % slower)
+constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
+void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t tri_test = base.tri_test;
+    uint32_t rb = 0, re = 0;  // the wave's range of slots not yet handed out
+    bool exhausted = false;
+    bool active = false;      // the lane walks the ray of `slot`
+    uint32_t slot = 0, rays = 0;
+    // the walk (traverse4's state)
+    local_ray lr;
+    wt_setup wt;
+    trav_state st;
+    f3 o_w = f3{0.0f, 0.0f, 0.0f}, inv = o_w, oi = o_w;
+    uint32_t stack[RBW_STACK];
+    int sp = 0;
+    int32_t cur = 0, leaf = 0;
+    bool done = true, has_leaf = false;
+    uint32_t* ring = rbw_lds_stack + threadIdx.x;
+    auto push = [&](uint32_t v) {
+        uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * 64u;
+        if (sp >= RBW_LDS_STACK) stack[sp - RBW_LDS_STACK] = *e;
+        *e = v;
+        sp++;
+    };
+    auto pop = [&]() -> uint32_t {
+        sp--;
+        uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * 64u;
+        const uint32_t v = *e;
+        if (sp >= RBW_LDS_STACK) *e = stack[sp - RBW_LDS_STACK];
+        return v;
+    };
+    auto advance = [&]() {
+        if (sp == 0) done = true;
+        else cur = (int32_t)pop();
+    };
+    trace_counters cnt{0, 0};
+    for (;;) {
+        // idle lanes take the next slots; a slot without a live path leaves its lane idle
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && (idle == ~0ull || (uint32_t)__popcll(idle) >= k_refill)) {
+            while (idle && !exhausted) {
+                if (rb == re) {
+                    uint32_t g = 0;
+                    if (lane == 0) g = atomicAdd(B.fetch, k_fetch_chunk);
+                    g = (uint32_t)__shfl((int)g, 0);
+                    if (g >= B.pool) {
+                        exhausted = true;
+                        break;
+                    }
+                    rb = g;
+                    re = g + k_fetch_chunk < B.pool ? g + k_fetch_chunk : B.pool;
+                }
+                const uint32_t take = min((uint32_t)__popcll(idle), re - rb);
+                const uint32_t rank = (uint32_t)__popcll(idle & below);
+                if (!active && rank < take) {
+                    slot = rb + rank;
+                    if (P.U(U_STATE, slot) & ST_ALIVE) {
+                        const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
+                        lr = make_local_ray(o, d);
+                        if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+                        o_w = o;
+                        inv = lr.inv;
+                        oi = f3{lr.o.x * inv.x, lr.o.y * inv.y, lr.o.z * inv.z};
+                        st.best.prim = RBW_MISS;
+                        st.best.t = 0.0f;
+                        st.best.b1 = st.best.b2 = 0.0f;
+                        st.best_inst_entity = RBW_NONE;
+                        st.len_w = __builtin_inff();
+                        st.thr_local = st.len_w * lr.L0;
+                        cur = (int32_t)B.jobs[P.U(U_JOB, slot)].root4;
+                        sp = 0;
+                        done = false;
+                        has_leaf = false;
+                        active = true;
+                        rays++;
+                    }
+                }
+                rb += take;
+                idle = __ballot(!active);
+            }
+        }
+        if (__ballot(active) == 0) break;  // exhausted, every walk finished
+        // walk inner nodes until every walking lane holds a leaf or is done (traverse4)
+        for (;;) {
+            if (active && !done && !has_leaf && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
+            }
+            const bool walk = active && !done && cur >= 0;
+            if (__ballot(walk) == 0) break;
+            if (walk && !visit4(S, cur, inv, oi, st.thr_local * 1.0000002f, push)) advance();
+            if (__ballot(active && !(has_leaf || done)) == 0) break;
+        }
+        while (has_leaf) {
+            has_leaf = false;
+            const uint32_t first = (uint32_t)leaf & 0x07FFFFFFu;
+            const uint32_t count = (((uint32_t)leaf >> 27) & 15u) + 1u;
+            test_leaf<false, false, false>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt, tri_test,
+                                           RBW_NONE, cnt, st);
+            if (!done && cur < 0) {
+                if (cur != k_empty_child) {
+                    leaf = cur;
+                    has_leaf = true;
+                }
+                advance();
+            }
+        }
+        if (active && done) {
+            P.U(U_HPRIM, slot) = st.best.prim;
+            P.F(F_HT, slot) = st.best.t;
+            active = false;
+        }
+    }
+    stripe_add64(B.counters, 0, rays);
 }
 
 template <int W>
@@ -502,7 +633,7 @@ struct rbw_batch_state {
     dbuf pf, pu, prng;
     uint32_t pool = 0;
     // per batch
-    dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, films, rgb;
+    dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, fetch, films, rgb;
     dctl* host_ctl = nullptr;  // pinned
     // guided jobs: per-slot guided vertex data and training records, per-job lists and rows
     dbuf g_surf, g_mat, g_query, g_mix, g_rec, g_flag, g_rows, g_count, t_in, t_dir, t_c, t_count;
@@ -637,6 +768,24 @@ struct run_result {
 };
 
 // one batch through the bounce loop; films must be cleared by the caller
+// blocks of a persistent kernel: as many as the device holds at once (cached per kernel)
+static uint32_t persistent_blocks(const void* kernel, uint32_t block, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, uint32_t> cache;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(dev, kernel);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t n = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    cache[key] = n;
+    return n;
+}
+
 void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, const batch_config& cfg, run_result& rr,
                bool want_job_times, kernel_timer* timer = nullptr) {
     rbw_batch_state& b = *ctx->batch;
@@ -662,6 +811,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     b.first_iter.ensure(size_t(nj) * 4);
     b.last_iter.ensure(size_t(nj) * 4);
     b.counters.ensure(RBW_STRIPES * RBW_STRIPE_WORDS * 8);
+    b.fetch.ensure(256);
     if (!b.host_ctl) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.host_ctl), sizeof(dctl), hipHostMallocDefault));
 
     dbatch B;
@@ -678,6 +828,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     B.first_iter = b.first_iter.as<uint32_t>();
     B.last_iter = b.last_iter.as<uint32_t>();
     B.counters = b.counters.as<unsigned long long>();
+    B.fetch = b.fetch.as<uint32_t>();
     dpaths P;
     P.f = b.pf.as<float>();
     P.u = b.pu.as<uint32_t>();
@@ -774,13 +925,21 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                        : (fast ? k_b_connect<2, true> : k_b_connect<2, false>);
     void (*extend_kernel)(dscene, dpaths, dparams, dbatch) =
         S.width == 8 ? k_b_extend<8> : (S.width == 4 ? k_b_extend<4> : k_b_extend<2>);
+    // the four-wide walks run persistent (debug-hit renders keep one slot per lane)
+    const bool dyn = S.width == 4 && base.debug == nullptr;
+    const uint32_t dyn_grid = dyn ? persistent_blocks(reinterpret_cast<const void*>(&k_b_extend_dyn), 64,
+                                                      lds_stack_bytes(64))
+                                  : 0u;
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
         hipLaunchKernelGGL(k_b_regen, dim3(ngroups), dim3(k_group), 0, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_EXTEND, s);
-        hipLaunchKernelGGL(extend_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
+        if (dyn)
+            hipLaunchKernelGGL(k_b_extend_dyn, dim3(dyn_grid), dim3(64), lds_stack_bytes(64), s, S, P, base, B);
+        else
+            hipLaunchKernelGGL(extend_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_SHADE, s);
         if (guided) {

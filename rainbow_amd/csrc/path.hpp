@@ -404,6 +404,54 @@ __device__ __forceinline__ hitrec traverse8(const dscene& S, f3 o_w, f3 d_w, flo
     return st.best;
 }
 
+// One four-wide node visit of a walk at node `cur`: the child boxes' slab tests against
+// [0, tmax], the hit children sorted by entry distance (a sorting network; misses last),
+// the nearest made `cur` and the others pushed far to near.  Returns false when no child
+// is hit (the caller pops).
+template <class PUSH>
+__device__ __forceinline__ bool visit4(const dscene& S, int32_t& cur, f3 inv, f3 oi, float tmax, PUSH&& push) {
+    const float4* n = S.nodes4 + 8 * (size_t)cur;
+    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+    const int4 ch = *reinterpret_cast<const int4*>(n + 6);
+    float d[4];
+    int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
+    const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
+    uint32_t nhit = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float x0 = __builtin_fmaf(blx[c], inv.x, -oi.x), x1 = __builtin_fmaf(bhx[c], inv.x, -oi.x);
+        const float y0 = __builtin_fmaf(bly[c], inv.y, -oi.y), y1 = __builtin_fmaf(bhy[c], inv.y, -oi.y);
+        const float z0 = __builtin_fmaf(blz[c], inv.z, -oi.z), z1 = __builtin_fmaf(bhz[c], inv.z, -oi.z);
+        const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+        const bool hit = r[c] != k_empty_child && tmin <= tfar * 1.0000004f;
+        d[c] = hit ? tmin : __builtin_inff();
+        nhit += hit ? 1u : 0u;
+    }
+    auto cswap = [&](int i, int j) {
+        const bool sw = d[j] < d[i];
+        const float di = d[i], dj = d[j];
+        const int32_t ri = r[i], rj = r[j];
+        d[i] = sw ? dj : di;
+        d[j] = sw ? di : dj;
+        r[i] = sw ? rj : ri;
+        r[j] = sw ? ri : rj;
+    };
+    cswap(0, 1);
+    cswap(2, 3);
+    cswap(0, 2);
+    cswap(1, 3);
+    cswap(1, 2);
+    if (nhit == 0) return false;
+    cur = r[0];
+    if (nhit > 3) push((uint32_t)r[3]);
+    if (nhit > 2) push((uint32_t)r[2]);
+    if (nhit > 1) push((uint32_t)r[1]);
+    return true;
+}
+
 // closest hit (or closest visible hit) through the four-wide BVH: the walk of traverse2
 // with four child boxes per node (one 128-byte line), hit children visited nearest
 // first and the others pushed far to near; postponed leaves as there
@@ -466,51 +514,8 @@ __device__ __forceinline__ hitrec traverse4(const dscene& S, f3 o_w, f3 d_w, flo
             const bool walk = !done && cur >= 0;
             if (__ballot(walk) == 0) break;
             if (walk) {
-                const float4* n = S.nodes4 + 8 * (size_t)cur;
-                const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-                const int4 ch = *reinterpret_cast<const int4*>(n + 6);
                 if (COUNT) cnt.nodes++;
-                const float tmax = st.thr_local * 1.0000002f;
-                float d[4];
-                int32_t r[4] = {ch.x, ch.y, ch.z, ch.w};
-                const float blx[4] = {lx.x, lx.y, lx.z, lx.w}, bhx[4] = {hx.x, hx.y, hx.z, hx.w};
-                const float bly[4] = {ly.x, ly.y, ly.z, ly.w}, bhy[4] = {hy.x, hy.y, hy.z, hy.w};
-                const float blz[4] = {lz.x, lz.y, lz.z, lz.w}, bhz[4] = {hz.x, hz.y, hz.z, hz.w};
-                uint32_t nhit = 0;
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const float x0 = __builtin_fmaf(blx[c], inv.x, -oi.x), x1 = __builtin_fmaf(bhx[c], inv.x, -oi.x);
-                    const float y0 = __builtin_fmaf(bly[c], inv.y, -oi.y), y1 = __builtin_fmaf(bhy[c], inv.y, -oi.y);
-                    const float z0 = __builtin_fmaf(blz[c], inv.z, -oi.z), z1 = __builtin_fmaf(bhz[c], inv.z, -oi.z);
-                    const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-                    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-                    const bool hit = r[c] != k_empty_child && tmin <= tfar * 1.0000004f;
-                    d[c] = hit ? tmin : __builtin_inff();
-                    nhit += hit ? 1u : 0u;
-                }
-                // sorting network on (distance, child): misses (infinite) go last
-                auto cswap = [&](int i, int j) {
-                    const bool sw = d[j] < d[i];
-                    const float di = d[i], dj = d[j];
-                    const int32_t ri = r[i], rj = r[j];
-                    d[i] = sw ? dj : di;
-                    d[j] = sw ? di : dj;
-                    r[i] = sw ? rj : ri;
-                    r[j] = sw ? ri : rj;
-                };
-                cswap(0, 1);
-                cswap(2, 3);
-                cswap(0, 2);
-                cswap(1, 3);
-                cswap(1, 2);
-                if (nhit == 0) {
-                    advance();
-                } else {
-                    cur = r[0];
-                    if (nhit > 3) push((uint32_t)r[3]);
-                    if (nhit > 2) push((uint32_t)r[2]);
-                    if (nhit > 1) push((uint32_t)r[1]);
-                }
+                if (!visit4(S, cur, inv, oi, st.thr_local * 1.0000002f, push)) advance();
             }
             if (__ballot(!(has_leaf || done)) == 0) break;
         }
