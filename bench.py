@@ -565,6 +565,17 @@ PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                        "tracer_persistent_bvh4_icosphere_1024_64spp.md")
 
 
+def profiled_kernel(stage, width):
+    """the profile's name of the persistent-slot kernel of a stage (extend, shade, connect)
+    at a BVH width: the four-wide extension walk runs persistent, connect is the
+    occlusion-only instantiation on the headline scene (one planar light)"""
+    if stage == "extend":
+        return "k_b_extend_dyn" if width == 4 else f"k_b_extend<{width}>"
+    if stage == "connect":
+        return f"k_b_connect<{width}, true>"
+    return "k_b_shade"
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` on the icosphere workload, from the committed
     rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
@@ -690,7 +701,7 @@ def main():
         achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(f"k_b_{dom}<{width}>") if args.workload == "icosphere" else None,
+                "traffic": profiled_traffic(profiled_kernel(dom, width)) if args.workload == "icosphere" else None,
                 "bvh_width": width,
                 "algorithmic_bytes_per_launch": int(kb[dom] / max(1, launches)),
                 "avg_launch_ms": kms[dom] / max(1, launches),
