@@ -814,7 +814,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     b.fetch.ensure(256);
     if (!b.host_ctl) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.host_ctl), sizeof(dctl), hipHostMallocDefault));
 
-    dbatch B;
+    dbatch B{};
     B.ctl = b.ctl.as<dctl>();
     B.jobs = b.jobs.as<djob>();
     B.njobs = nj;
@@ -925,11 +925,14 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                        : (fast ? k_b_connect<2, true> : k_b_connect<2, false>);
     void (*extend_kernel)(dscene, dpaths, dparams, dbatch) =
         S.width == 8 ? k_b_extend<8> : (S.width == 4 ? k_b_extend<4> : k_b_extend<2>);
-    // the four-wide walks run persistent (debug-hit renders keep one slot per lane)
-    const bool dyn = S.width == 4 && base.debug == nullptr;
-    const uint32_t dyn_grid = dyn ? persistent_blocks(reinterpret_cast<const void*>(&k_b_extend_dyn), 64,
+    // the four-wide extension walk runs persistent when the pool gives every resident wave a
+    // range or more (debug-hit renders keep one slot per lane); on the 64-job workload's small
+    // jobs, rendered one at a time, the persistent walk took 0.27 ms per launch against 0.14
+    const uint32_t dyn_grid = S.width == 4 && base.debug == nullptr
+                                  ? persistent_blocks(reinterpret_cast<const void*>(&k_b_extend_dyn), 64,
                                                       lds_stack_bytes(64))
                                   : 0u;
+    const bool dyn = dyn_grid && cfg.pool >= k_fetch_chunk * dyn_grid;
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);

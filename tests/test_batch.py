@@ -73,9 +73,11 @@ def test_batch_matches_single_renders(tracer, small_batch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("opts", [dict(use_graph=False), dict(pool_size=1024, block_size=64),
-                                  dict(pool_size=1 << 12, block_size=512)])
+                                  dict(pool_size=1 << 12, block_size=512), dict(pool_size=3001, block_size=128)])
 def test_batch_independent_of_schedule(tracer, small_batch, opts):
-    """the path-pool size, block size and graph capture change the schedule, not the images"""
+    """the path-pool size, block size and graph capture change the schedule, not the images
+    (3001 slots: the persistent extension walk's last slot range and the last group are
+    partial)"""
     jobs, ids = small_batch
     ref, _, _ = _submit(tracer, jobs, ids)
     imgs, _, bstats = _submit(tracer, jobs, ids, **opts)
@@ -163,3 +165,19 @@ def test_batch_64_jobs_match_single_renders(tracer):
         for sid in ids:
             tracer.remove_scene(sid)
     print(f"64 jobs checked in {time.perf_counter() - t0:.1f} s", flush=True)
+
+
+@pytest.mark.gpu
+def test_persistent_extension_walk_matches_one_slot_per_lane(tracer):
+    """A render whose pool gives every resident wave a slot range (1024x1024 x 4 spp: 4.2 M
+    slots) runs the persistent four-wide extension walk that refills idle lanes; the same
+    render with visit counting runs the pass schedule, one ray per lane.  Each ray's walk is
+    the same, so the images are equal bit for bit."""
+    from rainbow_amd.tracer import render_params
+    sc, cam = scenes.displaced_icosphere_box(level=6, width=1024, height=1024)
+    tracer.upload(sc)
+    a = tracer.render(cam, render_params(spp=4, max_depth=8, seed=21))
+    b = tracer.render(cam, render_params(spp=4, max_depth=8, seed=21, count_visits=1))
+    assert a["stats"]["bvh_width"] == 4 and b["stats"]["node_visits"] > 0
+    assert np.isfinite(a["image"]).all() and a["image"].mean() > 0
+    assert np.array_equal(a["image"], b["image"])
