@@ -122,3 +122,43 @@ def test_gpu_guided_batch_needs_a_network_per_job(tracer, rooms):
     tracer.guide_jobs_reset(2, seed=1)
     with pytest.raises(RbwError):
         tracer.submit_jobs(_jobs(rs, ids, 1, 1), batch_opts(guiding=guiding_opts()))
+
+
+def test_gpu_guided_rooms_at_the_configured_workload(tracer):
+    """The learned-guiding-plus-batching workload at its configured size: 16 narrow-aperture
+    room variants (gap 0.1-0.5, 1e5-1.3e6 triangles, GGX alpha 0.02-0.6, 256^2-1024^2) in
+    one submission, one network per job trained online by the batched kernel.  Every job's
+    samples are accounted for, the networks take Adam steps, and every job's guided image
+    mean (8 submissions of 8 spp while training) equals its plain 512-spp judge's within
+    four standard errors."""
+    from rainbow_amd.tracer import batch_opts, guiding_opts
+    rs = scenes.room_variants(n=16, res=(256, 1024))
+    ids = [tracer.add_scene(r["scene"]) for r in rs]
+    try:
+        def jobs(spp, seed0):
+            return [{"scene": ids[i], "camera": r["camera"], "spp": spp, "max_depth": 8, "seed": seed0 + 131 * i}
+                    for i, r in enumerate(rs)]
+        judge, _, _ = tracer.submit_jobs(jobs(512, 1 << 20), batch_opts())
+        tracer.guide_jobs_reset(len(rs), seed=7)
+        chunks, steps = [], 0
+        for k in range(8):
+            go = guiding_opts(train=True, steps_per_pass=1, batch=1 << 16)
+            imgs, jst, st = tracer.submit_jobs(jobs(8, 100000 + 1000 * k), batch_opts(guiding=go))
+            assert st["samples"] == sum(r["res"] ** 2 * 8 for r in rs)
+            for r, js in zip(rs, jst):
+                assert js["samples"] == r["res"] ** 2 * 8
+            steps += st["guide_steps"]
+            chunks.append(imgs)
+        print("guided rooms: Adam steps", steps, "resolutions", [r["res"] for r in rs])
+        assert steps > 0
+        for i in range(len(rs)):
+            means = np.array([float(c[i].mean()) for c in chunks])
+            m, se = means.mean(), means.std(ddof=1) / np.sqrt(len(means))
+            mr = float(judge[i].mean())
+            assert np.isfinite(chunks[-1][i]).all()
+            print(f"room {i} (gap {rs[i]['gap']:.2f}, {rs[i]['triangles']} tris, {rs[i]['res']}^2): "
+                  f"guided {m:.5f} +- {se:.1e}, plain 512 spp {mr:.5f}")
+            assert abs(m - mr) < 4 * se * np.sqrt(1 + 64 / 512) + 1e-6, i
+    finally:
+        for sid in ids:
+            tracer.remove_scene(sid)
