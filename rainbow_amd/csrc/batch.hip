@@ -95,6 +95,11 @@ struct dbatch {
     uint32_t t_cap;
 };
 
+// guided jobs: floats of a guided vertex's surface record (by slot): point, normal, wo, the
+// shading frame, the material's bits, one pad
+constexpr uint32_t G_REC = 20;
+static_assert(G_COUNT + 2 == G_REC, "record layout");
+
 // slot state (U_STATE): 0 = free
 constexpr uint32_t ST_ALIVE = 1u;    // trace an extension ray this iteration
 constexpr uint32_t ST_CONNECT = 2u;  // next-event estimation rays pending
@@ -515,13 +520,14 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
         shade_path<true>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
         if (to_guide) {
             const dguide& G = base.g;
-            const size_t cap = G.cap;
-            const float v[G_COUNT] = {gsi.point.x, gsi.point.y, gsi.point.z, gsi.normal.x, gsi.normal.y, gsi.normal.z,
-                                      gsi.wo.x, gsi.wo.y, gsi.wo.z, gsi.sh.x.x, gsi.sh.x.y, gsi.sh.x.z,
-                                      gsi.sh.y.x, gsi.sh.y.y, gsi.sh.y.z, gsi.sh.z.x, gsi.sh.z.y, gsi.sh.z.z};
-#pragma unroll
-            for (int k = 0; k < G_COUNT; k++) G.surf[k * cap + slot] = v[k];
-            G.surf_mat[slot] = gmat;
+            // the surface record by slot, 80 contiguous bytes (k_b_guided reads the scattered
+            // guided slots back: five 16-byte loads each instead of 19 gathers)
+            float4* r = reinterpret_cast<float4*>(G.surf + (size_t)slot * G_REC);
+            r[0] = make_float4(gsi.point.x, gsi.point.y, gsi.point.z, gsi.normal.x);
+            r[1] = make_float4(gsi.normal.y, gsi.normal.z, gsi.wo.x, gsi.wo.y);
+            r[2] = make_float4(gsi.wo.z, gsi.sh.x.x, gsi.sh.x.y, gsi.sh.x.z);
+            r[3] = make_float4(gsi.sh.y.x, gsi.sh.y.y, gsi.sh.y.z, gsi.sh.z.x);
+            r[4] = make_float4(gsi.sh.z.y, gsi.sh.z.z, __uint_as_float(gmat), 0.0f);
             float q[9];
             guide_query(gsi, J.gmin, J.ginv, q);
 #pragma unroll
@@ -539,22 +545,21 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
 __device__ __forceinline__ void guided_slot(const dscene& S, const dpaths& P, const dparams& base, const dbatch& B,
                                             uint32_t slot) {
     const dguide& G = base.g;
-    const size_t cap = G.cap;
-    float v[G_COUNT];
-#pragma unroll
-    for (int k = 0; k < G_COUNT; k++) v[k] = G.surf[k * cap + slot];
+    const float4* r = reinterpret_cast<const float4*>(G.surf + (size_t)slot * G_REC);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
     surf si;
-    si.point = f3{v[G_PX], v[G_PY], v[G_PZ]};
-    si.normal = f3{v[G_NX], v[G_NY], v[G_NZ]};
-    si.wo = f3{v[G_WX], v[G_WY], v[G_WZ]};
-    si.sh.x = f3{v[G_XX], v[G_XY], v[G_XZ]};
-    si.sh.y = f3{v[G_YX], v[G_YY], v[G_YZ]};
-    si.sh.z = f3{v[G_ZX], v[G_ZY], v[G_ZZ]};
+    si.point = f3{r0.x, r0.y, r0.z};
+    si.normal = f3{r0.w, r1.x, r1.y};
+    si.wo = f3{r1.z, r1.w, r2.x};
+    si.sh.x = f3{r2.y, r2.z, r2.w};
+    si.sh.y = f3{r3.x, r3.y, r3.z};
+    si.sh.z = f3{r3.w, r4.x, r4.y};
+    const uint32_t mat = __float_as_uint(r4.z);
     dscene Sj;
     dparams pj;
     job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
     bool alive = false, need_connect = false;
-    guided_vertex(Sj, P, pj, slot, si, G.surf_mat[slot], G.mix + (size_t)slot * guide::N_OUT_PAD,
+    guided_vertex(Sj, P, pj, slot, si, mat, G.mix + (size_t)slot * guide::N_OUT_PAD,
                   G.query + (size_t)slot * 9, 0u, alive, need_connect);
     P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
 }
@@ -636,7 +641,7 @@ struct rbw_batch_state {
     dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, fetch, films, rgb;
     dctl* host_ctl = nullptr;  // pinned
     // guided jobs: per-slot guided vertex data and training records, per-job lists and rows
-    dbuf g_surf, g_mat, g_query, g_mix, g_rec, g_flag, g_rows, g_count, t_in, t_dir, t_c, t_count;
+    dbuf g_surf, g_query, g_mix, g_rec, g_flag, g_rows, g_count, t_in, t_dir, t_c, t_count;
     uint32_t g_pool = 0, g_depth = 0, g_jobs = 0, g_tcap = 0;
     // captured bounce loop
     hipGraph_t graph = nullptr;
@@ -863,11 +868,10 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         if (b.g_pool != pool || b.g_depth < max_depth || b.g_jobs < nj || b.g_tcap < tcap) {
             const uint32_t depth = std::max(max_depth, b.g_depth);
             const uint32_t jobs = std::max(nj, b.g_jobs);
-            b.g_surf.ensure(size_t(G_COUNT) * pool * 4);
-            b.g_mat.ensure(size_t(pool) * 4);
+            b.g_surf.ensure(size_t(G_REC) * pool * 4);
             b.g_query.ensure(size_t(pool) * 9 * 4);
             b.g_mix.ensure(size_t(pool) * guide::N_OUT_PAD * 4);
-            b.g_rec.ensure(size_t(R_COUNT) * pool * depth * 4);
+            b.g_rec.ensure(size_t(R_STRIDE) * pool * depth * 4);
             b.g_flag.ensure(size_t(pool) * depth * 4);
             b.g_rows.ensure(size_t(jobs) * pool * 4);
             b.g_count.ensure(size_t(jobs) * 4);
@@ -890,7 +894,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         G.depth = go.guide_depth ? go.guide_depth : 0xFFFFFFFFu;
         G.product = go.product_targets ? 1u : 0u;
         G.surf = b.g_surf.as<float>();
-        G.surf_mat = b.g_mat.as<uint32_t>();
+        G.surf_mat = nullptr;  // the material is in the slot's surface record
         G.query = b.g_query.as<float>();
         G.mix = b.g_mix.as<float>();
         G.rec = b.g_rec.as<float>();

@@ -13,7 +13,11 @@
 
 namespace rbw {
 
-// the vMF-mixture head of the guiding MLP (same parameterisation as the training loss)
+// the vMF-mixture head of the guiding MLP (same parameterisation as the training loss),
+// evaluated with the hardware exponential, logarithm, sine and cosine (v_exp_f32,
+// v_log_f32, v_sin_f32, v_cos_f32; a few ulp): the sampler and the pdf share the lobes'
+// parameters, and the estimator divides by the density it evaluates, so the rounding
+// changes the variance a little, not the mean
 struct vmf_mixture {
     f3 mu[guide::N_LOBES];
     float kappa[guide::N_LOBES], w[guide::N_LOBES], norm[guide::N_LOBES];
@@ -22,22 +26,23 @@ struct vmf_mixture {
         for (int j = 0; j < guide::N_LOBES; j++) amax = fmaxf(amax, o[5 * j + 4]);
         float ws = 0.0f;
         for (int j = 0; j < guide::N_LOBES; j++) {
-            w[j] = expf(o[5 * j + 4] - amax);
+            w[j] = __expf(o[5 * j + 4] - amax);
             ws += w[j];
             const f3 m = f3{o[5 * j], o[5 * j + 1], o[5 * j + 2]};
             const float nm = fmaxf(sqrtf(m.x * m.x + m.y * m.y + m.z * m.z), 1e-12f);
             mu[j] = f3{m.x / nm, m.y / nm, m.z / nm};
             const float x = o[5 * j + 3];
-            const float sp = x > 20.0f ? x : log1pf(expf(x));
+            const float sp = x > 20.0f ? x : __logf(1.0f + __expf(x));
             kappa[j] = fminf(sp + guide::KAPPA_MIN, guide::KAPPA_MAX);
-            norm[j] = kappa[j] / (2.0f * 3.14159265f * (-expm1f(-2.0f * kappa[j])));
+            // 1 - e^(-2 kappa) in FP32 keeps ~5e-6 relative accuracy down to kappa = KAPPA_MIN
+            norm[j] = kappa[j] / (2.0f * 3.14159265f * (1.0f - __expf(-2.0f * kappa[j])));
         }
         for (int j = 0; j < guide::N_LOBES; j++) w[j] /= ws;
     }
     __device__ float pdf(f3 d) const {
         float q = 0.0f;
         for (int j = 0; j < guide::N_LOBES; j++)
-            q += w[j] * norm[j] * expf(kappa[j] * ((mu[j].x * d.x + mu[j].y * d.y + mu[j].z * d.z) - 1.0f));
+            q += w[j] * norm[j] * __expf(kappa[j] * ((mu[j].x * d.x + mu[j].y * d.y + mu[j].z * d.z) - 1.0f));
         return q;
     }
     // pick a lobe with u.x, then Jakob's vMF sampler around its mean
@@ -50,11 +55,11 @@ struct vmf_mixture {
         }
         const float ux = fminf(fmaxf((u.x - acc) / fmaxf(w[j], 1e-20f), 0.0f), 0.99999994f);
         const float k = kappa[j];
-        const float W = 1.0f + logf(ux + (1.0f - ux) * expf(-2.0f * k)) / k;
+        const float W = 1.0f + __logf(ux + (1.0f - ux) * __expf(-2.0f * k)) / k;
         const float sr = sqrtf(fmaxf(0.0f, 1.0f - W * W));
         const float phi = 2.0f * 3.14159265f * u.y;
         const frame fr = frame_from_normal(mu[j]);
-        return local_to_world(fr, f3{sr * cosf(phi), sr * sinf(phi), W});
+        return local_to_world(fr, f3{sr * __cosf(phi), sr * __sinf(phi), W});
     }
 };
 
@@ -192,15 +197,11 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
     alive = cont;
     if (alive && G.train) {
         const size_t ri = (size_t)slot * G.rec_depth + (uint32_t)b0;
-        const size_t rc = (size_t)P.cap * G.rec_depth;
-        for (int k = 0; k < 9; k++) G.rec[(R_Q0 + k) * rc + ri] = qrow[k];
-        G.rec[R_DX * rc + ri] = wi.x;
-        G.rec[R_DY * rc + ri] = wi.y;
-        G.rec[R_DZ * rc + ri] = wi.z;
-        G.rec[R_PDF * rc + ri] = rec_pdf;
-        G.rec[R_BX * rc + ri] = beta.x;
-        G.rec[R_BY * rc + ri] = beta.y;
-        G.rec[R_BZ * rc + ri] = beta.z;
+        float4* r = reinterpret_cast<float4*>(G.rec + ri * R_STRIDE);
+        r[0] = make_float4(qrow[0], qrow[1], qrow[2], qrow[3]);
+        r[1] = make_float4(qrow[4], qrow[5], qrow[6], qrow[7]);
+        r[2] = make_float4(qrow[8], wi.x, wi.y, wi.z);
+        r[3] = make_float4(rec_pdf, beta.x, beta.y, beta.z);
         G.rec_flag[ri] = 1;
         P.U(U_PENDING, slot) = (uint32_t)b0;
     }
@@ -214,15 +215,16 @@ __device__ __forceinline__ void guided_vertex(const dscene& S, const dpaths& P, 
 // brought back (final value minus the value at the next vertex's start, over the
 // throughput after the vertex) over the continuation's pdf; 0 when not usable.
 __device__ __forceinline__ float record_target(const dpaths& P, const dguide& G, uint32_t slot, size_t ri) {
-    const size_t rc = (size_t)P.cap * G.rec_depth;
     const f3 vf = P.F3(F_VX, slot);
-    const f3 m = f3{G.rec[R_MX * rc + ri], G.rec[R_MY * rc + ri], G.rec[R_MZ * rc + ri]};
-    const f3 b = f3{G.rec[R_BX * rc + ri], G.rec[R_BY * rc + ri], G.rec[R_BZ * rc + ri]};
+    const float4 pb = *reinterpret_cast<const float4*>(G.rec + ri * R_STRIDE + R_PDF);
+    const float4 mm = *reinterpret_cast<const float4*>(G.rec + ri * R_STRIDE + R_MX);
+    const f3 m = f3{mm.x, mm.y, mm.z};
+    const f3 b = f3{pb.y, pb.z, pb.w};
     const f3 d = vf - m;
     const float Lx = b.x > 0.0f ? d.x / b.x : 0.0f, Ly = b.y > 0.0f ? d.y / b.y : 0.0f,
                 Lz = b.z > 0.0f ? d.z / b.z : 0.0f;
     const float lum = 0.2126f * Lx + 0.7152f * Ly + 0.0722f * Lz;
-    const float pdf = G.rec[R_PDF * rc + ri];
+    const float pdf = pb.x;
     const float c = pdf > 0.0f ? lum / pdf : 0.0f;
     return c > 0.0f && isfinite(c) ? c : 0.0f;
 }
@@ -230,11 +232,13 @@ __device__ __forceinline__ float record_target(const dpaths& P, const dguide& G,
 // copies a record's query and direction with its weight c into training row k
 __device__ __forceinline__ void record_to_row(const dpaths& P, const dguide& G, size_t ri, float c, float* t_in,
                                               float* t_dir, float* t_c, size_t k) {
-    const size_t rc = (size_t)P.cap * G.rec_depth;
-    for (int q = 0; q < 9; q++) t_in[k * 9 + q] = G.rec[(R_Q0 + q) * rc + ri];
-    t_dir[k * 3 + 0] = G.rec[R_DX * rc + ri];
-    t_dir[k * 3 + 1] = G.rec[R_DY * rc + ri];
-    t_dir[k * 3 + 2] = G.rec[R_DZ * rc + ri];
+    const float4* r = reinterpret_cast<const float4*>(G.rec + ri * R_STRIDE);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    const float q[9] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x};
+    for (int j = 0; j < 9; j++) t_in[k * 9 + j] = q[j];
+    t_dir[k * 3 + 0] = r2.y;
+    t_dir[k * 3 + 1] = r2.z;
+    t_dir[k * 3 + 2] = r2.w;
     t_c[k] = c;
 }
 

@@ -684,6 +684,9 @@ struct dcamera {
 enum gsurf : uint32_t { G_PX, G_PY, G_PZ, G_NX, G_NY, G_NZ, G_WX, G_WY, G_WZ, G_XX, G_XY, G_XZ, G_YX, G_YY, G_YZ,
                         G_ZX, G_ZY, G_ZZ, G_COUNT };
 enum grec : uint32_t { R_Q0 = 0, R_DX = 9, R_DY, R_DZ, R_PDF, R_BX, R_BY, R_BZ, R_MX, R_MY, R_MZ, R_COUNT };
+// a training record's floats (R_COUNT and one pad: five 16-byte words, R_PDF and R_MX on word boundaries)
+constexpr uint32_t R_STRIDE = 20;
+static_assert(R_COUNT + 1 == R_STRIDE && R_PDF % 4 == 0 && R_MX % 4 == 0, "record words");
 
 struct dguide {
     uint32_t enable;
@@ -696,7 +699,7 @@ struct dguide {
     uint32_t* gq_count;
     float* query;                // 9 per row
     float* mix;                  // 48 per row (head outputs)
-    float* rec;                  // R_COUNT * (cap * rec_depth) (SoA)
+    float* rec;                  // R_STRIDE floats per record, record slot * rec_depth + bounce
     uint32_t* rec_flag;          // 0 none, 1 sampled, 2 closed
     uint32_t rec_depth;          // records per slot (at least every path's max depth)
     uint32_t cap;
@@ -984,10 +987,7 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
         const uint32_t pend = P.U(U_PENDING, slot);
         if (pend != RBW_NONE) {
             const size_t ri = (size_t)slot * prm.g.rec_depth + pend;
-            const size_t rc = (size_t)P.cap * prm.g.rec_depth;
-            prm.g.rec[R_MX * rc + ri] = value.x;
-            prm.g.rec[R_MY * rc + ri] = value.y;
-            prm.g.rec[R_MZ * rc + ri] = value.z;
+            *reinterpret_cast<float4*>(prm.g.rec + ri * R_STRIDE + R_MX) = make_float4(value.x, value.y, value.z, 0.0f);
             prm.g.rec_flag[ri] = 2;
             P.U(U_PENDING, slot) = RBW_NONE;
         }

@@ -794,7 +794,7 @@ void render(rbw_ctx* ctx, const rbw_camera_desc* cam, const rbw_render_params* p
         if (go->train) {
             const size_t nrec = size_t(ctx->path_cap) * pr.max_depth;
             if (ctx->g_rec_cap < nrec) {
-                ctx->g_rec.ensure(size_t(R_COUNT) * nrec * sizeof(float));
+                ctx->g_rec.ensure(size_t(R_STRIDE) * nrec * sizeof(float));
                 ctx->g_flag.ensure(nrec * 4);
                 ctx->g_rec_cap = uint32_t(nrec);
             }

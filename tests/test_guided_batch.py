@@ -6,7 +6,7 @@ launch over all networks, and all networks trained by one batched training kerne
   oracle (oracle/guiding_ref.py), networks with different parameters and ragged row counts;
 * per job, the guided batch's image mean equals the plain estimator's (unbiased) within
   stated standard errors, while the networks train online;
-* after online training, the guided batch's relMSE at equal spp is below the plain
+* after online training, the guided batch's relMSE at equal spp is below 0.9 of the plain
   batch's over the jobs.
 """
 import ctypes as C
@@ -96,24 +96,37 @@ def test_gpu_guided_batch_unbiased_per_job(tracer, rooms):
         assert abs(m - mr) < 4 * se * np.sqrt(1 + 512 / 4096) + 1e-6, i
 
 
-def test_gpu_guided_batch_reduces_error(tracer, rooms):
-    """after 24 training submissions (4 Adam steps of up to 2^16 rows per network after every
-    8 bounce iterations), the guided batch has lower relMSE than the plain batch at equal spp,
-    summed over the jobs; every job's image mean matches the judge's"""
+def test_gpu_guided_batch_reduces_error(tracer):
+    """Four narrow-aperture rooms whose beam is narrow (the light 2 m behind gaps of 0.2-0.35,
+    level-4 mesh, 64x64) in one batch: after 32 training submissions (up to 2^16 rows per
+    network after every 8 bounce iterations) the guided batch's relMSE at equal spp, summed
+    over the jobs and averaged over two independent 512-spp renders, is below 0.9 of the plain
+    batch's; every job's image mean matches the judge's"""
     from rainbow_amd.tracer import batch_opts, guiding_opts
-    rs, ids = rooms
-    judge, _, _ = tracer.submit_jobs(_jobs(rs, ids, 8192, 7), batch_opts())
-    plain, _, _ = tracer.submit_jobs(_jobs(rs, ids, 256, 11), batch_opts())
-    tracer.guide_jobs_reset(len(rs), seed=9)
-    for k in range(24):
-        tracer.submit_jobs(_jobs(rs, ids, 16, 2000 + k), batch_opts(guiding=guiding_opts(train=True, batch=1 << 16)))
-    guided, _, _ = tracer.submit_jobs(_jobs(rs, ids, 256, 13), batch_opts(guiding=guiding_opts(train=False)))
-    ep = sum(scenes.rel_mse(p, j) for p, j in zip(plain, judge))
-    eg = sum(scenes.rel_mse(g, j) for g, j in zip(guided, judge))
-    print("relMSE over jobs: plain", ep, "guided", eg, "ratio", eg / ep)
-    assert eg < ep
-    for g, j in zip(guided, judge):
-        assert abs(float(g.mean()) - float(j.mean())) < 0.02 * float(j.mean())
+    rs = [scenes.narrow_aperture_room(level=4, width=64, height=64, gap=g, light_tilt=45.0, light_dist=2.0, seed=3 + k)
+          for k, g in enumerate((0.2, 0.25, 0.3, 0.35))]
+    ids = [tracer.add_scene(sc) for sc, _ in rs]
+    try:
+        def jobs(spp, seed0):
+            return [{"scene": ids[i], "camera": cam, "spp": spp, "max_depth": 8, "seed": seed0 + 97 * i}
+                    for i, (_, cam) in enumerate(rs)]
+        judge, _, _ = tracer.submit_jobs(jobs(16384, 7), batch_opts())
+        tracer.guide_jobs_reset(len(rs), seed=9)
+        for k in range(32):
+            tracer.submit_jobs(jobs(16, 2000 + k), batch_opts(guiding=guiding_opts(train=True, batch=1 << 16)))
+        ep = eg = 0.0
+        for rep in range(2):
+            plain, _, _ = tracer.submit_jobs(jobs(512, 11 + rep), batch_opts())
+            guided, _, _ = tracer.submit_jobs(jobs(512, 13 + rep), batch_opts(guiding=guiding_opts(train=False)))
+            ep += sum(scenes.rel_mse(p, j) for p, j in zip(plain, judge))
+            eg += sum(scenes.rel_mse(g, j) for g, j in zip(guided, judge))
+            for g, j in zip(guided, judge):
+                assert abs(float(g.mean()) - float(j.mean())) < 0.02 * float(j.mean())
+        print("relMSE over jobs: plain", ep, "guided", eg, "ratio", eg / ep)
+        assert eg < 0.9 * ep
+    finally:
+        for sid in ids:
+            tracer.remove_scene(sid)
 
 
 def test_gpu_guided_batch_needs_a_network_per_job(tracer, rooms):
