@@ -125,6 +125,32 @@ __device__ __forceinline__ uint32_t append_by_job(bool take, uint32_t job, uint3
     return pos;
 }
 
+// reserves n consecutive entries of its job's list for every lane (n may be 0), one atomic
+// per job and wave; returns the lane's first position.  Call with the whole wave converged.
+__device__ __forceinline__ uint32_t append_by_job_n(uint32_t n, uint32_t job, uint32_t* counts) {
+    uint64_t pending = __ballot(n > 0);
+    uint32_t pos = 0;
+    const int lane = (int)__lane_id();
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t jl = __shfl(job, leader);
+        const bool mine = n > 0 && job == jl;
+        uint32_t x = mine ? n : 0u;  // inclusive scan of the job's counts over the wave
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)x, (unsigned)off);
+            if (lane >= off) x += v;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)x, 63);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counts[jl], total);
+        base = (uint32_t)__shfl((int)base, leader);
+        if (mine) pos = base + x - n;
+        pending &= ~__ballot(mine);
+    }
+    return pos;
+}
+
 __device__ __forceinline__ uint32_t find_job(const djob* jobs, uint32_t n, unsigned long long g) {
     uint32_t lo = 0, hi = n;  // jobs[lo].base <= g < jobs[hi].base
     while (hi - lo > 1) {
@@ -292,17 +318,26 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
         st = 0u;
     }
     if (B.train) {
-        // the finished path's closed training records into its job's training rows
+        // the finished path's closed training records into its job's training rows: count
+        // them, reserve the wave's rows of each job at once, then write them
         const dguide& G = base.g;
-        for (uint32_t b = 0; b < G.rec_depth; b++) {
-            const size_t ri = (size_t)slot * G.rec_depth + b;
-            float c = 0.0f;
-            if (dying && G.rec_flag[ri] == 2) c = record_target(P, G, slot, ri);
-            const bool take = c > 0.0f;
-            const uint32_t k = append_by_job(take, j, B.t_count);
-            if (take && k < B.t_cap) record_to_row(P, G, ri, c, B.t_in, B.t_dir, B.t_c, (size_t)j * B.t_cap + k);
-            if (dying) G.rec_flag[ri] = 0;
-        }
+        uint32_t n = 0;
+        if (dying)
+            for (uint32_t b = 0; b < G.rec_depth; b++) {
+                const size_t ri = (size_t)slot * G.rec_depth + b;
+                if (G.rec_flag[ri] == 2 && record_target(P, G, slot, ri) > 0.0f) n++;
+            }
+        uint32_t k = append_by_job_n(n, j, B.t_count);
+        if (dying)
+            for (uint32_t b = 0; b < G.rec_depth; b++) {
+                const size_t ri = (size_t)slot * G.rec_depth + b;
+                const float c = G.rec_flag[ri] == 2 ? record_target(P, G, slot, ri) : 0.0f;
+                if (c > 0.0f) {
+                    if (k < B.t_cap) record_to_row(P, G, ri, c, B.t_in, B.t_dir, B.t_c, (size_t)j * B.t_cap + k);
+                    k++;
+                }
+                G.rec_flag[ri] = 0;
+            }
     }
     // the iteration in which a job's last path retired: the wave records its first
     // retiring job (folded by the next plan); each other job with retiring lanes in the

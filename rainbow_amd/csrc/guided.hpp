@@ -14,10 +14,12 @@
 namespace rbw {
 
 // the vMF-mixture head of the guiding MLP (same parameterisation as the training loss),
-// evaluated with the hardware exponential, logarithm, sine and cosine (v_exp_f32,
-// v_log_f32, v_sin_f32, v_cos_f32; a few ulp): the sampler and the pdf share the lobes'
-// parameters, and the estimator divides by the density it evaluates, so the rounding
-// changes the variance a little, not the mean
+// evaluated with the hardware exponential, logarithm, sine, cosine, reciprocal and square
+// root (v_exp_f32, v_log_f32, v_sin_f32, v_cos_f32, v_rcp_f32, v_sqrt_f32; a few ulp): the
+// sampler and the pdf share the lobes' parameters, and the estimator divides by the
+// density it evaluates, so the rounding changes the variance a little, not the mean
+__device__ __forceinline__ float vrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 struct vmf_mixture {
     f3 mu[guide::N_LOBES];
     float kappa[guide::N_LOBES], w[guide::N_LOBES], norm[guide::N_LOBES];
@@ -29,15 +31,16 @@ struct vmf_mixture {
             w[j] = __expf(o[5 * j + 4] - amax);
             ws += w[j];
             const f3 m = f3{o[5 * j], o[5 * j + 1], o[5 * j + 2]};
-            const float nm = fmaxf(sqrtf(m.x * m.x + m.y * m.y + m.z * m.z), 1e-12f);
-            mu[j] = f3{m.x / nm, m.y / nm, m.z / nm};
+            const float inm = vrcp(fmaxf(vsqrt(m.x * m.x + m.y * m.y + m.z * m.z), 1e-12f));
+            mu[j] = f3{m.x * inm, m.y * inm, m.z * inm};
             const float x = o[5 * j + 3];
             const float sp = x > 20.0f ? x : __logf(1.0f + __expf(x));
             kappa[j] = fminf(sp + guide::KAPPA_MIN, guide::KAPPA_MAX);
             // 1 - e^(-2 kappa) in FP32 keeps ~5e-6 relative accuracy down to kappa = KAPPA_MIN
-            norm[j] = kappa[j] / (2.0f * 3.14159265f * (1.0f - __expf(-2.0f * kappa[j])));
+            norm[j] = kappa[j] * vrcp(2.0f * 3.14159265f * (1.0f - __expf(-2.0f * kappa[j])));
         }
-        for (int j = 0; j < guide::N_LOBES; j++) w[j] /= ws;
+        const float iws = vrcp(ws);
+        for (int j = 0; j < guide::N_LOBES; j++) w[j] *= iws;
     }
     __device__ float pdf(f3 d) const {
         float q = 0.0f;
@@ -53,10 +56,10 @@ struct vmf_mixture {
             if (u.x < acc + w[j]) break;
             acc += w[j];
         }
-        const float ux = fminf(fmaxf((u.x - acc) / fmaxf(w[j], 1e-20f), 0.0f), 0.99999994f);
+        const float ux = fminf(fmaxf((u.x - acc) * vrcp(fmaxf(w[j], 1e-20f)), 0.0f), 0.99999994f);
         const float k = kappa[j];
-        const float W = 1.0f + __logf(ux + (1.0f - ux) * __expf(-2.0f * k)) / k;
-        const float sr = sqrtf(fmaxf(0.0f, 1.0f - W * W));
+        const float W = 1.0f + __logf(ux + (1.0f - ux) * __expf(-2.0f * k)) * vrcp(k);
+        const float sr = vsqrt(fmaxf(0.0f, 1.0f - W * W));
         const float phi = 2.0f * 3.14159265f * u.y;
         const frame fr = frame_from_normal(mu[j]);
         return local_to_world(fr, f3{sr * __cosf(phi), sr * __sinf(phi), W});
