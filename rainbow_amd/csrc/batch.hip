@@ -65,7 +65,9 @@ struct dctl {
     unsigned long long total;   // samples of all jobs
     unsigned long long live;    // slots holding a path after this iteration's regeneration
     uint32_t iteration;
-    uint32_t pad;
+    // 1: the queue is exhausted and no slot holds a path after this iteration's regeneration
+    // (the stage kernels return at once); 2: that regeneration has run too (it returns too)
+    uint32_t done;
 };
 
 struct dbatch {
@@ -215,7 +217,7 @@ __global__ void k_b_init(dbatch B, dpaths P, unsigned long long total) {
         c.total = total;
         c.live = 0;
         c.iteration = 0;
-        c.pad = 0;
+        c.done = 0;
         *B.ctl = c;
     }
 }
@@ -261,11 +263,18 @@ __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     __shared__ uint32_t part[1024];
     const uint32_t t = threadIdx.x, n = B.ngroups;
     if (t == 0) B.fetch[0] = 0u;
+    if (B.guided)
+        for (uint32_t j = t; j < B.njobs; j += 1024) B.gcount[j] = 0u;
+    // the work is done: the iterations the host replays before it sees that cost only their
+    // launches (the last regeneration's retire records are folded after the loop)
+    if (B.ctl->done) {
+        __syncthreads();
+        if (t == 0) B.ctl->done = 2u;
+        return;
+    }
     fold_retire_records(B, B.ctl->iteration);
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b0 = t * per < n ? t * per : n, b1 = b0 + per < n ? b0 + per : n;
-    if (B.guided)
-        for (uint32_t j = t; j < B.njobs; j += 1024) B.gcount[j] = 0u;
     uint32_t sum = 0;
     for (uint32_t b = b0; b < b1; b++) sum += B.block_free[b];
     part[t] = sum;
@@ -295,6 +304,7 @@ __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
         B.ctl->cursor = cursor + granted;
         B.ctl->live = (unsigned long long)B.pool - (total_free - granted);
         B.ctl->iteration += 1;
+        if (B.ctl->live == 0 && cursor + granted >= total) B.ctl->done = 1u;
     }
 }
 
@@ -303,6 +313,7 @@ __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
 // get consecutive samples (neighbouring pixels of one job).
 __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
+    if (B.ctl->done == 2u) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = slot < B.pool;
     const uint32_t iter = B.ctl->iteration;
@@ -388,6 +399,7 @@ constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % 
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
 void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
     const uint32_t lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t tri_test = base.tri_test;
@@ -508,6 +520,7 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
 
 template <int W>
 __global__ __launch_bounds__(k_batch_max_block) RBW_EXTEND_OCCUPANCY void k_b_extend(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
     uint32_t rays = 0;
@@ -522,6 +535,7 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_EXTEND_OCCUPANCY void k_b_ex
 }
 
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
@@ -540,6 +554,7 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
 // the others shade as the plain estimator (shade_path).
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade_guided(dscene S, dpaths P,
                                                                                          dparams base, dbatch B) {
+    if (B.ctl->done) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     bool to_guide = false;
     uint32_t j = 0;
@@ -607,6 +622,7 @@ __device__ __forceinline__ void guided_slot(const dscene& S, const dpaths& P, co
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_guided(dscene S, dpaths P, dparams base,
                                                                                    dbatch B) {
     __shared__ uint32_t first_row[guide::MAX_NETS + 1];
+    if (B.ctl->done) return;
     if (threadIdx.x == 0) {
         uint32_t a = 0;
         for (uint32_t j = 0; j < B.njobs; j++) {
@@ -631,6 +647,7 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_gui
 template <int W, bool FAST>
 __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
+    if (B.ctl->done) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     trace_counters cnt{0, 0};
     uint32_t nshadow = 0, nmis = 0, nlight = 0;

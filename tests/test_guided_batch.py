@@ -142,8 +142,9 @@ def test_gpu_guided_rooms_at_the_configured_workload(tracer):
     room variants (gap 0.1-0.5, 1e5-1.3e6 triangles, GGX alpha 0.02-0.6, 256^2-1024^2) in
     one submission, one network per job trained online by the batched kernel.  Every job's
     samples are accounted for, the networks take Adam steps, and every job's guided image
-    mean (8 submissions of 8 spp while training) equals its plain 512-spp judge's within
-    four standard errors."""
+    mean (16 submissions of 8 spp, training in the first 8) equals its plain 2048-spp judge's
+    within five standard errors of the difference (the guided chunks' spread, and the judge's
+    error scaled from it by the sample counts)."""
     from rainbow_amd.tracer import batch_opts, guiding_opts
     rs = scenes.room_variants(n=16, res=(256, 1024))
     ids = [tracer.add_scene(r["scene"]) for r in rs]
@@ -151,11 +152,11 @@ def test_gpu_guided_rooms_at_the_configured_workload(tracer):
         def jobs(spp, seed0):
             return [{"scene": ids[i], "camera": r["camera"], "spp": spp, "max_depth": 8, "seed": seed0 + 131 * i}
                     for i, r in enumerate(rs)]
-        judge, _, _ = tracer.submit_jobs(jobs(512, 1 << 20), batch_opts())
+        judge, _, _ = tracer.submit_jobs(jobs(2048, 1 << 20), batch_opts())
         tracer.guide_jobs_reset(len(rs), seed=7)
         chunks, steps = [], 0
-        for k in range(8):
-            go = guiding_opts(train=True, steps_per_pass=1, batch=1 << 16)
+        for k in range(16):
+            go = guiding_opts(train=k < 8, steps_per_pass=1, batch=1 << 16)
             imgs, jst, st = tracer.submit_jobs(jobs(8, 100000 + 1000 * k), batch_opts(guiding=go))
             assert st["samples"] == sum(r["res"] ** 2 * 8 for r in rs)
             for r, js in zip(rs, jst):
@@ -170,8 +171,8 @@ def test_gpu_guided_rooms_at_the_configured_workload(tracer):
             mr = float(judge[i].mean())
             assert np.isfinite(chunks[-1][i]).all()
             print(f"room {i} (gap {rs[i]['gap']:.2f}, {rs[i]['triangles']} tris, {rs[i]['res']}^2): "
-                  f"guided {m:.5f} +- {se:.1e}, plain 512 spp {mr:.5f}")
-            assert abs(m - mr) < 4 * se * np.sqrt(1 + 64 / 512) + 1e-6, i
+                  f"guided {m:.5f} +- {se:.1e}, plain 2048 spp {mr:.5f}")
+            assert abs(m - mr) < 5 * se * np.sqrt(1 + 128 / 2048) + 1e-6, i
     finally:
         for sid in ids:
             tracer.remove_scene(sid)
