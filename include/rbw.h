@@ -247,7 +247,17 @@ typedef struct rbw_render_params {
      * pdf is not a number and fails the test at integrator.cpp:262); 1 = it contributes with
      * the power-heuristic weight, which makes the estimator unbiased */
     uint32_t medium_mis;
+    uint32_t schedule;               /* RBW_SCHEDULE_*: which wavefront schedule runs the plain estimator */
 } rbw_render_params;
+
+/* rbw_render_params.schedule: 0 picks per render; persistent path slots take the plain
+ * estimator with the per-sample or stratified sampler only (no replay, debug hits, visit
+ * counts, guiding, direct or volume integrator); passes of samples take every render */
+enum {
+    RBW_SCHEDULE_AUTO = 0,
+    RBW_SCHEDULE_PERSISTENT = 1,
+    RBW_SCHEDULE_PASSES = 2
+};
 /* Schedules: the plain estimator with the per-sample or stratified sampler (no replay, no debug
  * hits, no visit counts, no guiding) runs on persistent path slots regenerated in
  * place (the batched-jobs wavefront with one job, its bounce loop a hipGraph;

@@ -94,7 +94,7 @@ class rbw_render_params(C.Structure):
                 ("guiding", C.POINTER(rbw_guiding_opts)), ("time_kernels", C.c_uint32), ("bvh_width", C.c_uint32),
                 ("strata_x", C.c_uint32), ("strata_y", C.c_uint32), ("dims_1d", C.c_uint32), ("dims_2d", C.c_uint32),
                 ("integrator", C.c_uint32), ("emitter_samples", C.c_uint32), ("bsdf_samples", C.c_uint32),
-                ("medium_mis", C.c_uint32)]
+                ("medium_mis", C.c_uint32), ("schedule", C.c_uint32)]
 
 
 class rbw_stats(C.Structure):

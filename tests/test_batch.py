@@ -176,7 +176,7 @@ def test_persistent_extension_walk_matches_one_slot_per_lane(tracer):
     from rainbow_amd.tracer import render_params
     sc, cam = scenes.displaced_icosphere_box(level=6, width=1024, height=1024)
     tracer.upload(sc)
-    a = tracer.render(cam, render_params(spp=4, max_depth=8, seed=21))
+    a = tracer.render(cam, render_params(spp=4, max_depth=8, seed=21, schedule=1))
     b = tracer.render(cam, render_params(spp=4, max_depth=8, seed=21, count_visits=1))
     assert a["stats"]["bvh_width"] == 4 and b["stats"]["node_visits"] > 0
     assert np.isfinite(a["image"]).all() and a["image"].mean() > 0

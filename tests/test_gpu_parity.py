@@ -126,14 +126,17 @@ def test_render_cornell_per_sample_pcg(tracer, cornell, width):
 
 
 def test_render_deterministic(tracer, cornell):
-    """Same inputs, same image bit for bit, also with a different pass split: the film
-    sums are fixed-point integers, so the order samples arrive in does not matter."""
+    """Same inputs, same image bit for bit, also with a different pass split and on either
+    schedule: the film sums are fixed-point integers, so the order samples arrive in does
+    not matter."""
     sc, cam, os_ = cornell
     a = tracer.render(cam, render_params(spp=4, max_depth=5, seed=9))["image"]
     a2 = tracer.render(cam, render_params(spp=4, max_depth=5, seed=9))["image"]
-    b = tracer.render(cam, render_params(spp=4, max_depth=5, seed=9, samples_per_pass=1))["image"]
+    b = tracer.render(cam, render_params(spp=4, max_depth=5, seed=9, samples_per_pass=1, schedule=2))["image"]
+    c = tracer.render(cam, render_params(spp=4, max_depth=5, seed=9, schedule=1))["image"]
     assert np.array_equal(a, a2)
     assert np.array_equal(a, b)
+    assert np.array_equal(a, c)
 
 
 @pytest.fixture(scope="module")
@@ -242,3 +245,7 @@ def test_render_light_shortcuts_match_the_oracle(tracer, kind):
         assert abs(g["stats"][k] - c["stats"][k]) <= 1e-4 * c["stats"][k] + 2, k
     full = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3), debug=True)["image"]
     assert pixel_mismatch(g["image"], full).mean() < 1e-3
+    # both schedules take the shortcuts alike
+    persistent = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3, schedule=1))["image"]
+    passes = tracer.render(cam, render_params(spp=16, max_depth=5, seed=3, schedule=2))["image"]
+    assert np.array_equal(persistent, g["image"]) and np.array_equal(passes, g["image"])

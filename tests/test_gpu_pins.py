@@ -28,13 +28,13 @@ def test_gpu_sphere_closed_forms(tracer, width):
 
 @pytest.mark.parametrize("albedo,threshold,depth", RR_CASES)
 def test_gpu_furnace_with_russian_roulette(tracer, albedo, threshold, depth):
-    """persistent slots (the default schedule) and passes of samples (the instrumented
-    render, count_visits, runs there)"""
+    """persistent slots and passes of samples (the instrumented render, count_visits, runs
+    there)"""
     sc, cam = closed_box(albedo, 1.0)
     tracer.upload(sc)
     for counted in (0, 1):
         r = tracer.render(cam, render_params(spp=256, max_depth=depth, rr_threshold=threshold, seed=7,
-                                             count_visits=counted))
+                                             count_visits=counted, schedule=0 if counted else 1))
         st = r["stats"]
         print("schedule", "passes" if counted else "persistent", st["extension_rays"], st["samples"])
         check_furnace(r["image"], st["extension_rays"], st["samples"], albedo, threshold, depth)
@@ -81,5 +81,5 @@ def test_gpu_thin_lens_camera_matches_the_oracle(tracer, size):
     assert bad.sum() <= max(1, bad.size // 1000)
     assert ids <= 1e-5 * traced.sum()
     # the persistent-slot schedule (no debug hits) gives the same image bit for bit
-    p2 = tracer.render(cam, render_params(spp=16, max_depth=5, seed=13))["image"]
+    p2 = tracer.render(cam, render_params(spp=16, max_depth=5, seed=13, schedule=1))["image"]
     assert np.array_equal(p2, g["image"])

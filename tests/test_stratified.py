@@ -159,7 +159,7 @@ def test_gpu_stratified_schedules_agree(tracer):
     """Persistent slots and passes carry the per-sample dimension counters alike."""
     sc, cam = _cornell(48)
     tracer.upload(sc)
-    a = tracer.render(cam, _strat_params(spp=16, dims=(2, 3), seed=4, max_depth=5))["image"]
+    a = tracer.render(cam, _strat_params(spp=16, dims=(2, 3), seed=4, max_depth=5, schedule=1))["image"]
     b = tracer.render(cam, _strat_params(spp=16, dims=(2, 3), seed=4, max_depth=5, count_visits=1))["image"]
     assert np.array_equal(a, b)
 
