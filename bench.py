@@ -565,33 +565,40 @@ PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                        "tracer_persistent_bvh4_icosphere_1024_64spp.md")
 
 
-def profiled_kernel(stage, width):
-    """the profile's name of the persistent-slot kernel of a stage (extend, shade, connect)
-    at a BVH width: the four-wide extension walk runs persistent, connect is the
-    occlusion-only instantiation on the headline scene (one planar light)"""
+def profiled_kernels(stage, width):
+    """the profile's names of the persistent-slot kernels of a stage (extend, shade, connect)
+    at a BVH width on the headline scene (one planar light): at width 4 the extension walk
+    runs persistent, and the connect stage is the MIS rays' emitter test, the persistent
+    occlusion walk and the walk-free connect kernel; at widths 2 and 8 connect is the
+    occlusion-only instantiation"""
     if stage == "extend":
-        return "k_b_extend_dyn" if width == 4 else f"k_b_extend<{width}>"
+        return ("k_b_extend_dyn",) if width == 4 else (f"k_b_extend<{width}>",)
     if stage == "connect":
-        return f"k_b_connect<{width}, true>"
-    return "k_b_shade"
+        if width == 4:
+            return ("k_b_mis_prep", "k_b_occlusion_dyn", "k_b_connect<4, true, true>")
+        return (f"k_b_connect<{width}, true, false>",)
+    return ("k_b_shade",)
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the icosphere workload, from the committed
-    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
-    reads for a copy kernel of known size); None when the summary has no such row."""
+def profiled_traffic(kernels):
+    """HBM bytes per launch of the stage made of `kernels` on the icosphere workload, from
+    the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+    counter reads for a copy kernel of known size); None when the summary lacks a row."""
     try:
         lines = open(PROFILE).read().splitlines()
     except OSError:
         return None
+    rows = {}
     for line in lines:
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == kernel:
+        if len(cells) == 6 and cells[0] in kernels:
             try:
-                return int(float(cells[4]) * 1e6)
+                rows[cells[0]] = float(cells[4]) * 1e6
             except ValueError:
                 return None
-    return None
+    if len(rows) != len(kernels):
+        return None
+    return int(sum(rows.values()))
 
 
 def main():
@@ -701,7 +708,7 @@ def main():
         achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": roofline.HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(profiled_kernel(dom, width)) if args.workload == "icosphere" else None,
+                "traffic": profiled_traffic(profiled_kernels(dom, width)) if args.workload == "icosphere" else None,
                 "bvh_width": width,
                 "algorithmic_bytes_per_launch": int(kb[dom] / max(1, launches)),
                 "avg_launch_ms": kms[dom] / max(1, launches),

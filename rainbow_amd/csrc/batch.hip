@@ -84,7 +84,9 @@ struct dbatch {
     uint32_t* first_iter;           // per job
     uint32_t* last_iter;            // per job
     unsigned long long* counters;   // extension, shadow, MIS, light casts (striped)
-    uint32_t* fetch;                // next slot of the persistent extension walk (zeroed by the plan)
+    uint32_t* fetch;                // next item of the persistent extension [0] and occlusion [1] walks, MIS queue
+                                    // length [2] (zeroed by the plan)
+    uint32_t* mqueue;               // slots whose MIS ray reaches an emitter triangle (k_b_mis_prep)
     // guided jobs (job j uses network j): this iteration's guided vertices per job, as slots
     // (rows[j * pool + k], k < gcount[j]), and the training rows closed at retirement
     uint32_t guided, train;
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(1024) void k_b_fold(dbatch B) { fold_retire_records
 __global__ __launch_bounds__(1024) void k_b_plan(dbatch B) {
     __shared__ uint32_t part[1024];
     const uint32_t t = threadIdx.x, n = B.ngroups;
-    if (t == 0) B.fetch[0] = 0u;
+    if (t < 3) B.fetch[t] = 0u;
     if (B.guided)
         for (uint32_t j = t; j < B.njobs; j += 1024) B.gcount[j] = 0u;
     // the work is done: the iterations the host replays before it sees that cost only their
@@ -397,15 +399,26 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
 // waves (more are harmless: they find the counter past the pool and leave).
 constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % slower)
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
-void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
-    if (B.ctl->done) return;
+// CONNECT: the walks are the occlusion queries of the slots waiting for the connect stage
+// (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a nearest
+// visible emitter triangle (the n slots of k_b_mis_prep's queue; U_MPRIM, F_MLEN), walked up
+// to it with its entity the target, items [n, n + pool) the slots' shadow rays (C_SHADOW; the
+// light's entity is the target).  A shadow ray with nothing in the way sets C_LIT in the slot's connect flags,
+// an MIS ray with something in the way C_MOCC; the connect kernel then skips the walks.
+// Counter B.fetch[1].  Otherwise the extension walks of the live slots (closest hits),
+// counter B.fetch[0].
+template <bool CONNECT>
+__device__ __forceinline__ void persistent_walk(const dscene& S, const dpaths& P, const dparams& base,
+                                                const dbatch& B) {
     const uint32_t lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t tri_test = base.tri_test;
-    uint32_t rb = 0, re = 0;  // the wave's range of slots not yet handed out
+    const uint32_t nmis = CONNECT ? B.fetch[2] : 0u;
+    const uint32_t items = nmis + B.pool;
+    uint32_t rb = 0, re = 0;  // the wave's range of items not yet handed out
     bool exhausted = false;
     bool active = false;      // the lane walks the ray of `slot`
+    bool mis = false;         // CONNECT: the lane's ray is the MIS ray of `slot`
     uint32_t slot = 0, rays = 0;
     // the walk (traverse4's state)
     local_ray lr;
@@ -416,6 +429,7 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
     int sp = 0;
     int32_t cur = 0, leaf = 0;
     bool done = true, has_leaf = false;
+    uint32_t target = RBW_NONE;  // CONNECT: the entity the ray may hit
     uint32_t* ring = rbw_lds_stack + threadIdx.x;
     auto push = [&](uint32_t v) {
         uint32_t* e = ring + (uint32_t)(sp & (RBW_LDS_STACK - 1)) * 64u;
@@ -434,46 +448,84 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
         if (sp == 0) done = true;
         else cur = (int32_t)pop();
     };
+    // the walk of the ray (o, d) up to world length len_w (the caller sets `cur`, the root)
+    auto start = [&](f3 o, f3 d, float len_w) {
+        lr = make_local_ray(o, d);
+        if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+        o_w = o;
+        inv = lr.inv;
+        oi = f3{lr.o.x * inv.x, lr.o.y * inv.y, lr.o.z * inv.z};
+        st.best.prim = RBW_MISS;
+        st.best.t = 0.0f;
+        st.best.b1 = st.best.b2 = 0.0f;
+        st.best_inst_entity = RBW_NONE;
+        st.len_w = len_w;
+        st.thr_local = st.len_w * lr.L0;
+        sp = 0;
+        done = false;
+        has_leaf = false;
+        active = true;
+    };
     trace_counters cnt{0, 0};
     for (;;) {
-        // idle lanes take the next slots; a slot without a live path leaves its lane idle
+        // idle lanes take the next items; an item without a ray leaves its lane idle
         uint64_t idle = __ballot(!active);
         if (!exhausted && (idle == ~0ull || (uint32_t)__popcll(idle) >= k_refill)) {
             while (idle && !exhausted) {
                 if (rb == re) {
                     uint32_t g = 0;
-                    if (lane == 0) g = atomicAdd(B.fetch, k_fetch_chunk);
+                    if (lane == 0) g = atomicAdd(B.fetch + (CONNECT ? 1 : 0), k_fetch_chunk);
                     g = (uint32_t)__shfl((int)g, 0);
-                    if (g >= B.pool) {
+                    if (g >= items) {
                         exhausted = true;
                         break;
                     }
                     rb = g;
-                    re = g + k_fetch_chunk < B.pool ? g + k_fetch_chunk : B.pool;
+                    re = g + k_fetch_chunk < items ? g + k_fetch_chunk : items;
                 }
                 const uint32_t take = min((uint32_t)__popcll(idle), re - rb);
                 const uint32_t rank = (uint32_t)__popcll(idle & below);
                 if (!active && rank < take) {
-                    slot = rb + rank;
-                    if (P.U(U_STATE, slot) & ST_ALIVE) {
-                        const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
-                        lr = make_local_ray(o, d);
-                        if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
-                        o_w = o;
-                        inv = lr.inv;
-                        oi = f3{lr.o.x * inv.x, lr.o.y * inv.y, lr.o.z * inv.z};
-                        st.best.prim = RBW_MISS;
-                        st.best.t = 0.0f;
-                        st.best.b1 = st.best.b2 = 0.0f;
-                        st.best_inst_entity = RBW_NONE;
-                        st.len_w = __builtin_inff();
-                        st.thr_local = st.len_w * lr.L0;
-                        cur = (int32_t)B.jobs[P.U(U_JOB, slot)].root4;
-                        sp = 0;
-                        done = false;
-                        has_leaf = false;
-                        active = true;
-                        rays++;
+                    const uint32_t item = rb + rank;
+                    if constexpr (CONNECT) {
+                        mis = item < nmis;
+                        if (mis) {
+                            slot = B.mqueue[item];
+                            f3 eo, ed;
+                            spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
+                            start(eo, ed, P.F(F_MLEN, slot));
+                            target = S.tri_entity[P.U(U_MPRIM, slot)];
+                            cur = (int32_t)B.jobs[P.U(U_JOB, slot)].root4;
+                        } else {
+                            slot = item - nmis;
+                            if ((P.U(U_STATE, slot) & ST_CONNECT) && (P.U(U_CFLAGS, slot) & C_SHADOW)) {
+                                start(P.F3(F_SOX, slot), P.F3(F_SDX, slot), P.F(F_SLEN, slot));
+                                target = P.U(U_SENT, slot);
+                                cur = (int32_t)B.jobs[P.U(U_JOB, slot)].root4;
+                            }
+                        }
+                    } else {
+                        slot = item;
+                        if (P.U(U_STATE, slot) & ST_ALIVE) {
+                            const f3 o = P.F3(F_OX, slot), d = P.F3(F_DX, slot);
+                            lr = make_local_ray(o, d);
+                            if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
+                            o_w = o;
+                            inv = lr.inv;
+                            oi = f3{lr.o.x * inv.x, lr.o.y * inv.y, lr.o.z * inv.z};
+                            st.best.prim = RBW_MISS;
+                            st.best.t = 0.0f;
+                            st.best.b1 = st.best.b2 = 0.0f;
+                            st.best_inst_entity = RBW_NONE;
+                            st.len_w = __builtin_inff();
+                            st.thr_local = st.len_w * lr.L0;
+                            cur = (int32_t)B.jobs[P.U(U_JOB, slot)].root4;
+                            sp = 0;
+                            done = false;
+                            has_leaf = false;
+                            active = true;
+                            rays++;
+                        }
                     }
                 }
                 rb += take;
@@ -499,8 +551,16 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
             has_leaf = false;
             const uint32_t first = (uint32_t)leaf & 0x07FFFFFFu;
             const uint32_t count = (((uint32_t)leaf >> 27) & 15u) + 1u;
-            test_leaf<false, false, false>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt, tri_test,
-                                           RBW_NONE, cnt, st);
+            if constexpr (CONNECT) {
+                if (test_leaf<true, true, false>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt,
+                                                 tri_test, target, cnt, st)) {
+                    done = true;  // occluded
+                    break;
+                }
+            } else {
+                test_leaf<false, false, false>(S, S.prim_a, S.prim_b, S.prim_c, first, count, lr, o_w, wt, tri_test,
+                                               RBW_NONE, cnt, st);
+            }
             if (!done && cur < 0) {
                 if (cur != k_empty_child) {
                     leaf = cur;
@@ -510,12 +570,60 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
             }
         }
         if (active && done) {
-            P.U(U_HPRIM, slot) = st.best.prim;
-            P.F(F_HT, slot) = st.best.t;
+            if constexpr (CONNECT) {
+                // (the slot's other item may be finishing at the same time)
+                const bool open = st.best.prim == RBW_MISS;
+                if (open != mis) atomicOr(&P.U(U_CFLAGS, slot), mis ? C_MOCC : C_LIT);
+            } else {
+                P.U(U_HPRIM, slot) = st.best.prim;
+                P.F(F_HT, slot) = st.best.t;
+            }
             active = false;
         }
     }
-    stripe_add64(B.counters, 0, rays);
+    if (!CONNECT) stripe_add64(B.counters, 0, rays);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
+void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
+    persistent_walk<false>(S, P, base, B);
+}
+
+// the MIS rays' nearest visible emitter triangles (connect_path's FAST form) for
+// k_b_occlusion_dyn: U_MPRIM (RBW_MISS: none) and F_MLEN, the ray's world length up to it;
+// the slots with one join the MIS queue (B.mqueue, length B.fetch[2])
+__global__ __launch_bounds__(k_batch_max_block) void k_b_mis_prep(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool reach = false;
+    if (slot < B.pool && (P.U(U_STATE, slot) & ST_CONNECT) && (P.U(U_CFLAGS, slot) & C_MIS)) {
+        const djob& J = B.jobs[P.U(U_JOB, slot)];
+        f3 eo, ed;
+        spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
+        const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
+        P.U(U_MPRIM, slot) = h.prim;
+        if (h.prim != RBW_MISS) {
+            P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
+            reach = true;
+        }
+    }
+    // one atomic per wave
+    const uint64_t m = __ballot(reach);
+    if (m == 0) return;
+    const uint32_t lane = __lane_id();
+    uint32_t base_ix = 0;
+    if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) base_ix = atomicAdd(B.fetch + 2, (uint32_t)__popcll(m));
+    base_ix = (uint32_t)__shfl((int)base_ix, __ffsll((unsigned long long)m) - 1);
+    if (reach) B.mqueue[base_ix + (uint32_t)__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = slot;
+}
+
+// the occlusion walks of the connect stage, persistent (four-wide BVH, every job's scene with
+// few planar lights, no debug hits)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
+void k_b_occlusion_dyn(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
+    persistent_walk<true>(S, P, base, B);
 }
 
 template <int W>
@@ -643,8 +751,9 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_gui
 
 // next-event estimation rays; then the block's count of slots the next
 // regeneration will find free (free or finished)
-// FAST: every job's scene has few planar lights (connect_path's occlusion-only form)
-template <int W, bool FAST>
+// FAST: every job's scene has few planar lights (connect_path's occlusion-only form);
+// WALKED: k_b_occlusion_dyn ran the occlusion walks
+template <int W, bool FAST, bool WALKED = false>
 __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_connect(dscene S, dpaths P, dparams base, dbatch B) {
     __shared__ uint32_t lds[k_batch_max_block / 64];
     if (B.ctl->done) return;
@@ -658,7 +767,7 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_CONNECT_OCCUPANCY void k_b_c
             dscene Sj;
             dparams pj;
             job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-            connect_path<W, false, true, FAST>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
+            connect_path<W, false, true, FAST, WALKED>(Sj, P, pj, slot, cnt, nshadow, nmis, nlight);
             st &= ~ST_CONNECT;
             P.U(U_STATE, slot) = st;
         }
@@ -690,7 +799,7 @@ struct rbw_batch_state {
     dbuf pf, pu, prng;
     uint32_t pool = 0;
     // per batch
-    dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, fetch, films, rgb;
+    dbuf ctl, jobs, block_free, block_n, block_base, wave_job, first_iter, last_iter, counters, fetch, mqueue, films, rgb;
     dctl* host_ctl = nullptr;  // pinned
     // guided jobs: per-slot guided vertex data and training records, per-job lists and rows
     dbuf g_surf, g_query, g_mix, g_rec, g_flag, g_rows, g_count, t_in, t_dir, t_c, t_count;
@@ -869,6 +978,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     b.last_iter.ensure(size_t(nj) * 4);
     b.counters.ensure(RBW_STRIPES * RBW_STRIPE_WORDS * 8);
     b.fetch.ensure(256);
+    b.mqueue.ensure(size_t(cfg.pool) * 4);
     if (!b.host_ctl) HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b.host_ctl), sizeof(dctl), hipHostMallocDefault));
 
     dbatch B{};
@@ -886,6 +996,7 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     B.last_iter = b.last_iter.as<uint32_t>();
     B.counters = b.counters.as<unsigned long long>();
     B.fetch = b.fetch.as<uint32_t>();
+    B.mqueue = b.mqueue.as<uint32_t>();
     dpaths P;
     P.f = b.pf.as<float>();
     P.u = b.pu.as<uint32_t>();
@@ -989,6 +1100,11 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                                                       lds_stack_bytes(64))
                                   : 0u;
     const bool dyn = dyn_grid && cfg.pool >= k_fetch_chunk * dyn_grid;
+    // the connect stage's walks run persistent too where they are occlusion queries
+    const uint32_t occl_grid =
+        dyn && fast ? persistent_blocks(reinterpret_cast<const void*>(&k_b_occlusion_dyn), 64, lds_stack_bytes(64))
+                    : 0u;
+    if (occl_grid) connect_kernel = k_b_connect<4, true, true>;
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
         hipLaunchKernelGGL(k_b_plan, dim3(1), dim3(1024), 0, s, B);
@@ -1017,6 +1133,10 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
+        if (occl_grid) {
+            hipLaunchKernelGGL(k_b_mis_prep, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+            hipLaunchKernelGGL(k_b_occlusion_dyn, dim3(occl_grid), dim3(64), lds_stack_bytes(64), s, S, P, base, B);
+        }
         hipLaunchKernelGGL(connect_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
         if (tm) tm->end(s);
     };

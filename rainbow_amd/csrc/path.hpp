@@ -117,7 +117,8 @@ __device__ __forceinline__ bool test_leaf(const dscene& S, const float4* __restr
 
 // the nearest hit among the visible emitters' triangles (mesh lights, world space), with the
 // leaf test's acceptance rule; used only while few emitter triangles exist (k_lights_few)
-__device__ __forceinline__ hitrec nearest_emitter_hit(const dscene& S, f3 o_w, f3 d_w, uint32_t tri_test) {
+__device__ __forceinline__ hitrec nearest_emitter_hit(const dscene& S, const uint32_t* emitters, uint32_t num_emitters,
+                                                      f3 o_w, f3 d_w, uint32_t tri_test) {
     const local_ray lr = make_local_ray(o_w, d_w);
     wt_setup wt;
     if (tri_test == RBW_TRI_WATERTIGHT) wt = make_wt(lr.d);
@@ -126,8 +127,8 @@ __device__ __forceinline__ hitrec nearest_emitter_hit(const dscene& S, f3 o_w, f
     best.t = 0.0f;
     best.b1 = best.b2 = 0.0f;
     float len_w = __builtin_inff();
-    for (uint32_t k = 0; k < S.num_emitters; k++) {
-        const dentity E = S.ents[S.emitters[k]];
+    for (uint32_t k = 0; k < num_emitters; k++) {
+        const dentity E = S.ents[emitters[k]];
         if (E.material == RBW_NONE) continue;
         for (uint32_t i = 0; i < E.count; i++) {
             const uint32_t tri = E.first + i;
@@ -557,7 +558,7 @@ template <int W, bool COUNT, bool LDS>
 __device__ __forceinline__ hitrec find_emitter_walk(const dscene& S, f3 o, f3 d, uint32_t tri_test,
                                                     trace_counters& cnt) {
     if (S.light_flags & k_lights_few) {
-        hitrec h = nearest_emitter_hit(S, o, d, tri_test);
+        hitrec h = nearest_emitter_hit(S, S.emitters, S.num_emitters, o, d, tri_test);
         if (h.prim != RBW_MISS) {
             const uint32_t le = S.tri_entity[h.prim];
             const hitrec occ = traverse<W, true, true, COUNT, LDS>(S, o, d, h.t * make_local_ray(o, d).L1, tri_test,
@@ -646,15 +647,21 @@ enum fidx : uint32_t {
     F_SFX, F_SFY, F_SFZ,                         // f * Le * w / pdf of the light sample
     F_MPX, F_MPY, F_MPZ, F_MNX, F_MNY, F_MNZ,    // vertex point and normal (MIS ray origin)
     F_MWX, F_MWY, F_MWZ, F_MFX, F_MFY, F_MFZ, F_MPDF,  // BSDF sample direction, f*|cos|, pdf
+    F_MLEN,                                      // world length of the MIS ray to its nearest emitter triangle
     F_CBX, F_CBY, F_CBZ,                         // beta at the vertex
     F_COUNT
 };
 // U_DIMS: the stratified sampler's draws made by the sample (1D | 2D << 16)
 // U_MEDIUM: the medium the path's ray travels in (volume path integrator; RBW_NONE = vacuum)
+// U_MPRIM: the MIS ray's nearest visible emitter triangle (RBW_MISS: none), with F_MLEN
+// (batched jobs' persistent occlusion walk)
 enum uidx : uint32_t { U_PIXEL, U_SAMPLE, U_BOUNCE, U_HPRIM, U_CFLAGS, U_SENT, U_PENDING, U_JOB, U_STATE, U_DIMS, U_MEDIUM,
-                       U_COUNT };
+                       U_MPRIM, U_COUNT };
 
-constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u;
+// connect flags (bits 0-7 of U_CFLAGS; the bounce above): the shadow and MIS rays to trace;
+// the answers of the persistent occlusion walks: C_LIT, the shadow ray reaches the light;
+// C_MOCC, something is in the way of the MIS ray before its nearest emitter triangle
+constexpr uint32_t C_SHADOW = 1u, C_MIS = 2u, C_LIT = 4u, C_MOCC = 8u;
 
 struct dpaths {
     float* f;      // F_COUNT * cap
@@ -1122,8 +1129,9 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
 // few (k_lights_planar | k_lights_few) and no debug hits are recorded, so both rays are
 // occlusion queries (any-hit walks) and the closest-hit walks are not instantiated.  The
 // MIS ray's vertex (point, normal, direction) is read again after its walk rather than
-// held in registers across it.
-template <int W, bool COUNT, bool LDS = false, bool FAST = false>
+// held in registers across it.  WALKED (with FAST): a persistent walk ran both occlusion
+// queries (C_LIT, U_MPRIM and C_MOCC hold their answers), and no walk is instantiated here.
+template <int W, bool COUNT, bool LDS = false, bool FAST = false, bool WALKED = false>
 __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                              trace_counters& cnt, uint32_t& nshadow, uint32_t& nmis,
                                              uint32_t& nlight, bool* lit = nullptr) {
@@ -1131,7 +1139,11 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
     const int bounce = (int)(flags >> 8);
     uint32_t* dbg = FAST ? nullptr : debug_slot(P, prm, slot, bounce);
     f3 L = f3{0.0f, 0.0f, 0.0f};
-    if (flags & C_SHADOW) {
+    if (WALKED && (flags & C_SHADOW)) {
+        nshadow = 1;
+        if (flags & C_LIT) L = L + P.F3(F_SFX, slot);
+        if (lit) *lit = (flags & C_LIT) != 0u;
+    } else if (flags & C_SHADOW) {
         nshadow = 1;
         const uint32_t lid = P.U(U_SENT, slot);
         const f3 so = P.F3(F_SOX, slot), sd = P.F3(F_SDX, slot);
@@ -1158,7 +1170,9 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
         nmis = 1;
         // find_emitter (integrator.cpp:49-82)
         uint32_t prim;
-        {
+        if (WALKED) {
+            prim = (flags & C_MOCC) ? RBW_MISS : P.U(U_MPRIM, slot);
+        } else {
             const f3 mp = P.F3(F_MPX, slot), mn = P.F3(F_MNX, slot), mw = P.F3(F_MWX, slot);
             f3 eo, ed;
             spawn_ray(mp, mn, mw, eo, ed);
@@ -1168,7 +1182,7 @@ __device__ __forceinline__ void connect_path(const dscene& S, const dpaths& P, c
                 // emitter triangle along the ray has no other visible surface before it: test
                 // the few emitter triangles, then walk the tree only for rays that reach one,
                 // as an occlusion query up to it
-                h = nearest_emitter_hit(S, eo, ed, prm.tri_test);
+                h = nearest_emitter_hit(S, S.emitters, S.num_emitters, eo, ed, prm.tri_test);
                 if (h.prim != RBW_MISS) {
                     const uint32_t le = S.tri_entity[h.prim];
                     const local_ray lr = make_local_ray(eo, ed);
