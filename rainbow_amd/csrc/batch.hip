@@ -86,7 +86,7 @@ struct dbatch {
     unsigned long long* counters;   // extension, shadow, MIS, light casts (striped)
     uint32_t* fetch;                // next item of the persistent extension [0] and occlusion [1] walks, MIS queue
                                     // length [2] (zeroed by the plan)
-    uint32_t* mqueue;               // slots whose MIS ray reaches an emitter triangle (k_b_mis_prep)
+    uint32_t* mqueue;               // slots whose MIS ray reaches an emitter triangle (queue_mis)
     // guided jobs (job j uses network j): this iteration's guided vertices per job, as slots
     // (rows[j * pool + k], k < gcount[j]), and the training rows closed at retirement
     uint32_t guided, train;
@@ -400,13 +400,14 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
 constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % slower)
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
 // CONNECT: the walks are the occlusion queries of the slots waiting for the connect stage
-// (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a nearest
-// visible emitter triangle (the n slots of k_b_mis_prep's queue; U_MPRIM, F_MLEN), walked up
-// to it with its entity the target, items [n, n + pool) the slots' shadow rays (C_SHADOW; the
-// light's entity is the target).  A shadow ray with nothing in the way sets C_LIT in the slot's connect flags,
-// an MIS ray with something in the way C_MOCC; the connect kernel then skips the walks.
-// Counter B.fetch[1].  Otherwise the extension walks of the live slots (closest hits),
-// counter B.fetch[0].
+// (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a
+// nearest visible emitter triangle (the n slots of the MIS queue, filled by shading or
+// k_b_mis_reach; U_MPRIM, F_MLEN), walked up to it with its entity the target, items
+// [n, n + pool) the slots' shadow rays (C_SHADOW; the light's entity is the target).  A
+// shadow ray with nothing in the way sets C_LIT in the slot's connect flags, an MIS ray
+// with something in the way C_MOCC; the connect kernel then skips the walks.  Counter
+// B.fetch[1].  Otherwise the extension walks of the live slots (closest hits), counter
+// B.fetch[0].
 template <bool CONNECT>
 __device__ __forceinline__ void persistent_walk(const dscene& S, const dpaths& P, const dparams& base,
                                                 const dbatch& B) {
@@ -590,34 +591,6 @@ void k_b_extend_dyn(dscene S, dpaths P, dparams base, dbatch B) {
     persistent_walk<false>(S, P, base, B);
 }
 
-// the MIS rays' nearest visible emitter triangles (connect_path's FAST form) for
-// k_b_occlusion_dyn: U_MPRIM (RBW_MISS: none) and F_MLEN, the ray's world length up to it;
-// the slots with one join the MIS queue (B.mqueue, length B.fetch[2])
-__global__ __launch_bounds__(k_batch_max_block) void k_b_mis_prep(dscene S, dpaths P, dparams base, dbatch B) {
-    if (B.ctl->done) return;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    bool reach = false;
-    if (slot < B.pool && (P.U(U_STATE, slot) & ST_CONNECT) && (P.U(U_CFLAGS, slot) & C_MIS)) {
-        const djob& J = B.jobs[P.U(U_JOB, slot)];
-        f3 eo, ed;
-        spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
-        const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
-        P.U(U_MPRIM, slot) = h.prim;
-        if (h.prim != RBW_MISS) {
-            P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
-            reach = true;
-        }
-    }
-    // one atomic per wave
-    const uint64_t m = __ballot(reach);
-    if (m == 0) return;
-    const uint32_t lane = __lane_id();
-    uint32_t base_ix = 0;
-    if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1)) base_ix = atomicAdd(B.fetch + 2, (uint32_t)__popcll(m));
-    base_ix = (uint32_t)__shfl((int)base_ix, __ffsll((unsigned long long)m) - 1);
-    if (reach) B.mqueue[base_ix + (uint32_t)__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = slot;
-}
-
 // the occlusion walks of the connect stage, persistent (four-wide BVH, every job's scene with
 // few planar lights, no debug hits)
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RBW_EXTEND4_WAVES)))
@@ -642,9 +615,25 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_EXTEND_OCCUPANCY void k_b_ex
     stripe_add64(B.counters, 0, rays);
 }
 
+// the slots whose MIS ray reaches an emitter triangle join the MIS queue (one atomic per wave)
+__device__ __forceinline__ void queue_mis(const dbatch& B, bool reach, uint32_t slot) {
+    const uint64_t m = __ballot(reach);
+    if (m == 0) return;
+    const uint32_t lane = __lane_id();
+    const int first = __ffsll((unsigned long long)m) - 1;
+    uint32_t at = 0;
+    if ((int)lane == first) at = atomicAdd(B.fetch + 2, (uint32_t)__popcll(m));
+    at = (uint32_t)__shfl((int)at, first);
+    if (reach) B.mqueue[at + (uint32_t)__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = slot;
+}
+
+// MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn (shade_path's
+// MIS_REACH; the MIS queue)
+template <bool MIS_REACH>
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
     if (B.ctl->done) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool reach = false;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
         uint32_t gmat = 0;
@@ -652,9 +641,10 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
         dscene Sj;
         dparams pj;
         job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-        shade_path<false>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
+        shade_path<false, MIS_REACH>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat, &reach);
         P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
     }
+    if (MIS_REACH) queue_mis(B, reach, slot);
 }
 
 // guided jobs: the vertex up to the guided stage.  A guided vertex stores its surface
@@ -697,6 +687,27 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
     }
     const uint32_t k = append_by_job(to_guide, j, B.gcount);
     if (to_guide) B.grows[(size_t)j * B.pool + k] = slot;
+}
+
+// shade_path's MIS_REACH test as a kernel of its own, for guided jobs (in k_b_shade_guided
+// it made the register allocator spill): the slots waiting for the connect stage with an
+// MIS ray
+__global__ __launch_bounds__(k_batch_max_block) void k_b_mis_reach(dscene S, dpaths P, dparams base, dbatch B) {
+    if (B.ctl->done) return;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool reach = false;
+    if (slot < B.pool && (P.U(U_STATE, slot) & ST_CONNECT) && (P.U(U_CFLAGS, slot) & C_MIS)) {
+        const djob& J = B.jobs[P.U(U_JOB, slot)];
+        f3 eo, ed;
+        spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
+        const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
+        P.U(U_MPRIM, slot) = h.prim;
+        if (h.prim != RBW_MISS) {
+            P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
+            reach = true;
+        }
+    }
+    queue_mis(B, reach, slot);
 }
 
 // the guided vertex of the slot's path with its job's network outputs
@@ -1129,12 +1140,13 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
             // a grid-stride walk over the guided rows: 8192 blocks cover an iteration's typical rows
             hipLaunchKernelGGL(k_b_guided, dim3(std::min<uint32_t>(grid, 8192u)), dim3(bs), 0, s, S, P, base, B);
         } else {
-            hipLaunchKernelGGL(k_b_shade, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+            hipLaunchKernelGGL(occl_grid ? k_b_shade<true> : k_b_shade<false>, dim3(grid), dim3(bs), 0, s, S, P, base,
+                               B);
         }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
         if (occl_grid) {
-            hipLaunchKernelGGL(k_b_mis_prep, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+            if (guided) hipLaunchKernelGGL(k_b_mis_reach, dim3(grid), dim3(bs), 0, s, S, P, base, B);
             hipLaunchKernelGGL(k_b_occlusion_dyn, dim3(occl_grid), dim3(64), lds_stack_bytes(64), s, S, P, base, B);
         }
         hipLaunchKernelGGL(connect_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
