@@ -401,13 +401,12 @@ constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % 
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
 // CONNECT: the walks are the occlusion queries of the slots waiting for the connect stage
 // (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a
-// nearest visible emitter triangle (the n slots of the MIS queue, filled by shading or
-// k_b_mis_reach; U_MPRIM, F_MLEN), walked up to it with its entity the target, items
-// [n, n + pool) the slots' shadow rays (C_SHADOW; the light's entity is the target).  A
-// shadow ray with nothing in the way sets C_LIT in the slot's connect flags, an MIS ray
-// with something in the way C_MOCC; the connect kernel then skips the walks.  Counter
-// B.fetch[1].  Otherwise the extension walks of the live slots (closest hits), counter
-// B.fetch[0].
+// nearest visible emitter triangle (the n slots of the MIS queue, filled by k_b_shade;
+// U_MPRIM, F_MLEN), walked up to it with its entity the target, items [n, n + pool) the
+// slots' shadow rays (C_SHADOW; the light's entity is the target).  A shadow ray with
+// nothing in the way sets C_LIT in the slot's connect flags, an MIS ray with something in
+// the way C_MOCC; the connect kernel then skips the walks.  Counter B.fetch[1].  Otherwise
+// the extension walks of the live slots (closest hits), counter B.fetch[0].
 template <bool CONNECT>
 __device__ __forceinline__ void persistent_walk(const dscene& S, const dpaths& P, const dparams& base,
                                                 const dbatch& B) {
@@ -687,27 +686,6 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
     }
     const uint32_t k = append_by_job(to_guide, j, B.gcount);
     if (to_guide) B.grows[(size_t)j * B.pool + k] = slot;
-}
-
-// shade_path's MIS_REACH test as a kernel of its own, for guided jobs (in k_b_shade_guided
-// it made the register allocator spill): the slots waiting for the connect stage with an
-// MIS ray
-__global__ __launch_bounds__(k_batch_max_block) void k_b_mis_reach(dscene S, dpaths P, dparams base, dbatch B) {
-    if (B.ctl->done) return;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    bool reach = false;
-    if (slot < B.pool && (P.U(U_STATE, slot) & ST_CONNECT) && (P.U(U_CFLAGS, slot) & C_MIS)) {
-        const djob& J = B.jobs[P.U(U_JOB, slot)];
-        f3 eo, ed;
-        spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
-        const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
-        P.U(U_MPRIM, slot) = h.prim;
-        if (h.prim != RBW_MISS) {
-            P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
-            reach = true;
-        }
-    }
-    queue_mis(B, reach, slot);
 }
 
 // the guided vertex of the slot's path with its job's network outputs
@@ -1111,10 +1089,13 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
                                                       lds_stack_bytes(64))
                                   : 0u;
     const bool dyn = dyn_grid && cfg.pool >= k_fetch_chunk * dyn_grid;
-    // the connect stage's walks run persistent too where they are occlusion queries
+    // the connect stage's walks run persistent too where they are occlusion queries, except in
+    // guided batches (the narrow-aperture rooms, 16 guided jobs at 1024 spp: 30.8 s against
+    // 29.8 s with the one-slot-per-lane connect kernel)
     const uint32_t occl_grid =
-        dyn && fast ? persistent_blocks(reinterpret_cast<const void*>(&k_b_occlusion_dyn), 64, lds_stack_bytes(64))
-                    : 0u;
+        dyn && fast && !guided
+            ? persistent_blocks(reinterpret_cast<const void*>(&k_b_occlusion_dyn), 64, lds_stack_bytes(64))
+            : 0u;
     if (occl_grid) connect_kernel = k_b_connect<4, true, true>;
     auto iteration = [&]() {
         if (tm) tm->begin(RBW_KERNEL_RAYGEN, s);
@@ -1145,10 +1126,8 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
         }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
-        if (occl_grid) {
-            if (guided) hipLaunchKernelGGL(k_b_mis_reach, dim3(grid), dim3(bs), 0, s, S, P, base, B);
+        if (occl_grid)
             hipLaunchKernelGGL(k_b_occlusion_dyn, dim3(occl_grid), dim3(64), lds_stack_bytes(64), s, S, P, base, B);
-        }
         hipLaunchKernelGGL(connect_kernel, dim3(grid), dim3(bs), trav_lds, s, S, P, base, B);
         if (tm) tm->end(s);
     };
