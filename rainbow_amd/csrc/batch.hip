@@ -626,8 +626,11 @@ __device__ __forceinline__ void queue_mis(const dbatch& B, bool reach, uint32_t 
     if (reach) B.mqueue[at + (uint32_t)__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = slot;
 }
 
-// MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn (shade_path's
-// MIS_REACH; the MIS queue)
+// MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn, and the MIS ray of
+// a vertex is tested against the emitter triangles here (connect_path's FAST form): U_MPRIM
+// (RBW_MISS: none) and F_MLEN, the ray's world length up to it, and the slots with one join
+// the MIS queue.  The test reads the ray back from the path state after shading (the lines
+// are in L2): inside shade_path it made the register allocator spill.
 template <bool MIS_REACH>
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
     if (B.ctl->done) return;
@@ -640,8 +643,21 @@ __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_sha
         dscene Sj;
         dparams pj;
         job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
-        shade_path<false, MIS_REACH>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat, &reach);
+        shade_path<false>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
         P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
+        // (the job's emitter range is read again too: held across shading it spilled)
+        __asm__ volatile("" ::: "memory");
+        if (MIS_REACH && need_connect && (P.U(U_CFLAGS, slot) & C_MIS)) {
+            const djob& J = B.jobs[P.U(U_JOB, slot)];
+            f3 eo, ed;
+            spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
+            const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
+            P.U(U_MPRIM, slot) = h.prim;
+            if (h.prim != RBW_MISS) {
+                P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
+                reach = true;
+            }
+        }
     }
     if (MIS_REACH) queue_mis(B, reach, slot);
 }

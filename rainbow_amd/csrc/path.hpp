@@ -980,14 +980,11 @@ __device__ __forceinline__ void extend_path(const dscene& S, const dpaths& P, co
 // sample_surface_interaction (integrator.cpp:329-404) up to the light and BSDF
 // samples of next-event estimation, then path_integrator.cpp:35-43, for the path in
 // `slot`.  GUIDED: vertices with a BSDF are handed back in gsi/guide_mat (to_guide)
-// for the guided vertex stage instead.  MIS_REACH (the scene's lights are few and planar):
-// the BSDF sample's ray is tested against the emitter triangles here, U_MPRIM and F_MLEN
-// record the nearest one, and mis_reach says whether it reaches one (the MIS ray's
-// occlusion walk then runs in the persistent occlusion kernel).
-template <bool GUIDED, bool MIS_REACH = false>
+// for the guided vertex stage instead.
+template <bool GUIDED>
 __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, const dparams& prm, uint32_t slot,
                                            bool& alive, bool& need_connect, bool& to_guide, surf& gsi,
-                                           uint32_t& guide_mat, bool* mis_reach = nullptr) {
+                                           uint32_t& guide_mat) {
     const uint32_t prim = P.U(U_HPRIM, slot);
     int bounce = (int)P.U(U_BOUNCE, slot);
     f3 value = P.F3(F_VX, slot);
@@ -1076,16 +1073,6 @@ __device__ __forceinline__ void shade_path(const dscene& S, const dpaths& P, con
                         P.setF3(F_MFX, slot, mf);
                         P.F(F_MPDF, slot) = mpdf;
                         cflags |= C_MIS;
-                        if (MIS_REACH) {
-                            f3 eo, ed;
-                            spawn_ray(si.point, si.normal, mwi, eo, ed);
-                            const hitrec h = nearest_emitter_hit(S, S.emitters, S.num_emitters, eo, ed, prm.tri_test);
-                            P.U(U_MPRIM, slot) = h.prim;
-                            if (h.prim != RBW_MISS) {
-                                P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
-                                *mis_reach = true;
-                            }
-                        }
                     }
                 }
                 if (cflags) {
