@@ -568,16 +568,16 @@ PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
 def profiled_kernels(stage, width):
     """the profile's names of the persistent-slot kernels of a stage (extend, shade, connect)
     at a BVH width on the headline scene (one planar light): at width 4 the extension walk
-    runs persistent, shading also tests the MIS rays against the emitter triangles, and the
-    connect stage is the persistent occlusion walk and the walk-free connect kernel; at
-    widths 2 and 8 connect is the occlusion-only instantiation"""
+    runs persistent, shading (grouped by material kind) also tests the MIS rays against the
+    emitter triangles, and the connect stage is the persistent occlusion walk and the
+    walk-free connect kernel; at widths 2 and 8 connect is the occlusion-only instantiation"""
     if stage == "extend":
         return ("k_b_extend_dyn",) if width == 4 else (f"k_b_extend<{width}>",)
     if stage == "connect":
         if width == 4:
             return ("k_b_occlusion_dyn", "k_b_connect<4, true, true>")
         return (f"k_b_connect<{width}, true, false>",)
-    return ("k_b_shade<true>",) if width == 4 else ("k_b_shade<false>",)
+    return ("k_b_shade_sorted<true>",) if width == 4 else ("k_b_shade_sorted<false>",)
 
 
 def profiled_traffic(kernels):

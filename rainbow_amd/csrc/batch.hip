@@ -401,7 +401,7 @@ constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % 
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
 // CONNECT: the walks are the occlusion queries of the slots waiting for the connect stage
 // (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a
-// nearest visible emitter triangle (the n slots of the MIS queue, filled by k_b_shade;
+// nearest visible emitter triangle (the n slots of the MIS queue, filled by shading;
 // U_MPRIM, F_MLEN), walked up to it with its entity the target, items [n, n + pool) the
 // slots' shadow rays (C_SHADOW; the light's entity is the target).  A shadow ray with
 // nothing in the way sets C_LIT in the slot's connect flags, an MIS ray with something in
@@ -626,15 +626,63 @@ __device__ __forceinline__ void queue_mis(const dbatch& B, bool reach, uint32_t 
     if (reach) B.mqueue[at + (uint32_t)__popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane))))] = slot;
 }
 
-// MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn, and the MIS ray of
-// a vertex is tested against the emitter triangles here (connect_path's FAST form): U_MPRIM
-// (RBW_MISS: none) and F_MLEN, the ray's world length up to it, and the slots with one join
-// the MIS queue.  The test reads the ray back from the path state after shading (the lines
-// are in L2): inside shade_path it made the register allocator spill.
+// The vertex's material kind, for grouping vertices by the code they run: 0 no surface
+// (missed, invisible entity), else 1 + the material type (matte, metal, plastic).  A GGX
+// lobe's evaluation and samples run several times the instructions of a Lambert lobe, and
+// a wave runs every kind any of its lanes needs.
+__device__ __forceinline__ uint32_t shade_kind(const dscene& S, uint32_t prim) {
+    if (prim == RBW_MISS) return 0u;
+    uint32_t mid;
+    if (prim < S.num_tris) {
+        mid = S.tri_material ? S.tri_material[prim] : RBW_NONE;
+        if (mid == RBW_NONE) mid = S.ents[S.tri_entity[prim]].material;
+    } else {
+        mid = S.ents[S.sphere_entity[prim - S.num_tris]].material;
+    }
+    if (mid == RBW_NONE) return 0u;
+    const uint32_t t = S.mats[mid].type;
+    return t < 1u ? 1u : (t > 3u ? 3u : t);
+}
+
+// The vertices of the live slots (shade_path), grouped by material kind: each block's
+// threads take its 256-slot window's slots in kind order (a stable counting sort in LDS),
+// so most waves run one kind's code.  A slot's shading does not depend on the thread that
+// runs it.  MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn, and
+// the MIS ray of a vertex is tested against the emitter triangles here (connect_path's
+// FAST form): U_MPRIM (RBW_MISS: none) and F_MLEN, the ray's world length up to it, and
+// the slots with one join the MIS queue.  The test reads the ray back from the path state
+// after shading (the lines are in L2): inside shade_path it made the register allocator
+// spill.
+// (measured on the icosphere workload, shading per render: unsorted 51.6 ms; windows of
+// 128 / 256 / 512 / 1024 slots 51.3 / 49.8 / 55.5 / 60.3 ms: wider windows scatter each
+// wave's path-state accesses over more cache lines)
+constexpr uint32_t k_sort_window = 256;
 template <bool MIS_REACH>
-__global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade(dscene S, dpaths P, dparams base, dbatch B) {
+__global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_sorted(dscene S, dpaths P, dparams base,
+                                                                                      dbatch B) {
+    __shared__ uint32_t counts[4][k_sort_window / 64];
+    __shared__ uint32_t order[k_sort_window];
     if (B.ctl->done) return;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t own = blockIdx.x * k_sort_window + t;
+    uint32_t kind = 0;
+    if (own < B.pool && (P.U(U_STATE, own) & ST_ALIVE)) kind = shade_kind(S, P.U(U_HPRIM, own));
+    uint32_t rank = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; c++) {
+        const uint64_t m = __ballot(kind == c);
+        if (lane == 0) counts[c][wave] = (uint32_t)__popcll(m);
+        if (kind == c) rank = (uint32_t)__popcll(m & below);
+    }
+    __syncthreads();
+    uint32_t at = rank;
+    for (uint32_t c = 0; c < kind; c++)
+        for (uint32_t w = 0; w < k_sort_window / 64; w++) at += counts[c][w];
+    for (uint32_t w = 0; w < wave; w++) at += counts[kind][w];
+    order[at] = own;
+    __syncthreads();
+    const uint32_t slot = order[t];
     bool reach = false;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
@@ -1137,8 +1185,8 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
             // a grid-stride walk over the guided rows: 8192 blocks cover an iteration's typical rows
             hipLaunchKernelGGL(k_b_guided, dim3(std::min<uint32_t>(grid, 8192u)), dim3(bs), 0, s, S, P, base, B);
         } else {
-            hipLaunchKernelGGL(occl_grid ? k_b_shade<true> : k_b_shade<false>, dim3(grid), dim3(bs), 0, s, S, P, base,
-                               B);
+            hipLaunchKernelGGL(occl_grid ? k_b_shade_sorted<true> : k_b_shade_sorted<false>,
+                               dim3(blocks(cfg.pool, k_sort_window)), dim3(k_sort_window), 0, s, S, P, base, B);
         }
         if (tm) tm->end(s);
         if (tm) tm->begin(RBW_KERNEL_CONNECT, s);
