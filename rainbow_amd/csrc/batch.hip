@@ -644,25 +644,16 @@ __device__ __forceinline__ uint32_t shade_kind(const dscene& S, uint32_t prim) {
     return t < 1u ? 1u : (t > 3u ? 3u : t);
 }
 
-// The vertices of the live slots (shade_path), grouped by material kind: each block's
-// threads take its 256-slot window's slots in kind order (a stable counting sort in LDS),
-// so most waves run one kind's code.  A slot's shading does not depend on the thread that
-// runs it.  MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn, and
-// the MIS ray of a vertex is tested against the emitter triangles here (connect_path's
-// FAST form): U_MPRIM (RBW_MISS: none) and F_MLEN, the ray's world length up to it, and
-// the slots with one join the MIS queue.  The test reads the ray back from the path state
-// after shading (the lines are in L2): inside shade_path it made the register allocator
-// spill.
+// The slot whose vertex this thread shades when each block of k_sort_window threads takes
+// its window's live slots grouped by material kind (a stable counting sort of the slot
+// indices in LDS), so most waves run one kind's code.  A slot's shading does not depend on
+// the thread that runs it.  Every thread of the block calls it.
 // (measured on the icosphere workload, shading per render: unsorted 51.6 ms; windows of
 // 128 / 256 / 512 / 1024 slots 51.3 / 49.8 / 55.5 / 60.3 ms: wider windows scatter each
 // wave's path-state accesses over more cache lines)
 constexpr uint32_t k_sort_window = 256;
-template <bool MIS_REACH>
-__global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_sorted(dscene S, dpaths P, dparams base,
-                                                                                      dbatch B) {
-    __shared__ uint32_t counts[4][k_sort_window / 64];
-    __shared__ uint32_t order[k_sort_window];
-    if (B.ctl->done) return;
+__device__ __forceinline__ uint32_t grouped_slot(const dscene& S, const dpaths& P, const dbatch& B,
+                                                 uint32_t (*counts)[k_sort_window / 64], uint32_t* order) {
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t own = blockIdx.x * k_sort_window + t;
@@ -682,7 +673,22 @@ __global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_s
     for (uint32_t w = 0; w < wave; w++) at += counts[kind][w];
     order[at] = own;
     __syncthreads();
-    const uint32_t slot = order[t];
+    return order[t];
+}
+
+// The vertices of the live slots (shade_path), grouped by material kind (grouped_slot).
+// MIS_REACH: the connect stage's occlusion walks run in k_b_occlusion_dyn, and the MIS ray
+// of a vertex is tested against the emitter triangles here (connect_path's FAST form):
+// U_MPRIM (RBW_MISS: none) and F_MLEN, the ray's world length up to it, and the slots with
+// one join the MIS queue.  The test reads the ray back from the path state after shading
+// (the lines are in L2): inside shade_path it made the register allocator spill.
+template <bool MIS_REACH>
+__global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_sorted(dscene S, dpaths P, dparams base,
+                                                                                      dbatch B) {
+    __shared__ uint32_t counts[4][k_sort_window / 64];
+    __shared__ uint32_t order[k_sort_window];
+    if (B.ctl->done) return;
+    const uint32_t slot = grouped_slot(S, P, B, counts, order);
     bool reach = false;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
@@ -712,7 +718,9 @@ __global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_s
 
 // guided jobs: the vertex up to the guided stage.  A guided vertex stores its surface
 // record, material and network query by slot and joins its job's list of guided slots;
-// the others shade as the plain estimator (shade_path).
+// the others shade as the plain estimator (shade_path).  (Grouping the slots by material
+// kind, as k_b_shade_sorted does, left the 16 guided rooms at 1024 spp unchanged: 29.9 s
+// against 29.8 s.)
 __global__ __launch_bounds__(k_batch_max_block) RBW_SHADE_OCCUPANCY void k_b_shade_guided(dscene S, dpaths P,
                                                                                          dparams base, dbatch B) {
     if (B.ctl->done) return;
