@@ -689,7 +689,7 @@ __global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_s
     __shared__ uint32_t order[k_sort_window];
     if (B.ctl->done) return;
     const uint32_t slot = grouped_slot(S, P, B, counts, order);
-    bool reach = false;
+    bool test = false;
     if (slot < B.pool && (P.U(U_STATE, slot) & ST_ALIVE)) {
         bool alive = false, need_connect = false, to_guide = false;
         uint32_t gmat = 0;
@@ -699,21 +699,27 @@ __global__ __launch_bounds__(k_sort_window) RBW_SHADE_OCCUPANCY void k_b_shade_s
         job_view(S, base, B.jobs[P.U(U_JOB, slot)], Sj, pj);
         shade_path<false>(Sj, P, pj, slot, alive, need_connect, to_guide, gsi, gmat);
         P.U(U_STATE, slot) = (alive ? ST_ALIVE : ST_DYING) | (need_connect ? ST_CONNECT : 0u);
-        // (the job's emitter range is read again too: held across shading it spilled)
+        test = MIS_REACH && need_connect;
+    }
+    if (MIS_REACH) {
+        // the slot index and the job's emitter range are read again (LDS, L2): held across
+        // shading they spilled
         __asm__ volatile("" ::: "memory");
-        if (MIS_REACH && need_connect && (P.U(U_CFLAGS, slot) & C_MIS)) {
-            const djob& J = B.jobs[P.U(U_JOB, slot)];
+        const uint32_t ms = order[threadIdx.x];
+        bool reach = false;
+        if (test && (P.U(U_CFLAGS, ms) & C_MIS)) {
+            const djob& J = B.jobs[P.U(U_JOB, ms)];
             f3 eo, ed;
-            spawn_ray(P.F3(F_MPX, slot), P.F3(F_MNX, slot), P.F3(F_MWX, slot), eo, ed);
+            spawn_ray(P.F3(F_MPX, ms), P.F3(F_MNX, ms), P.F3(F_MWX, ms), eo, ed);
             const hitrec h = nearest_emitter_hit(S, S.emitters + J.em_first, J.em_count, eo, ed, base.tri_test);
-            P.U(U_MPRIM, slot) = h.prim;
+            P.U(U_MPRIM, ms) = h.prim;
             if (h.prim != RBW_MISS) {
-                P.F(F_MLEN, slot) = h.t * make_local_ray(eo, ed).L1;
+                P.F(F_MLEN, ms) = h.t * make_local_ray(eo, ed).L1;
                 reach = true;
             }
         }
+        queue_mis(B, reach, ms);
     }
-    if (MIS_REACH) queue_mis(B, reach, slot);
 }
 
 // guided jobs: the vertex up to the guided stage.  A guided vertex stores its surface
