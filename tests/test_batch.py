@@ -170,9 +170,12 @@ def test_batch_64_jobs_match_single_renders(tracer):
 @pytest.mark.gpu
 def test_persistent_extension_walk_matches_one_slot_per_lane(tracer):
     """A render whose pool gives every resident wave a slot range (1024x1024 x 4 spp: 4.2 M
-    slots) runs the persistent four-wide extension walk that refills idle lanes; the same
-    render with visit counting runs the pass schedule, one ray per lane.  Each ray's walk is
-    the same, so the images are equal bit for bit."""
+    slots) runs the persistent four-wide extension walk and the persistent occlusion walk
+    (shadow rays and the queued MIS rays that reach an emitter triangle) that refill idle
+    lanes, and shades each 256-slot window grouped by material kind (the icosphere's
+    triangles are half Lambert, half GGX); the same render with visit counting runs the pass
+    schedule, one ray per lane, in slot order.  Each ray's walk and each vertex's shading are
+    the same, so the images and the ray counts are equal bit for bit."""
     from rainbow_amd.tracer import render_params
     sc, cam = scenes.displaced_icosphere_box(level=6, width=1024, height=1024)
     tracer.upload(sc)
@@ -181,3 +184,6 @@ def test_persistent_extension_walk_matches_one_slot_per_lane(tracer):
     assert a["stats"]["bvh_width"] == 4 and b["stats"]["node_visits"] > 0
     assert np.isfinite(a["image"]).all() and a["image"].mean() > 0
     assert np.array_equal(a["image"], b["image"])
+    for k in ("extension_rays", "shadow_rays", "mis_rays", "light_casts"):
+        print(k, a["stats"][k], b["stats"][k])
+        assert a["stats"][k] == b["stats"][k], k
