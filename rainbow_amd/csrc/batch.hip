@@ -1169,7 +1169,8 @@ void run_batch(rbw_ctx* ctx, const dscene& S, const std::vector<djob>& hjobs, co
     const bool dyn = dyn_grid && cfg.pool >= k_fetch_chunk * dyn_grid;
     // the connect stage's walks run persistent too where they are occlusion queries, except in
     // guided batches (the narrow-aperture rooms, 16 guided jobs at 1024 spp: 30.8 s against
-    // 29.8 s with the one-slot-per-lane connect kernel)
+    // 29.8 s with the one-slot-per-lane connect kernel when a kernel of its own filled the MIS
+    // queue, 29.6 s against 29.7 s when the guided shading kernel did)
     const uint32_t occl_grid =
         dyn && fast && !guided
             ? persistent_blocks(reinterpret_cast<const void*>(&k_b_occlusion_dyn), 64, lds_stack_bytes(64))
