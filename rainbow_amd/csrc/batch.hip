@@ -397,7 +397,7 @@ __global__ __launch_bounds__(k_batch_max_block) void k_b_regen(dscene S, dpaths 
 // (one slot per lane for the whole launch kept ~20 % of the lanes busy).  A slot's walk is
 // traverse4's, so its hit is the same.  One wave per block; launch at most the resident
 // waves (more are harmless: they find the counter past the pool and leave).
-constexpr uint32_t k_fetch_chunk = 256;  // slots per range (1024 measured 11 % slower)
+constexpr uint32_t k_fetch_chunk = 256;  // items per range (1024, 128, 64 measured slower)
 constexpr uint32_t k_refill = 32;        // idle lanes that make a wave refill (8, 16: the same)
 // CONNECT: the walks are the occlusion queries of the slots waiting for the connect stage
 // (ST_CONNECT), connect_path's FAST forms: items [0, n) are the MIS rays that reach a
