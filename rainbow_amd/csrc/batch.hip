@@ -790,7 +790,8 @@ __device__ __forceinline__ void guided_slot(const dscene& S, const dpaths& P, co
 }
 
 // guided jobs: the guided vertex (guided.hpp) of every slot that waits for it, with its job's
-// network outputs.  The threads walk the jobs' lists of guided slots (rows of job 0, then
+// network outputs.  (Grouping each block's rows by material kind, as the plain shading
+// kernel does, left the 16 guided rooms at 1024 spp unchanged, 29.8 s.)  The threads walk the jobs' lists of guided slots (rows of job 0, then
 // job 1, ...) rather than the pool: the guided vertices of an iteration are a scattered
 // minority of the slots (with the first vertex guided, the paths regenerated in it), so a
 // walk over the pool would run each wave's few of them with most lanes idle.
