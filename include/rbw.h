@@ -414,7 +414,8 @@ typedef struct rbw_batch_opts {
     uint32_t block_size;             /* threads per block of the wavefront kernels: 64, 128, 256 or 512; 0 = 256 */
     uint32_t pool_size;              /* path slots of the shared launch; 0 = default 2^21.  Jobs with at
                                         least this many samples run alone, one after another, with at
-                                        least a single render's 2^23 slots; the rest share the launch */
+                                        least a single render's 2^24 slots (fewer for smaller jobs); the
+                                        rest share the launch */
     uint32_t bvh_width;              /* 0 (default: 4), 2, 4 or 8; the autotuner tries all three */
     uint32_t use_graph;              /* 1: replay the bounce loop as a hipGraph; 0: plain launches */
     uint32_t autotune;               /* 1: first probe the jobs themselves (their real spp, each probe

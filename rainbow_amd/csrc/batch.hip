@@ -896,9 +896,11 @@ namespace {
 
 // the single render's schedule (measured best on the 1.3M-triangle icosphere at 1024^2, 64 spp:
 // block 64 / 128 / 256 with 2^23 slots: 348 / 332 / 309 Msamples/s; block 64 with 2^22 / 2^24 /
-// 2^25 slots: 332 / 337 / 312)
+// 2^25 slots: 332 / 337 / 312, one slot per lane; with the persistent extension and occlusion
+// walks and grouped shading, 2^23 / 2^24 slots: 430 / 445: the persistent walks gain from
+// more items in flight)
 constexpr uint32_t k_render_block = 64;
-constexpr uint32_t k_render_pool = 1u << 23;
+constexpr uint32_t k_render_pool = 1u << 24;
 
 struct batch_config {
     uint32_t block = 256, pool = 1u << 21, width = k_default_width;
