@@ -1446,7 +1446,8 @@ void autotune_jobs(rbw_ctx* ctx, const rbw_job* jobs, size_t n, rbw_batch_opts* 
             }
         }
     }
-    // the pool: 2^20, 2^21 or 2^22 slots
+    // the pool: 2^20, 2^21 or 2^22 slots (adding 2^23 on the 64-job workload: chosen, the same
+    // 476 Msamples/s, autotuning 0.91 s against 0.59)
     if (n > 1) {
         const uint32_t cands[3] = {1u << 20, 1u << 21, 1u << 22};
         std::vector<double> alone_rate(n, 0.0);
